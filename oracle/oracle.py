@@ -1,0 +1,306 @@
+"""ctypes binding of the CPU oracle (oracle/gs_oracle.c) and of the reference
+compiled from source (oracle/_ref/libgs_ref.so, when it was built).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+REF_LIB = os.path.join(HERE, "_ref", "libgs_ref.so")
+
+_i32 = ctypes.c_int
+_i64 = ctypes.c_longlong
+_f32 = ctypes.c_float
+_vp = ctypes.c_void_p
+
+
+class _Cam(ctypes.Structure):
+    _fields_ = [("view_matrix", ctypes.c_float * 16), ("viewproj_matrix", ctypes.c_float * 16),
+                ("cam_pos", ctypes.c_float * 3), ("tan_fovx", ctypes.c_float), ("tan_fovy", ctypes.c_float),
+                ("width", ctypes.c_int), ("height", ctypes.c_int)]
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [("bg_color", ctypes.c_float * 3), ("scale_modifier", ctypes.c_float),
+                ("antialiasing", ctypes.c_int), ("training", ctypes.c_int), ("sh_degree", ctypes.c_int),
+                ("tile_w", ctypes.c_int), ("tile_h", ctypes.c_int)]
+
+
+def cam_struct(cam) -> _Cam:
+    c = _Cam()
+    for i in range(16):
+        c.view_matrix[i] = float(cam.view[i])
+        c.viewproj_matrix[i] = float(cam.viewproj[i])
+    for i in range(3):
+        c.cam_pos[i] = float(cam.cam_pos[i])
+    c.tan_fovx, c.tan_fovy = float(cam.tan_fovx), float(cam.tan_fovy)
+    c.width, c.height = int(cam.width), int(cam.height)
+    return c
+
+
+def cfg_struct(cfg) -> _Cfg:
+    c = _Cfg()
+    for i in range(3):
+        c.bg_color[i] = float(cfg.bg_color[i])
+    c.scale_modifier = float(cfg.scale_modifier)
+    c.antialiasing, c.training = int(cfg.antialiasing), int(cfg.training)
+    c.sh_degree, c.tile_w, c.tile_h = int(cfg.sh_degree), int(cfg.tile_w), int(cfg.tile_h)
+    return c
+
+
+def build():
+    """Compiles the oracle (and, where the reference tree is present, oracle/_ref)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.isdir("/root/reference/harmonyos_3dgs/src"):
+        subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile.ref")], check=True)
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+class Oracle:
+    """The float (``double=False``) or FP64 (``double=True``) oracle."""
+
+    def __init__(self, double: bool = False):
+        name = "libgs_oracle64.so" if double else "libgs_oracle.so"
+        path = os.path.join(BUILD, name)
+        if not os.path.exists(path):
+            build()
+        self.lib = ctypes.CDLL(path)
+        self.double = double
+        self.dt = np.float64 if double else np.float32
+        p = "gsd_" if double else "gso_"
+        L = self.lib
+        self._activate = getattr(L, p + "activate")
+        self._preprocess = getattr(L, p + "preprocess")
+        self._count = getattr(L, p + "count_pairs")
+        self._count.restype = _i64
+        self._dup = getattr(L, p + "duplicate")
+        self._sort = getattr(L, p + "sort_pairs")
+        self._sort.argtypes = [_i64, _vp, _vp, _vp, _vp, _i32, _vp]
+        self._raster = getattr(L, p + "rasterize")
+        self._l1 = getattr(L, p + "l1_loss")
+        self._l1.restype = _f32
+        self._l1d = getattr(L, p + "l1_loss_double")
+        self._l1d.restype = ctypes.c_double
+        self._rbwd = getattr(L, p + "raster_backward")
+        self._pbwd = getattr(L, p + "preprocess_backward")
+        self._adam = getattr(L, p + "adam_update")
+        self._adam.argtypes = [_i64, _vp, _vp, _vp, _vp, _f32, _f32, _f32, _f32, _f32, _f32]
+        self._bc = getattr(L, p + "adam_bias_corrections")
+        self._bc.argtypes = [_f32, _f32, _i32, _vp, _vp]
+        self._lrs = getattr(L, p + "lr_schedule")
+        self._lrs.argtypes = [_f32, _f32, _i32, _i32]
+        self._lrs.restype = _f32
+
+    def _a(self, x, shape=None):
+        a = np.ascontiguousarray(x, dtype=self.dt)
+        return a if shape is None else a.reshape(shape)
+
+    # --- stages -----------------------------------------------------------
+    def activate(self, scene):
+        n, k = scene.n, scene.max_coeffs
+        raw = [self._a(a) for a in scene.arrays()]
+        pos, sc, rot, sh, op = (np.empty_like(raw[0]), np.empty_like(raw[1]), np.empty_like(raw[2]),
+                                np.empty_like(raw[3]), np.empty_like(raw[4]))
+        self._activate(_i32(n), _i32(k), *[_p(a) for a in raw], _p(pos), _p(sc), _p(rot), _p(sh), _p(op))
+        return dict(pos=pos, scale=sc, rot=rot, sh=sh, op=op)
+
+    def preprocess(self, g, n, model_deg, max_coeffs, cam, cfg, cache=True):
+        dt = self.dt
+        out = dict(means2D=np.zeros((n, 2), dt), depths=np.zeros(n, dt), conics=np.zeros((n, 3), dt),
+                   op2d=np.zeros(n, dt), rgb=np.zeros((n, 3), dt), radii=np.zeros(n, np.int32),
+                   tiles=np.zeros(n, np.int32))
+        if cache:
+            out.update(cov3D=np.zeros((n, 6), dt), p_view=np.zeros((n, 3), dt), p_hom_w=np.zeros(n, dt),
+                       cov2D=np.zeros((n, 3), dt), det=np.zeros(n, dt))
+        c, f = cam_struct(cam), cfg_struct(cfg)
+        self._preprocess(_i32(n), _i32(model_deg), _i32(max_coeffs), _p(g["pos"]), _p(g["sh"]),
+                         _p(g["scale"]), _p(g["rot"]), _p(g["op"]), ctypes.byref(c), ctypes.byref(f),
+                         _p(out["means2D"]), _p(out["depths"]), _p(out["conics"]), _p(out["op2d"]),
+                         _p(out["rgb"]), _p(out["radii"]), _p(out["tiles"]),
+                         _p(out.get("cov3D")), _p(out.get("p_view")), _p(out.get("p_hom_w")),
+                         _p(out.get("cov2D")), _p(out.get("det")))
+        return out
+
+    def duplicate(self, pre, n, width, height, tile=16):
+        tiles = np.ascontiguousarray(pre["tiles"], np.int32)
+        K = int(self._count(_i32(n), _p(tiles)))
+        keys = np.zeros(max(K, 1), np.uint64)
+        vals = np.zeros(max(K, 1), np.uint32)
+        self._dup(_i32(n), _p(self._a(pre["means2D"])), _p(self._a(pre["depths"])),
+                  _p(np.ascontiguousarray(pre["radii"], np.int32)), _p(tiles), _i32(width), _i32(height),
+                  _i32(tile), _i32(tile), _p(keys), _p(vals))
+        return keys[:K], vals[:K]
+
+    def sort_pairs(self, keys, vals, num_tiles):
+        K = len(keys)
+        keys = np.ascontiguousarray(keys, np.uint64)
+        vals = np.ascontiguousarray(vals, np.uint32)
+        ko = np.zeros(max(K, 1), np.uint64)
+        vo = np.zeros(max(K, 1), np.uint32)
+        ranges = np.zeros(num_tiles * 2, np.uint32)
+        self._sort(K, _p(keys), _p(vals), _p(ko), _p(vo), num_tiles, _p(ranges))
+        return ko[:K], vo[:K], ranges
+
+    def rasterize(self, pre, values_sorted, ranges, width, height, bg=(0, 0, 0), tile=16):
+        dt = self.dt
+        img = np.zeros((height, width, 3), dt)
+        T = np.zeros((height, width), dt)
+        nc = np.zeros((height, width), np.int32)
+        last = np.zeros((height, width), np.uint32)
+        bgv = np.asarray(bg, np.float32)
+        vs = np.ascontiguousarray(values_sorted, np.uint32)
+        if vs.size == 0:
+            vs = np.zeros(1, np.uint32)
+        self._raster(_p(self._a(pre["means2D"])), _p(self._a(pre["conics"])), _p(self._a(pre["op2d"])),
+                     _p(self._a(pre["rgb"])), _p(vs), _p(np.ascontiguousarray(ranges, np.uint32)),
+                     _i32(width), _i32(height), _i32(tile), _i32(tile), _p(bgv), _p(img), _p(T), _p(nc),
+                     _p(last))
+        return img, T, nc, last
+
+    def l1_loss(self, rendered, gt, double_sum=False):
+        r = self._a(rendered)
+        g = self._a(gt)
+        h, w = r.shape[0], r.shape[1]
+        d = np.zeros_like(r)
+        fn = self._l1d if double_sum else self._l1
+        loss = fn(_p(r), _p(g), _i32(h), _i32(w), _p(d))
+        return loss, d
+
+    def raster_backward(self, pre, values_sorted, ranges, width, height, T_final, n_contrib, d_image,
+                        bg=(0, 0, 0), tile=16):
+        n = pre["radii"].shape[0]
+        dt = self.dt
+        g = dict(d_means2D=np.zeros((n, 2), dt), d_conics=np.zeros((n, 3), dt), d_rgb=np.zeros((n, 3), dt),
+                 d_op2d=np.zeros(n, dt))
+        vs = np.ascontiguousarray(values_sorted, np.uint32)
+        if vs.size == 0:
+            vs = np.zeros(1, np.uint32)
+        bgv = np.asarray(bg, np.float32)
+        self._rbwd(_p(self._a(pre["means2D"])), _p(self._a(pre["conics"])), _p(self._a(pre["op2d"])),
+                   _p(self._a(pre["rgb"])), _p(vs), _p(np.ascontiguousarray(ranges, np.uint32)), _i32(width),
+                   _i32(height), _i32(tile), _i32(tile), _p(bgv), _p(self._a(T_final)),
+                   _p(np.ascontiguousarray(n_contrib, np.int32)), _p(self._a(d_image)), _p(g["d_means2D"]),
+                   _p(g["d_conics"]), _p(g["d_rgb"]), _p(g["d_op2d"]))
+        return g
+
+    def preprocess_backward(self, g, raw_rot, n, model_deg, max_coeffs, cam, cfg, pre, rg):
+        dt = self.dt
+        out = dict(pos=np.zeros((n, 3), dt), scale=np.zeros((n, 3), dt), rot=np.zeros((n, 4), dt),
+                   sh=np.zeros((n, max_coeffs, 3), dt), op=np.zeros(n, dt))
+        c, f = cam_struct(cam), cfg_struct(cfg)
+        self._pbwd(_i32(n), _i32(model_deg), _i32(max_coeffs), _p(g["pos"]), _p(g["sh"]), _p(g["scale"]),
+                   _p(g["rot"]), _p(g["op"]), _p(self._a(raw_rot)), ctypes.byref(c), ctypes.byref(f),
+                   _p(np.ascontiguousarray(pre["radii"], np.int32)), _p(self._a(pre["means2D"])),
+                   _p(self._a(pre["cov3D"])), _p(self._a(pre["p_view"])), _p(self._a(pre["p_hom_w"])),
+                   _p(self._a(pre["cov2D"])), _p(self._a(pre["det"])), _p(self._a(rg["d_means2D"])),
+                   _p(self._a(rg["d_conics"])), _p(self._a(rg["d_rgb"])), _p(self._a(rg["d_op2d"])),
+                   _p(out["pos"]), _p(out["scale"]), _p(out["rot"]), _p(out["sh"]), _p(out["op"]))
+        return out
+
+    def adam_update(self, p, g, m, v, lr, b1, b2, eps, bc1, bc2):
+        """In place on p, m, v (float32 contiguous arrays)."""
+        self._adam(p.size, _p(p), _p(np.ascontiguousarray(g, self.dt)), _p(m), _p(v), lr, b1, b2, eps, bc1, bc2)
+
+    def bias_corrections(self, b1, b2, t):
+        a, b = _f32(), _f32()
+        self._bc(b1, b2, t, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def lr_schedule(self, lr_init, lr_final, step, max_steps):
+        return self._lrs(lr_init, lr_final, step, max_steps)
+
+    # --- whole path -------------------------------------------------------
+    def forward(self, scene, cam, cfg, backward_cache=True):
+        g = self.activate(scene)
+        n = scene.n
+        pre = self.preprocess(g, n, scene.sh_degree, scene.max_coeffs, cam, cfg, cache=backward_cache)
+        keys, vals = self.duplicate(pre, n, cam.width, cam.height, cfg.tile_w)
+        ks, vs, ranges = self.sort_pairs(keys, vals, cam.num_tiles)
+        img, T, nc, last = self.rasterize(pre, vs, ranges, cam.width, cam.height, cfg.bg_color, cfg.tile_w)
+        return dict(g=g, pre=pre, keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges,
+                    img=img, T_final=T, n_contrib=nc, last_contrib=last)
+
+    def forward_backward(self, scene, cam, cfg, gt, double_loss=True):
+        f = self.forward(scene, cam, cfg)
+        loss, d_img = self.l1_loss(f["img"], gt, double_sum=double_loss)
+        rg = self.raster_backward(f["pre"], f["values_sorted"], f["ranges"], cam.width, cam.height,
+                                  f["T_final"], f["n_contrib"], d_img, cfg.bg_color, cfg.tile_w)
+        grads = self.preprocess_backward(f["g"], scene.rot, scene.n, scene.sh_degree, scene.max_coeffs, cam,
+                                         cfg, f["pre"], rg)
+        f.update(loss=loss, d_image=d_img, rgrad=rg, grads=grads)
+        return f
+
+    def loss(self, scene, cam, cfg, gt):
+        """L1 loss of the full forward (double accumulation)."""
+        f = self.forward(scene, cam, cfg, backward_cache=False)
+        return self.l1_loss(f["img"], gt, double_sum=True)[0]
+
+
+class RefLib:
+    """The reference's own CPU classes compiled from source (oracle/_ref)."""
+
+    def __init__(self):
+        if not os.path.exists(REF_LIB):
+            raise FileNotFoundError(REF_LIB)
+        self.lib = ctypes.CDLL(REF_LIB)
+        self.lib.ref_forward_backward.restype = _i64
+        self.lib.ref_train_steps.restype = _f32
+
+    @staticmethod
+    def available() -> bool:
+        return os.path.exists(REF_LIB)
+
+    def forward_backward(self, scene, cam, cfg, gt, arena_bytes=None):
+        n, k = scene.n, scene.max_coeffs
+        raw = [np.ascontiguousarray(a, np.float32).copy() for a in scene.arrays()]
+        H, W = cam.height, cam.width
+        o = dict(img=np.zeros((H, W, 3), np.float32), T_final=np.zeros((H, W), np.float32),
+                 n_contrib=np.zeros((H, W), np.int32), means2D=np.zeros((n, 2), np.float32),
+                 depths=np.zeros(n, np.float32), conics=np.zeros((n, 3), np.float32),
+                 op2d=np.zeros(n, np.float32), rgb=np.zeros((n, 3), np.float32), radii=np.zeros(n, np.int32),
+                 tiles=np.zeros(n, np.int32), g_pos=np.zeros((n, 3), np.float32),
+                 g_scale=np.zeros((n, 3), np.float32), g_rot=np.zeros((n, 4), np.float32),
+                 g_sh=np.zeros((n, k, 3), np.float32), g_op=np.zeros(n, np.float32),
+                 d_means2D=np.zeros((n, 2), np.float32), d_conics=np.zeros((n, 3), np.float32),
+                 d_rgb=np.zeros((n, 3), np.float32), d_op2d=np.zeros(n, np.float32))
+        loss = ctypes.c_double()
+        c, f = cam_struct(cam), cfg_struct(cfg)
+        if arena_bytes is None:
+            arena_bytes = 64 * 1024 * 1024 + n * 2048 + H * W * 64
+        K = self.lib.ref_forward_backward(
+            _i32(n), _i32(scene.sh_degree), _i32(k), *[_p(a) for a in raw], ctypes.byref(c), ctypes.byref(f),
+            _p(np.ascontiguousarray(gt, np.float32)), ctypes.c_size_t(arena_bytes), _p(o["img"]),
+            ctypes.byref(loss), _p(o["T_final"]), _p(o["n_contrib"]), _p(o["means2D"]), _p(o["depths"]),
+            _p(o["conics"]), _p(o["op2d"]), _p(o["rgb"]), _p(o["radii"]), _p(o["tiles"]), _p(o["g_pos"]),
+            _p(o["g_scale"]), _p(o["g_rot"]), _p(o["g_sh"]), _p(o["g_op"]), _p(o["d_means2D"]),
+            _p(o["d_conics"]), _p(o["d_rgb"]), _p(o["d_op2d"]))
+        o["loss"] = loss.value
+        o["K"] = int(K)
+        return o
+
+    def train_steps(self, scene, cam, cfg, gt, tc, first_iteration, steps, arena_bytes=None):
+        """Runs Trainer::step with Adam ``steps`` times; returns (loss, raw arrays, image)."""
+        n, k = scene.n, scene.max_coeffs
+        raw = [np.ascontiguousarray(a, np.float32).copy() for a in scene.arrays()]
+        img = np.zeros((cam.height, cam.width, 3), np.float32)
+        c, f = cam_struct(cam), cfg_struct(cfg)
+        if arena_bytes is None:
+            arena_bytes = 64 * 1024 * 1024 + n * 2048 + cam.width * cam.height * 64
+        loss = self.lib.ref_train_steps(
+            _i32(n), _i32(scene.sh_degree), _i32(k), *[_p(a) for a in raw], ctypes.byref(c), ctypes.byref(f),
+            _p(np.ascontiguousarray(gt, np.float32)), _f32(tc["lr_pos_init"]), _f32(tc["lr_pos_final"]),
+            _f32(tc["lr_feature"]), _f32(tc["lr_opacity"]), _f32(tc["lr_scaling"]), _f32(tc["lr_rotation"]),
+            _i32(tc["max_steps"]), _f32(tc["beta1"]), _f32(tc["beta2"]), _f32(tc["eps"]), _i32(first_iteration),
+            _i32(steps), ctypes.c_size_t(arena_bytes), _p(img))
+        return float(loss), raw, img

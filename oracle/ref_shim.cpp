@@ -1,0 +1,147 @@
+// ref_shim.cpp -- extern "C" driver around the reference's own CPU classes.
+//
+// TEST INFRASTRUCTURE ONLY (the checker, never the product).  oracle/Makefile.ref
+// compiles this file together with the reference's CPU sources where they lie
+// under harmonyos_3dgs/src (nothing is copied), into oracle/_ref/libgs_ref.so.
+// It is used to validate the restatement in gs_oracle.c and as the "reference"
+// CPU baseline.  It follows the call sequence of harmonyos_3dgs/src/
+// verify_grads_main.cpp:83-177 (forward + L1 + backward) and of
+// Trainer::step (trainer.cpp:7-70) for full training steps.
+#include "types.h"
+#include "train_types.h"
+#include "loss.h"
+#include "optimizer.h"
+#include "trainer.h"
+#include "cpu/preprocessor_cpu.h"
+#include "cpu/tile_binner_cpu.h"
+#include "cpu/sorter_cpu.h"
+#include "cpu/rasterizer_cpu.h"
+#include "cpu/rasterizer_backward_cpu.h"
+#include "cpu/preprocessor_backward_cpu.h"
+#include <cstring>
+#include <memory>
+#include <vector>
+
+struct gs_camera {
+    float view_matrix[16], viewproj_matrix[16], cam_pos[3], tan_fovx, tan_fovy;
+    int width, height;
+};
+struct gs_render_cfg {
+    float bg_color[3];
+    float scale_modifier;
+    int antialiasing, training, sh_degree, tile_w, tile_h;
+};
+
+static Camera to_cam(const gs_camera* c) {
+    Camera cam{};
+    std::memcpy(cam.view_matrix, c->view_matrix, 64);
+    std::memcpy(cam.viewproj_matrix, c->viewproj_matrix, 64);
+    std::memcpy(cam.cam_pos, c->cam_pos, 12);
+    cam.tan_fovx = c->tan_fovx; cam.tan_fovy = c->tan_fovy;
+    cam.width = c->width; cam.height = c->height;
+    return cam;
+}
+static RenderConfig to_cfg(const gs_render_cfg* c) {
+    RenderConfig cfg;
+    for (int i = 0; i < 3; i++) cfg.bg_color[i] = c->bg_color[i];
+    cfg.scale_modifier = c->scale_modifier;
+    cfg.antialiasing = c->antialiasing != 0;
+    cfg.eval_3D = false;
+    cfg.training = c->training != 0;
+    cfg.sh_degree = c->sh_degree;
+    cfg.tile_w = c->tile_w; cfg.tile_h = c->tile_h;
+    return cfg;
+}
+
+extern "C" {
+
+// Forward (activation, preprocess with cache, bin, sort, raster), L1 loss in
+// double, raster backward and preprocess backward.  Returns the pair count, or
+// -1 if an output buffer of `max_pairs` entries is too small.
+long long ref_forward_backward(int N, int sh_degree, int max_coeffs, float* raw_pos, float* raw_sc,
+                               float* raw_rot, float* raw_sh, float* raw_op, const gs_camera* camp,
+                               const gs_render_cfg* cfgp, const float* gt, size_t arena_bytes,
+                               float* img, double* loss, float* T_final, int* n_contrib,
+                               float* means2D, float* depths, float* conics, float* op2d, float* rgb,
+                               int* radii, int* tiles, float* g_pos, float* g_sc, float* g_rot,
+                               float* g_sh, float* g_op, float* d_means2D, float* d_conics,
+                               float* d_rgb, float* d_op2d) {
+    RawGaussianParams raw;
+    raw.count = N; raw.sh_degree = sh_degree; raw.max_coeffs = max_coeffs;
+    raw.raw_positions = raw_pos; raw.raw_scales = raw_sc; raw.raw_rotations = raw_rot;
+    raw.raw_sh_coeffs = raw_sh; raw.raw_opacities = raw_op;
+    std::vector<float> ap(N * 3), as(N * 3), ar(N * 4), ash((size_t)N * max_coeffs * 3), ao(N);
+    GaussianData g;
+    g.count = N; g.sh_degree = sh_degree; g.max_coeffs = max_coeffs;
+    g.positions = ap.data(); g.scales = as.data(); g.rotations = ar.data();
+    g.sh_coeffs = ash.data(); g.opacities = ao.data(); g.filter_3D = nullptr;
+    raw.activate(g);
+    Camera cam = to_cam(camp);
+    RenderConfig cfg = to_cfg(cfgp);
+    FrameAllocator alloc(arena_bytes);
+    PreprocessorCPU pp; TileBinnerCPU bn; SorterCPU sr; RasterizerCPU rs;
+    ForwardCache cache{};
+    PreprocessOutput pre = pp.process(g, cam, cfg, alloc, &cache);
+    BinningOutput bin = bn.bin(pre, N, cam, cfg, alloc);
+    if (bin.total_pairs > 0) sr.sort(bin, alloc);
+    cache.pre = &pre; cache.bin = &bin;
+    int npix = cam.width * cam.height;
+    rs.rasterize(pre, bin, cam, cfg, img, nullptr, &cache, &alloc);
+    float* d_image = alloc.allocate_array<float>((size_t)npix * 3);
+    *loss = l1_loss_double(img, gt, cam.height, cam.width, d_image);
+    RasterGradOutput rg; rg.allocate_and_zero(alloc, N);
+    RasterizerBackwardCPU rb; rb.backward(pre, bin, cam, cfg, cache, d_image, rg);
+    GradientOutput gr; gr.allocate_and_zero(alloc, N, max_coeffs);
+    PreprocessorBackwardCPU pb; pb.backward(g, cam, cfg, cache, rg, raw, gr);
+
+    std::memcpy(T_final, cache.T_final, npix * 4);
+    std::memcpy(n_contrib, cache.n_contrib, npix * 4);
+    for (int i = 0; i < N; i++) {
+        bool vis = pre.radii[i] > 0;
+        radii[i] = pre.radii[i]; tiles[i] = pre.tiles_touched[i];
+        means2D[2 * i] = vis ? pre.means2D[2 * i] : 0; means2D[2 * i + 1] = vis ? pre.means2D[2 * i + 1] : 0;
+        depths[i] = vis ? pre.depths[i] : 0; op2d[i] = vis ? pre.opacities_2d[i] : 0;
+        for (int k = 0; k < 3; k++) {
+            conics[3 * i + k] = vis ? pre.conics[3 * i + k] : 0;
+            rgb[3 * i + k] = vis ? pre.rgb[3 * i + k] : 0;
+        }
+    }
+    std::memcpy(g_pos, gr.d_raw_positions, N * 12);
+    std::memcpy(g_sc, gr.d_raw_scales, N * 12);
+    std::memcpy(g_rot, gr.d_raw_rotations, N * 16);
+    std::memcpy(g_sh, gr.d_raw_sh_coeffs, (size_t)N * max_coeffs * 12);
+    std::memcpy(g_op, gr.d_raw_opacities, N * 4);
+    std::memcpy(d_means2D, rg.d_means2D, N * 8);
+    std::memcpy(d_conics, rg.d_conics, N * 12);
+    std::memcpy(d_rgb, rg.d_rgb, N * 12);
+    std::memcpy(d_op2d, rg.d_opacities_2d, N * 4);
+    return bin.total_pairs;
+}
+
+// `steps` calls of Trainer::step with Adam on one camera; raw params are
+// updated in place.  Returns the loss of the last step.
+float ref_train_steps(int N, int sh_degree, int max_coeffs, float* raw_pos, float* raw_sc,
+                      float* raw_rot, float* raw_sh, float* raw_op, const gs_camera* camp,
+                      const gs_render_cfg* cfgp, const float* gt, float lr_pos_init,
+                      float lr_pos_final, float lr_feature, float lr_opacity, float lr_scaling,
+                      float lr_rotation, int max_steps, float beta1, float beta2, float eps,
+                      int first_iteration, int steps, size_t arena_bytes, float* img_out) {
+    RawGaussianParams raw;
+    raw.count = N; raw.sh_degree = sh_degree; raw.max_coeffs = max_coeffs;
+    raw.raw_positions = raw_pos; raw.raw_scales = raw_sc; raw.raw_rotations = raw_rot;
+    raw.raw_sh_coeffs = raw_sh; raw.raw_opacities = raw_op;
+    Camera cam = to_cam(camp);
+    RenderConfig cfg = to_cfg(cfgp);
+    TrainConfig tc;
+    tc.lr_position_init = lr_pos_init; tc.lr_position_final = lr_pos_final;
+    tc.lr_feature = lr_feature; tc.lr_opacity = lr_opacity; tc.lr_scaling = lr_scaling;
+    tc.lr_rotation = lr_rotation; tc.max_steps = max_steps; tc.use_adam = true;
+    tc.adam_beta1 = beta1; tc.adam_beta2 = beta2; tc.adam_eps = eps;
+    std::unique_ptr<Trainer> tr(new Trainer(arena_bytes));
+    float loss = 0.f;
+    for (int s = 0; s < steps; s++) loss = tr->step(raw, cam, gt, cfg, tc, first_iteration + s).loss;
+    if (img_out) std::memcpy(img_out, tr->last_rendered(), (size_t)cam.width * cam.height * 12);
+    return loss;
+}
+
+}  // extern "C"
