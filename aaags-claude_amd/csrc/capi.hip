@@ -1,0 +1,446 @@
+// capi.hip -- the C ABI (include/gs_hip.h): argument checks, the context's
+// workspace, and kernel launches on the context stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gs_device.h"
+#include "gs_internal.h"
+
+static thread_local std::string g_err;
+
+int gs_set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int gs_hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return GS_OK;
+    return gs_set_error(GS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+extern "C" const char* gs_last_error(void) { return g_err.c_str(); }
+
+#define GS_CHECK_ARG(cond, ...)                                    \
+    do {                                                           \
+        if (!(cond)) return gs_set_error(GS_ERR_INVALID, __VA_ARGS__); \
+    } while (0)
+#define GS_HIP(expr, what)                            \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return gs_hip_status(e_, what); \
+    } while (0)
+
+static bool aligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+// ------------------------------------------------------------- context ----
+extern "C" int gs_ctx_create(int device, void* stream, gs_ctx** out) {
+    GS_CHECK_ARG(out != nullptr, "gs_ctx_create: out is null");
+    int n = 0;
+    GS_HIP(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    GS_CHECK_ARG(device >= 0 && device < n, "gs_ctx_create: device %d out of range (%d devices)", device, n);
+    GS_HIP(hipSetDevice(device), "hipSetDevice");
+    gs_ctx* c = new gs_ctx();
+    c->device = device;
+    c->stream = static_cast<hipStream_t>(stream);
+    *out = c;
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_destroy(gs_ctx* c) {
+    if (!c) return GS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->ws) (void)hipFree(c->ws);
+    if (c->small) (void)hipFree(c->small);
+    delete c;
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_set_stream(gs_ctx* c, void* stream) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    c->stream = static_cast<hipStream_t>(stream);
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_synchronize(gs_ctx* c) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_set_tuning(gs_ctx* c, int radix_bits, int raster_batch, int raster_bwd_batch) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    if (radix_bits) {
+        GS_CHECK_ARG(radix_bits >= 4 && radix_bits <= 8, "radix_bits must be in [4, 8]");
+        c->radix_bits = radix_bits;
+    }
+    if (raster_batch) {
+        GS_CHECK_ARG(raster_batch == 64 || raster_batch == 128 || raster_batch == 256 || raster_batch == 512,
+                     "raster_batch must be 64, 128, 256 or 512");
+        c->raster_batch = raster_batch;
+    }
+    if (raster_bwd_batch) {
+        GS_CHECK_ARG(raster_bwd_batch == 64 || raster_bwd_batch == 128 || raster_bwd_batch == 256,
+                     "raster_bwd_batch must be 64, 128 or 256");
+        c->raster_bwd_batch = raster_bwd_batch;
+    }
+    return GS_OK;
+}
+
+// Grows the workspace to at least `bytes`.  Not for use inside graph capture:
+// callers that capture reserve the workspace beforehand (gs_ctx_reserve).
+int gs_ws_get(gs_ctx* c, size_t bytes, void** out) {
+    if (bytes > c->ws_bytes) {
+        if (c->capturing)
+            return gs_set_error(GS_ERR_CAPACITY, "workspace of %zu bytes needed during capture (%zu reserved)", bytes,
+                                c->ws_bytes);
+        GS_HIP(hipSetDevice(c->device), "hipSetDevice");
+        if (c->ws) {
+            GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            GS_HIP(hipFree(c->ws), "hipFree(workspace)");
+            c->ws = nullptr;
+            c->ws_bytes = 0;
+        }
+        size_t nb = bytes + bytes / 4 + (1 << 20);
+        GS_HIP(hipMalloc(&c->ws, nb), "hipMalloc(workspace)");
+        c->ws_bytes = nb;
+    }
+    *out = c->ws;
+    return GS_OK;
+}
+
+// Small fixed scratch: scan partials, overflow flags, loss partials.
+int gs_small_get(gs_ctx* c, void** out) {
+    if (!c->small) {
+        GS_HIP(hipSetDevice(c->device), "hipSetDevice");
+        GS_HIP(hipMalloc(&c->small, GS_SMALL_BYTES), "hipMalloc(small scratch)");
+        GS_HIP(hipMemsetAsync(c->small, 0, GS_SMALL_BYTES, c->stream), "hipMemsetAsync(small scratch)");
+    }
+    *out = c->small;
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_reserve(gs_ctx* c, size_t bytes) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    void* p;
+    int st = gs_ws_get(c, bytes, &p);
+    if (st) return st;
+    return gs_small_get(c, &p);
+}
+
+extern "C" int gs_zero(gs_ctx* c, void* ptr, size_t bytes) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    if (bytes == 0) return GS_OK;
+    GS_CHECK_ARG(ptr != nullptr, "gs_zero: null pointer");
+    GS_HIP(launch_zero(ptr, bytes, c->stream), "zero kernel");
+    return GS_OK;
+}
+
+// --------------------------------------------------------------- helpers ----
+gsd::CamArgs make_cam_args(const gs_camera* cam, const gs_render_cfg* cfg) {
+    gsd::CamArgs a;
+    std::memcpy(a.view, cam->view_matrix, sizeof(a.view));
+    std::memcpy(a.vp, cam->viewproj_matrix, sizeof(a.vp));
+    std::memcpy(a.campos, cam->cam_pos, sizeof(a.campos));
+    a.tanx = cam->tan_fovx;
+    a.tany = cam->tan_fovy;
+    // preprocessor_cpu.cpp:50-51 (host float arithmetic, identical on the device)
+    a.fx = (float)cam->width / (2.0f * cam->tan_fovx);
+    a.fy = (float)cam->height / (2.0f * cam->tan_fovy);
+    a.W = cam->width;
+    a.H = cam->height;
+    a.gx = (cam->width + gsd::TILE - 1) / gsd::TILE;
+    a.gy = (cam->height + gsd::TILE - 1) / gsd::TILE;
+    a.scale_mod = cfg->scale_modifier;
+    a.antialiasing = cfg->antialiasing;
+    a.training = cfg->training;
+    a.deg_cfg = cfg->sh_degree;
+    return a;
+}
+
+int check_cam_cfg(const gs_camera* cam, const gs_render_cfg* cfg) {
+    GS_CHECK_ARG(cam && cfg, "null camera or render config");
+    GS_CHECK_ARG(cam->width > 0 && cam->height > 0, "camera size %dx%d", cam->width, cam->height);
+    GS_CHECK_ARG(cfg->tile_w == 16 && cfg->tile_h == 16, "tiles must be 16x16 (got %dx%d)", cfg->tile_w, cfg->tile_h);
+    return GS_OK;
+}
+
+static int check_gaussians(const gs_gaussians* g) {
+    GS_CHECK_ARG(g != nullptr, "null gaussians");
+    GS_CHECK_ARG(g->count >= 0, "negative Gaussian count");
+    GS_CHECK_ARG(g->max_coeffs == 1 || g->max_coeffs == 4 || g->max_coeffs == 9 || g->max_coeffs == 16,
+                 "max_coeffs must be 1, 4, 9 or 16 (got %d)", g->max_coeffs);
+    GS_CHECK_ARG(g->sh_degree >= 0 && (g->sh_degree + 1) * (g->sh_degree + 1) <= g->max_coeffs,
+                 "sh_degree %d does not fit max_coeffs %d", g->sh_degree, g->max_coeffs);
+    if (g->count == 0) return GS_OK;
+    GS_CHECK_ARG(g->positions && g->scales && g->rotations && g->sh_coeffs && g->opacities,
+                 "null Gaussian parameter array");
+    GS_CHECK_ARG(aligned(g->rotations, 16), "rotations must be 16-byte aligned");
+    GS_CHECK_ARG(aligned(g->sh_coeffs, 16), "sh_coeffs must be 16-byte aligned");
+    return GS_OK;
+}
+
+static int check_pre(const gs_preprocessed* p, bool need_all) {
+    GS_CHECK_ARG(p != nullptr, "null preprocess buffers");
+    GS_CHECK_ARG(p->means2D && p->conics && p->opacities && p->rgb, "null preprocess array");
+    GS_CHECK_ARG(aligned(p->means2D, 8), "means2D must be 8-byte aligned");
+    if (need_all) GS_CHECK_ARG(p->depths && p->radii && p->tiles_touched, "null preprocess array");
+    return GS_OK;
+}
+
+// ---------------------------------------------------------- entry points ----
+extern "C" int gs_activate(gs_ctx* c, const gs_gaussians* raw, float* pos, float* sc, float* rot, float* sh, float* op) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    int st = check_gaussians(raw);
+    if (st) return st;
+    if (raw->count == 0) return GS_OK;
+    GS_CHECK_ARG(pos && sc && rot && sh && op, "null output");
+    GS_CHECK_ARG(aligned(rot, 16), "rotations output must be 16-byte aligned");
+    GS_HIP(launch_activate(*raw, pos, sc, rot, sh, op, c->stream), "activate kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_preprocess_fwd(gs_ctx* c, const gs_gaussians* g, const gs_camera* cam, const gs_render_cfg* cfg,
+                                 const gs_preprocessed* out) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    int st = check_gaussians(g);
+    if (st) return st;
+    if ((st = check_cam_cfg(cam, cfg))) return st;
+    if (g->count == 0) return GS_OK;
+    if ((st = check_pre(out, true))) return st;
+    gsd::CamArgs a = make_cam_args(cam, cfg);
+    int deg = cfg->sh_degree < g->sh_degree ? cfg->sh_degree : g->sh_degree;  // preprocessor_cpu.cpp:55
+    GS_HIP(launch_preprocess_fwd(*g, a, *out, deg, c->stream), "preprocess kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_scan_tiles(gs_ctx* c, int n, const int* tiles, uint32_t* offsets, uint32_t* total_dev,
+                             long long* total_host) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(n >= 0, "negative count");
+    void* sm;
+    int st = gs_small_get(c, &sm);
+    if (st) return st;
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
+    if (n == 0) {
+        GS_HIP(launch_zero(total64, 8, c->stream), "zero kernel");
+        if (total_dev) GS_HIP(launch_zero(total_dev, 4, c->stream), "zero kernel");
+    } else {
+        GS_CHECK_ARG(tiles && offsets, "null array");
+        uint32_t np = scan_partials_needed((uint32_t)n);
+        void* ws;
+        if ((st = gs_ws_get(c, (size_t)(np + 1) * 8, &ws))) return st;
+        GS_HIP(launch_scan_tiles(tiles, (uint32_t)n, offsets, static_cast<unsigned long long*>(ws), total_dev, total64,
+                                 c->stream),
+               "scan kernels");
+    }
+    if (total_host) {
+        unsigned long long t = 0;
+        GS_HIP(hipMemcpyAsync(&t, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
+        GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        *total_host = (long long)t;
+        GS_CHECK_ARG(t <= 0xFFFFFFFFull, "pair count %llu exceeds 2^32-1", t);
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_duplicate_keys(gs_ctx* c, int n, const float* means2D, const float* depths, const int* radii,
+                                 const uint32_t* offsets, int width, int height, int tile_w, int tile_h,
+                                 uint64_t* keys, uint32_t* values, uint32_t capacity) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(tile_w == 16 && tile_h == 16, "tiles must be 16x16");
+    GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
+    if (n <= 0) return GS_OK;
+    GS_CHECK_ARG(means2D && depths && radii && offsets && keys && values, "null array");
+    GS_CHECK_ARG(aligned(means2D, 8), "means2D must be 8-byte aligned");
+    void* sm;
+    int st = gs_small_get(c, &sm);
+    if (st) return st;
+    uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_OVERFLOW);
+    int gx = (width + 15) / 16, gy = (height + 15) / 16;
+    GS_HIP(launch_duplicate(n, means2D, depths, radii, offsets, gx, gy, keys, values, capacity, ovf, c->stream),
+           "duplicate kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_check_overflow(gs_ctx* c, int* overflowed) {
+    GS_CHECK_ARG(c != nullptr && overflowed != nullptr, "null argument");
+    void* sm;
+    int st = gs_small_get(c, &sm);
+    if (st) return st;
+    uint32_t v = 0;
+    uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_OVERFLOW);
+    GS_HIP(hipMemcpyAsync(&v, ovf, 4, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(overflow)");
+    GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    *overflowed = v != 0;
+    if (v) GS_HIP(launch_zero(ovf, 4, c->stream), "zero kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_radix_sort_kv(gs_ctx* c, uint32_t n, const uint64_t* ki, const uint32_t* vi, uint64_t* ko,
+                                uint32_t* vo, int key_bits, int radix_bits) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(key_bits >= 0 && key_bits <= 64, "key_bits must be in [0, 64]");
+    if (radix_bits == 0) radix_bits = c->radix_bits;
+    GS_CHECK_ARG(radix_bits >= 4 && radix_bits <= 8, "radix_bits must be in [4, 8]");
+    if (n == 0) return GS_OK;
+    GS_CHECK_ARG(ki && vi && ko && vo, "null array");
+    GS_CHECK_ARG(ko != ki && vo != vi, "sort outputs must not alias the inputs");
+    void* ws;
+    int st = gs_ws_get(c, radix_workspace_bytes(n, radix_bits), &ws);
+    if (st) return st;
+    GS_HIP(launch_radix_sort(ki, vi, ko, vo, n, key_bits, radix_bits, ws, c->stream), "radix sort kernels");
+    return GS_OK;
+}
+
+extern "C" int gs_tile_ranges(gs_ctx* c, uint32_t n, const uint64_t* keys_sorted, int num_tiles, uint32_t* ranges) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(num_tiles > 0 && ranges, "bad tile range buffer");
+    GS_HIP(launch_zero(ranges, (size_t)num_tiles * 8, c->stream), "zero kernel");
+    if (n == 0) return GS_OK;
+    GS_CHECK_ARG(keys_sorted != nullptr, "null keys");
+    GS_HIP(launch_tile_ranges(keys_sorted, n, (uint32_t)num_tiles, ranges, c->stream), "tile ranges kernel");
+    return GS_OK;
+}
+
+int gs_tile_key_bits(int num_tiles) {
+    int b = 0;
+    while ((1ll << b) < (long long)num_tiles) b++;
+    return 32 + b;
+}
+
+extern "C" int gs_sort_pairs(gs_ctx* c, uint32_t n, const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo,
+                             int num_tiles, uint32_t* ranges) {
+    int st = gs_radix_sort_kv(c, n, ki, vi, ko, vo, gs_tile_key_bits(num_tiles), 0);
+    if (st) return st;
+    return gs_tile_ranges(c, n, ko, num_tiles, ranges);
+}
+
+static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                                   const uint32_t* ranges, const float bg[3]) {
+    gsr::RasterArgs a;
+    a.W = width;
+    a.H = height;
+    a.gx = (width + 15) / 16;
+    a.means2D = reinterpret_cast<const float2*>(pre->means2D);
+    a.conics = pre->conics;
+    a.opac = pre->opacities;
+    a.rgb = pre->rgb;
+    a.values = values;
+    a.ranges = ranges;
+    a.bg0 = bg ? bg[0] : 0.0f;
+    a.bg1 = bg ? bg[1] : 0.0f;
+    a.bg2 = bg ? bg[2] : 0.0f;
+    return a;
+}
+
+extern "C" int gs_raster_fwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                             const uint32_t* ranges, const float bg[3], float* image, const gs_pixel_state* px) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
+    int st = check_pre(pre, false);
+    if (st) return st;
+    GS_CHECK_ARG(ranges && image, "null ranges or image");
+    int tiles = ((width + 15) / 16) * ((height + 15) / 16);
+    gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg);
+    GS_HIP(launch_raster_fwd(a, tiles, c->raster_batch, image, px ? px->T_final : nullptr,
+                             px ? px->n_contrib : nullptr, px ? px->last_contrib : nullptr, c->stream),
+           "raster forward kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_l1_loss(gs_ctx* c, int width, int height, const float* r, const float* g, float* d, double* loss) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
+    GS_CHECK_ARG(r && g && d && loss, "null array");
+    GS_CHECK_ARG(aligned(r, 16) && aligned(g, 16) && aligned(d, 16), "images must be 16-byte aligned");
+    uint32_t n = (uint32_t)width * (uint32_t)height * 3u;
+    void* ws;
+    int st = gs_ws_get(c, (size_t)l1_partials_needed(n) * 8, &ws);
+    if (st) return st;
+    GS_HIP(launch_l1(r, g, d, n, static_cast<double*>(ws), loss, c->stream), "l1 kernels");
+    return GS_OK;
+}
+
+extern "C" int gs_raster_bwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                             const uint32_t* ranges, const float bg[3], const gs_pixel_state* px, const float* d_image,
+                             const gs_raster_grads* g) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
+    int st = check_pre(pre, false);
+    if (st) return st;
+    GS_CHECK_ARG(ranges && px && px->T_final && px->last_contrib && d_image && g, "null array");
+    GS_CHECK_ARG(g->d_means2D && g->d_conics && g->d_rgb && g->d_opacities_2d, "null gradient array");
+    GS_CHECK_ARG(aligned(g->d_means2D, 8), "d_means2D must be 8-byte aligned");
+    int tiles = ((width + 15) / 16) * ((height + 15) / 16);
+    gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg);
+    gsr::GradArgs ga{reinterpret_cast<float2*>(g->d_means2D), g->d_conics, g->d_rgb, g->d_opacities_2d};
+    GS_HIP(launch_raster_bwd(a, tiles, c->raster_bwd_batch, px->T_final, px->last_contrib, d_image, ga, c->stream),
+           "raster backward kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_preprocess_bwd(gs_ctx* c, const gs_gaussians* g, const gs_camera* cam, const gs_render_cfg* cfg,
+                                 const int* radii, const gs_raster_grads* rg, const gs_param_grads* out,
+                                 int accumulate) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    int st = check_gaussians(g);
+    if (st) return st;
+    if ((st = check_cam_cfg(cam, cfg))) return st;
+    GS_CHECK_ARG(g->raw != 0, "gs_preprocess_bwd needs raw (pre-activation) parameters");
+    if (g->count == 0) return GS_OK;
+    GS_CHECK_ARG(radii && rg && out, "null argument");
+    GS_CHECK_ARG(rg->d_means2D && rg->d_conics && rg->d_rgb && rg->d_opacities_2d, "null raster gradient");
+    GS_CHECK_ARG(out->d_positions && out->d_scales && out->d_rotations && out->d_sh_coeffs && out->d_opacities,
+                 "null output gradient");
+    GS_CHECK_ARG(aligned(out->d_rotations, 16), "d_rotations must be 16-byte aligned");
+    gsd::CamArgs a = make_cam_args(cam, cfg);
+    // The reference evaluates the SH backward with the model degree
+    // (preprocessor_backward_cpu.cpp:219-221), not min(cfg, model).
+    GS_HIP(launch_preprocess_bwd(*g, a, radii, *rg, *out, accumulate, g->sh_degree, c->stream),
+           "preprocess backward kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_adam_step(gs_ctx* c, int n, int max_coeffs, float* const params[5], const float* const grads[5],
+                            float* const m[5], float* const v[5], const gs_adam_hparams* hp) {
+    GS_CHECK_ARG(c != nullptr && hp != nullptr, "null argument");
+    GS_CHECK_ARG(n >= 0, "negative count");
+    if (n == 0) return GS_OK;
+    const uint32_t sizes[5] = {(uint32_t)n * 3u, (uint32_t)n * 3u, (uint32_t)n * 4u,
+                               (uint32_t)n * (uint32_t)max_coeffs * 3u, (uint32_t)n};
+    gso::AdamArgs a;
+    for (int k = 0; k < 5; k++) {
+        GS_CHECK_ARG(params[k] && grads[k] && m[k] && v[k], "null Adam array (tensor %d)", k);
+        GS_CHECK_ARG(aligned(params[k], 16) && aligned(grads[k], 16) && aligned(m[k], 16) && aligned(v[k], 16),
+                     "Adam arrays must be 16-byte aligned (tensor %d)", k);
+        a.seg[k] = gso::AdamSeg{params[k], grads[k], m[k], v[k], sizes[k], hp->lr[k]};
+    }
+    a.b1 = hp->beta1;
+    a.b2 = hp->beta2;
+    a.eps = hp->eps;
+    a.bc1 = hp->bc1;
+    a.bc2 = hp->bc2;
+    GS_HIP(launch_adam(a, c->stream), "adam kernel");
+    return GS_OK;
+}
+
+// train_types.cpp:64-68 and optimizer.cpp:73-76, host float arithmetic.
+extern "C" float gs_lr_schedule(float lr_init, float lr_final, int step, int max_steps) {
+    float t = static_cast<float>(step) / static_cast<float>(max_steps);
+    t = std::max(0.0f, std::min(1.0f, t));
+    return std::exp(std::log(lr_init) * (1.0f - t) + std::log(lr_final) * t);
+}
+
+extern "C" void gs_adam_bias_corrections(float b1, float b2, int step, float* bc1, float* bc2) {
+    *bc1 = 1.0f / (1.0f - std::pow(b1, (float)step));
+    *bc2 = 1.0f / (1.0f - std::pow(b2, (float)step));
+}

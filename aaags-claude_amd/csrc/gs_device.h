@@ -1,0 +1,257 @@
+// gs_device.h -- device-side math of the 3DGS 2D path, written for gfx950.
+//
+// Every function keeps the floating-point operation order of the reference's
+// CPU path (harmonyos_3dgs/src/math_utils.cpp, sh_eval.cpp,
+// cpu/preprocessor_cpu.cpp) so that, with -ffp-contract=off and IEEE division
+// and square root, the GPU reproduces the CPU results bit for bit; only expf
+// (device libm, <= 1 ulp) can differ in the last bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gs_hip.h"
+
+namespace gsd {
+
+// std::min / std::max semantics: the first argument wins unless strictly beaten.
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ int smin(int a, int b) { return (b < a) ? b : a; }
+__device__ __forceinline__ int smax(int a, int b) { return (a < b) ? b : a; }
+
+// SH basis constants (sh_eval.cpp:5-22).
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f,
+                SH_C2_2 = 0.31539156525252005f, SH_C2_3 = -1.0925484305920792f,
+                SH_C2_4 = 0.5462742152960396f;
+constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
+                SH_C3_2 = -0.4570457994644658f, SH_C3_3 = 0.3731763325901154f,
+                SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
+                SH_C3_6 = -0.5900435899266435f;
+
+constexpr int TILE = 16;
+
+// ---- math_utils.cpp:8-19 ----
+__device__ __forceinline__ void xform43(const float p[3], const float* m, float o[3]) {
+    o[0] = m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12];
+    o[1] = m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13];
+    o[2] = m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14];
+}
+__device__ __forceinline__ void xform44(const float p[3], const float* m, float o[4]) {
+    o[0] = m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12];
+    o[1] = m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13];
+    o[2] = m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14];
+    o[3] = m[3] * p[0] + m[7] * p[1] + m[11] * p[2] + m[15];
+}
+
+// ---- math_utils.h:12-14 ----
+__device__ __forceinline__ float ndc2pix(float v, int S) { return ((v + 1.0f) * (float)S - 1.0f) * 0.5f; }
+
+// ---- math_utils.cpp:21-28 (truncate toward zero, then clamp) ----
+__device__ __forceinline__ void get_rect(float px, float py, int r, int gx, int gy, int mn[2], int mx[2]) {
+    const float tw = (float)TILE;
+    mn[0] = smin(gx, smax(0, (int)((px - (float)r) / tw)));
+    mn[1] = smin(gy, smax(0, (int)((py - (float)r) / tw)));
+    mx[0] = smin(gx, smax(0, (int)((px + (float)r + tw - 1.0f) / tw)));
+    mx[1] = smin(gy, smax(0, (int)((py + (float)r + tw - 1.0f) / tw)));
+}
+
+// ---- math_utils.cpp:35-45 (R[row][col]) ----
+__device__ __forceinline__ void rot_matrix(float r, float x, float y, float z, float R[3][3]) {
+    R[0][0] = 1.f - 2.f * (y * y + z * z);
+    R[0][1] = 2.f * (x * y + r * z);
+    R[0][2] = 2.f * (x * z - r * y);
+    R[1][0] = 2.f * (x * y - r * z);
+    R[1][1] = 1.f - 2.f * (x * x + z * z);
+    R[1][2] = 2.f * (y * z + r * x);
+    R[2][0] = 2.f * (x * z + r * y);
+    R[2][1] = 2.f * (y * z - r * x);
+    R[2][2] = 1.f - 2.f * (x * x + y * y);
+}
+
+// ---- math_utils.cpp:30-62 ----
+__device__ __forceinline__ void cov3d_of(const float s_in[3], float mod, const float q[4], float cov[6]) {
+    float s[3] = {mod * s_in[0], mod * s_in[1], mod * s_in[2]};
+    float R[3][3], M[3][3];
+    rot_matrix(q[0], q[1], q[2], q[3], R);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) M[i][j] = s[i] * R[i][j];
+    cov[0] = M[0][0] * M[0][0] + M[1][0] * M[1][0] + M[2][0] * M[2][0];
+    cov[1] = M[0][0] * M[0][1] + M[1][0] * M[1][1] + M[2][0] * M[2][1];
+    cov[2] = M[0][0] * M[0][2] + M[1][0] * M[1][2] + M[2][0] * M[2][2];
+    cov[3] = M[0][1] * M[0][1] + M[1][1] * M[1][1] + M[2][1] * M[2][1];
+    cov[4] = M[0][1] * M[0][2] + M[1][1] * M[1][2] + M[2][1] * M[2][2];
+    cov[5] = M[0][2] * M[0][2] + M[1][2] * M[1][2] + M[2][2] * M[2][2];
+}
+
+// Clamped view point, J, W and T = W*J, all stored [col][row]
+// (math_utils.cpp:69-109, reused by preprocessor_backward_cpu.cpp:316-351).
+struct ProjJac {
+    float t[3], txtz, tytz, limx, limy;
+    float J00, J02, J11, J12;  // the non-zero entries of J[col][row]
+    float T[3][3];
+};
+
+__device__ __forceinline__ void proj_jacobian(const float tv[3], const float* vm, float tanx, float tany,
+                                              float fx, float fy, ProjJac& o) {
+    o.t[0] = tv[0]; o.t[1] = tv[1]; o.t[2] = tv[2];
+    o.limx = 1.3f * tanx;
+    o.limy = 1.3f * tany;
+    o.txtz = o.t[0] / o.t[2];
+    o.tytz = o.t[1] / o.t[2];
+    o.t[0] = smin(o.limx, smax(-o.limx, o.txtz)) * o.t[2];
+    o.t[1] = smin(o.limy, smax(-o.limy, o.tytz)) * o.t[2];
+    const float* t = o.t;
+    o.J00 = fx / t[2];
+    o.J02 = -(fx * t[0]) / (t[2] * t[2]);
+    o.J11 = fy / t[2];
+    o.J12 = -(fy * t[1]) / (t[2] * t[2]);
+    // J[col][row]: col0 = (J00, 0, J02), col1 = (0, J11, J12), col2 = 0.
+    // T[col][row] = sum_k W[k][row] * J[col][k] accumulated from 0 in k order;
+    // the zero terms are kept so the additions round exactly as the reference.
+    const float J[3][3] = {{o.J00, 0.0f, o.J02}, {0.0f, o.J11, o.J12}, {0.0f, 0.0f, 0.0f}};
+    const float W[3][3] = {{vm[0], vm[4], vm[8]}, {vm[1], vm[5], vm[9]}, {vm[2], vm[6], vm[10]}};
+#pragma unroll
+    for (int col = 0; col < 3; col++)
+#pragma unroll
+        for (int row = 0; row < 3; row++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc += W[k][row] * J[col][k];
+            o.T[col][row] = acc;
+        }
+}
+
+// ---- math_utils.cpp:64-144: cov2D entries (a, b, c) before the +0.3 ----
+__device__ __forceinline__ void cov2d_from(const ProjJac& pj, const float c3[6], float out[3]) {
+    const float V[3][3] = {{c3[0], c3[1], c3[2]}, {c3[1], c3[3], c3[4]}, {c3[2], c3[4], c3[5]}};
+    float tmp[2][3];
+#pragma unroll
+    for (int col = 0; col < 2; col++)
+#pragma unroll
+        for (int row = 0; row < 3; row++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc += V[row][k] * pj.T[col][k];
+            tmp[col][row] = acc;
+        }
+    // res[col][row] = sum_k T[row][k] * tmp[col][k]
+    float r00 = 0.0f, r10 = 0.0f, r11 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) r00 += pj.T[0][k] * tmp[0][k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) r10 += pj.T[0][k] * tmp[1][k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) r11 += pj.T[1][k] * tmp[1][k];
+    out[0] = r00;
+    out[1] = r10;
+    out[2] = r11;
+}
+
+// SH direction terms: dir = normalize(pos - cam), and the basis values the
+// forward and backward share (sh_eval.cpp:30-37).
+struct ShDir {
+    float x, y, z, len;
+};
+__device__ __forceinline__ ShDir sh_dir(const float p[3], const float cam[3]) {
+    float dx = p[0] - cam[0], dy = p[1] - cam[1], dz = p[2] - cam[2];
+    float len = sqrtf(dx * dx + dy * dy + dz * dz);
+    dx /= len; dy /= len; dz /= len;
+    return ShDir{dx, dy, dz, len};
+}
+
+// ---- sh_eval.cpp:39-77: the colour before +0.5 and clamping, one channel.
+// sh points at coefficient 0 of this channel; stride 3 floats per basis. ----
+template <int DEG>
+__device__ __forceinline__ float sh_raw_channel(const ShDir& d, const float* sh) {
+    const float x = d.x, y = d.y, z = d.z;
+    float v = SH_C0 * sh[0];
+    if (DEG > 0) {
+        v += -SH_C1 * y * sh[1 * 3] + SH_C1 * z * sh[2 * 3] + -SH_C1 * x * sh[3 * 3];
+        if (DEG > 1) {
+            float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
+            v += SH_C2_0 * xy * sh[4 * 3] + SH_C2_1 * yz * sh[5 * 3] + SH_C2_2 * (2.0f * zz - xx - yy) * sh[6 * 3] +
+                 SH_C2_3 * xz * sh[7 * 3] + SH_C2_4 * (xx - yy) * sh[8 * 3];
+            if (DEG > 2) {
+                v += SH_C3_0 * y * (3.0f * xx - yy) * sh[9 * 3] + SH_C3_1 * xy * z * sh[10 * 3] +
+                     SH_C3_2 * y * (4.0f * zz - xx - yy) * sh[11 * 3] +
+                     SH_C3_3 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * sh[12 * 3] +
+                     SH_C3_4 * x * (4.0f * zz - xx - yy) * sh[13 * 3] + SH_C3_5 * z * (xx - yy) * sh[14 * 3] +
+                     SH_C3_6 * x * (xx - 3.0f * yy) * sh[15 * 3];
+            }
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ float sh_raw_channel_rt(int deg, const ShDir& d, const float* sh) {
+    switch (deg) {
+        case 0: return sh_raw_channel<0>(d, sh);
+        case 1: return sh_raw_channel<1>(d, sh);
+        case 2: return sh_raw_channel<2>(d, sh);
+        default: return sh_raw_channel<3>(d, sh);
+    }
+}
+
+// Activation of one Gaussian's raw parameters (train_types.cpp:6-36).
+__device__ __forceinline__ void activate_scale(const float rs[3], float s[3]) {
+    s[0] = expf(rs[0]); s[1] = expf(rs[1]); s[2] = expf(rs[2]);
+}
+__device__ __forceinline__ void activate_rot(const float q[4], float o[4]) {
+    float len = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (len < 1e-12f) len = 1e-12f;
+    float inv = 1.0f / len;
+    o[0] = q[0] * inv; o[1] = q[1] * inv; o[2] = q[2] * inv; o[3] = q[3] * inv;
+}
+__device__ __forceinline__ float activate_opacity(float l) { return 1.0f / (1.0f + expf(-l)); }
+
+// Camera constants shared by the per-Gaussian kernels, passed by value.
+struct CamArgs {
+    float view[16];
+    float vp[16];
+    float campos[3];
+    float tanx, tany, fx, fy;
+    int W, H, gx, gy;
+    float scale_mod;
+    int antialiasing, training, deg_cfg;
+};
+
+}  // namespace gsd
+
+// Kernel argument blocks shared by the launchers and the C ABI layer.
+namespace gsr {
+struct RasterArgs {
+    int W, H, gx;
+    const float2* means2D;
+    const float* conics;
+    const float* opac;
+    const float* rgb;
+    const uint32_t* values;
+    const uint32_t* ranges;
+    float bg0, bg1, bg2;
+};
+struct GradArgs {
+    float2* d_means2D;
+    float* d_conics;
+    float* d_rgb;
+    float* d_op;
+};
+}  // namespace gsr
+
+namespace gso {
+struct AdamSeg {
+    float* p;
+    const float* g;
+    float* m;
+    float* v;
+    uint32_t n;
+    float lr;
+};
+struct AdamArgs {
+    AdamSeg seg[5];
+    float b1, b2, eps, bc1, bc2;
+};
+}  // namespace gso

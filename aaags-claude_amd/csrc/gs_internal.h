@@ -1,0 +1,72 @@
+// gs_internal.h -- host-side declarations shared by the C ABI layer, the
+// kernel launchers and the C++ stage classes.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstddef>
+#include <cstdint>
+
+#include "gs_hip.h"
+
+namespace gsd { struct CamArgs; }
+namespace gsr { struct RasterArgs; struct GradArgs; }
+namespace gso { struct AdamArgs; }
+
+// Small fixed scratch owned by each context.
+constexpr size_t GS_SMALL_BYTES = 4096;
+constexpr size_t GS_SMALL_TOTAL64 = 0;    // uint64 pair total of the last scan
+constexpr size_t GS_SMALL_OVERFLOW = 64;  // uint32 capacity-overflow flag
+
+struct gs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    void* small = nullptr;
+    bool capturing = false;
+    int radix_bits = 8;
+    int raster_batch = 256;
+    int raster_bwd_batch = 256;
+};
+
+int gs_set_error(int code, const char* fmt, ...);
+int gs_hip_status(hipError_t e, const char* what);
+int gs_ws_get(gs_ctx* c, size_t bytes, void** out);
+int gs_small_get(gs_ctx* c, void** out);
+int check_cam_cfg(const gs_camera* cam, const gs_render_cfg* cfg);
+int gs_tile_key_bits(int num_tiles);
+
+extern "C" int gs_ctx_set_tuning(gs_ctx* c, int radix_bits, int raster_batch, int raster_bwd_batch);
+extern "C" int gs_ctx_reserve(gs_ctx* c, size_t bytes);
+extern "C" int gs_check_overflow(gs_ctx* c, int* overflowed);
+
+// kernel launchers
+hipError_t launch_preprocess_fwd(const gs_gaussians& G, const gsd::CamArgs& c, const gs_preprocessed& out, int deg,
+                                 hipStream_t s);
+hipError_t launch_activate(const gs_gaussians& G, float* pos, float* sc, float* rot, float* sh, float* op,
+                           hipStream_t s);
+hipError_t launch_preprocess_bwd(const gs_gaussians& G, const gsd::CamArgs& c, const int* radii,
+                                 const gs_raster_grads& rg, const gs_param_grads& out, int accumulate, int sh_deg,
+                                 hipStream_t s);
+uint32_t scan_partials_needed(uint32_t n);
+hipError_t launch_scan_tiles(const int* in, uint32_t n, uint32_t* out, unsigned long long* partial, uint32_t* total32,
+                             unsigned long long* total64, hipStream_t s);
+hipError_t launch_duplicate(int n, const float* means2D, const float* depths, const int* radii,
+                            const uint32_t* offsets, int gx, int gy, uint64_t* keys, uint32_t* values,
+                            uint32_t capacity, uint32_t* overflow, hipStream_t s);
+size_t radix_workspace_bytes(uint32_t n, int radix_bits);
+hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out, uint32_t* vals_out,
+                             uint32_t n, int key_bits, int radix_bits, void* workspace, hipStream_t s);
+hipError_t launch_tile_ranges(const uint64_t* keys_sorted, uint32_t n, uint32_t num_tiles, uint32_t* ranges,
+                              hipStream_t s);
+hipError_t launch_raster_fwd(const gsr::RasterArgs& a, int num_tiles, int batch, float* img, float* T_final,
+                             int* n_contrib, uint32_t* last_contrib, hipStream_t s);
+hipError_t launch_raster_bwd(const gsr::RasterArgs& a, int num_tiles, int batch, const float* T_final,
+                             const uint32_t* last_contrib, const float* d_image, const gsr::GradArgs& g,
+                             hipStream_t s);
+int l1_partials_needed(uint32_t n);
+hipError_t launch_l1(const float* r, const float* g, float* d, uint32_t n, double* partial, double* loss,
+                     hipStream_t s);
+hipError_t launch_adam(const gso::AdamArgs& a, hipStream_t s);
+hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);

@@ -1,0 +1,451 @@
+// k_sort.hip -- tile binning and the stable LSD radix sort, written for wave64.
+//
+// scan:       reduce-then-scan over 4096-element blocks (three launches); the
+//             wave-level prefix uses lane shuffles, the block level LDS.
+// duplicate:  one thread per Gaussian writes its tile|depth keys at the
+//             scanned offset (row-major over its tile rectangle).
+// radix sort: per pass an upsweep (per-block digit histogram), a scan of the
+//             digit-major histogram table, and a downsweep that ranks keys
+//             stably with wave ballots (match-any on the digit bits), stages
+//             the block's keys in LDS in digit order and writes them out in
+//             runs, so global writes are contiguous within each digit bucket.
+//             4096 keys per 256-thread block; 48 KB of LDS per block.
+#include "gs_device.h"
+
+namespace gsk {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_ITEMS;  // 4096
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;  // 4096 keys per block
+constexpr int RS_WAVE_SPAN = RS_ROUNDS * 64;     // 1024 contiguous keys per wave
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    int l = lane_id();
+    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_scan(T v) {
+    int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T u = __shfl_up(v, o, 64);
+        if (l >= o) v += u;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Exclusive block scan of one value per thread (256 threads); returns the
+// exclusive prefix and writes the block total to *total.
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T* total) {
+    int w = threadIdx.x >> 6, l = lane_id();
+    T inc = wave_inclusive_scan(v);
+    if (l == 63) s_wave[w] = inc;
+    __syncthreads();
+    T wbase = 0, tot = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+        T s = s_wave[k];
+        if (k < w) wbase += s;
+        tot += s;
+    }
+    *total = tot;
+    __syncthreads();
+    return wbase + inc - v;
+}
+
+// ---------------------------------------------------------------- scan ----
+// Phase 1: per-block sums of `in` (int32 or uint32 values treated as uint64).
+template <typename TIn>
+__global__ __launch_bounds__(SCAN_THREADS) void scan_reduce_kernel(const TIn* __restrict__ in, uint32_t n,
+                                                                   unsigned long long* __restrict__ partial) {
+    __shared__ unsigned long long s_w[SCAN_THREADS / 64];
+    uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    unsigned long long s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint32_t i = base + k;
+        if (i < n) s += (unsigned long long)(uint32_t)in[i];
+    }
+    s = wave_sum(s);
+    if (lane_id() == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < SCAN_THREADS / 64; k++) t += s_w[k];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// Phase 2: one block turns the partial sums into exclusive block offsets and
+// writes the grand total (uint32 + overflow flag).
+__global__ __launch_bounds__(1024) void scan_partials_kernel(unsigned long long* __restrict__ partial, uint32_t nb,
+                                                             uint32_t* __restrict__ total32,
+                                                             unsigned long long* __restrict__ total64) {
+    __shared__ unsigned long long s_w[16];
+    uint32_t per = (nb + 1023) / 1024;
+    uint32_t b0 = threadIdx.x * per;
+    unsigned long long s = 0;
+    for (uint32_t k = 0; k < per; k++)
+        if (b0 + k < nb) s += partial[b0 + k];
+    int w = threadIdx.x >> 6, l = lane_id();
+    unsigned long long inc = wave_inclusive_scan(s);
+    if (l == 63) s_w[w] = inc;
+    __syncthreads();
+    unsigned long long wb = 0, tot = 0;
+    for (int k = 0; k < 16; k++) {
+        if (k < w) wb += s_w[k];
+        tot += s_w[k];
+    }
+    unsigned long long run = wb + inc - s;
+    for (uint32_t k = 0; k < per; k++)
+        if (b0 + k < nb) {
+            unsigned long long v = partial[b0 + k];
+            partial[b0 + k] = run;
+            run += v;
+        }
+    if (threadIdx.x == 0) {
+        if (total32) *total32 = (uint32_t)tot;
+        if (total64) *total64 = tot;
+    }
+}
+
+// Phase 3: block-local scan plus the block offset.  INCLUSIVE selects the
+// inclusive (binning offsets) or exclusive (radix histogram) variant.
+template <typename TIn, bool INCLUSIVE>
+__global__ __launch_bounds__(SCAN_THREADS) void scan_downsweep_kernel(const TIn* __restrict__ in, uint32_t n,
+                                                                      const unsigned long long* __restrict__ partial,
+                                                                      uint32_t* __restrict__ out) {
+    __shared__ unsigned long long s_w[SCAN_THREADS / 64];
+    uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    unsigned long long s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint32_t i = base + k;
+        v[k] = (i < n) ? (uint32_t)in[i] : 0u;
+        s += v[k];
+    }
+    unsigned long long tot;
+    unsigned long long run = block_exclusive_scan<unsigned long long>(s, s_w, &tot) + partial[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        uint32_t i = base + k;
+        if (INCLUSIVE) run += v[k];
+        if (i < n) out[i] = (uint32_t)run;
+        if (!INCLUSIVE) run += v[k];
+    }
+}
+
+// ----------------------------------------------------------- duplicate ----
+// tile_binner_cpu.cpp:59-99: keys tile_id << 32 | float bits of the depth.
+__global__ __launch_bounds__(256) void duplicate_kernel(int n, const float2* __restrict__ means2D,
+                                                        const float* __restrict__ depths, const int* __restrict__ radii,
+                                                        const uint32_t* __restrict__ offsets_incl, int gx, int gy,
+                                                        uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
+                                                        uint32_t capacity, uint32_t* __restrict__ overflow) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int r = radii[i];
+    if (r <= 0) return;
+    float2 m = means2D[i];
+    int mn[2], mx[2];
+    gsd::get_rect(m.x, m.y, r, gx, gy, mn, mx);
+    uint32_t cnt = (uint32_t)((mx[1] - mn[1]) * (mx[0] - mn[0]));
+    uint32_t off = offsets_incl[i] - cnt;
+    uint64_t db = (uint64_t)__float_as_uint(depths[i]);
+    for (int y = mn[1]; y < mx[1]; y++)
+        for (int x = mn[0]; x < mx[0]; x++) {
+            if (off < capacity) {
+                keys[off] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | db;
+                values[off] = (uint32_t)i;
+            } else if (overflow) {
+                *overflow = 1u;
+            }
+            off++;
+        }
+}
+
+// ---------------------------------------------------------- radix sort ----
+template <int BITS>
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) {
+    return (uint32_t)(k >> shift) & ((1u << BITS) - 1u);
+}
+
+// Lanes (among `active`) whose digit equals this lane's.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
+    uint64_t peers = active;
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        uint64_t bal = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    return peers;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint32_t n, int shift,
+                                                                uint32_t nb, uint32_t* __restrict__ hist) {
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t s_h[RS_WAVES][RADIX];
+    const int w = threadIdx.x >> 6, l = lane_id();
+    for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_h[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t wbase = blockIdx.x * RS_TILE + w * RS_WAVE_SPAN;
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        uint32_t i = wbase + r * 64 + l;
+        bool valid = i < n;
+        uint64_t active = __ballot(valid);
+        if (active == 0) break;
+        uint32_t d = valid ? digit_of<BITS>(keys[i], shift) : 0u;
+        uint64_t peers = match_digit<BITS>(d, active);
+        if (valid && (peers & lanemask_lt()) == 0) s_h[w][d] += (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < RADIX; d += RS_THREADS) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; k++) s += s_h[k][d];
+        hist[(size_t)d * nb + blockIdx.x] = s;
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t* __restrict__ keys_in,
+                                                                  const uint32_t* __restrict__ vals_in, uint32_t n,
+                                                                  int shift, uint32_t nb,
+                                                                  const uint32_t* __restrict__ goff,
+                                                                  uint64_t* __restrict__ keys_out,
+                                                                  uint32_t* __restrict__ vals_out) {
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint64_t s_k[RS_TILE];
+    __shared__ uint32_t s_v[RS_TILE];
+    __shared__ uint32_t s_wh[RS_WAVES][RADIX];
+    __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+    __shared__ uint32_t s_scan[RS_WAVES];
+
+    const int w = threadIdx.x >> 6, l = lane_id();
+    for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_wh[0][0])[i] = 0u;
+    for (int d = threadIdx.x; d < RADIX; d += RS_THREADS) s_goff[d] = goff[(size_t)d * nb + blockIdx.x];
+    __syncthreads();
+
+    const uint32_t bbase = blockIdx.x * RS_TILE;
+    const uint32_t wbase = bbase + w * RS_WAVE_SPAN;
+    uint64_t k_r[RS_ROUNDS];
+    uint32_t v_r[RS_ROUNDS];
+    uint32_t rank_r[RS_ROUNDS];
+    uint32_t d_r[RS_ROUNDS];
+    // 1. per-wave stable ranks: rounds in key order, lanes in key order.
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        uint32_t i = wbase + r * 64 + l;
+        bool valid = i < n;
+        uint64_t active = __ballot(valid);
+        k_r[r] = valid ? keys_in[i] : 0ull;
+        v_r[r] = valid ? vals_in[i] : 0u;
+        uint32_t d = digit_of<BITS>(k_r[r], shift);
+        d_r[r] = d;
+        rank_r[r] = 0u;
+        if (active != 0) {
+            uint64_t peers = match_digit<BITS>(d, active);
+            uint32_t before = 0u;
+            if (valid) before = s_wh[w][d];
+            // every lane has read the counter before the leader bumps it
+            __builtin_amdgcn_wave_barrier();
+            if (valid && (peers & lanemask_lt()) == 0) s_wh[w][d] = before + (uint32_t)__popcll(peers);
+            __builtin_amdgcn_wave_barrier();
+            rank_r[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
+        }
+    }
+    __syncthreads();
+    // 2. digit totals, per-wave offsets inside each digit, digit starts.
+    for (int d = threadIdx.x; d < RADIX; d += RS_THREADS) {
+        uint32_t run = 0;
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; k++) {
+            uint32_t c = s_wh[k][d];
+            s_wh[k][d] = run;
+            run += c;
+        }
+        s_dstart[d] = run;  // digit total for now
+    }
+    __syncthreads();
+    {
+        uint32_t tot = (threadIdx.x < RADIX) ? s_dstart[threadIdx.x] : 0u;
+        uint32_t total;
+        uint32_t ex = block_exclusive_scan<uint32_t>(tot, s_scan, &total);
+        if (threadIdx.x < RADIX) s_dstart[threadIdx.x] = ex;
+    }
+    __syncthreads();
+    // 3. stage the block's pairs in LDS in (digit, input order) order.
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; r++) {
+        uint32_t i = wbase + r * 64 + l;
+        if (i < n) {
+            uint32_t p = s_dstart[d_r[r]] + s_wh[w][d_r[r]] + rank_r[r];
+            s_k[p] = k_r[r];
+            s_v[p] = v_r[r];
+        }
+    }
+    __syncthreads();
+    // 4. write out: consecutive threads write consecutive slots of a bucket.
+    uint32_t cnt = (n - bbase < (uint32_t)RS_TILE) ? (n - bbase) : (uint32_t)RS_TILE;
+    for (uint32_t p = threadIdx.x; p < cnt; p += RS_THREADS) {
+        uint64_t k = s_k[p];
+        uint32_t d = digit_of<BITS>(k, shift);
+        uint32_t g = s_goff[d] + (p - s_dstart[d]);
+        if (g < n) {
+            keys_out[g] = k;
+            vals_out[g] = s_v[p];
+        }
+    }
+}
+
+// identify_tile_ranges (radix_sort.cl:14-46), ranges pre-zeroed.
+// Tile ids at or beyond num_tiles are ignored (they cannot come from binning).
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t* __restrict__ keys, uint32_t n,
+                                                          uint32_t num_tiles, uint32_t* __restrict__ ranges) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t cur = (uint32_t)(keys[i] >> 32);
+    if (i == 0) {
+        if (cur < num_tiles) ranges[cur * 2] = 0;
+    } else {
+        uint32_t prev = (uint32_t)(keys[i - 1] >> 32);
+        if (cur != prev) {
+            if (prev < num_tiles) ranges[prev * 2 + 1] = i;
+            if (cur < num_tiles) ranges[cur * 2] = i;
+        }
+    }
+    if (i == n - 1 && cur < num_tiles) ranges[cur * 2 + 1] = n;
+}
+
+}  // namespace gsk
+
+using namespace gsk;
+
+uint32_t scan_partials_needed(uint32_t n) { return (n + SCAN_BLOCK - 1) / SCAN_BLOCK; }
+
+// Inclusive scan of int32 tiles_touched into uint32 offsets.
+hipError_t launch_scan_tiles(const int* in, uint32_t n, uint32_t* out, unsigned long long* partial,
+                             uint32_t* total32, unsigned long long* total64, hipStream_t s) {
+    uint32_t nb = scan_partials_needed(n);
+    if (nb == 0) nb = 1;
+    scan_reduce_kernel<int><<<nb, SCAN_THREADS, 0, s>>>(in, n, partial);
+    scan_partials_kernel<<<1, 1024, 0, s>>>(partial, nb, total32, total64);
+    scan_downsweep_kernel<int, true><<<nb, SCAN_THREADS, 0, s>>>(in, n, partial, out);
+    return hipGetLastError();
+}
+
+// Exclusive scan of uint32 counts in place-compatible buffers (in != out).
+static hipError_t launch_scan_u32_exclusive(const uint32_t* in, uint32_t n, uint32_t* out,
+                                            unsigned long long* partial, hipStream_t s) {
+    uint32_t nb = scan_partials_needed(n);
+    scan_reduce_kernel<uint32_t><<<nb, SCAN_THREADS, 0, s>>>(in, n, partial);
+    scan_partials_kernel<<<1, 1024, 0, s>>>(partial, nb, nullptr, nullptr);
+    scan_downsweep_kernel<uint32_t, false><<<nb, SCAN_THREADS, 0, s>>>(in, n, partial, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_duplicate(int n, const float* means2D, const float* depths, const int* radii,
+                            const uint32_t* offsets, int gx, int gy, uint64_t* keys, uint32_t* values,
+                            uint32_t capacity, uint32_t* overflow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    duplicate_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, reinterpret_cast<const float2*>(means2D), depths, radii,
+                                                     offsets, gx, gy, keys, values, capacity, overflow);
+    return hipGetLastError();
+}
+
+uint32_t radix_blocks(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+// Workspace layout for one sort of n pairs with the given radix:
+//   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][partials]
+size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
+    size_t nb = radix_blocks(n);
+    size_t R = (size_t)1 << radix_bits;
+    size_t tab = R * nb;
+    size_t np = (tab + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    return al((size_t)n * 8) + al((size_t)n * 4) + 2 * al(tab * 4) + al(np * 8);
+}
+
+template <int BITS>
+static hipError_t sort_pass(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n, int shift,
+                            uint32_t* hist, uint32_t* goff, unsigned long long* partial, hipStream_t s) {
+    uint32_t nb = radix_blocks(n);
+    rs_upsweep_kernel<BITS><<<nb, RS_THREADS, 0, s>>>(ki, n, shift, nb, hist);
+    hipError_t e = launch_scan_u32_exclusive(hist, (uint32_t)((1u << BITS) * nb), goff, partial, s);
+    if (e != hipSuccess) return e;
+    rs_downsweep_kernel<BITS><<<nb, RS_THREADS, 0, s>>>(ki, vi, n, shift, nb, goff, ko, vo);
+    return hipGetLastError();
+}
+
+hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out, uint32_t* vals_out,
+                             uint32_t n, int key_bits, int radix_bits, void* workspace, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t nb = radix_blocks(n);
+    size_t tab = ((size_t)1 << radix_bits) * nb;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    char* w = static_cast<char*>(workspace);
+    uint64_t* pk = reinterpret_cast<uint64_t*>(w);
+    w += al((size_t)n * 8);
+    uint32_t* pv = reinterpret_cast<uint32_t*>(w);
+    w += al((size_t)n * 4);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w);
+    w += al(tab * 4);
+    uint32_t* goff = reinterpret_cast<uint32_t*>(w);
+    w += al(tab * 4);
+    unsigned long long* partial = reinterpret_cast<unsigned long long*>(w);
+
+    int passes = (key_bits + radix_bits - 1) / radix_bits;
+    if (passes <= 0) {
+        hipError_t e = hipMemcpyAsync(keys_out, keys_in, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(vals_out, vals_in, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
+    }
+    const uint64_t* ki = keys_in;
+    const uint32_t* vi = vals_in;
+    for (int p = 0; p < passes; p++) {
+        // the last pass lands in the output; earlier passes alternate so that it does
+        bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t* ko = to_out ? keys_out : pk;
+        uint32_t* vo = to_out ? vals_out : pv;
+        hipError_t e;
+        int shift = p * radix_bits;
+        switch (radix_bits) {
+            case 4: e = sort_pass<4>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
+            case 5: e = sort_pass<5>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
+            case 6: e = sort_pass<6>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
+            case 7: e = sort_pass<7>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
+            case 8: e = sort_pass<8>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
+            default: return hipErrorInvalidValue;
+        }
+        if (e != hipSuccess) return e;
+        ki = ko;
+        vi = vo;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_tile_ranges(const uint64_t* keys_sorted, uint32_t n, uint32_t num_tiles, uint32_t* ranges,
+                              hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    tile_ranges_kernel<<<(n + 255) / 256, 256, 0, s>>>(keys_sorted, n, num_tiles, ranges);
+    return hipGetLastError();
+}

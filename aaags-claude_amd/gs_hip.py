@@ -1,0 +1,307 @@
+"""ctypes binding of libgs_hip.so (the C ABI in include/gs_hip.h).
+
+Device memory and streams come from PyTorch (plumbing only); every computation
+runs in the library's HIP kernels.  The library is required: if it has not been
+built (``make -C aaags-claude_amd`` or ``__graft_entry__.build()``) importing
+:class:`GsHip` raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from gs_scenes import GsCamera, GsRenderCfg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libgs_hip.so")
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int
+_u32 = ctypes.c_uint32
+
+
+class _Gaussians(ctypes.Structure):
+    _fields_ = [("count", _i32), ("sh_degree", _i32), ("max_coeffs", _i32), ("raw", _i32), ("positions", _vp),
+                ("scales", _vp), ("rotations", _vp), ("sh_coeffs", _vp), ("opacities", _vp)]
+
+
+class _Pre(ctypes.Structure):
+    _fields_ = [("means2D", _vp), ("depths", _vp), ("conics", _vp), ("opacities", _vp), ("rgb", _vp),
+                ("radii", _vp), ("tiles_touched", _vp)]
+
+
+class _RGrads(ctypes.Structure):
+    _fields_ = [("d_means2D", _vp), ("d_conics", _vp), ("d_rgb", _vp), ("d_opacities_2d", _vp)]
+
+
+class _PGrads(ctypes.Structure):
+    _fields_ = [("d_positions", _vp), ("d_scales", _vp), ("d_rotations", _vp), ("d_sh_coeffs", _vp),
+                ("d_opacities", _vp)]
+
+
+class _PixelState(ctypes.Structure):
+    _fields_ = [("T_final", _vp), ("n_contrib", _vp), ("last_contrib", _vp)]
+
+
+class _AdamHP(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_float * 5), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float), ("bc1", ctypes.c_float), ("bc2", ctypes.c_float)]
+
+
+class GsError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libgs_hip.so once; raises if it is missing (no fallback path)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise GsError(f"{path} is not built; run `make -C {HERE}` or __graft_entry__.build()")
+        _LIB = ctypes.CDLL(path)
+        _LIB.gs_last_error.restype = ctypes.c_char_p
+        _LIB.gs_lr_schedule.restype = ctypes.c_float
+        _LIB.gs_lr_schedule.argtypes = [ctypes.c_float, ctypes.c_float, _i32, _i32]
+        _LIB.gs_adam_bias_corrections.argtypes = [ctypes.c_float, ctypes.c_float, _i32, _vp, _vp]
+        _LIB.gs_zero.argtypes = [_vp, _vp, ctypes.c_size_t]
+        _LIB.gs_ctx_reserve.argtypes = [_vp, ctypes.c_size_t]
+        _LIB.gs_radix_sort_kv.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _i32, _i32]
+        _LIB.gs_tile_ranges.argtypes = [_vp, _u32, _vp, _i32, _vp]
+        _LIB.gs_sort_pairs.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _i32, _vp]
+        _LIB.gs_duplicate_keys.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _u32]
+        _LIB.gs_scan_tiles.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
+    return _LIB
+
+
+def ptr(t) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def _f32(x, dev):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float32), device=dev)
+
+
+class GsHip:
+    """One library context bound to a device and to torch's current stream."""
+
+    def __init__(self, device: int = 0, radix_bits: int = 8, raster_batch: int = 256, raster_bwd_batch: int = 256):
+        self.lib = load_library()
+        self.dev = torch.device("cuda", device)
+        self.ctx = _vp()
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        self._check(self.lib.gs_ctx_create(_i32(device), _vp(stream), ctypes.byref(self.ctx)), "gs_ctx_create")
+        self.set_tuning(radix_bits, raster_batch, raster_bwd_batch)
+
+    def __del__(self):
+        try:
+            if self.ctx:
+                self.lib.gs_ctx_destroy(self.ctx)
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != 0:
+            raise GsError(f"{what} failed ({st}): {self.lib.gs_last_error().decode()}")
+
+    def set_tuning(self, radix_bits=0, raster_batch=0, raster_bwd_batch=0):
+        self._check(self.lib.gs_ctx_set_tuning(self.ctx, _i32(radix_bits), _i32(raster_batch), _i32(raster_bwd_batch)),
+                    "gs_ctx_set_tuning")
+
+    def sync(self):
+        self._check(self.lib.gs_ctx_synchronize(self.ctx), "gs_ctx_synchronize")
+
+    # ------------------------------------------------------------ tensors --
+    def upload_scene(self, scene) -> dict:
+        d = self.dev
+        return dict(pos=_f32(scene.pos, d), scale=_f32(scene.log_scale, d), rot=_f32(scene.rot, d),
+                    sh=_f32(scene.sh, d), op=_f32(scene.opacity_logit, d), sh_degree=scene.sh_degree,
+                    max_coeffs=scene.max_coeffs, n=scene.n)
+
+    def _gauss(self, p, raw=True) -> _Gaussians:
+        return _Gaussians(p["n"], p["sh_degree"], p["max_coeffs"], int(raw), ptr(p["pos"]), ptr(p["scale"]),
+                          ptr(p["rot"]), ptr(p["sh"]), ptr(p["op"]))
+
+    def alloc_pre(self, n):
+        d = self.dev
+        return dict(means2D=torch.empty((n, 2), device=d), depths=torch.empty(n, device=d),
+                    conics=torch.empty((n, 3), device=d), op2d=torch.empty(n, device=d),
+                    rgb=torch.empty((n, 3), device=d), radii=torch.empty(n, dtype=torch.int32, device=d),
+                    tiles=torch.empty(n, dtype=torch.int32, device=d))
+
+    @staticmethod
+    def _pre_struct(pre) -> _Pre:
+        return _Pre(ptr(pre["means2D"]), ptr(pre.get("depths")), ptr(pre["conics"]), ptr(pre["op2d"]),
+                    ptr(pre["rgb"]), ptr(pre.get("radii")), ptr(pre.get("tiles")))
+
+    # ------------------------------------------------------------- stages --
+    def activate(self, p):
+        n, k = p["n"], p["max_coeffs"]
+        d = self.dev
+        out = dict(pos=torch.empty((n, 3), device=d), scale=torch.empty((n, 3), device=d),
+                   rot=torch.empty((n, 4), device=d), sh=torch.empty((n, k, 3), device=d),
+                   op=torch.empty(n, device=d))
+        g = self._gauss(p, raw=True)
+        self._check(self.lib.gs_activate(self.ctx, ctypes.byref(g), _vp(ptr(out["pos"])), _vp(ptr(out["scale"])),
+                                         _vp(ptr(out["rot"])), _vp(ptr(out["sh"])), _vp(ptr(out["op"]))),
+                    "gs_activate")
+        return out
+
+    def preprocess(self, p, cam, cfg, raw=True, pre=None):
+        pre = pre if pre is not None else self.alloc_pre(p["n"])
+        g = self._gauss(p, raw)
+        c, f = cam.to_c(), cfg.to_c()
+        ps = self._pre_struct(pre)
+        self._check(self.lib.gs_preprocess_fwd(self.ctx, ctypes.byref(g), ctypes.byref(c), ctypes.byref(f),
+                                               ctypes.byref(ps)), "gs_preprocess_fwd")
+        return pre
+
+    def scan_tiles(self, tiles):
+        n = tiles.numel()
+        offsets = torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
+        total_dev = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        total = ctypes.c_longlong()
+        self._check(self.lib.gs_scan_tiles(self.ctx, n, ptr(tiles), ptr(offsets), ptr(total_dev), ctypes.byref(total)),
+                    "gs_scan_tiles")
+        return offsets[:n], int(total.value)
+
+    def duplicate(self, pre, offsets, K, width, height, capacity=None):
+        n = pre["radii"].numel()
+        cap = K if capacity is None else capacity
+        keys = torch.empty(max(cap, 1), dtype=torch.int64, device=self.dev)
+        vals = torch.empty(max(cap, 1), dtype=torch.int32, device=self.dev)
+        self._check(self.lib.gs_duplicate_keys(self.ctx, n, ptr(pre["means2D"]), ptr(pre["depths"]), ptr(pre["radii"]),
+                                               ptr(offsets), width, height, 16, 16, ptr(keys), ptr(vals), cap),
+                    "gs_duplicate_keys")
+        return keys[:K], vals[:K]
+
+    def check_overflow(self) -> bool:
+        o = ctypes.c_int()
+        self._check(self.lib.gs_check_overflow(self.ctx, ctypes.byref(o)), "gs_check_overflow")
+        return bool(o.value)
+
+    def radix_sort(self, keys, vals, key_bits=64, radix_bits=0):
+        n = keys.numel()
+        ko = torch.empty_like(keys)
+        vo = torch.empty_like(vals)
+        if n:
+            self._check(self.lib.gs_radix_sort_kv(self.ctx, n, ptr(keys), ptr(vals), ptr(ko), ptr(vo), key_bits,
+                                                  radix_bits), "gs_radix_sort_kv")
+        return ko, vo
+
+    def sort_pairs(self, keys, vals, num_tiles):
+        n = keys.numel()
+        ko = torch.empty(max(n, 1), dtype=torch.int64, device=self.dev)
+        vo = torch.empty(max(n, 1), dtype=torch.int32, device=self.dev)
+        ranges = torch.empty(num_tiles * 2, dtype=torch.int32, device=self.dev)
+        self._check(self.lib.gs_sort_pairs(self.ctx, n, ptr(keys) if n else None, ptr(vals) if n else None, ptr(ko),
+                                           ptr(vo), num_tiles, ptr(ranges)), "gs_sort_pairs")
+        return ko[:n], vo[:n], ranges
+
+    def tile_ranges(self, keys_sorted, num_tiles):
+        ranges = torch.empty(num_tiles * 2, dtype=torch.int32, device=self.dev)
+        n = keys_sorted.numel()
+        self._check(self.lib.gs_tile_ranges(self.ctx, n, ptr(keys_sorted) if n else None, num_tiles, ptr(ranges)),
+                    "gs_tile_ranges")
+        return ranges
+
+    def raster_fwd(self, pre, values_sorted, ranges, width, height, bg=(0.0, 0.0, 0.0)):
+        d = self.dev
+        img = torch.empty((height, width, 3), device=d)
+        px = dict(T_final=torch.empty((height, width), device=d),
+                  n_contrib=torch.empty((height, width), dtype=torch.int32, device=d),
+                  last_contrib=torch.empty((height, width), dtype=torch.int32, device=d))
+        ps = self._pre_struct(pre)
+        st = _PixelState(ptr(px["T_final"]), ptr(px["n_contrib"]), ptr(px["last_contrib"]))
+        bgv = (ctypes.c_float * 3)(*bg)
+        vs = values_sorted if values_sorted.numel() else torch.zeros(1, dtype=torch.int32, device=d)
+        self._check(self.lib.gs_raster_fwd(self.ctx, width, height, ctypes.byref(ps), _vp(ptr(vs)), _vp(ptr(ranges)),
+                                           bgv, _vp(ptr(img)), ctypes.byref(st)), "gs_raster_fwd")
+        return img, px
+
+    def l1_loss(self, img, gt):
+        H, W = img.shape[0], img.shape[1]
+        d_img = torch.empty_like(img)
+        loss = torch.zeros(1, dtype=torch.float64, device=self.dev)
+        self._check(self.lib.gs_l1_loss(self.ctx, W, H, _vp(ptr(img)), _vp(ptr(gt)), _vp(ptr(d_img)), _vp(ptr(loss))),
+                    "gs_l1_loss")
+        return loss, d_img
+
+    def alloc_rgrads(self, n):
+        d = self.dev
+        return dict(d_means2D=torch.zeros((n, 2), device=d), d_conics=torch.zeros((n, 3), device=d),
+                    d_rgb=torch.zeros((n, 3), device=d), d_op2d=torch.zeros(n, device=d))
+
+    def raster_bwd(self, pre, values_sorted, ranges, width, height, px, d_image, bg=(0.0, 0.0, 0.0), rg=None):
+        n = pre["radii"].numel() if "radii" in pre else pre["op2d"].numel()
+        rg = rg if rg is not None else self.alloc_rgrads(n)
+        ps = self._pre_struct(pre)
+        st = _PixelState(ptr(px["T_final"]), ptr(px.get("n_contrib")), ptr(px["last_contrib"]))
+        gs = _RGrads(ptr(rg["d_means2D"]), ptr(rg["d_conics"]), ptr(rg["d_rgb"]), ptr(rg["d_op2d"]))
+        bgv = (ctypes.c_float * 3)(*bg)
+        vs = values_sorted if values_sorted.numel() else torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self._check(self.lib.gs_raster_bwd(self.ctx, width, height, ctypes.byref(ps), _vp(ptr(vs)), _vp(ptr(ranges)),
+                                           bgv, ctypes.byref(st), _vp(ptr(d_image)), ctypes.byref(gs)),
+                    "gs_raster_bwd")
+        return rg
+
+    def alloc_pgrads(self, n, k):
+        d = self.dev
+        return dict(pos=torch.zeros((n, 3), device=d), scale=torch.zeros((n, 3), device=d),
+                    rot=torch.zeros((n, 4), device=d), sh=torch.zeros((n, k, 3), device=d), op=torch.zeros(n, device=d))
+
+    def preprocess_bwd(self, p, cam, cfg, radii, rg, out=None, accumulate=False):
+        out = out if out is not None else self.alloc_pgrads(p["n"], p["max_coeffs"])
+        g = self._gauss(p, raw=True)
+        c, f = cam.to_c(), cfg.to_c()
+        rgs = _RGrads(ptr(rg["d_means2D"]), ptr(rg["d_conics"]), ptr(rg["d_rgb"]), ptr(rg["d_op2d"]))
+        pgs = _PGrads(ptr(out["pos"]), ptr(out["scale"]), ptr(out["rot"]), ptr(out["sh"]), ptr(out["op"]))
+        self._check(self.lib.gs_preprocess_bwd(self.ctx, ctypes.byref(g), ctypes.byref(c), ctypes.byref(f),
+                                               _vp(ptr(radii)), ctypes.byref(rgs), ctypes.byref(pgs),
+                                               int(accumulate)), "gs_preprocess_bwd")
+        return out
+
+    def adam(self, p, grads, m, v, lrs, beta1, beta2, eps, step):
+        bc1, bc2 = ctypes.c_float(), ctypes.c_float()
+        self.lib.gs_adam_bias_corrections(ctypes.c_float(beta1), ctypes.c_float(beta2), step, ctypes.byref(bc1),
+                                          ctypes.byref(bc2))
+        hp = _AdamHP((ctypes.c_float * 5)(*lrs), beta1, beta2, eps, bc1.value, bc2.value)
+        names = ["pos", "scale", "rot", "sh", "op"]
+        P = (_vp * 5)(*[ptr(p[k]) for k in names])
+        G = (_vp * 5)(*[ptr(grads[k]) for k in names])
+        M = (_vp * 5)(*[ptr(m[k]) for k in names])
+        V = (_vp * 5)(*[ptr(v[k]) for k in names])
+        self._check(self.lib.gs_adam_step(self.ctx, p["n"], p["max_coeffs"], P, G, M, V, ctypes.byref(hp)),
+                    "gs_adam_step")
+
+    def lr_schedule(self, lr_init, lr_final, step, max_steps):
+        return self.lib.gs_lr_schedule(lr_init, lr_final, step, max_steps)
+
+    # -------------------------------------------------------- whole paths --
+    def forward(self, p, cam, cfg):
+        """Plain single-view forward (host readback of the pair count)."""
+        pre = self.preprocess(p, cam, cfg)
+        offsets, K = self.scan_tiles(pre["tiles"])
+        keys, vals = self.duplicate(pre, offsets, K, cam.width, cam.height)
+        ks, vs, ranges = self.sort_pairs(keys, vals, cam.num_tiles)
+        img, px = self.raster_fwd(pre, vs, ranges, cam.width, cam.height, cfg.bg_color)
+        return dict(pre=pre, K=K, keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges, img=img,
+                    px=px)
+
+    def forward_backward(self, p, cam, cfg, gt):
+        f = self.forward(p, cam, cfg)
+        loss, d_img = self.l1_loss(f["img"], gt)
+        rg = self.raster_bwd(f["pre"], f["values_sorted"], f["ranges"], cam.width, cam.height, f["px"], d_img,
+                             cfg.bg_color)
+        grads = self.preprocess_bwd(p, cam, cfg, f["pre"]["radii"], rg)
+        f.update(loss=loss, d_image=d_img, rgrad=rg, grads=grads)
+        return f

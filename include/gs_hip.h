@@ -1,0 +1,231 @@
+/*
+ * gs_hip.h -- C ABI of the MI355X (gfx950) 3D Gaussian Splatting render-and-train
+ * hot path.
+ *
+ * Every entry point is asynchronous on the context's HIP stream, takes device
+ * pointers and plain sizes, and returns a status code (GS_OK == 0).  Nothing
+ * here allocates caller-visible memory: outputs are caller-owned device
+ * buffers; temporaries live in a workspace owned by the context.
+ *
+ * Data layout (identical to the reference's public structs, so host arrays of
+ * the reference can be copied across unchanged):
+ *   positions  float[N*3]            (GaussianData::positions, types.h:39)
+ *   scales     float[N*3]            (types.h:41; raw: log-space, train_types.h:15)
+ *   rotations  float[N*4] (r,x,y,z)  (types.h:42; raw: unnormalised)
+ *   sh_coeffs  float[N*max_coeffs*3] interleaved RGB per basis (types.h:40)
+ *   opacities  float[N]              (types.h:43; raw: logit)
+ *   images     float[H*W*3] HWC
+ *   keys       uint64[K] = tile_id << 32 | float_bits(view depth)
+ *   values     uint32[K] = Gaussian index
+ *   ranges     uint32[num_tiles*2] = [start, end) into the sorted pair list
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef GS_HIP_H
+#define GS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define GS_OK 0
+#define GS_ERR_INVALID 1     /* bad argument (null pointer, bad size, unsupported mode) */
+#define GS_ERR_HIP 2         /* a HIP runtime call failed */
+#define GS_ERR_CAPACITY 3    /* a pre-sized buffer was too small (pairs > capacity) */
+#define GS_ERR_UNSUPPORTED 4 /* feature outside this path (eval_3D) */
+
+/* Camera: same fields and order as the reference Camera (types.h:48-54).
+ * Matrices are column-major. */
+typedef struct gs_camera {
+    float view_matrix[16];
+    float viewproj_matrix[16];
+    float cam_pos[3];
+    float tan_fovx, tan_fovy;
+    int width, height;
+} gs_camera;
+
+/* The 2D-path fields of the reference RenderConfig (types.h:57-66).
+ * eval_3D is not part of this path. */
+typedef struct gs_render_cfg {
+    float bg_color[3];
+    float scale_modifier;
+    int antialiasing;
+    int training;
+    int sh_degree;
+    int tile_w, tile_h; /* must be 16 x 16 */
+} gs_render_cfg;
+
+/* Gaussian parameter arrays.  raw != 0: pre-activation parameters
+ * (RawGaussianParams, train_types.h:9-18), activated in-kernel exactly as
+ * RawGaussianParams::activate (train_types.cpp:6-36).  raw == 0: activated
+ * GaussianData (types.h:34-45). */
+typedef struct gs_gaussians {
+    int count;
+    int sh_degree;
+    int max_coeffs;
+    int raw;
+    const float* positions;
+    const float* scales;
+    const float* rotations;
+    const float* sh_coeffs;
+    const float* opacities;
+} gs_gaussians;
+
+/* Per-Gaussian preprocess outputs (PreprocessOutput, types.h:69-82, 2D path). */
+typedef struct gs_preprocessed {
+    float* means2D;   /* [N*2] */
+    float* depths;    /* [N]   */
+    float* conics;    /* [N*3] */
+    float* opacities; /* [N]   */
+    float* rgb;       /* [N*3] */
+    int* radii;       /* [N]   */
+    int* tiles_touched; /* [N] */
+} gs_preprocessed;
+
+/* Screen-space gradients (RasterGradOutput, train_types.h:25-31). */
+typedef struct gs_raster_grads {
+    float* d_means2D;      /* [N*2] */
+    float* d_conics;       /* [N*3] */
+    float* d_rgb;          /* [N*3] */
+    float* d_opacities_2d; /* [N]   */
+} gs_raster_grads;
+
+/* Raw-parameter gradients (GradientOutput, train_types.h:35-43). */
+typedef struct gs_param_grads {
+    float* d_positions;  /* [N*3] */
+    float* d_scales;     /* [N*3] */
+    float* d_rotations;  /* [N*4] */
+    float* d_sh_coeffs;  /* [N*max_coeffs*3] */
+    float* d_opacities;  /* [N] */
+} gs_param_grads;
+
+/* Per-pixel forward state kept for the backward (ForwardCache::T_final and
+ * ::n_contrib, types.h:97-99, plus the sorted-list index + 1 of each pixel's
+ * last contributor, which lets the backward start there without a replay). */
+typedef struct gs_pixel_state {
+    float* T_final;          /* [H*W] */
+    int* n_contrib;          /* [H*W] */
+    uint32_t* last_contrib;  /* [H*W] */
+} gs_pixel_state;
+
+typedef struct gs_ctx gs_ctx;
+
+/* Context: device, stream and a growable device workspace.
+ * Replaces OpenCLContext::init (gpu/opencl_context.cpp:13-80). */
+int gs_ctx_create(int device, void* hip_stream, gs_ctx** out);
+int gs_ctx_destroy(gs_ctx* ctx);
+int gs_ctx_set_stream(gs_ctx* ctx, void* hip_stream);
+int gs_ctx_synchronize(gs_ctx* ctx);
+const char* gs_last_error(void);
+
+/* Fills `bytes` bytes with zero using a kernel (safe inside graph capture). */
+int gs_zero(gs_ctx* ctx, void* ptr, size_t bytes);
+
+/* Activation only: RawGaussianParams::activate (train_types.cpp:6-36). */
+int gs_activate(gs_ctx* ctx, const gs_gaussians* raw, float* positions, float* scales,
+                float* rotations, float* sh_coeffs, float* opacities);
+
+/* Preprocess forward: Preprocessor::process (preprocessor.h:7-12), 2D branch
+ * of PreprocessorCPU::process (cpu/preprocessor_cpu.cpp:11-62,188-265) and of
+ * the preprocess kernel (gpu/kernels/preprocess.cl:583-620).  Activation is
+ * fused when g->raw != 0.  Culled Gaussians get zeros in every output. */
+int gs_preprocess_fwd(gs_ctx* ctx, const gs_gaussians* g, const gs_camera* cam,
+                      const gs_render_cfg* cfg, const gs_preprocessed* out);
+
+/* Inclusive prefix sum of tiles_touched into uint32 offsets; the total pair
+ * count is written to *total_dev (device uint32) and, if total_host != NULL,
+ * copied to the host after a stream sync.  Replaces the host scan of
+ * TileBinnerGPU::bin (gpu/tile_binner_gpu.cpp:282-292) and
+ * tile_binner_cpu.cpp:33-41. */
+int gs_scan_tiles(gs_ctx* ctx, int n, const int* tiles_touched, uint32_t* offsets,
+                  uint32_t* total_dev, long long* total_host);
+
+/* Duplicate-with-keys: TileBinner::bin (tile_binner.h:7-12), the scatter
+ * kernel (gpu/kernels/scatter.cl:73-154) and tile_binner_cpu.cpp:59-99.
+ * `offsets` is the inclusive scan from gs_scan_tiles.  Pairs at positions >=
+ * capacity are dropped and GS_ERR_CAPACITY is reported by gs_check_overflow. */
+int gs_duplicate_keys(gs_ctx* ctx, int n, const float* means2D, const float* depths,
+                      const int* radii, const uint32_t* offsets, int width, int height,
+                      int tile_w, int tile_h, uint64_t* keys, uint32_t* values,
+                      uint32_t capacity);
+
+/* Stable LSD radix sort of (uint64 key, uint32 value) pairs on key bits
+ * [0, key_bits), radix_bits (4..8) bits per pass.  Inputs are left unchanged.
+ * Replaces rs_histogram/rs_compute_offsets/rs_scatter_kv
+ * (gpu/kernels/radix_sort_kv_cl.h:24-181) driven by SorterGPU::sort
+ * (gpu/sorter_gpu.cpp:175-303).  Equal keys keep their input order
+ * (std::stable_sort semantics). */
+int gs_radix_sort_kv(gs_ctx* ctx, uint32_t count, const uint64_t* keys_in,
+                     const uint32_t* values_in, uint64_t* keys_out, uint32_t* values_out,
+                     int key_bits, int radix_bits);
+
+/* Tile ranges from sorted keys: identify_tile_ranges
+ * (gpu/kernels/radix_sort.cl:14-46) and sorter_cpu.cpp:28-43.  Empty tiles get
+ * (0,0). */
+int gs_tile_ranges(gs_ctx* ctx, uint32_t count, const uint64_t* keys_sorted, int num_tiles,
+                   uint32_t* ranges);
+
+/* Sorter::sort (sorter.h:7): radix sort on tile-id + depth bits, then tile
+ * ranges. */
+int gs_sort_pairs(gs_ctx* ctx, uint32_t count, const uint64_t* keys_in, const uint32_t* values_in,
+                  uint64_t* keys_out, uint32_t* values_out, int num_tiles, uint32_t* ranges);
+
+/* Forward rasterization: Rasterizer::rasterize (rasterizer.h:7-15), the
+ * rasterize kernel (gpu/kernels/rasterize.cl:10-274, 2D lines) and
+ * RasterizerCPU::rasterize (cpu/rasterizer_cpu.cpp:137-198).  One workgroup
+ * per 16x16 tile.  `px` may be NULL fields if not needed. */
+int gs_raster_fwd(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre,
+                  const uint32_t* values_sorted, const uint32_t* ranges, const float bg[3],
+                  float* image, const gs_pixel_state* px);
+
+/* L1 loss and its image gradient: l1_loss / l1_loss_double (loss.cpp:4-30).
+ * loss_dev: device double, the mean absolute difference (sum in FP64). */
+int gs_l1_loss(gs_ctx* ctx, int width, int height, const float* rendered, const float* gt,
+               float* d_image, double* loss_dev);
+
+/* Backward rasterization: RasterizerBackwardCPU::backward
+ * (cpu/rasterizer_backward_cpu.cpp:7-138) and rasterize_backward
+ * (gpu/kernels/rasterize_backward.cl:23-203; the CPU recurrence is the one
+ * followed).  Gradients are ACCUMULATED into `grads` (zero them first). */
+int gs_raster_bwd(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre,
+                  const uint32_t* values_sorted, const uint32_t* ranges, const float bg[3],
+                  const gs_pixel_state* px, const float* d_image, const gs_raster_grads* grads);
+
+/* Preprocess backward: PreprocessorBackwardCPU::backward
+ * (cpu/preprocessor_backward_cpu.cpp:205-645) and preprocess_backward
+ * (gpu/kernels/preprocess_backward.cl:27-410; the CPU chain is the one
+ * followed).  g must hold RAW parameters.  accumulate != 0 adds into `out`
+ * (multi-view gradient accumulation); otherwise `out` is overwritten
+ * (culled Gaussians get 0). */
+int gs_preprocess_bwd(gs_ctx* ctx, const gs_gaussians* g, const gs_camera* cam,
+                      const gs_render_cfg* cfg, const int* radii, const gs_raster_grads* rg,
+                      const gs_param_grads* out, int accumulate);
+
+/* Adam hyper-parameters for one step (TrainConfig, train_types.h:46-60;
+ * AdamOptimizer::step, optimizer.cpp:60-101).  lr[] is per tensor in the order
+ * positions, scales, rotations, sh_coeffs, opacities (positions already
+ * scheduled); bc1/bc2 = 1/(1-beta^t). */
+typedef struct gs_adam_hparams {
+    float lr[5];
+    float beta1, beta2, eps;
+    float bc1, bc2;
+} gs_adam_hparams;
+
+/* Fused Adam over the five parameter tensors (one launch). */
+int gs_adam_step(gs_ctx* ctx, int n, int max_coeffs, float* const params[5],
+                 const float* const grads[5], float* const m[5], float* const v[5],
+                 const gs_adam_hparams* hp);
+
+/* Host helpers with the reference's host arithmetic: lr_schedule
+ * (train_types.cpp:64-68) and Adam bias corrections (optimizer.cpp:73-76). */
+float gs_lr_schedule(float lr_init, float lr_final, int step, int max_steps);
+void gs_adam_bias_corrections(float beta1, float beta2, int step, float* bc1, float* bc2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GS_HIP_H */

@@ -1,0 +1,298 @@
+"""GPU parity tests: every HIP stage against the CPU oracle on the same inputs,
+and the whole GPU path against the reference's golden vectors.
+
+Bars: binning/sort/tile ranges bit-exact given identical inputs; images and
+per-Gaussian float outputs within 1e-4 max-abs; gradients within 1e-3 relative
+(normalised by the largest entry of each array); Adam bit-exact.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from gs_scenes import RenderCfg, make_workload, random_small_scene
+from helpers import (IMAGE_TOL, assert_grad_close, assert_image_close, load_golden, rel_err, scene_dirs, u32)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+SMALL_CASES = [(n, deg, seed) for n, deg, seed in [(3, 0, 1), (40, 1, 2), (200, 2, 3), (500, 3, 4), (2000, 3, 5)]]
+
+
+@pytest.mark.parametrize("n,deg,seed", SMALL_CASES)
+def test_preprocess_matches_oracle(hip, oracle, n, deg, seed):
+    _say("preprocess", n, deg, seed)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=96, height=72)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    p = hip.upload_scene(scene)
+    pre = hip.preprocess(p, cam, cfg)
+    g = oracle.activate(scene)
+    ref = oracle.preprocess(g, n, deg, scene.max_coeffs, cam, cfg, cache=False)
+    assert np.array_equal(_np(pre["radii"]), ref["radii"])
+    assert np.array_equal(_np(pre["tiles"]), ref["tiles"])
+    for k in ["means2D", "depths", "conics", "op2d", "rgb"]:
+        d = np.abs(_np(pre[k]) - ref[k]).max()
+        assert d <= 1e-4, (k, d)
+
+
+def test_activation_matches_oracle(hip, oracle):
+    _say("activate")
+    scene, cam, gt = random_small_scene(5000, 3, 11)
+    p = hip.upload_scene(scene)
+    a = hip.activate(p)
+    ref = oracle.activate(scene)
+    for k in ["pos", "rot", "sh"]:
+        assert np.array_equal(_np(a[k]), ref[k].reshape(_np(a[k]).shape)), k
+    for k in ["scale", "op"]:  # device expf vs glibc expf: at most 1 ulp
+        got, want = _np(a[k]).ravel(), ref[k].ravel()
+        ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+        assert ulps.max() <= 1, (k, ulps.max())
+
+
+@pytest.mark.parametrize("n,deg,seed", SMALL_CASES[1:] + [(10000, 0, 0)])
+def test_binning_and_sort_bitexact(hip, oracle, n, deg, seed):
+    """Identical preprocess outputs in -> identical keys, values, ranges out."""
+    _say("binning+sort", n, deg, seed)
+    if n == 10000:
+        scene, cams, _ = make_workload("c1_10k_256")
+        cam = cams[0]
+    else:
+        scene, cam, _ = random_small_scene(n, deg, seed, width=160, height=120)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    g = oracle.activate(scene)
+    ref = oracle.preprocess(g, scene.n, scene.sh_degree, scene.max_coeffs, cam, cfg, cache=False)
+    keys_ref, vals_ref = oracle.duplicate(ref, scene.n, cam.width, cam.height)
+    ks_ref, vs_ref, rg_ref = oracle.sort_pairs(keys_ref, vals_ref, cam.num_tiles)
+    d = hip.dev
+    pre = dict(means2D=torch.tensor(ref["means2D"], device=d), depths=torch.tensor(ref["depths"], device=d),
+               radii=torch.tensor(ref["radii"], device=d), tiles=torch.tensor(ref["tiles"], device=d))
+    offsets, K = hip.scan_tiles(pre["tiles"])
+    assert K == len(keys_ref)
+    assert np.array_equal(u32(offsets), np.cumsum(ref["tiles"]).astype(np.uint32))
+    keys, vals = hip.duplicate(pre, offsets, K, cam.width, cam.height)
+    assert np.array_equal(u32(keys), keys_ref)
+    assert np.array_equal(u32(vals), vals_ref)
+    ks, vs, ranges = hip.sort_pairs(keys, vals, cam.num_tiles)
+    assert np.array_equal(u32(ks), ks_ref)
+    assert np.array_equal(u32(vs), vs_ref)
+    assert np.array_equal(u32(ranges), rg_ref)
+
+
+@pytest.mark.parametrize("radix_bits", [4, 5, 6, 7, 8])
+@pytest.mark.parametrize("count", [1, 63, 4096, 4097, 100_003])
+def test_radix_sort_stable_random_keys(hip, count, radix_bits):
+    """Random 64-bit keys with many ties: equal to numpy's stable argsort."""
+    _say("radix", count, radix_bits)
+    rng = np.random.default_rng(count * 10 + radix_bits)
+    hi = rng.integers(0, 50, count, dtype=np.uint64)
+    lo = rng.integers(0, 1 << 20, count, dtype=np.uint64) & np.uint64(0xFFFF0)  # ties
+    keys = (hi << np.uint64(32)) | lo
+    vals = np.arange(count, dtype=np.uint32)
+    order = np.argsort(keys, kind="stable")
+    kt = torch.tensor(keys.view(np.int64), device=hip.dev)
+    vt = torch.tensor(vals.view(np.int32), device=hip.dev)
+    ko, vo = hip.radix_sort(kt, vt, key_bits=38, radix_bits=radix_bits)
+    assert np.array_equal(u32(ko), keys[order])
+    assert np.array_equal(u32(vo), vals[order])
+
+
+def test_radix_sort_full_64bit(hip):
+    _say("radix 64-bit")
+    rng = np.random.default_rng(7)
+    keys = rng.integers(0, np.iinfo(np.int64).max, 50_000, dtype=np.int64).view(np.uint64)
+    keys[::7] = keys[3]  # ties
+    keys[5] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    vals = np.arange(keys.size, dtype=np.uint32)
+    order = np.argsort(keys, kind="stable")
+    ko, vo = hip.radix_sort(torch.tensor(keys.view(np.int64), device=hip.dev),
+                            torch.tensor(vals.view(np.int32), device=hip.dev), key_bits=64)
+    assert np.array_equal(u32(ko), keys[order])
+    assert np.array_equal(u32(vo), vals[order])
+
+
+def test_sort_empty_and_tile_ranges_edge_cases(hip, oracle):
+    _say("sort edge cases")
+    # sorter_cpu known answers (test_sorter.cpp:6-102): empty, single, empty tile in the middle
+    for keys, vals, tiles in [
+        (np.array([], np.uint64), np.array([], np.uint32), 1),
+        (np.array([42], np.uint64), np.array([7], np.uint32), 1),
+        (np.array([(1 << 32) | 3, 1, 2], np.uint64), np.array([0, 1, 2], np.uint32), 2),
+        (np.array([1, (2 << 32) | 2], np.uint64), np.array([0, 1], np.uint32), 3),
+    ]:
+        ks_ref, vs_ref, rg_ref = oracle.sort_pairs(keys, vals, tiles)
+        ks, vs, rg = hip.sort_pairs(torch.tensor(keys.view(np.int64), device=hip.dev),
+                                    torch.tensor(vals.view(np.int32), device=hip.dev), tiles)
+        assert np.array_equal(u32(ks), ks_ref)
+        assert np.array_equal(u32(vs), vs_ref)
+        assert np.array_equal(u32(rg), rg_ref)
+
+
+def _oracle_stage_inputs(oracle, scene, cam, cfg):
+    f = oracle.forward(scene, cam, cfg)
+    return f
+
+
+def _to_dev(hip, pre):
+    d = hip.dev
+    out = {}
+    for k, v in pre.items():
+        out[k] = torch.tensor(np.ascontiguousarray(v), device=d)
+    return out
+
+
+@pytest.mark.parametrize("n,deg,seed,bg", [(40, 1, 2, (0, 0, 0)), (500, 3, 4, (0.2, 0.5, 1.0)), (2000, 3, 5, (0, 0, 0))])
+@pytest.mark.parametrize("batch", [64, 256, 512])
+def test_raster_fwd_matches_oracle(hip, oracle, n, deg, seed, bg, batch):
+    _say("raster fwd", n, deg, seed, batch)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=100, height=75)
+    cfg = RenderCfg(sh_degree=3, training=True, bg_color=bg)
+    f = _oracle_stage_inputs(oracle, scene, cam, cfg)
+    pre = _to_dev(hip, {k: f["pre"][k] for k in ["means2D", "conics", "op2d", "rgb", "radii"]})
+    vs = torch.tensor(f["values_sorted"].view(np.int32), device=hip.dev)
+    rg = torch.tensor(f["ranges"].view(np.int32), device=hip.dev)
+    hip.set_tuning(raster_batch=batch)
+    img, px = hip.raster_fwd(pre, vs, rg, cam.width, cam.height, bg)
+    hip.set_tuning(raster_batch=256)
+    assert_image_close(_np(img), f["img"])
+    assert np.abs(_np(px["T_final"]) - f["T_final"]).max() <= 1e-5
+    mism = int((_np(px["n_contrib"]) != f["n_contrib"]).sum())
+    assert mism == 0, mism
+    assert np.array_equal(u32(px["last_contrib"]), f["last_contrib"])
+
+
+def test_l1_loss_fixtures(hip):
+    """tests/test_data/l1_verify of the reference: bit-exact d_image, loss 1e-6 relative."""
+    import json
+    import os
+    from helpers import GOLDEN
+    base = os.path.join(GOLDEN, "l1_verify")
+    for case in json.load(open(os.path.join(base, "manifest.json"))):
+        _say("l1", case["name"])
+        H, W = case["H"], case["W"]
+        r = np.fromfile(os.path.join(base, case["name"] + "_rendered.bin"), np.float32).reshape(H, W, 3)
+        g = np.fromfile(os.path.join(base, case["name"] + "_gt.bin"), np.float32).reshape(H, W, 3)
+        d = np.fromfile(os.path.join(base, case["name"] + "_d_image.bin"), np.float32).reshape(H, W, 3)
+        loss, d_img = hip.l1_loss(torch.tensor(r, device=hip.dev), torch.tensor(g, device=hip.dev))
+        assert np.array_equal(_np(d_img), d)
+        exact = np.abs(r.astype(np.float64) - g.astype(np.float64)).sum() / r.size
+        assert abs(float(loss.item()) - exact) <= 1e-12 + 1e-9 * exact
+        assert abs(float(loss.item()) - case["loss_serial"]) <= 1e-6
+
+
+@pytest.mark.parametrize("n,deg,seed,bg", [(40, 1, 2, (0, 0, 0)), (500, 3, 4, (0.3, 0.1, 0.7)), (2000, 3, 5, (0, 0, 0))])
+@pytest.mark.parametrize("batch", [64, 256])
+def test_raster_bwd_matches_oracle(hip, oracle, n, deg, seed, bg, batch):
+    _say("raster bwd", n, deg, seed, batch)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=100, height=75)
+    cfg = RenderCfg(sh_degree=3, training=True, bg_color=bg)
+    f = oracle.forward_backward(scene, cam, cfg, gt)
+    pre = _to_dev(hip, {k: f["pre"][k] for k in ["means2D", "conics", "op2d", "rgb", "radii"]})
+    vs = torch.tensor(f["values_sorted"].view(np.int32), device=hip.dev)
+    rg = torch.tensor(f["ranges"].view(np.int32), device=hip.dev)
+    px = dict(T_final=torch.tensor(f["T_final"], device=hip.dev),
+              last_contrib=torch.tensor(f["last_contrib"].view(np.int32), device=hip.dev))
+    d_img = torch.tensor(f["d_image"], device=hip.dev)
+    hip.set_tuning(raster_bwd_batch=batch)
+    g = hip.raster_bwd(pre, vs, rg, cam.width, cam.height, px, d_img, bg)
+    hip.set_tuning(raster_bwd_batch=256)
+    for k in ["d_means2D", "d_conics", "d_rgb", "d_op2d"]:
+        assert_grad_close(_np(g[k]), f["rgrad"][k], k)
+
+
+@pytest.mark.parametrize("n,deg,seed", [(40, 0, 2), (300, 1, 3), (300, 2, 6), (2000, 3, 5)])
+def test_preprocess_bwd_matches_oracle(hip, oracle, n, deg, seed):
+    _say("preprocess bwd", n, deg, seed)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=100, height=75)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    f = oracle.forward_backward(scene, cam, cfg, gt)
+    p = hip.upload_scene(scene)
+    rg = _to_dev(hip, f["rgrad"])
+    radii = torch.tensor(f["pre"]["radii"], device=hip.dev)
+    out = hip.preprocess_bwd(p, cam, cfg, radii, rg)
+    for k in ["pos", "scale", "rot", "sh", "op"]:
+        assert_grad_close(_np(out[k]), f["grads"][k], "grad_" + k)
+
+
+def test_adam_bitexact(hip, oracle):
+    _say("adam")
+    rng = np.random.default_rng(3)
+    n, kc = 1001, 16
+    shapes = dict(pos=(n, 3), scale=(n, 3), rot=(n, 4), sh=(n, kc, 3), op=(n,))
+    p = {k: rng.standard_normal(s).astype(np.float32) for k, s in shapes.items()}
+    m = {k: np.zeros(s, np.float32) for k, s in shapes.items()}
+    v = {k: np.zeros(s, np.float32) for k, s in shapes.items()}
+    d = hip.dev
+    pt = {k: torch.tensor(a, device=d) for k, a in p.items()}
+    mt = {k: torch.zeros(s, device=d) for k, s in shapes.items()}
+    vt = {k: torch.zeros(s, device=d) for k, s in shapes.items()}
+    pt["n"], pt["max_coeffs"] = n, kc
+    lrs = [oracle.lr_schedule(1.6e-4, 1.6e-6, 0, 30000), 5e-3, 1e-3, 2.5e-3, 2.5e-2]
+    for step in range(1, 4):
+        g = {k: rng.standard_normal(s).astype(np.float32) for k, s in shapes.items()}
+        g["sh"][::3] = 0.0
+        gt_ = {k: torch.tensor(a, device=d) for k, a in g.items()}
+        hip.adam(pt, gt_, mt, vt, lrs, 0.9, 0.999, 1e-8, step)
+        bc1, bc2 = oracle.bias_corrections(0.9, 0.999, step)
+        for i, k in enumerate(["pos", "scale", "rot", "sh", "op"]):
+            oracle.adam_update(p[k], g[k], m[k], v[k], lrs[i], 0.9, 0.999, 1e-8, bc1, bc2)
+    for k in shapes:
+        assert np.array_equal(_np(pt[k]), p[k]), k
+        assert np.array_equal(_np(mt[k]), m[k]), k
+        assert np.array_equal(_np(vt[k]), v[k]), k
+
+
+@pytest.mark.parametrize("name", scene_dirs())
+def test_golden_scenes_full_path(hip, name):
+    """The whole GPU path against the reference's own golden vectors
+    (train/verify_grads_ext/*, train/verify_forward)."""
+    _say("golden", name)
+    scene, cam, cfg, gt, exp = load_golden(name)
+    p = hip.upload_scene(scene)
+    f = hip.forward_backward(p, cam, cfg, torch.tensor(gt, device=hip.dev))
+    assert_image_close(_np(f["img"]), exp["img"])
+    if "loss" in exp:
+        assert abs(float(f["loss"].item()) - exp["loss"]) <= 1e-6
+    for k in ["pos", "scale", "rot", "sh", "op"]:
+        if "grad_" + k in exp:
+            assert_grad_close(_np(f["grads"][k]), exp["grad_" + k], "grad_" + k)
+
+
+@pytest.mark.parametrize("n,deg,seed", [(500, 3, 21), (3000, 3, 22), (3000, 1, 23)])
+def test_full_path_matches_oracle(hip, oracle, n, deg, seed):
+    _say("full path", n, deg, seed)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=128, height=96)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    ref = oracle.forward_backward(scene, cam, cfg, gt)
+    p = hip.upload_scene(scene)
+    f = hip.forward_backward(p, cam, cfg, torch.tensor(gt, device=hip.dev))
+    assert_image_close(_np(f["img"]), ref["img"])
+    assert abs(float(f["loss"].item()) - ref["loss"]) <= 1e-6
+    for k in ["pos", "scale", "rot", "sh", "op"]:
+        assert_grad_close(_np(f["grads"][k]), ref["grads"][k], "grad_" + k)
+
+
+def test_c1_workload_full_path(hip, oracle):
+    """10k Gaussians, 256x256, SH0: the reference's own CPU-runnable workload."""
+    _say("c1 workload")
+    scene, cams, gts = make_workload("c1_10k_256")
+    cam, gt = cams[0], gts[0]
+    cfg = RenderCfg(sh_degree=3, training=True)
+    ref = oracle.forward_backward(scene, cam, cfg, gt)
+    p = hip.upload_scene(scene)
+    f = hip.forward_backward(p, cam, cfg, torch.tensor(gt, device=hip.dev))
+    img_err = np.abs(_np(f["img"]) - ref["img"]).max()
+    assert f["K"] == len(ref["keys"])
+    assert img_err <= IMAGE_TOL, img_err
+    assert abs(float(f["loss"].item()) - ref["loss"]) <= 1e-6
+    for k in ["pos", "scale", "rot", "sh", "op"]:
+        assert_grad_close(_np(f["grads"][k]), ref["grads"][k], "grad_" + k)
