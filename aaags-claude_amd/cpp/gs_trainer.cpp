@@ -1,0 +1,332 @@
+// gs_trainer.cpp -- native single-view training driver over the C ABI.
+//
+// Mirrors Trainer::step (harmonyos_3dgs/src/trainer.cpp:7-70): activation is
+// fused into the preprocess kernel, the forward keeps T_final / contributor
+// state on the device, the backward and the optimizer run on the same stream,
+// and the only host round trips are the pair-count readback used to size the
+// pair buffers (as TileBinnerGPU::bin does, gpu/tile_binner_gpu.cpp:292-316)
+// and the loss readback of StepResult.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "gs_internal.h"
+#include "gs_trainer_impl.h"
+
+using gs::DevBuf;
+
+namespace gs {
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes) return GS_OK;
+    if (p) {
+        hipError_t e = hipFree(p);
+        if (e != hipSuccess) return gs_hip_status(e, "hipFree");
+        p = nullptr;
+        bytes = 0;
+    }
+    size_t nb = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) return gs_hip_status(e, "hipMalloc");
+    bytes = nb;
+    return GS_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+}  // namespace gs
+
+#define TR_TRY(x)                 \
+    do {                          \
+        int st_ = (x);            \
+        if (st_ != GS_OK) return st_; \
+    } while (0)
+
+static int tr_alloc_view(gs_trainer* t, gs::ViewBufs& v, int W, int H) {
+    size_t n = (size_t)t->n;
+    size_t npix = (size_t)W * H;
+    int tiles = ((W + 15) / 16) * ((H + 15) / 16);
+    TR_TRY(v.means2D.ensure(n * 8));
+    TR_TRY(v.depths.ensure(n * 4));
+    TR_TRY(v.conics.ensure(n * 12));
+    TR_TRY(v.op2d.ensure(n * 4));
+    TR_TRY(v.rgb.ensure(n * 12));
+    TR_TRY(v.radii.ensure(n * 4));
+    TR_TRY(v.tiles.ensure(n * 4));
+    TR_TRY(v.offsets.ensure(n * 4 + 4));
+    TR_TRY(v.ranges.ensure((size_t)tiles * 8));
+    TR_TRY(v.image.ensure(npix * 12));
+    TR_TRY(v.d_image.ensure(npix * 12));
+    TR_TRY(v.T_final.ensure(npix * 4));
+    TR_TRY(v.n_contrib.ensure(npix * 4));
+    TR_TRY(v.last.ensure(npix * 4));
+    TR_TRY(v.loss.ensure(8));
+    TR_TRY(v.rg_means2D.ensure(n * 8));
+    TR_TRY(v.rg_conics.ensure(n * 12));
+    TR_TRY(v.rg_rgb.ensure(n * 12));
+    TR_TRY(v.rg_op.ensure(n * 4));
+    return GS_OK;
+}
+
+gs_preprocessed gs::ViewBufs::pre() {
+    gs_preprocessed p;
+    p.means2D = means2D.f();
+    p.depths = depths.f();
+    p.conics = conics.f();
+    p.opacities = op2d.f();
+    p.rgb = rgb.f();
+    p.radii = radii.i();
+    p.tiles_touched = tiles.i();
+    return p;
+}
+
+gs_raster_grads gs::ViewBufs::rgrads() {
+    gs_raster_grads g;
+    g.d_means2D = rg_means2D.f();
+    g.d_conics = rg_conics.f();
+    g.d_rgb = rg_rgb.f();
+    g.d_opacities_2d = rg_op.f();
+    return g;
+}
+
+gs_gaussians gs_trainer::gaussians() const {
+    gs_gaussians g;
+    g.count = n;
+    g.sh_degree = sh_degree;
+    g.max_coeffs = max_coeffs;
+    g.raw = 1;
+    g.positions = static_cast<const float*>(param[0].p);
+    g.scales = static_cast<const float*>(param[1].p);
+    g.rotations = static_cast<const float*>(param[2].p);
+    g.sh_coeffs = static_cast<const float*>(param[3].p);
+    g.opacities = static_cast<const float*>(param[4].p);
+    return g;
+}
+
+size_t gs_trainer::tensor_elems(int k) const {
+    const size_t per[5] = {3, 3, 4, (size_t)max_coeffs * 3, 1};
+    return (size_t)n * per[k];
+}
+
+extern "C" int gs_trainer_create(gs_ctx* ctx, int n, int sh_degree, int max_coeffs, gs_trainer** out) {
+    if (!ctx || !out) return gs_set_error(GS_ERR_INVALID, "gs_trainer_create: null argument");
+    if (n < 0) return gs_set_error(GS_ERR_INVALID, "gs_trainer_create: negative count");
+    if (!(max_coeffs == 1 || max_coeffs == 4 || max_coeffs == 9 || max_coeffs == 16) || sh_degree < 0 ||
+        (sh_degree + 1) * (sh_degree + 1) > max_coeffs)
+        return gs_set_error(GS_ERR_INVALID, "gs_trainer_create: bad SH layout (degree %d, %d coefficients)", sh_degree,
+                            max_coeffs);
+    gs_trainer* t = new gs_trainer();
+    t->ctx = ctx;
+    t->n = n;
+    t->sh_degree = sh_degree;
+    t->max_coeffs = max_coeffs;
+    for (int k = 0; k < 5; k++) {
+        size_t b = t->tensor_elems(k) * 4;
+        int st;
+        if ((st = t->param[k].ensure(b)) || (st = t->grad[k].ensure(b)) || (st = t->adam_m[k].ensure(b)) ||
+            (st = t->adam_v[k].ensure(b))) {
+            gs_trainer_destroy(t);
+            return st;
+        }
+        gs_zero(ctx, t->adam_m[k].p, b);
+        gs_zero(ctx, t->adam_v[k].p, b);
+        gs_zero(ctx, t->grad[k].p, b);
+    }
+    *out = t;
+    return GS_OK;
+}
+
+gs_trainer::~gs_trainer() {
+    for (int k = 0; k < 5; k++) {
+        param[k].release();
+        grad[k].release();
+        adam_m[k].release();
+        adam_v[k].release();
+    }
+    for (auto& v : views) v.release_all();
+    keys.release();
+    vals.release();
+    keys_sorted.release();
+    vals_sorted.release();
+    for (auto& g : graphs) g.release();
+}
+
+void gs::ViewBufs::release_all() {
+    DevBuf* all[] = {&means2D, &depths, &conics, &op2d, &rgb, &radii, &tiles, &offsets, &ranges, &image, &d_image,
+                     &T_final, &n_contrib, &last, &loss, &rg_means2D, &rg_conics, &rg_rgb, &rg_op};
+    for (DevBuf* b : all) b->release();
+}
+
+extern "C" int gs_trainer_destroy(gs_trainer* t) {
+    if (!t) return GS_OK;
+    if (t->ctx) (void)hipStreamSynchronize(t->ctx->stream);
+    delete t;
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_set_params(gs_trainer* t, const float* pos, const float* ls, const float* rot,
+                                     const float* sh, const float* op, int from_host) {
+    if (!t) return gs_set_error(GS_ERR_INVALID, "null trainer");
+    const float* src[5] = {pos, ls, rot, sh, op};
+    for (int k = 0; k < 5; k++) {
+        if (!src[k]) return gs_set_error(GS_ERR_INVALID, "gs_trainer_set_params: null array %d", k);
+        hipError_t e = hipMemcpyAsync(t->param[k].p, src[k], t->tensor_elems(k) * 4,
+                                      from_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, t->ctx->stream);
+        if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(params)");
+    }
+    hipError_t e = hipStreamSynchronize(t->ctx->stream);
+    if (e != hipSuccess) return gs_hip_status(e, "hipStreamSynchronize");
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_get_params(gs_trainer* t, float* pos, float* ls, float* rot, float* sh, float* op,
+                                     int to_host) {
+    if (!t) return gs_set_error(GS_ERR_INVALID, "null trainer");
+    float* dst[5] = {pos, ls, rot, sh, op};
+    for (int k = 0; k < 5; k++) {
+        if (!dst[k]) continue;
+        hipError_t e = hipMemcpyAsync(dst[k], t->param[k].p, t->tensor_elems(k) * 4,
+                                      to_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, t->ctx->stream);
+        if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(params)");
+    }
+    hipError_t e = hipStreamSynchronize(t->ctx->stream);
+    if (e != hipSuccess) return gs_hip_status(e, "hipStreamSynchronize");
+    return GS_OK;
+}
+
+extern "C" void* gs_trainer_buffer(gs_trainer* t, const char* which) {
+    if (!t || !which) return nullptr;
+    std::string w(which);
+    const char* pn[5] = {"pos", "scale", "rot", "sh", "op"};
+    for (int k = 0; k < 5; k++) {
+        if (w == std::string("param_") + pn[k]) return t->param[k].p;
+        if (w == std::string("grad_") + pn[k]) return t->grad[k].p;
+        if (w == std::string("adam_m_") + pn[k]) return t->adam_m[k].p;
+        if (w == std::string("adam_v_") + pn[k]) return t->adam_v[k].p;
+    }
+    if (t->views.empty()) return nullptr;
+    gs::ViewBufs& v = t->views[0];
+    if (w == "image") return v.image.p;
+    if (w == "d_image") return v.d_image.p;
+    if (w == "T_final") return v.T_final.p;
+    if (w == "n_contrib") return v.n_contrib.p;
+    if (w == "last_contrib") return v.last.p;
+    if (w == "means2D") return v.means2D.p;
+    if (w == "depths") return v.depths.p;
+    if (w == "conics") return v.conics.p;
+    if (w == "opacities_2d") return v.op2d.p;
+    if (w == "rgb") return v.rgb.p;
+    if (w == "radii") return v.radii.p;
+    if (w == "tiles_touched") return v.tiles.p;
+    if (w == "ranges") return v.ranges.p;
+    if (w == "values_sorted") return t->vals_sorted.p;
+    if (w == "keys_sorted") return t->keys_sorted.p;
+    if (w == "loss") return v.loss.p;
+    return nullptr;
+}
+
+// Forward of one view into view buffers; returns the pair count via *pairs.
+static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs_render_cfg* cfg,
+                      long long* pairs) {
+    gs_ctx* c = t->ctx;
+    TR_TRY(tr_alloc_view(t, v, cam->width, cam->height));
+    gs_gaussians g = t->gaussians();
+    gs_preprocessed pre = v.pre();
+    TR_TRY(gs_preprocess_fwd(c, &g, cam, cfg, &pre));
+    long long K = 0;
+    TR_TRY(gs_scan_tiles(c, t->n, pre.tiles_touched, v.offsets.u(), nullptr, &K));
+    TR_TRY(t->keys.ensure((size_t)K * 8 + 8));
+    TR_TRY(t->vals.ensure((size_t)K * 4 + 4));
+    TR_TRY(t->keys_sorted.ensure((size_t)K * 8 + 8));
+    TR_TRY(t->vals_sorted.ensure((size_t)K * 4 + 4));
+    TR_TRY(gs_duplicate_keys(c, t->n, pre.means2D, pre.depths, pre.radii, v.offsets.u(), cam->width, cam->height, 16,
+                             16, static_cast<uint64_t*>(t->keys.p), t->vals.u(), (uint32_t)K));
+    int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
+    TR_TRY(gs_sort_pairs(c, (uint32_t)K, static_cast<uint64_t*>(t->keys.p), t->vals.u(),
+                         static_cast<uint64_t*>(t->keys_sorted.p), t->vals_sorted.u(), tiles, v.ranges.u()));
+    gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
+    TR_TRY(gs_raster_fwd(c, cam->width, cam->height, &pre, t->vals_sorted.u(), v.ranges.u(), cfg->bg_color,
+                         v.image.f(), &px));
+    *pairs = K;
+    return GS_OK;
+}
+
+static int tr_optimizer(gs_trainer* t, const gs_train_cfg* tc, int iteration) {
+    gs_ctx* c = t->ctx;
+    float lr_pos = gs_lr_schedule(tc->lr_position_init, tc->lr_position_final, iteration, tc->max_steps);
+    const float lrs[5] = {lr_pos, tc->lr_scaling, tc->lr_rotation, tc->lr_feature, tc->lr_opacity};
+    float* P[5];
+    const float* G[5];
+    float* M[5];
+    float* V[5];
+    for (int k = 0; k < 5; k++) {
+        P[k] = t->param[k].f();
+        G[k] = t->grad[k].f();
+        M[k] = t->adam_m[k].f();
+        V[k] = t->adam_v[k].f();
+    }
+    if (tc->use_adam) {
+        t->adam_steps++;  // AdamOptimizer::step_count_ (optimizer.cpp:72)
+        gs_adam_hparams hp;
+        for (int k = 0; k < 5; k++) hp.lr[k] = lrs[k];
+        hp.beta1 = tc->adam_beta1;
+        hp.beta2 = tc->adam_beta2;
+        hp.eps = tc->adam_eps;
+        gs_adam_bias_corrections(hp.beta1, hp.beta2, t->adam_steps, &hp.bc1, &hp.bc2);
+        return gs_adam_step(c, t->n, t->max_coeffs, P, G, M, V, &hp);
+    }
+    return gs_sgd_step(c, t->n, t->max_coeffs, P, G, lrs);
+}
+
+extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                               const gs_train_cfg* tc, int iteration, gs_step_result* out) {
+    if (!t || !cam || !cfg || !gt || !tc) return gs_set_error(GS_ERR_INVALID, "gs_trainer_step: null argument");
+    TR_TRY(check_cam_cfg(cam, cfg));
+    gs_ctx* c = t->ctx;
+    if (t->views.empty()) t->views.resize(1);
+    gs::ViewBufs& v = t->views[0];
+    long long K = 0;
+    TR_TRY(tr_forward(t, v, cam, cfg, &K));
+    TR_TRY(gs_l1_loss(c, cam->width, cam->height, v.image.f(), gt, v.d_image.f(), static_cast<double*>(v.loss.p)));
+    size_t n = (size_t)t->n;
+    TR_TRY(gs_zero(c, v.rg_means2D.p, n * 8));
+    TR_TRY(gs_zero(c, v.rg_conics.p, n * 12));
+    TR_TRY(gs_zero(c, v.rg_rgb.p, n * 12));
+    TR_TRY(gs_zero(c, v.rg_op.p, n * 4));
+    gs_preprocessed pre = v.pre();
+    gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
+    gs_raster_grads rg = v.rgrads();
+    TR_TRY(gs_raster_bwd(c, cam->width, cam->height, &pre, t->vals_sorted.u(), v.ranges.u(), cfg->bg_color, &px,
+                         v.d_image.f(), &rg));
+    gs_gaussians g = t->gaussians();
+    gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
+    TR_TRY(gs_preprocess_bwd(c, &g, cam, cfg, pre.radii, &rg, &pg, 0));
+    TR_TRY(tr_optimizer(t, tc, iteration));
+    double loss = 0.0;
+    hipError_t e = hipMemcpyAsync(&loss, v.loss.p, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(loss)");
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return gs_hip_status(e, "hipStreamSynchronize");
+    if (out) {
+        out->loss = loss;
+        out->iteration = iteration;
+        out->n_gaussians = t->n;
+        out->pairs = K;
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_render(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, long long* pairs) {
+    if (!t || !cam || !cfg) return gs_set_error(GS_ERR_INVALID, "gs_trainer_render: null argument");
+    TR_TRY(check_cam_cfg(cam, cfg));
+    if (t->views.empty()) t->views.resize(1);
+    long long K = 0;
+    TR_TRY(tr_forward(t, t->views[0], cam, cfg, &K));
+    if (pairs) *pairs = K;
+    return GS_OK;
+}

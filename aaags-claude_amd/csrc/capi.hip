@@ -433,6 +433,22 @@ extern "C" int gs_adam_step(gs_ctx* c, int n, int max_coeffs, float* const param
     return GS_OK;
 }
 
+extern "C" int gs_sgd_step(gs_ctx* c, int n, int max_coeffs, float* const params[5], const float* const grads[5],
+                           const float lr[5]) {
+    GS_CHECK_ARG(c != nullptr && lr != nullptr, "null argument");
+    GS_CHECK_ARG(n >= 0, "negative count");
+    if (n == 0) return GS_OK;
+    const uint32_t sizes[5] = {(uint32_t)n * 3u, (uint32_t)n * 3u, (uint32_t)n * 4u,
+                               (uint32_t)n * (uint32_t)max_coeffs * 3u, (uint32_t)n};
+    gso::AdamArgs a{};
+    for (int k = 0; k < 5; k++) {
+        GS_CHECK_ARG(params[k] && grads[k], "null SGD array (tensor %d)", k);
+        a.seg[k] = gso::AdamSeg{params[k], grads[k], nullptr, nullptr, sizes[k], lr[k]};
+    }
+    GS_HIP(launch_sgd(a, c->stream), "sgd kernel");
+    return GS_OK;
+}
+
 // train_types.cpp:64-68 and optimizer.cpp:73-76, host float arithmetic.
 extern "C" float gs_lr_schedule(float lr_init, float lr_final, int step, int max_steps) {
     float t = static_cast<float>(step) / static_cast<float>(max_steps);

@@ -69,4 +69,5 @@ int l1_partials_needed(uint32_t n);
 hipError_t launch_l1(const float* r, const float* g, float* d, uint32_t n, double* partial, double* loss,
                      hipStream_t s);
 hipError_t launch_adam(const gso::AdamArgs& a, hipStream_t s);
+hipError_t launch_sgd(const gso::AdamArgs& a, hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);

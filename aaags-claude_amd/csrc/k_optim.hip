@@ -99,6 +99,13 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
     }
 }
 
+// optimizer.cpp:9-26: p -= lr * g, per tensor.
+__global__ __launch_bounds__(256) void sgd_kernel(AdamArgs a) {
+    const AdamSeg& s = a.seg[blockIdx.y];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.n; i += gridDim.x * blockDim.x)
+        s.p[i] -= s.lr * s.g[i];
+}
+
 __global__ __launch_bounds__(256) void zero_kernel(uint4* __restrict__ p16, size_t n16, unsigned char* __restrict__ tail,
                                                    size_t ntail) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
@@ -135,6 +142,16 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     adam_kernel<<<dim3(blocks, 5), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sgd(const AdamArgs& a, hipStream_t s) {
+    uint32_t mx = 0;
+    for (int k = 0; k < 5; k++) mx = a.seg[k].n > mx ? a.seg[k].n : mx;
+    uint32_t blocks = (mx + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    sgd_kernel<<<dim3(blocks, 5), 256, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -305,3 +305,86 @@ class GsHip:
         grads = self.preprocess_bwd(p, cam, cfg, f["pre"]["radii"], rg)
         f.update(loss=loss, d_image=d_img, rgrad=rg, grads=grads)
         return f
+
+
+class _TrainCfg(ctypes.Structure):
+    _fields_ = [("lr_position_init", ctypes.c_float), ("lr_position_final", ctypes.c_float),
+                ("lr_feature", ctypes.c_float), ("lr_opacity", ctypes.c_float), ("lr_scaling", ctypes.c_float),
+                ("lr_rotation", ctypes.c_float), ("max_steps", _i32), ("use_adam", _i32),
+                ("adam_beta1", ctypes.c_float), ("adam_beta2", ctypes.c_float), ("adam_eps", ctypes.c_float)]
+
+
+class _StepResult(ctypes.Structure):
+    _fields_ = [("loss", ctypes.c_double), ("iteration", _i32), ("n_gaussians", _i32), ("pairs", ctypes.c_longlong)]
+
+
+def train_cfg(lr=None, use_adam=True, max_steps=30000, beta1=0.9, beta2=0.999, eps=1e-8, **lrs) -> _TrainCfg:
+    """TrainConfig (train_types.h:46-60).  ``lr`` sets every learning rate
+    (position schedule constant); keyword overrides set individual ones."""
+    d = dict(lr_position_init=0.00016, lr_position_final=0.0000016, lr_feature=0.0025, lr_opacity=0.025,
+             lr_scaling=0.005, lr_rotation=0.001)
+    if lr is not None:
+        d = {k: lr for k in d}
+    d.update(lrs)
+    return _TrainCfg(d["lr_position_init"], d["lr_position_final"], d["lr_feature"], d["lr_opacity"],
+                     d["lr_scaling"], d["lr_rotation"], max_steps, int(use_adam), beta1, beta2, eps)
+
+
+class GsTrainer:
+    """The native training driver (gs_trainer_*): parameters, gradients and
+    optimizer state stay in HBM; each step is one call."""
+
+    def __init__(self, hip: GsHip, scene):
+        self.hip = hip
+        self.lib = hip.lib
+        self.n, self.k = scene.n, scene.max_coeffs
+        self.tr = _vp()
+        hip._check(self.lib.gs_trainer_create(hip.ctx, scene.n, scene.sh_degree, scene.max_coeffs,
+                                              ctypes.byref(self.tr)), "gs_trainer_create")
+        self.lib.gs_trainer_buffer.restype = _vp
+        arrs = [np.ascontiguousarray(a, np.float32) for a in scene.arrays()]
+        hip._check(self.lib.gs_trainer_set_params(self.tr, *[a.ctypes.data_as(_vp) for a in arrs], 1),
+                   "gs_trainer_set_params")
+
+    def __del__(self):
+        try:
+            if self.tr:
+                self.lib.gs_trainer_destroy(self.tr)
+        except Exception:
+            pass
+
+    def step(self, cam, cfg, gt, tc, iteration):
+        res = _StepResult()
+        c, f = cam.to_c(), cfg.to_c()
+        self.hip._check(self.lib.gs_trainer_step(self.tr, ctypes.byref(c), ctypes.byref(f), _vp(ptr(gt)),
+                                                 ctypes.byref(tc), iteration, ctypes.byref(res)), "gs_trainer_step")
+        return res
+
+    def render(self, cam, cfg):
+        k = ctypes.c_longlong()
+        c, f = cam.to_c(), cfg.to_c()
+        self.hip._check(self.lib.gs_trainer_render(self.tr, ctypes.byref(c), ctypes.byref(f), ctypes.byref(k)),
+                        "gs_trainer_render")
+        return int(k.value)
+
+    def buffer(self, name, shape, dtype=torch.float32):
+        """Wraps a trainer-owned device buffer as a torch tensor (a copy)."""
+        p = self.lib.gs_trainer_buffer(self.tr, name.encode())
+        if not p:
+            raise GsError(f"no trainer buffer {name}")
+        numel = int(np.prod(shape))
+        out = torch.empty(numel, dtype=dtype, device=self.hip.dev)
+        torch.cuda.synchronize(self.hip.dev)
+        hipMemcpy = ctypes.CDLL("libamdhip64.so").hipMemcpy
+        hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, _i32]
+        st = hipMemcpy(_vp(out.data_ptr()), _vp(p), numel * out.element_size(), 3)  # device to device
+        if st != 0:
+            raise GsError(f"hipMemcpy failed ({st})")
+        return out.view(*shape)
+
+    def params(self):
+        out = [np.empty((self.n, 3), np.float32), np.empty((self.n, 3), np.float32), np.empty((self.n, 4), np.float32),
+               np.empty((self.n, self.k, 3), np.float32), np.empty(self.n, np.float32)]
+        self.hip._check(self.lib.gs_trainer_get_params(self.tr, *[a.ctypes.data_as(_vp) for a in out], 1),
+                        "gs_trainer_get_params")
+        return out

@@ -1,0 +1,212 @@
+"""Benchmark of the 3DGS render-and-train hot path on one MI355X.
+
+Workload: 1M Gaussians (SH degree 3; means U[-1,1]^3, log-scales N(-4,0.5),
+opacity logits N(0,1)), one 1920x1080 pinhole camera (fov 60 deg, distance 3),
+L1 against a uniform random target, Adam (lr 1e-3, betas 0.9/0.999).  A step
+is preprocess -> bin -> sort -> raster -> L1 -> raster backward -> preprocess
+backward -> Adam, run by the native driver (gs_trainer_step) with inputs
+resident in HBM.
+
+Prints one JSON line: train steps/s (value), forward Mpix/s, a per-stage
+breakdown, the roofline of the dominant kernel, and the reference CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "aaags-claude_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_breakdown(hip, p, cam, cfg, gt, reps):
+    """Per-stage device time (ms) of the single-view step, HIP events on the
+    library's stream (torch's current stream)."""
+    import torch
+
+    names = ["preprocess", "scan", "duplicate", "sort", "raster_fwd", "l1", "zero_grads", "raster_bwd",
+             "preprocess_bwd", "adam"]
+    acc = {k: 0.0 for k in names}
+    n, kc = p["n"], p["max_coeffs"]
+    m = hip.alloc_pgrads(n, kc)
+    v = hip.alloc_pgrads(n, kc)
+    K = 0
+    for r in range(reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        ev[0].record()
+        pre = hip.preprocess(p, cam, cfg)
+        ev[1].record()
+        offsets, K = hip.scan_tiles(pre["tiles"])  # includes the pair-count readback
+        ev[2].record()
+        keys, vals = hip.duplicate(pre, offsets, K, cam.width, cam.height)
+        ev[3].record()
+        ks, vs, ranges = hip.sort_pairs(keys, vals, cam.num_tiles)
+        ev[4].record()
+        img, px = hip.raster_fwd(pre, vs, ranges, cam.width, cam.height, cfg.bg_color)
+        ev[5].record()
+        loss, d_img = hip.l1_loss(img, gt)
+        ev[6].record()
+        rg = hip.alloc_rgrads(n)
+        ev[7].record()
+        rg = hip.raster_bwd(pre, vs, ranges, cam.width, cam.height, px, d_img, cfg.bg_color, rg=rg)
+        ev[8].record()
+        grads = hip.preprocess_bwd(p, cam, cfg, pre["radii"], rg)
+        ev[9].record()
+        hip.adam(p, grads, m, v, [1e-3] * 5, 0.9, 0.999, 1e-8, r + 1)
+        ev[10].record()
+        torch.cuda.synchronize()
+        for i, k in enumerate(names):
+            acc[k] += ev[i].elapsed_time(ev[i + 1])
+        del keys, vals, ks, vs, ranges, img, px, d_img, rg, grads, pre
+    return {k: acc[k] / reps for k in names}, K
+
+
+def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles):
+    """Bytes each stage must move at minimum (per launch), from the per-unit
+    figures of the hot-path analysis: preprocess 236 B/G read (SH3) + 48 B/G
+    written; sort K*(24*P8 + 8), P8 = ceil((ceil(log2 tiles) + 32) / 8);
+    raster forward 40 B/pair + 20 B/px; raster backward 40 B/pair + 20 B/px +
+    72 B/G; preprocess backward ~580 B/G; Adam 28 B per parameter element."""
+    import math
+    params_per_g = 3 + 3 + 4 + max_coeffs * 3 + 1
+    p8 = math.ceil((math.ceil(math.log2(max(num_tiles, 2))) + 32) / 8)
+    return {
+        "preprocess": N * (4 * params_per_g + 48),
+        "scan": N * 8,
+        "duplicate": N * 20 + K * 12,
+        "sort": K * (24 * p8 + 8),
+        "raster_fwd": K * 40 + npix * 20,
+        "l1": npix * 36,
+        "raster_bwd": K * 40 + npix * 20 + N * 72,
+        "preprocess_bwd": N * 580,
+        "adam": N * params_per_g * 28,
+    }.get(stage)
+
+
+def cpu_baseline(scene, cam, cfg, gt):
+    """The reference's own Trainer::step (compiled from source into
+    oracle/_ref) on the same workload, one full step on one core; falls back
+    to the oracle (C restatement) if that library is absent."""
+    from oracle.oracle import Oracle, RefLib
+    tc = dict(lr_pos_init=1e-3, lr_pos_final=1e-3, lr_feature=1e-3, lr_opacity=1e-3, lr_scaling=1e-3,
+              lr_rotation=1e-3, max_steps=30000, beta1=0.9, beta2=0.999, eps=1e-8)
+    t0 = time.perf_counter()
+    if RefLib.available():
+        RefLib().train_steps(scene, cam, cfg, gt, tc, 0, 1)
+        kind = "reference"
+        sample = "one full Trainer::step (reference CPU classes, Adam) at 1M Gaussians, 1920x1080"
+    else:
+        Oracle().forward_backward(scene, cam, cfg, gt)
+        kind = "port"
+        sample = "one forward+backward of the C oracle at 1M Gaussians, 1920x1080 (no optimizer)"
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 5), "unit": "train steps/s", "cores": 1, "kind": kind, "sample": sample,
+            "seconds": round(dt, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c3_1m_1080p")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown-reps", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is design-only",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gs_hip import GsHip, GsTrainer, train_cfg
+    from gs_scenes import RenderCfg, make_workload
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    t0 = time.perf_counter()
+    scene, cams, gts = make_workload(args.workload)
+    cam, gt_np = cams[0], gts[0]
+    log(f"workload {args.workload}: N={scene.n} {cam.width}x{cam.height} generated in {time.perf_counter() - t0:.1f}s")
+    cfg = RenderCfg(sh_degree=3, training=True)
+    tc = train_cfg(lr=1e-3)
+    hip = GsHip(0)
+    gt = torch.tensor(gt_np, device=hip.dev)
+    tr = GsTrainer(hip, scene)
+
+    for i in range(args.warmup):
+        r = tr.step(cam, cfg, gt, tc, i)
+    torch.cuda.synchronize()
+    log(f"warmup done: loss {r.loss:.5f} pairs {r.pairs}")
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = tr.step(cam, cfg, gt, tc, args.warmup + i)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    steps_s = args.steps / dt
+    log(f"train: {steps_s:.2f} steps/s ({1e3 * dt / args.steps:.3f} ms/step), last loss {r.loss:.5f}")
+
+    for i in range(2):
+        tr.render(cam, cfg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        K = tr.render(cam, cfg)
+    torch.cuda.synchronize()
+    dtf = time.perf_counter() - t0
+    mpix_s = cam.width * cam.height * args.steps / dtf / 1e6
+    log(f"forward: {mpix_s:.1f} Mpix/s ({1e3 * dtf / args.steps:.3f} ms/frame), pairs {K}")
+
+    p = hip.upload_scene(scene)
+    stages, K = stage_breakdown(hip, p, cam, cfg, gt, args.breakdown_reps)
+    log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
+    npix = cam.width * cam.height
+    dom = max((k for k in stages if k != "scan"), key=lambda k: stages[k])
+    byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
+    achieved = byts / (stages[dom] * 1e-3) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": int(byts), "ms": round(stages[dom], 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, cam, cfg, gt_np)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "train steps/s + fwd Mpix/s, 1M Gaussians 1080p; batched vs single-view",
+        "value": round(steps_s, 3),
+        "unit": "train steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox scene and target image)",
+        "config": {"workload": args.workload, "gaussians": scene.n, "sh_degree": scene.sh_degree,
+                   "width": cam.width, "height": cam.height, "pairs": int(K), "optimizer": "adam lr 1e-3"},
+        "fwd_mpix_s": round(mpix_s, 2),
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

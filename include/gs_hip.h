@@ -220,10 +220,63 @@ int gs_adam_step(gs_ctx* ctx, int n, int max_coeffs, float* const params[5],
                  const float* const grads[5], float* const m[5], float* const v[5],
                  const gs_adam_hparams* hp);
 
+/* SGDOptimizer::step (optimizer.cpp:9-26): p -= lr[k] * g over the five
+ * tensors (used by TrainerGPU, trainer_gpu.h:44). */
+int gs_sgd_step(gs_ctx* ctx, int n, int max_coeffs, float* const params[5], const float* const grads[5],
+                const float lr[5]);
+
 /* Host helpers with the reference's host arithmetic: lr_schedule
  * (train_types.cpp:64-68) and Adam bias corrections (optimizer.cpp:73-76). */
 float gs_lr_schedule(float lr_init, float lr_final, int step, int max_steps);
 void gs_adam_bias_corrections(float beta1, float beta2, int step, float* bc1, float* bc2);
+
+
+/* ------------------------------------------------------------------------
+ * Native training driver: the whole step (preprocess -> bin -> sort -> raster
+ * -> L1 -> raster backward -> preprocess backward -> optimizer) on one stream,
+ * with parameters, gradients and optimizer state resident in HBM.
+ * ---------------------------------------------------------------------- */
+
+/* TrainConfig (train_types.h:46-60). */
+typedef struct gs_train_cfg {
+    float lr_position_init, lr_position_final;
+    float lr_feature, lr_opacity, lr_scaling, lr_rotation;
+    int max_steps;
+    int use_adam;
+    float adam_beta1, adam_beta2, adam_eps;
+} gs_train_cfg;
+
+/* Trainer::StepResult (trainer.h:18) plus the pair count of the step. */
+typedef struct gs_step_result {
+    double loss;     /* mean over views for a batched step */
+    int iteration;
+    int n_gaussians;
+    long long pairs; /* tile-Gaussian pairs (summed over views) */
+} gs_step_result;
+
+typedef struct gs_trainer gs_trainer;
+
+int gs_trainer_create(gs_ctx* ctx, int n, int sh_degree, int max_coeffs, gs_trainer** out);
+int gs_trainer_destroy(gs_trainer* tr);
+/* Raw parameters in (from_host != 0: host arrays) / out (to_host != 0). */
+int gs_trainer_set_params(gs_trainer* tr, const float* pos, const float* log_scales, const float* rot,
+                          const float* sh, const float* opacity_logits, int from_host);
+int gs_trainer_get_params(gs_trainer* tr, float* pos, float* log_scales, float* rot, float* sh,
+                          float* opacity_logits, int to_host);
+/* Device pointers of the trainer's resident buffers: which = "image",
+ * "grad_pos", "grad_scale", "grad_rot", "grad_sh", "grad_op", "means2D",
+ * "radii", ... (NULL if unknown). */
+void* gs_trainer_buffer(gs_trainer* tr, const char* which);
+
+/* Trainer::step (trainer.h:21-23) / TrainerGPU::step (trainer_gpu.h:27-29):
+ * one single-view step.  The pair count is read back to size the pair
+ * buffers, as the reference GPU stages do; gt is a device HWC image. */
+int gs_trainer_step(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                    const gs_train_cfg* tc, int iteration, gs_step_result* out);
+
+/* Renderer::render (renderer.h:17-21) on the trainer's parameters: forward
+ * only, image into the trainer's "image" buffer. */
+int gs_trainer_render(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, long long* pairs);
 
 #ifdef __cplusplus
 }

@@ -53,10 +53,12 @@ def test_activation_matches_oracle(hip, oracle):
     ref = oracle.activate(scene)
     for k in ["pos", "rot", "sh"]:
         assert np.array_equal(_np(a[k]), ref[k].reshape(_np(a[k]).shape)), k
-    for k in ["scale", "op"]:  # device expf vs glibc expf: at most 1 ulp
+    # device expf vs glibc expf differ by at most 1 ulp; the sigmoid
+    # 1/(1+exp(-x)) carries that through one more rounding (<= 2 ulp)
+    for k, bound in [("scale", 1), ("op", 2)]:
         got, want = _np(a[k]).ravel(), ref[k].ravel()
         ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
-        assert ulps.max() <= 1, (k, ulps.max())
+        assert ulps.max() <= bound, (k, ulps.max())
 
 
 @pytest.mark.parametrize("n,deg,seed", SMALL_CASES[1:] + [(10000, 0, 0)])
@@ -184,7 +186,8 @@ def test_l1_loss_fixtures(hip):
         d = np.fromfile(os.path.join(base, case["name"] + "_d_image.bin"), np.float32).reshape(H, W, 3)
         loss, d_img = hip.l1_loss(torch.tensor(r, device=hip.dev), torch.tensor(g, device=hip.dev))
         assert np.array_equal(_np(d_img), d)
-        exact = np.abs(r.astype(np.float64) - g.astype(np.float64)).sum() / r.size
+        # loss.cpp:18-30: float difference, |.| accumulated in double
+        exact = np.abs((r - g).astype(np.float64)).sum() / r.size
         assert abs(float(loss.item()) - exact) <= 1e-12 + 1e-9 * exact
         assert abs(float(loss.item()) - case["loss_serial"]) <= 1e-6
 
@@ -251,6 +254,13 @@ def test_adam_bitexact(hip, oracle):
         assert np.array_equal(_np(vt[k]), v[k]), k
 
 
+# verify_forward is a forward-render fixture: its target image is the rendered
+# image itself (max |rendered - target| = 2.4e-7, 653 of 768 values equal), so
+# sign(rendered - target) in the L1 gradient is decided by last-bit rounding
+# and its gradients say nothing about the path.  Only its image is compared.
+GRADIENT_UNDEFINED = {"verify_forward"}
+
+
 @pytest.mark.parametrize("name", scene_dirs())
 def test_golden_scenes_full_path(hip, name):
     """The whole GPU path against the reference's own golden vectors
@@ -262,6 +272,8 @@ def test_golden_scenes_full_path(hip, name):
     assert_image_close(_np(f["img"]), exp["img"])
     if "loss" in exp:
         assert abs(float(f["loss"].item()) - exp["loss"]) <= 1e-6
+    if name in GRADIENT_UNDEFINED:
+        return
     for k in ["pos", "scale", "rot", "sh", "op"]:
         if "grad_" + k in exp:
             assert_grad_close(_np(f["grads"][k]), exp["grad_" + k], "grad_" + k)
