@@ -224,13 +224,14 @@ extern "C" void* gs_trainer_buffer(gs_trainer* t, const char* which) {
     if (w == "radii") return v.radii.p;
     if (w == "tiles_touched") return v.tiles.p;
     if (w == "ranges") return v.ranges.p;
-    if (w == "values_sorted") return t->vals_sorted.p;
-    if (w == "keys_sorted") return t->keys_sorted.p;
+    if (w == "pair_words") return t->keys_sorted.p;
     if (w == "loss") return v.loss.p;
     return nullptr;
 }
 
 // Forward of one view into view buffers; returns the pair count via *pairs.
+// Binning is the depth-ordered path (gs_bin_sorted_words); the pair buffer
+// grows when a view needs more pairs than it holds.
 static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs_render_cfg* cfg,
                       long long* pairs) {
     gs_ctx* c = t->ctx;
@@ -239,19 +240,19 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
     gs_preprocessed pre = v.pre();
     TR_TRY(gs_preprocess_fwd(c, &g, cam, cfg, &pre));
     long long K = 0;
-    TR_TRY(gs_scan_tiles(c, t->n, pre.tiles_touched, v.offsets.u(), nullptr, &K));
-    TR_TRY(t->keys.ensure((size_t)K * 8 + 8));
-    TR_TRY(t->vals.ensure((size_t)K * 4 + 4));
-    TR_TRY(t->keys_sorted.ensure((size_t)K * 8 + 8));
-    TR_TRY(t->vals_sorted.ensure((size_t)K * 4 + 4));
-    TR_TRY(gs_duplicate_keys(c, t->n, pre.means2D, pre.depths, pre.radii, v.offsets.u(), cam->width, cam->height, 16,
-                             16, static_cast<uint64_t*>(t->keys.p), t->vals.u(), (uint32_t)K));
-    int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
-    TR_TRY(gs_sort_pairs(c, (uint32_t)K, static_cast<uint64_t*>(t->keys.p), t->vals.u(),
-                         static_cast<uint64_t*>(t->keys_sorted.p), t->vals_sorted.u(), tiles, v.ranges.u()));
+    uint32_t cap = (uint32_t)(t->keys_sorted.bytes / 8);
+    int st = gs_bin_sorted_words(c, t->n, &pre, cam->width, cam->height, static_cast<uint64_t*>(t->keys_sorted.p), cap,
+                                 v.ranges.u(), &K);
+    if (st == GS_ERR_CAPACITY && K > (long long)cap) {
+        TR_TRY(t->keys_sorted.ensure((size_t)(K + K / 8 + 1024) * 8));
+        cap = (uint32_t)(t->keys_sorted.bytes / 8);
+        st = gs_bin_sorted_words(c, t->n, &pre, cam->width, cam->height, static_cast<uint64_t*>(t->keys_sorted.p), cap,
+                                 v.ranges.u(), &K);
+    }
+    TR_TRY(st);
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
-    TR_TRY(gs_raster_fwd(c, cam->width, cam->height, &pre, t->vals_sorted.u(), v.ranges.u(), cfg->bg_color,
-                         v.image.f(), &px));
+    TR_TRY(gs_raster_fwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
+                               v.ranges.u(), cfg->bg_color, v.image.f(), &px));
     *pairs = K;
     return GS_OK;
 }
@@ -301,8 +302,8 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
     gs_preprocessed pre = v.pre();
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
     gs_raster_grads rg = v.rgrads();
-    TR_TRY(gs_raster_bwd(c, cam->width, cam->height, &pre, t->vals_sorted.u(), v.ranges.u(), cfg->bg_color, &px,
-                         v.d_image.f(), &rg));
+    TR_TRY(gs_raster_bwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
+                               v.ranges.u(), cfg->bg_color, &px, v.d_image.f(), &rg));
     gs_gaussians g = t->gaussians();
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
     TR_TRY(gs_preprocess_bwd(c, &g, cam, cfg, pre.radii, &rg, &pg, 0));

@@ -59,6 +59,7 @@ extern "C" int gs_ctx_destroy(gs_ctx* c) {
     if (!c) return GS_OK;
     (void)hipSetDevice(c->device);
     if (c->ws) (void)hipFree(c->ws);
+    if (c->ws2) (void)hipFree(c->ws2);
     if (c->small) (void)hipFree(c->small);
     delete c;
     return GS_OK;
@@ -95,27 +96,30 @@ extern "C" int gs_ctx_set_tuning(gs_ctx* c, int radix_bits, int raster_batch, in
     return GS_OK;
 }
 
-// Grows the workspace to at least `bytes`.  Not for use inside graph capture:
-// callers that capture reserve the workspace beforehand (gs_ctx_reserve).
-int gs_ws_get(gs_ctx* c, size_t bytes, void** out) {
-    if (bytes > c->ws_bytes) {
+// Grows a workspace to at least `bytes`.  Not for use inside graph capture:
+// callers that capture reserve the workspaces beforehand (gs_ctx_reserve).
+static int grow(gs_ctx* c, void** buf, size_t* have, size_t bytes, void** out) {
+    if (bytes > *have) {
         if (c->capturing)
             return gs_set_error(GS_ERR_CAPACITY, "workspace of %zu bytes needed during capture (%zu reserved)", bytes,
-                                c->ws_bytes);
+                                *have);
         GS_HIP(hipSetDevice(c->device), "hipSetDevice");
-        if (c->ws) {
+        if (*buf) {
             GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-            GS_HIP(hipFree(c->ws), "hipFree(workspace)");
-            c->ws = nullptr;
-            c->ws_bytes = 0;
+            GS_HIP(hipFree(*buf), "hipFree(workspace)");
+            *buf = nullptr;
+            *have = 0;
         }
         size_t nb = bytes + bytes / 4 + (1 << 20);
-        GS_HIP(hipMalloc(&c->ws, nb), "hipMalloc(workspace)");
-        c->ws_bytes = nb;
+        GS_HIP(hipMalloc(buf, nb), "hipMalloc(workspace)");
+        *have = nb;
     }
-    *out = c->ws;
+    *out = *buf;
     return GS_OK;
 }
+
+int gs_ws_get(gs_ctx* c, size_t bytes, void** out) { return grow(c, &c->ws, &c->ws_bytes, bytes, out); }
+int gs_ws2_get(gs_ctx* c, size_t bytes, void** out) { return grow(c, &c->ws2, &c->ws2_bytes, bytes, out); }
 
 // Small fixed scratch: scan partials, overflow flags, loss partials.
 int gs_small_get(gs_ctx* c, void** out) {
@@ -324,8 +328,76 @@ extern "C" int gs_sort_pairs(gs_ctx* c, uint32_t n, const uint64_t* ki, const ui
     return gs_tile_ranges(c, n, ko, num_tiles, ranges);
 }
 
+// Depth-ordered binning: depth sort of the Gaussians, pairs emitted in depth
+// order as tile << 32 | index words, stable tile sort, tile ranges.
+extern "C" int gs_bin_sorted_words(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height,
+                                   uint64_t* words, uint32_t capacity, uint32_t* ranges, long long* total_host) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
+    GS_CHECK_ARG(n >= 0, "negative count");
+    GS_CHECK_ARG(ranges != nullptr, "null tile ranges");
+    int st = check_pre(pre, true);
+    if (st) return st;
+    const int gx = (width + 15) / 16, gy = (height + 15) / 16;
+    const int num_tiles = gx * gy;
+    GS_HIP(launch_zero(ranges, (size_t)num_tiles * 8, c->stream), "zero kernel");
+    if (n == 0) {
+        if (total_host) *total_host = 0;
+        return GS_OK;
+    }
+    void* sm;
+    if ((st = gs_small_get(c, &sm))) return st;
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
+    uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_OVERFLOW);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t un = (size_t)n;
+    // Gaussian-sized arrays: depth keys/indices (+ sorted), ranked counts/offsets, scan partials, sort scratch
+    const size_t np = scan_partials_needed((uint32_t)n) + 1;
+    void* ws2;
+    if ((st = gs_ws2_get(c, 2 * al(un * 8) + 4 * al(un * 4) + al(np * 8) + radix_workspace_bytes((uint32_t)n, 8),
+                         &ws2)))
+        return st;
+    char* w = static_cast<char*>(ws2);
+    uint64_t* dkeys = reinterpret_cast<uint64_t*>(w); w += al(un * 8);
+    uint64_t* dkeys_s = reinterpret_cast<uint64_t*>(w); w += al(un * 8);
+    uint32_t* didx = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
+    uint32_t* order = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
+    int* rtiles = reinterpret_cast<int*>(w); w += al(un * 4);
+    uint32_t* roff = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
+    unsigned long long* partial = reinterpret_cast<unsigned long long*>(w); w += al(np * 8);
+    void* dsort_ws = w;
+    GS_HIP(launch_depth_keys(n, pre->depths, pre->radii, dkeys, didx, c->stream), "depth keys kernel");
+    GS_HIP(launch_radix_sort_bits(dkeys, didx, dkeys_s, order, (uint32_t)n, 0, 32, 8, dsort_ws, c->stream, nullptr),
+           "depth sort");
+    GS_HIP(launch_gather_tiles(n, order, pre->tiles_touched, rtiles, c->stream), "gather kernel");
+    GS_HIP(launch_scan_tiles(rtiles, (uint32_t)n, roff, partial, nullptr, total64, c->stream), "scan kernels");
+    unsigned long long K = 0;
+    GS_HIP(hipMemcpyAsync(&K, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
+    GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (total_host) *total_host = (long long)K;
+    if (K > capacity)
+        return gs_set_error(GS_ERR_CAPACITY, "gs_bin_sorted_words: %llu pairs exceed capacity %u", K, capacity);
+    if (K == 0) return GS_OK;
+    GS_CHECK_ARG(words != nullptr, "null words buffer");
+    // pair-sized arrays: the unsorted words and the tile sort's scratch
+    const int tile_bits = gs_tile_key_bits(num_tiles) - 32;
+    const int rb = c->radix_bits;
+    void* ws;
+    if ((st = gs_ws_get(c, al((size_t)K * 8) + radix_workspace_bytes((uint32_t)K, rb), &ws))) return st;
+    uint64_t* unsorted = static_cast<uint64_t*>(ws);
+    void* tsort_ws = static_cast<char*>(ws) + al((size_t)K * 8);
+    GS_HIP(launch_duplicate_ranked(n, order, pre->means2D, pre->radii, roff, gx, gy, unsorted, (uint32_t)K, ovf,
+                                   c->stream),
+           "duplicate kernel");
+    GS_HIP(launch_radix_sort_bits(unsorted, nullptr, words, nullptr, (uint32_t)K, 32, 32 + tile_bits, rb, tsort_ws,
+                                  c->stream, nullptr),
+           "tile sort");
+    GS_HIP(launch_tile_ranges(words, (uint32_t)K, (uint32_t)num_tiles, ranges, c->stream), "tile ranges kernel");
+    return GS_OK;
+}
+
 static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed* pre, const uint32_t* values,
-                                   const uint32_t* ranges, const float bg[3]) {
+                                   const uint32_t* ranges, const float bg[3], int vstride = 1) {
     gsr::RasterArgs a;
     a.W = width;
     a.H = height;
@@ -339,22 +411,35 @@ static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed*
     a.bg0 = bg ? bg[0] : 0.0f;
     a.bg1 = bg ? bg[1] : 0.0f;
     a.bg2 = bg ? bg[2] : 0.0f;
+    a.vstride = vstride;
     return a;
 }
 
-extern "C" int gs_raster_fwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
-                             const uint32_t* ranges, const float bg[3], float* image, const gs_pixel_state* px) {
+static int raster_fwd_impl(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                           int vstride, const uint32_t* ranges, const float bg[3], float* image,
+                           const gs_pixel_state* px) {
     GS_CHECK_ARG(c != nullptr, "null context");
     GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
     int st = check_pre(pre, false);
     if (st) return st;
     GS_CHECK_ARG(ranges && image, "null ranges or image");
     int tiles = ((width + 15) / 16) * ((height + 15) / 16);
-    gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg);
+    gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg, vstride);
     GS_HIP(launch_raster_fwd(a, tiles, c->raster_batch, image, px ? px->T_final : nullptr,
                              px ? px->n_contrib : nullptr, px ? px->last_contrib : nullptr, c->stream),
            "raster forward kernel");
     return GS_OK;
+}
+
+extern "C" int gs_raster_fwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                             const uint32_t* ranges, const float bg[3], float* image, const gs_pixel_state* px) {
+    return raster_fwd_impl(c, width, height, pre, values, 1, ranges, bg, image, px);
+}
+
+extern "C" int gs_raster_fwd_words(gs_ctx* c, int width, int height, const gs_preprocessed* pre,
+                                   const uint64_t* words, const uint32_t* ranges, const float bg[3], float* image,
+                                   const gs_pixel_state* px) {
+    return raster_fwd_impl(c, width, height, pre, reinterpret_cast<const uint32_t*>(words), 2, ranges, bg, image, px);
 }
 
 extern "C" int gs_l1_loss(gs_ctx* c, int width, int height, const float* r, const float* g, float* d, double* loss) {
@@ -370,9 +455,9 @@ extern "C" int gs_l1_loss(gs_ctx* c, int width, int height, const float* r, cons
     return GS_OK;
 }
 
-extern "C" int gs_raster_bwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
-                             const uint32_t* ranges, const float bg[3], const gs_pixel_state* px, const float* d_image,
-                             const gs_raster_grads* g) {
+static int raster_bwd_impl(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                           int vstride, const uint32_t* ranges, const float bg[3], const gs_pixel_state* px,
+                           const float* d_image, const gs_raster_grads* g) {
     GS_CHECK_ARG(c != nullptr, "null context");
     GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
     int st = check_pre(pre, false);
@@ -381,11 +466,24 @@ extern "C" int gs_raster_bwd(gs_ctx* c, int width, int height, const gs_preproce
     GS_CHECK_ARG(g->d_means2D && g->d_conics && g->d_rgb && g->d_opacities_2d, "null gradient array");
     GS_CHECK_ARG(aligned(g->d_means2D, 8), "d_means2D must be 8-byte aligned");
     int tiles = ((width + 15) / 16) * ((height + 15) / 16);
-    gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg);
+    gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg, vstride);
     gsr::GradArgs ga{reinterpret_cast<float2*>(g->d_means2D), g->d_conics, g->d_rgb, g->d_opacities_2d};
     GS_HIP(launch_raster_bwd(a, tiles, c->raster_bwd_batch, px->T_final, px->last_contrib, d_image, ga, c->stream),
            "raster backward kernel");
     return GS_OK;
+}
+
+extern "C" int gs_raster_bwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                             const uint32_t* ranges, const float bg[3], const gs_pixel_state* px, const float* d_image,
+                             const gs_raster_grads* g) {
+    return raster_bwd_impl(c, width, height, pre, values, 1, ranges, bg, px, d_image, g);
+}
+
+extern "C" int gs_raster_bwd_words(gs_ctx* c, int width, int height, const gs_preprocessed* pre,
+                                   const uint64_t* words, const uint32_t* ranges, const float bg[3],
+                                   const gs_pixel_state* px, const float* d_image, const gs_raster_grads* g) {
+    return raster_bwd_impl(c, width, height, pre, reinterpret_cast<const uint32_t*>(words), 2, ranges, bg, px,
+                           d_image, g);
 }
 
 extern "C" int gs_preprocess_bwd(gs_ctx* c, const gs_gaussians* g, const gs_camera* cam, const gs_render_cfg* cfg,

@@ -229,9 +229,10 @@ struct RasterArgs {
     const float* conics;
     const float* opac;
     const float* rgb;
-    const uint32_t* values;
+    const uint32_t* values;  // Gaussian index of sorted pair j at values[j * vstride]
     const uint32_t* ranges;
     float bg0, bg1, bg2;
+    int vstride;  // 1: uint32 value array; 2: low words of tile<<32|index pairs
 };
 struct GradArgs {
     float2* d_means2D;

@@ -21,8 +21,10 @@ constexpr size_t GS_SMALL_OVERFLOW = 64;  // uint32 capacity-overflow flag
 struct gs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    void* ws = nullptr;
+    void* ws = nullptr;  // pair-count-sized scratch (sorts, pair words)
     size_t ws_bytes = 0;
+    void* ws2 = nullptr;  // Gaussian-count-sized scratch (depth order, ranked offsets)
+    size_t ws2_bytes = 0;
     void* small = nullptr;
     bool capturing = false;
     int radix_bits = 8;
@@ -33,6 +35,7 @@ struct gs_ctx {
 int gs_set_error(int code, const char* fmt, ...);
 int gs_hip_status(hipError_t e, const char* what);
 int gs_ws_get(gs_ctx* c, size_t bytes, void** out);
+int gs_ws2_get(gs_ctx* c, size_t bytes, void** out);
 int gs_small_get(gs_ctx* c, void** out);
 int check_cam_cfg(const gs_camera* cam, const gs_render_cfg* cfg);
 int gs_tile_key_bits(int num_tiles);
@@ -58,8 +61,17 @@ hipError_t launch_duplicate(int n, const float* means2D, const float* depths, co
 size_t radix_workspace_bytes(uint32_t n, int radix_bits);
 hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out, uint32_t* vals_out,
                              uint32_t n, int key_bits, int radix_bits, void* workspace, hipStream_t s);
+hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                                  uint32_t* vals_out, uint32_t n, int begin_bit, int end_bit, int radix_bits,
+                                  void* workspace, hipStream_t s, const uint32_t* n_dev);
 hipError_t launch_tile_ranges(const uint64_t* keys_sorted, uint32_t n, uint32_t num_tiles, uint32_t* ranges,
-                              hipStream_t s);
+                              hipStream_t s, const uint32_t* n_dev = nullptr);
+hipError_t launch_depth_keys(int n, const float* depths, const int* radii, uint64_t* keys, uint32_t* idx,
+                             hipStream_t s);
+hipError_t launch_gather_tiles(int n, const uint32_t* order, const int* tiles, int* out, hipStream_t s);
+hipError_t launch_duplicate_ranked(int n, const uint32_t* order, const float* means2D, const int* radii,
+                                   const uint32_t* offsets, int gx, int gy, uint64_t* words, uint32_t capacity,
+                                   uint32_t* overflow, hipStream_t s);
 hipError_t launch_raster_fwd(const gsr::RasterArgs& a, int num_tiles, int batch, float* img, float* T_final,
                              int* n_contrib, uint32_t* last_contrib, hipStream_t s);
 hipError_t launch_raster_bwd(const gsr::RasterArgs& a, int num_tiles, int batch, const float* T_final,

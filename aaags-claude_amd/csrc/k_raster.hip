@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, float* __
         for (int k = t; k < BATCH; k += 256) {
             uint32_t j = base + k;
             if (j < end) {
-                uint32_t id = a.values[j];
+                uint32_t id = a.values[(size_t)j * a.vstride];
                 s_xy[k] = a.means2D[id];
                 s_co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
                 s_rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const flo
         __syncthreads();
         for (int k = t; k < BATCH; k += 256) {
             if (k < n) {
-                uint32_t id = a.values[lo + k];
+                uint32_t id = a.values[(size_t)(lo + k) * a.vstride];
                 s_id[k] = id;
                 s_xy[k] = a.means2D[id];
                 s_co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);

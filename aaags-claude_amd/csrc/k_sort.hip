@@ -180,6 +180,56 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int n, const float2* __r
         }
 }
 
+// ------------------------------------------------- depth-ordered binning ----
+// Stable-sorting pairs by (tile << 32 | depth bits) orders each tile's pairs
+// by (depth bits, Gaussian index).  The same order results from sorting the
+// visible Gaussians by depth bits (stable, so ties keep index order), emitting
+// each Gaussian's pairs in that order, and stable-sorting the pairs by tile
+// alone.  These kernels implement the first two steps; the pairs are 64-bit
+// words tile << 32 | index, so the tile sort moves 8 bytes per pair.
+
+// Depth keys: culled Gaussians get the largest key and emit no pairs.
+__global__ __launch_bounds__(256) void depth_keys_kernel(int n, const float* __restrict__ depths,
+                                                         const int* __restrict__ radii, uint64_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ idx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = radii[i] > 0 ? (uint64_t)__float_as_uint(depths[i]) : 0xFFFFFFFFull;
+    idx[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void gather_tiles_kernel(int n, const uint32_t* __restrict__ order,
+                                                           const int* __restrict__ tiles, int* __restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    out[r] = tiles[order[r]];
+}
+
+// Pairs of the Gaussian of depth rank r at [offsets_incl[r] - count, offsets_incl[r]).
+__global__ __launch_bounds__(256) void duplicate_ranked_kernel(int n, const uint32_t* __restrict__ order,
+                                                               const float2* __restrict__ means2D,
+                                                               const int* __restrict__ radii,
+                                                               const uint32_t* __restrict__ offsets_incl, int gx,
+                                                               int gy, uint64_t* __restrict__ words,
+                                                               uint32_t capacity, uint32_t* __restrict__ overflow) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    uint32_t i = order[r];
+    int rad = radii[i];
+    if (rad <= 0) return;
+    float2 m = means2D[i];
+    int mn[2], mx[2];
+    gsd::get_rect(m.x, m.y, rad, gx, gy, mn, mx);
+    uint32_t cnt = (uint32_t)((mx[1] - mn[1]) * (mx[0] - mn[0]));
+    uint32_t off = offsets_incl[r] - cnt;
+    for (int y = mn[1]; y < mx[1]; y++)
+        for (int x = mn[0]; x < mx[0]; x++) {
+            if (off < capacity) words[off] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | i;
+            else if (overflow) *overflow = 1u;
+            off++;
+        }
+}
+
 // ---------------------------------------------------------- radix sort ----
 template <int BITS>
 __device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) {
@@ -198,10 +248,22 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
     return peers;
 }
 
+// n_dev (optional): the item count lives on the device (graph capture); the
+// launch is sized for n (the capacity) and the kernels use min(*n_dev, n).
+__device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* n_dev) {
+    if (n_dev) {
+        uint32_t d = *n_dev;
+        return d < n ? d : n;
+    }
+    return n;
+}
+
 template <int BITS>
-__global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint32_t n, int shift,
+__global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
+                                                                const uint32_t* __restrict__ n_dev, int shift,
                                                                 uint32_t nb, uint32_t* __restrict__ hist) {
     constexpr int RADIX = 1 << BITS;
+    const uint32_t n = live_count(n_cap, n_dev);
     __shared__ uint32_t s_h[RS_WAVES][RADIX];
     const int w = threadIdx.x >> 6, l = lane_id();
     for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_h[0][0])[i] = 0u;
@@ -225,16 +287,19 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* 
     }
 }
 
-template <int BITS>
+// VALS = false sorts self-contained 64-bit words (no separate value array).
+template <int BITS, bool VALS>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t* __restrict__ keys_in,
-                                                                  const uint32_t* __restrict__ vals_in, uint32_t n,
-                                                                  int shift, uint32_t nb,
-                                                                  const uint32_t* __restrict__ goff,
+                                                                  const uint32_t* __restrict__ vals_in, uint32_t n_cap,
+                                                                  const uint32_t* __restrict__ n_dev, int shift,
+                                                                  uint32_t nb, const uint32_t* __restrict__ goff,
                                                                   uint64_t* __restrict__ keys_out,
                                                                   uint32_t* __restrict__ vals_out) {
     constexpr int RADIX = 1 << BITS;
+    const uint32_t n = live_count(n_cap, n_dev);
+    if (blockIdx.x * (uint32_t)RS_TILE >= n) return;  // whole block past the live count
     __shared__ uint64_t s_k[RS_TILE];
-    __shared__ uint32_t s_v[RS_TILE];
+    __shared__ uint32_t s_v[VALS ? RS_TILE : 1];
     __shared__ uint32_t s_wh[RS_WAVES][RADIX];
     __shared__ uint32_t s_dstart[RADIX];
     __shared__ uint32_t s_goff[RADIX];
@@ -248,7 +313,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
     const uint32_t bbase = blockIdx.x * RS_TILE;
     const uint32_t wbase = bbase + w * RS_WAVE_SPAN;
     uint64_t k_r[RS_ROUNDS];
-    uint32_t v_r[RS_ROUNDS];
+    uint32_t v_r[VALS ? RS_ROUNDS : 1];
     uint32_t rank_r[RS_ROUNDS];
     uint32_t d_r[RS_ROUNDS];
     // 1. per-wave stable ranks: rounds in key order, lanes in key order.
@@ -258,7 +323,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
         bool valid = i < n;
         uint64_t active = __ballot(valid);
         k_r[r] = valid ? keys_in[i] : 0ull;
-        v_r[r] = valid ? vals_in[i] : 0u;
+        if constexpr (VALS) v_r[r] = valid ? vals_in[i] : 0u;
         uint32_t d = digit_of<BITS>(k_r[r], shift);
         d_r[r] = d;
         rank_r[r] = 0u;
@@ -299,8 +364,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
         uint32_t i = wbase + r * 64 + l;
         if (i < n) {
             uint32_t p = s_dstart[d_r[r]] + s_wh[w][d_r[r]] + rank_r[r];
-            s_k[p] = k_r[r];
-            s_v[p] = v_r[r];
+            if (p < (uint32_t)RS_TILE) {
+                s_k[p] = k_r[r];
+                if constexpr (VALS) s_v[p] = v_r[r];
+            }
         }
     }
     __syncthreads();
@@ -312,15 +379,17 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
         uint32_t g = s_goff[d] + (p - s_dstart[d]);
         if (g < n) {
             keys_out[g] = k;
-            vals_out[g] = s_v[p];
+            if constexpr (VALS) vals_out[g] = s_v[p];
         }
     }
 }
 
 // identify_tile_ranges (radix_sort.cl:14-46), ranges pre-zeroed.
 // Tile ids at or beyond num_tiles are ignored (they cannot come from binning).
-__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t* __restrict__ keys, uint32_t n,
-                                                          uint32_t num_tiles, uint32_t* __restrict__ ranges) {
+__global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
+                                                          const uint32_t* __restrict__ n_dev, uint32_t num_tiles,
+                                                          uint32_t* __restrict__ ranges) {
+    const uint32_t n = live_count(n_cap, n_dev);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t cur = (uint32_t)(keys[i] >> 32);
@@ -374,7 +443,7 @@ hipError_t launch_duplicate(int n, const float* means2D, const float* depths, co
 
 uint32_t radix_blocks(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
 
-// Workspace layout for one sort of n pairs with the given radix:
+// Workspace layout for one sort of n items with the given radix:
 //   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][partials]
 size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
     size_t nb = radix_blocks(n);
@@ -386,18 +455,23 @@ size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
 }
 
 template <int BITS>
-static hipError_t sort_pass(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n, int shift,
-                            uint32_t* hist, uint32_t* goff, unsigned long long* partial, hipStream_t s) {
+static hipError_t sort_pass(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n,
+                            const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff,
+                            unsigned long long* partial, hipStream_t s) {
     uint32_t nb = radix_blocks(n);
-    rs_upsweep_kernel<BITS><<<nb, RS_THREADS, 0, s>>>(ki, n, shift, nb, hist);
+    rs_upsweep_kernel<BITS><<<nb, RS_THREADS, 0, s>>>(ki, n, n_dev, shift, nb, hist);
     hipError_t e = launch_scan_u32_exclusive(hist, (uint32_t)((1u << BITS) * nb), goff, partial, s);
     if (e != hipSuccess) return e;
-    rs_downsweep_kernel<BITS><<<nb, RS_THREADS, 0, s>>>(ki, vi, n, shift, nb, goff, ko, vo);
+    if (vi) rs_downsweep_kernel<BITS, true><<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, nb, goff, ko, vo);
+    else rs_downsweep_kernel<BITS, false><<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, nb, goff, ko, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out, uint32_t* vals_out,
-                             uint32_t n, int key_bits, int radix_bits, void* workspace, hipStream_t s) {
+// Stable LSD sort of n keys (with optional uint32 values) on key bits
+// [begin_bit, end_bit).  The last pass lands in keys_out / vals_out.
+hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                                  uint32_t* vals_out, uint32_t n, int begin_bit, int end_bit, int radix_bits,
+                                  void* workspace, hipStream_t s, const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
     size_t nb = radix_blocks(n);
     size_t tab = ((size_t)1 << radix_bits) * nb;
@@ -413,10 +487,12 @@ hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, u
     w += al(tab * 4);
     unsigned long long* partial = reinterpret_cast<unsigned long long*>(w);
 
-    int passes = (key_bits + radix_bits - 1) / radix_bits;
+    const bool vals = vals_in != nullptr;
+    int bits = end_bit - begin_bit;
+    int passes = bits > 0 ? (bits + radix_bits - 1) / radix_bits : 0;
     if (passes <= 0) {
         hipError_t e = hipMemcpyAsync(keys_out, keys_in, (size_t)n * 8, hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess || !vals) return e;
         return hipMemcpyAsync(vals_out, vals_in, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
     }
     const uint64_t* ki = keys_in;
@@ -425,15 +501,15 @@ hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, u
         // the last pass lands in the output; earlier passes alternate so that it does
         bool to_out = ((passes - 1 - p) % 2) == 0;
         uint64_t* ko = to_out ? keys_out : pk;
-        uint32_t* vo = to_out ? vals_out : pv;
+        uint32_t* vo = vals ? (to_out ? vals_out : pv) : nullptr;
         hipError_t e;
-        int shift = p * radix_bits;
+        int shift = begin_bit + p * radix_bits;
         switch (radix_bits) {
-            case 4: e = sort_pass<4>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
-            case 5: e = sort_pass<5>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
-            case 6: e = sort_pass<6>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
-            case 7: e = sort_pass<7>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
-            case 8: e = sort_pass<8>(ki, vi, ko, vo, n, shift, hist, goff, partial, s); break;
+            case 4: e = sort_pass<4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
+            case 5: e = sort_pass<5>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
+            case 6: e = sort_pass<6>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
+            case 7: e = sort_pass<7>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
+            case 8: e = sort_pass<8>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
             default: return hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
@@ -443,9 +519,37 @@ hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, u
     return hipSuccess;
 }
 
+hipError_t launch_radix_sort(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out, uint32_t* vals_out,
+                             uint32_t n, int key_bits, int radix_bits, void* workspace, hipStream_t s) {
+    return launch_radix_sort_bits(keys_in, vals_in, keys_out, vals_out, n, 0, key_bits, radix_bits, workspace, s,
+                                  nullptr);
+}
+
+hipError_t launch_depth_keys(int n, const float* depths, const int* radii, uint64_t* keys, uint32_t* idx,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    depth_keys_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, depths, radii, keys, idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_tiles(int n, const uint32_t* order, const int* tiles, int* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    gather_tiles_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, order, tiles, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_duplicate_ranked(int n, const uint32_t* order, const float* means2D, const int* radii,
+                                   const uint32_t* offsets, int gx, int gy, uint64_t* words, uint32_t capacity,
+                                   uint32_t* overflow, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    duplicate_ranked_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, order, reinterpret_cast<const float2*>(means2D), radii,
+                                                            offsets, gx, gy, words, capacity, overflow);
+    return hipGetLastError();
+}
+
 hipError_t launch_tile_ranges(const uint64_t* keys_sorted, uint32_t n, uint32_t num_tiles, uint32_t* ranges,
-                              hipStream_t s) {
+                              hipStream_t s, const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
-    tile_ranges_kernel<<<(n + 255) / 256, 256, 0, s>>>(keys_sorted, n, num_tiles, ranges);
+    tile_ranges_kernel<<<(n + 255) / 256, 256, 0, s>>>(keys_sorted, n, n_dev, num_tiles, ranges);
     return hipGetLastError();
 }

@@ -76,6 +76,7 @@ def load_library(path: str = LIB_PATH):
         _LIB.gs_sort_pairs.argtypes = [_vp, _u32, _vp, _vp, _vp, _vp, _i32, _vp]
         _LIB.gs_duplicate_keys.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _u32]
         _LIB.gs_scan_tiles.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
+        _LIB.gs_bin_sorted_words.argtypes = [_vp, _i32, _vp, _i32, _i32, _vp, _u32, _vp, _vp]
     return _LIB
 
 
@@ -214,6 +215,51 @@ class GsHip:
                     "gs_tile_ranges")
         return ranges
 
+    def bin_sorted_words(self, pre, width, height, capacity=None):
+        """Depth-ordered binning: (words[K] = tile << 32 | index, ranges, K)."""
+        n = pre["radii"].numel()
+        num_tiles = ((width + 15) // 16) * ((height + 15) // 16)
+        ranges = torch.empty(num_tiles * 2, dtype=torch.int32, device=self.dev)
+        K = ctypes.c_longlong()
+        if capacity is None:
+            # an upper bound: run once to learn K when it is unknown
+            st = self.lib.gs_bin_sorted_words(self.ctx, n, ctypes.byref(self._pre_struct(pre)), width, height, None,
+                                              _u32(0), _vp(ptr(ranges)), ctypes.byref(K))
+            if st not in (0, 3):
+                self._check(st, "gs_bin_sorted_words")
+            capacity = max(int(K.value), 1)
+        words = torch.empty(capacity, dtype=torch.int64, device=self.dev)
+        self._check(self.lib.gs_bin_sorted_words(self.ctx, n, ctypes.byref(self._pre_struct(pre)), width, height,
+                                                 _vp(ptr(words)), _u32(capacity), _vp(ptr(ranges)), ctypes.byref(K)),
+                    "gs_bin_sorted_words")
+        return words[:K.value], ranges, int(K.value)
+
+    def raster_fwd_words(self, pre, words, ranges, width, height, bg=(0.0, 0.0, 0.0)):
+        d = self.dev
+        img = torch.empty((height, width, 3), device=d)
+        px = dict(T_final=torch.empty((height, width), device=d),
+                  n_contrib=torch.empty((height, width), dtype=torch.int32, device=d),
+                  last_contrib=torch.empty((height, width), dtype=torch.int32, device=d))
+        st = _PixelState(ptr(px["T_final"]), ptr(px["n_contrib"]), ptr(px["last_contrib"]))
+        bgv = (ctypes.c_float * 3)(*bg)
+        w = words if words.numel() else torch.zeros(1, dtype=torch.int64, device=d)
+        self._check(self.lib.gs_raster_fwd_words(self.ctx, width, height, ctypes.byref(self._pre_struct(pre)),
+                                                 _vp(ptr(w)), _vp(ptr(ranges)), bgv, _vp(ptr(img)), ctypes.byref(st)),
+                    "gs_raster_fwd_words")
+        return img, px
+
+    def raster_bwd_words(self, pre, words, ranges, width, height, px, d_image, bg=(0.0, 0.0, 0.0), rg=None):
+        n = pre["op2d"].numel()
+        rg = rg if rg is not None else self.alloc_rgrads(n)
+        st = _PixelState(ptr(px["T_final"]), ptr(px.get("n_contrib")), ptr(px["last_contrib"]))
+        gs = _RGrads(ptr(rg["d_means2D"]), ptr(rg["d_conics"]), ptr(rg["d_rgb"]), ptr(rg["d_op2d"]))
+        bgv = (ctypes.c_float * 3)(*bg)
+        w = words if words.numel() else torch.zeros(1, dtype=torch.int64, device=self.dev)
+        self._check(self.lib.gs_raster_bwd_words(self.ctx, width, height, ctypes.byref(self._pre_struct(pre)),
+                                                 _vp(ptr(w)), _vp(ptr(ranges)), bgv, ctypes.byref(st),
+                                                 _vp(ptr(d_image)), ctypes.byref(gs)), "gs_raster_bwd_words")
+        return rg
+
     def raster_fwd(self, pre, values_sorted, ranges, width, height, bg=(0.0, 0.0, 0.0)):
         d = self.dev
         img = torch.empty((height, width, 3), device=d)
@@ -296,6 +342,22 @@ class GsHip:
         img, px = self.raster_fwd(pre, vs, ranges, cam.width, cam.height, cfg.bg_color)
         return dict(pre=pre, K=K, keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges, img=img,
                     px=px)
+
+    def forward_fast(self, p, cam, cfg):
+        """Single-view forward with depth-ordered binning (the trainer's path)."""
+        pre = self.preprocess(p, cam, cfg)
+        words, ranges, K = self.bin_sorted_words(pre, cam.width, cam.height)
+        img, px = self.raster_fwd_words(pre, words, ranges, cam.width, cam.height, cfg.bg_color)
+        return dict(pre=pre, K=K, words=words, ranges=ranges, img=img, px=px)
+
+    def forward_backward_fast(self, p, cam, cfg, gt):
+        f = self.forward_fast(p, cam, cfg)
+        loss, d_img = self.l1_loss(f["img"], gt)
+        rg = self.raster_bwd_words(f["pre"], f["words"], f["ranges"], cam.width, cam.height, f["px"], d_img,
+                                   cfg.bg_color)
+        grads = self.preprocess_bwd(p, cam, cfg, f["pre"]["radii"], rg)
+        f.update(loss=loss, d_image=d_img, rgrad=rg, grads=grads)
+        return f
 
     def forward_backward(self, p, cam, cfg, gt):
         f = self.forward(p, cam, cfg)

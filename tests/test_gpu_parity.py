@@ -90,6 +90,35 @@ def test_binning_and_sort_bitexact(hip, oracle, n, deg, seed):
     assert np.array_equal(u32(ranges), rg_ref)
 
 
+@pytest.mark.parametrize("n,deg,seed", SMALL_CASES[1:] + [(10000, 0, 0)])
+@pytest.mark.parametrize("radix_bits", [6, 8])
+def test_depth_ordered_binning_bitexact(hip, oracle, n, deg, seed, radix_bits):
+    """gs_bin_sorted_words: same per-tile Gaussian lists as the reference's
+    duplicate-with-keys + stable 64-bit sort, bit for bit."""
+    _say("depth-ordered binning", n, deg, seed, radix_bits)
+    if n == 10000:
+        scene, cams, _ = make_workload("c1_10k_256")
+        cam = cams[0]
+    else:
+        scene, cam, _ = random_small_scene(n, deg, seed, width=160, height=120)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    g = oracle.activate(scene)
+    ref = oracle.preprocess(g, scene.n, scene.sh_degree, scene.max_coeffs, cam, cfg, cache=False)
+    if n == 200:  # force depth ties: equal depths must keep index order
+        ref["depths"][::3] = ref["depths"][0]
+    keys_ref, vals_ref = oracle.duplicate(ref, scene.n, cam.width, cam.height)
+    ks_ref, vs_ref, rg_ref = oracle.sort_pairs(keys_ref, vals_ref, cam.num_tiles)
+    pre = _to_dev(hip, {k: ref[k] for k in ["means2D", "depths", "conics", "op2d", "rgb", "radii", "tiles"]})
+    hip.set_tuning(radix_bits=radix_bits)
+    words, ranges, K = hip.bin_sorted_words(pre, cam.width, cam.height)
+    hip.set_tuning(radix_bits=8)
+    assert K == len(keys_ref)
+    w = u32(words)
+    assert np.array_equal((w & np.uint64(0xFFFFFFFF)).astype(np.uint32), vs_ref)
+    assert np.array_equal((w >> np.uint64(32)).astype(np.uint32), (ks_ref >> np.uint64(32)).astype(np.uint32))
+    assert np.array_equal(u32(ranges), rg_ref)
+
+
 @pytest.mark.parametrize("radix_bits", [4, 5, 6, 7, 8])
 @pytest.mark.parametrize("count", [1, 63, 4096, 4097, 100_003])
 def test_radix_sort_stable_random_keys(hip, count, radix_bits):
@@ -279,14 +308,16 @@ def test_golden_scenes_full_path(hip, name):
             assert_grad_close(_np(f["grads"][k]), exp["grad_" + k], "grad_" + k)
 
 
+@pytest.mark.parametrize("fast", [False, True])
 @pytest.mark.parametrize("n,deg,seed", [(500, 3, 21), (3000, 3, 22), (3000, 1, 23)])
-def test_full_path_matches_oracle(hip, oracle, n, deg, seed):
-    _say("full path", n, deg, seed)
+def test_full_path_matches_oracle(hip, oracle, n, deg, seed, fast):
+    _say("full path", n, deg, seed, fast)
     scene, cam, gt = random_small_scene(n, deg, seed, width=128, height=96)
     cfg = RenderCfg(sh_degree=3, training=True)
     ref = oracle.forward_backward(scene, cam, cfg, gt)
     p = hip.upload_scene(scene)
-    f = hip.forward_backward(p, cam, cfg, torch.tensor(gt, device=hip.dev))
+    run = hip.forward_backward_fast if fast else hip.forward_backward
+    f = run(p, cam, cfg, torch.tensor(gt, device=hip.dev))
     assert_image_close(_np(f["img"]), ref["img"])
     assert abs(float(f["loss"].item()) - ref["loss"]) <= 1e-6
     for k in ["pos", "scale", "rot", "sh", "op"]:
@@ -301,7 +332,7 @@ def test_c1_workload_full_path(hip, oracle):
     cfg = RenderCfg(sh_degree=3, training=True)
     ref = oracle.forward_backward(scene, cam, cfg, gt)
     p = hip.upload_scene(scene)
-    f = hip.forward_backward(p, cam, cfg, torch.tensor(gt, device=hip.dev))
+    f = hip.forward_backward_fast(p, cam, cfg, torch.tensor(gt, device=hip.dev))
     img_err = np.abs(_np(f["img"]) - ref["img"]).max()
     assert f["K"] == len(ref["keys"])
     assert img_err <= IMAGE_TOL, img_err

@@ -1,0 +1,65 @@
+"""The native training driver (gs_trainer_step): one full step against the
+oracle (Trainer::step restated), and the optimizer update against the oracle
+Adam applied to the driver's own gradients (bit-exact)."""
+import numpy as np
+import pytest
+import torch
+
+from gs_scenes import RenderCfg, make_workload, random_small_scene
+from helpers import assert_grad_close, assert_image_close
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["pos", "scale", "rot", "sh", "op"]
+
+
+@pytest.mark.parametrize("use_adam", [True, False])
+@pytest.mark.parametrize("n,deg,seed", [(800, 3, 31), (3000, 1, 32)])
+def test_trainer_step_matches_oracle(hip, oracle, n, deg, seed, use_adam):
+    from gs_hip import GsTrainer, train_cfg
+    print("[case] trainer step", n, deg, seed, use_adam, flush=True)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=128, height=96)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    tc = train_cfg(use_adam=use_adam)
+    tr = GsTrainer(hip, scene)
+    gt_t = torch.tensor(gt, device=hip.dev)
+    res = tr.step(cam, cfg, gt_t, tc, 7)
+    torch.cuda.synchronize()
+    ref = oracle.forward_backward(scene, cam, cfg, gt)
+    assert abs(res.loss - ref["loss"]) <= 1e-6
+    assert res.pairs == len(ref["keys"])
+    assert_image_close(tr.buffer("image", (cam.height, cam.width, 3)).cpu().numpy(), ref["img"])
+    shapes = dict(pos=(n, 3), scale=(n, 3), rot=(n, 4), sh=(n, scene.max_coeffs, 3), op=(n,))
+    grads = {k: tr.buffer("grad_" + k, shapes[k]).cpu().numpy() for k in NAMES}
+    for k in NAMES:
+        assert_grad_close(grads[k], ref["grads"][k], "grad_" + k)
+    # optimizer: the oracle's update applied to the driver's own gradients
+    after = tr.params()
+    lr_pos = oracle.lr_schedule(tc.lr_position_init, tc.lr_position_final, 7, tc.max_steps)
+    lrs = [lr_pos, tc.lr_scaling, tc.lr_rotation, tc.lr_feature, tc.lr_opacity]
+    raw = [a.copy() for a in scene.arrays()]
+    if use_adam:
+        bc1, bc2 = oracle.bias_corrections(0.9, 0.999, 1)
+        for i, k in enumerate(NAMES):
+            m = np.zeros_like(raw[i])
+            v = np.zeros_like(raw[i])
+            oracle.adam_update(raw[i], grads[k].reshape(raw[i].shape), m, v, lrs[i], 0.9, 0.999, 1e-8, bc1, bc2)
+    else:
+        for i, k in enumerate(NAMES):
+            raw[i] = raw[i] - np.float32(lrs[i]) * grads[k].reshape(raw[i].shape)
+    for i, k in enumerate(NAMES):
+        assert np.array_equal(after[i].reshape(raw[i].shape), raw[i]), k
+
+
+def test_trainer_loss_decreases_on_c1_workload(hip):
+    """test_trainer_gpu.cpp's criterion: the loss does not increase over steps
+    (10k Gaussians, 256x256, Adam)."""
+    from gs_hip import GsTrainer, train_cfg
+    print("[case] trainer c1 convergence", flush=True)
+    scene, cams, gts = make_workload("c1_10k_256")
+    cfg = RenderCfg(sh_degree=3, training=True)
+    tr = GsTrainer(hip, scene)
+    gt = torch.tensor(gts[0], device=hip.dev)
+    tc = train_cfg(lr=1e-2)
+    losses = [tr.step(cams[0], cfg, gt, tc, i).loss for i in range(30)]
+    assert losses[-1] < losses[0] * 0.97, losses
