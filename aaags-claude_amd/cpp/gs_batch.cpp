@@ -1,0 +1,512 @@
+// gs_batch.cpp -- batched multi-view training step.
+//
+// One step over V views of uneven resolution: the Gaussians' parameters are
+// read once per step by a multi-view preprocess, all views are binned and
+// sorted together (depth sort on view << 32 | depth bits, tile sort on the
+// local tile bits of view << 48 | tile << 32 | index words), the forward and
+// backward raster each run as one launch over the global (view, tile) list
+// (optionally a persistent grid pulling tiles from an atomic counter), and the
+// multi-view preprocess backward sums the views' gradients in registers and
+// applies Adam in place.  The whole step can be captured once into a hipGraph
+// and replayed: pair buffers are pre-sized from a measured step, every kernel
+// reads the live pair count from device memory, and per-step Adam
+// hyper-parameters are copied to the device before each replay.
+//
+// The multi-view loss is the sum of the views' mean L1 losses (each view's
+// image gradient is sign(r - g) / n_view), so the gradients are the sum of the
+// single-view gradients Trainer::step would compute per view
+// (harmonyos_3dgs/src/trainer.cpp:7-70).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gs_device.h"
+#include "gs_internal.h"
+#include "gs_trainer_impl.h"
+
+using gs::DevBuf;
+
+gsd::CamArgs make_cam_args(const gs_camera* cam, const gs_render_cfg* cfg);
+
+namespace {
+
+constexpr int SCAN_BLOCK = 4096;  // k_sort.hip scan block size
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct gs_batch {
+    gs_trainer* tr = nullptr;
+    int V = 0, N = 0;
+    std::vector<gs_camera> cams;
+    gs_render_cfg cfg{};
+    std::vector<int4> views;  // W, H, gx, first global tile
+    std::vector<int> pix_off; // V + 1
+    int total_tiles = 0;
+    long long total_pix = 0;
+    int tile_bits = 0, view_bits = 0;
+    int radix_bits = 8;
+    int raster_batch = 256, raster_bwd_batch = 256;
+    int persistent = 1, fused_adam = 1, use_graph = 1;
+    int persistent_groups = 0;
+    uint32_t capacity = 0;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    gs::GraphSlot graph;
+    bool graph_ready = false;
+
+    // device buffers
+    DevBuf cams_dev, views_dev, pixoff_dev, hp_dev, counters;
+    DevBuf means2D, depths, conics, op2d, rgb, radii, tiles;
+    DevBuf dkeys, didx, dkeys_s, didx_s, rtiles, roff, scan_partial, dsort_ws;
+    DevBuf words, words_tmp, tsort_ws, ranges;
+    DevBuf image, d_image, gt, T_final, n_contrib, last;
+    DevBuf rg_means2D, rg_conics, rg_rgb, rg_op;
+    DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16
+    int l1_nb = 0;
+
+    gs_preprocessed pre() {
+        return gs_preprocessed{means2D.f(), depths.f(), conics.f(), op2d.f(), rgb.f(), radii.i(), tiles.i()};
+    }
+    gs_raster_grads rgrads() { return gs_raster_grads{rg_means2D.f(), rg_conics.f(), rg_rgb.f(), rg_op.f()}; }
+    uint32_t* total32() { return reinterpret_cast<uint32_t*>(small.p); }
+    unsigned long long* total64() { return reinterpret_cast<unsigned long long*>(static_cast<char*>(small.p) + 8); }
+    uint32_t* overflow() { return reinterpret_cast<uint32_t*>(static_cast<char*>(small.p) + 16); }
+    ~gs_batch();
+};
+
+gs_batch::~gs_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    graph.release();
+    DevBuf* all[] = {&cams_dev, &views_dev, &pixoff_dev, &hp_dev, &counters, &means2D, &depths, &conics, &op2d, &rgb,
+                     &radii, &tiles, &dkeys, &didx, &dkeys_s, &didx_s, &rtiles, &roff, &scan_partial, &dsort_ws,
+                     &words, &words_tmp, &tsort_ws, &ranges, &image, &d_image, &gt, &T_final, &n_contrib, &last,
+                     &rg_means2D, &rg_conics, &rg_rgb, &rg_op, &losses, &l1_partial, &small};
+    for (DevBuf* b : all) b->release();
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+#define B_TRY(x)                      \
+    do {                              \
+        int st_ = (x);                \
+        if (st_ != GS_OK) return st_; \
+    } while (0)
+#define B_HIP(expr, what)                                     \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return gs_hip_status(e_, what); \
+    } while (0)
+
+static int bits_for(long long n) {
+    int b = 0;
+    while ((1ll << b) < n) b++;
+    return b;
+}
+
+// Pair-count-sized buffers for `cap` pairs.
+static int batch_size_pairs(gs_batch* b, uint32_t cap) {
+    if (cap <= b->capacity && b->words.p) return GS_OK;
+    b->graph.release();
+    b->graph_ready = false;
+    B_TRY(b->words.ensure((size_t)cap * 8));
+    B_TRY(b->words_tmp.ensure((size_t)cap * 8));
+    B_TRY(b->tsort_ws.ensure(radix_workspace_bytes(cap, b->radix_bits)));
+    b->capacity = cap;
+    return GS_OK;
+}
+
+extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cams, const gs_render_cfg* cfg,
+                               gs_batch** out) {
+    if (!tr || !cams || !cfg || !out || n_views <= 0)
+        return gs_set_error(GS_ERR_INVALID, "gs_batch_create: bad argument");
+    if (n_views > 65535) return gs_set_error(GS_ERR_INVALID, "gs_batch_create: at most 65535 views");
+    for (int v = 0; v < n_views; v++) B_TRY(check_cam_cfg(&cams[v], cfg));
+    gs_batch* b = new gs_batch();
+    b->tr = tr;
+    b->V = n_views;
+    b->N = tr->n;
+    b->cams.assign(cams, cams + n_views);
+    b->cfg = *cfg;
+    b->radix_bits = tr->ctx->radix_bits;
+    b->raster_batch = tr->ctx->raster_batch;
+    b->raster_bwd_batch = tr->ctx->raster_bwd_batch;
+    int max_tiles = 0;
+    b->pix_off.push_back(0);
+    for (int v = 0; v < n_views; v++) {
+        int W = cams[v].width, H = cams[v].height;
+        int gx = (W + 15) / 16, gy = (H + 15) / 16;
+        if (gx * gy > 65536) {
+            delete b;
+            return gs_set_error(GS_ERR_INVALID, "gs_batch_create: view %d has more than 65536 tiles", v);
+        }
+        b->views.push_back(int4{W, H, gx, b->total_tiles});
+        b->total_tiles += gx * gy;
+        max_tiles = std::max(max_tiles, gx * gy);
+        b->pix_off.push_back(b->pix_off.back() + W * H);
+    }
+    b->total_pix = b->pix_off.back();
+    b->tile_bits = std::max(1, bits_for(max_tiles));
+    b->view_bits = bits_for(n_views);
+    int st = GS_OK;
+    auto fail = [&](int s) {
+        delete b;
+        return s;
+    };
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, tr->ctx->device) != hipSuccess) return fail(gs_set_error(GS_ERR_HIP, "props"));
+    b->persistent_groups = prop.multiProcessorCount * 6;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
+        return fail(gs_set_error(GS_ERR_HIP, "gs_batch_create: stream/event creation failed"));
+
+    const size_t VN = (size_t)n_views * b->N;
+    const size_t P = (size_t)b->total_pix;
+    std::vector<gsd::CamArgs> ca(n_views);
+    for (int v = 0; v < n_views; v++) ca[v] = make_cam_args(&cams[v], cfg);
+    if ((st = b->cams_dev.ensure(sizeof(gsd::CamArgs) * n_views)) || (st = b->views_dev.ensure(sizeof(int4) * n_views)) ||
+        (st = b->pixoff_dev.ensure(sizeof(int) * (n_views + 1))) || (st = b->hp_dev.ensure(sizeof(gso::AdamDev))) ||
+        (st = b->counters.ensure(256)) || (st = b->means2D.ensure(VN * 8)) || (st = b->depths.ensure(VN * 4)) ||
+        (st = b->conics.ensure(VN * 12)) || (st = b->op2d.ensure(VN * 4)) || (st = b->rgb.ensure(VN * 12)) ||
+        (st = b->radii.ensure(VN * 4)) || (st = b->tiles.ensure(VN * 4)) || (st = b->dkeys.ensure(VN * 8)) ||
+        (st = b->didx.ensure(VN * 4)) || (st = b->dkeys_s.ensure(VN * 8)) || (st = b->didx_s.ensure(VN * 4)) ||
+        (st = b->rtiles.ensure(VN * 4)) || (st = b->roff.ensure(VN * 4)) ||
+        (st = b->scan_partial.ensure(((VN + SCAN_BLOCK - 1) / SCAN_BLOCK + 1) * 8)) ||
+        (st = b->dsort_ws.ensure(radix_workspace_bytes((uint32_t)VN, b->radix_bits))) ||
+        (st = b->ranges.ensure((size_t)b->total_tiles * 8)) || (st = b->image.ensure(P * 12)) ||
+        (st = b->d_image.ensure(P * 12)) || (st = b->gt.ensure(P * 12)) || (st = b->T_final.ensure(P * 4)) ||
+        (st = b->n_contrib.ensure(P * 4)) || (st = b->last.ensure(P * 4)) || (st = b->rg_means2D.ensure(VN * 8)) ||
+        (st = b->rg_conics.ensure(VN * 12)) || (st = b->rg_rgb.ensure(VN * 12)) || (st = b->rg_op.ensure(VN * 4)) ||
+        (st = b->losses.ensure(8 * n_views)) || (st = b->small.ensure(256)))
+        return fail(st);
+    int max_pix = 0;
+    for (int v = 0; v < n_views; v++) max_pix = std::max(max_pix, b->pix_off[v + 1] - b->pix_off[v]);
+    b->l1_nb = std::max(1, std::min(512, (max_pix * 3 + 255) / 256 / 4));
+    if ((st = b->l1_partial.ensure((size_t)n_views * b->l1_nb * 8))) return fail(st);
+    hipStream_t s = b->stream;
+    if (hipMemcpyAsync(b->cams_dev.p, ca.data(), sizeof(gsd::CamArgs) * n_views, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->views_dev.p, b->views.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->pixoff_dev.p, b->pix_off.data(), sizeof(int) * (n_views + 1), hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        launch_zero(b->small.p, 256, s) != hipSuccess || launch_zero(b->gt.p, P * 12, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(gs_set_error(GS_ERR_HIP, "gs_batch_create: upload failed"));
+    *out = b;
+    return GS_OK;
+}
+
+extern "C" int gs_batch_destroy(gs_batch* b) {
+    delete b;
+    return GS_OK;
+}
+
+extern "C" int gs_batch_set_options(gs_batch* b, int persistent, int fused_adam, int use_graph) {
+    if (!b) return gs_set_error(GS_ERR_INVALID, "null batch");
+    if (b->persistent != persistent || b->fused_adam != fused_adam) {
+        b->graph.release();
+        b->graph_ready = false;
+    }
+    b->persistent = persistent;
+    b->fused_adam = fused_adam;
+    b->use_graph = use_graph;
+    return GS_OK;
+}
+
+extern "C" int gs_batch_set_targets(gs_batch* b, const float* const* gts, int from_host) {
+    if (!b || !gts) return gs_set_error(GS_ERR_INVALID, "null argument");
+    for (int v = 0; v < b->V; v++) {
+        size_t off = (size_t)b->pix_off[v] * 3, n = (size_t)(b->pix_off[v + 1] - b->pix_off[v]) * 3;
+        B_HIP(hipMemcpyAsync(b->gt.f() + off, gts[v], n * 4, from_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                             b->stream),
+              "hipMemcpyAsync(targets)");
+    }
+    B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    return GS_OK;
+}
+
+// Everything from preprocess to the tile ranges (the forward's binning).
+static int enqueue_binning(gs_batch* b, hipStream_t s) {
+    gs_trainer* t = b->tr;
+    const uint32_t VN = (uint32_t)((size_t)b->V * b->N);
+    gs_gaussians g = t->gaussians();
+    gs_preprocessed pre = b->pre();
+    const int deg = std::min(b->cfg.sh_degree, t->sh_degree);
+    B_HIP(launch_preprocess_fwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), b->V, pre,
+                                      static_cast<uint64_t*>(b->dkeys.p), b->didx.u(), deg, s),
+          "multi-view preprocess");
+    B_HIP(launch_radix_sort_bits(static_cast<const uint64_t*>(b->dkeys.p), b->didx.u(),
+                                 static_cast<uint64_t*>(b->dkeys_s.p), b->didx_s.u(), VN, 0, 32 + b->view_bits,
+                                 b->radix_bits, b->dsort_ws.p, s, nullptr),
+          "depth sort");
+    B_HIP(launch_gather_tiles_multi(VN, b->N, static_cast<const uint64_t*>(b->dkeys_s.p), b->didx_s.u(), pre.tiles_touched,
+                                    b->rtiles.i(), s),
+          "gather tiles");
+    B_HIP(launch_scan_tiles(b->rtiles.i(), VN, b->roff.u(), static_cast<unsigned long long*>(b->scan_partial.p),
+                            b->total32(), b->total64(), s),
+          "scan");
+    B_HIP(launch_duplicate_multi(VN, b->N, static_cast<const uint64_t*>(b->dkeys_s.p), b->didx_s.u(), pre.means2D,
+                                 pre.radii, b->roff.u(), static_cast<const int4*>(b->views_dev.p),
+                                 static_cast<uint64_t*>(b->words_tmp.p), b->capacity, b->overflow(), s),
+          "duplicate");
+    B_HIP(launch_radix_sort_bits(static_cast<const uint64_t*>(b->words_tmp.p), nullptr,
+                                 static_cast<uint64_t*>(b->words.p), nullptr, b->capacity, 32, 32 + b->tile_bits,
+                                 b->radix_bits, b->tsort_ws.p, s, b->total32()),
+          "tile sort");
+    B_HIP(launch_zero(b->ranges.p, (size_t)b->total_tiles * 8, s), "zero ranges");
+    B_HIP(launch_tile_ranges_multi(static_cast<const uint64_t*>(b->words.p), b->capacity, b->total32(),
+                                   static_cast<const int4*>(b->views_dev.p), b->ranges.u(), s),
+          "tile ranges");
+    return GS_OK;
+}
+
+static gsr::MultiRaster multi_raster(gs_batch* b, unsigned* counter) {
+    gsr::MultiRaster m;
+    m.V = b->V;
+    m.N = b->N;
+    m.total_tiles = b->total_tiles;
+    m.views = static_cast<const int4*>(b->views_dev.p);
+    m.pix_off = b->pixoff_dev.i();
+    m.means2D = reinterpret_cast<const float2*>(b->means2D.p);
+    m.conics = b->conics.f();
+    m.opac = b->op2d.f();
+    m.rgb = b->rgb.f();
+    m.values = b->words.u();
+    m.vstride = 2;
+    m.ranges = b->ranges.u();
+    m.bg0 = b->cfg.bg_color[0];
+    m.bg1 = b->cfg.bg_color[1];
+    m.bg2 = b->cfg.bg_color[2];
+    m.counter = counter;
+    return m;
+}
+
+static int enqueue_forward(gs_batch* b, hipStream_t s) {
+    B_TRY(enqueue_binning(b, s));
+    unsigned* ctr = b->persistent ? static_cast<unsigned*>(b->counters.p) : nullptr;
+    if (ctr) B_HIP(launch_zero(ctr, 4, s), "zero counter");
+    gsr::MultiRaster m = multi_raster(b, ctr);
+    gsr::MultiPix px{b->image.f(), b->T_final.f(), b->n_contrib.i(), b->last.u(), b->d_image.f()};
+    B_HIP(launch_raster_fwd_multi(m, px, b->raster_batch, b->persistent_groups, s), "multi-view raster forward");
+    return GS_OK;
+}
+
+static int enqueue_step(gs_batch* b, hipStream_t s) {
+    gs_trainer* t = b->tr;
+    const size_t VN = (size_t)b->V * b->N;
+    B_TRY(enqueue_forward(b, s));
+    B_HIP(launch_l1_multi(b->image.f(), b->gt.f(), b->d_image.f(), b->pixoff_dev.i(), b->V, b->l1_nb,
+                          static_cast<double*>(b->l1_partial.p), static_cast<double*>(b->losses.p), s),
+          "multi-view l1");
+    B_HIP(launch_zero(b->rg_means2D.p, VN * 8, s), "zero");
+    B_HIP(launch_zero(b->rg_conics.p, VN * 12, s), "zero");
+    B_HIP(launch_zero(b->rg_rgb.p, VN * 12, s), "zero");
+    B_HIP(launch_zero(b->rg_op.p, VN * 4, s), "zero");
+    unsigned* ctr = b->persistent ? static_cast<unsigned*>(b->counters.p) + 16 : nullptr;
+    if (ctr) B_HIP(launch_zero(ctr, 4, s), "zero counter");
+    gsr::MultiRaster m = multi_raster(b, ctr);
+    gsr::MultiPix px{b->image.f(), b->T_final.f(), b->n_contrib.i(), b->last.u(), b->d_image.f()};
+    gs_raster_grads rg = b->rgrads();
+    gsr::MultiGrads mg{reinterpret_cast<float2*>(rg.d_means2D), rg.d_conics, rg.d_rgb, rg.d_opacities_2d};
+    B_HIP(launch_raster_bwd_multi(m, px, mg, b->raster_bwd_batch, b->persistent_groups, s),
+          "multi-view raster backward");
+    gs_gaussians g = t->gaussians();
+    gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
+    float* P[5];
+    float* M[5];
+    float* Vv[5];
+    for (int k = 0; k < 5; k++) {
+        P[k] = t->param[k].f();
+        M[k] = t->adam_m[k].f();
+        Vv[k] = t->adam_v[k].f();
+    }
+    const gso::AdamDev* hp = static_cast<const gso::AdamDev*>(b->hp_dev.p);
+    if (b->fused_adam) {
+        B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), b->V, b->radii.i(), rg, pg,
+                                          P, M, Vv, hp, t->sh_degree, s),
+              "multi-view preprocess backward + adam");
+    } else {
+        B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), b->V, b->radii.i(), rg, pg,
+                                          nullptr, nullptr, nullptr, nullptr, t->sh_degree, s),
+              "multi-view preprocess backward");
+        gso::AdamArgs a{};
+        const uint32_t sizes[5] = {(uint32_t)t->n * 3u, (uint32_t)t->n * 3u, (uint32_t)t->n * 4u,
+                                   (uint32_t)t->n * (uint32_t)t->max_coeffs * 3u, (uint32_t)t->n};
+        for (int k = 0; k < 5; k++) a.seg[k] = gso::AdamSeg{P[k], pg.d_positions, M[k], Vv[k], sizes[k], 0.0f};
+        a.seg[0].g = pg.d_positions;
+        a.seg[1].g = pg.d_scales;
+        a.seg[2].g = pg.d_rotations;
+        a.seg[3].g = pg.d_sh_coeffs;
+        a.seg[4].g = pg.d_opacities;
+        a.hp = hp;
+        B_HIP(launch_adam(a, s), "adam");
+    }
+    return GS_OK;
+}
+
+// Sizes the pair buffers with one binning pass (eager, synchronous).
+static int batch_measure(gs_batch* b, long long* pairs) {
+    if (b->capacity == 0) B_TRY(batch_size_pairs(b, 1024));
+    B_TRY(enqueue_binning(b, b->stream));
+    unsigned long long K = 0;
+    B_HIP(hipMemcpyAsync(&K, b->total64(), 8, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+    B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (K > 0xFFFFFFFFull) return gs_set_error(GS_ERR_CAPACITY, "more than 2^32-1 pairs");
+    *pairs = (long long)K;
+    if (K > b->capacity) {
+        unsigned long long cap = K + K / 4 + 65536;
+        if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
+        B_TRY(batch_size_pairs(b, (uint32_t)cap));
+        B_HIP(launch_zero(b->overflow(), 4, b->stream), "zero overflow");
+    }
+    return GS_OK;
+}
+
+static int batch_upload_hparams(gs_batch* b, const gs_train_cfg* tc, int iteration) {
+    gs_trainer* t = b->tr;
+    gso::AdamDev h{};
+    h.lr[0] = gs_lr_schedule(tc->lr_position_init, tc->lr_position_final, iteration, tc->max_steps);
+    h.lr[1] = tc->lr_scaling;
+    h.lr[2] = tc->lr_rotation;
+    h.lr[3] = tc->lr_feature;
+    h.lr[4] = tc->lr_opacity;
+    h.b1 = tc->adam_beta1;
+    h.b2 = tc->adam_beta2;
+    h.eps = tc->adam_eps;
+    t->adam_steps++;
+    gs_adam_bias_corrections(h.b1, h.b2, t->adam_steps, &h.bc1, &h.bc2);
+    // pageable source: the runtime stages it before returning, so `h` may die
+    B_HIP(hipMemcpyAsync(b->hp_dev.p, &h, sizeof(h), hipMemcpyHostToDevice, b->stream), "hipMemcpyAsync(hparams)");
+    return GS_OK;
+}
+
+// Reads the losses / pair count / overflow flag of the last step (synchronous).
+static int batch_result(gs_batch* b, gs_step_result* out, int iteration) {
+    std::vector<double> L(b->V);
+    unsigned long long K = 0;
+    uint32_t ovf = 0;
+    B_HIP(hipMemcpyAsync(L.data(), b->losses.p, 8 * b->V, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+    B_HIP(hipMemcpyAsync(&K, b->total64(), 8, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+    B_HIP(hipMemcpyAsync(&ovf, b->overflow(), 4, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+    B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    double s = 0.0;
+    for (double x : L) s += x;
+    if (out) {
+        out->loss = s / b->V;
+        out->iteration = iteration;
+        out->n_gaussians = b->N;
+        out->pairs = (long long)K;
+    }
+    if (ovf) {
+        // some step since the last check dropped pairs: grow for the next one
+        B_HIP(launch_zero(b->overflow(), 4, b->stream), "zero overflow");
+        unsigned long long cap = K + K / 4 + 65536;
+        if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
+        B_TRY(batch_size_pairs(b, (uint32_t)cap));
+        return gs_set_error(GS_ERR_CAPACITY, "a batched step had %llu pairs for a capacity of %u; buffers regrown", K,
+                            b->capacity);
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_batch_step(gs_batch* b, const gs_train_cfg* tc, int iteration, gs_step_result* out) {
+    if (!b || !tc) return gs_set_error(GS_ERR_INVALID, "gs_batch_step: null argument");
+    if (!tc->use_adam) return gs_set_error(GS_ERR_INVALID, "gs_batch_step: the batched step uses Adam");
+    gs_ctx* c = b->tr->ctx;
+    B_HIP(hipEventRecord(b->ev_in, c->stream), "hipEventRecord");
+    B_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0), "hipStreamWaitEvent");
+    if (b->capacity == 0) {
+        long long K;
+        B_TRY(batch_measure(b, &K));
+    }
+    B_TRY(batch_upload_hparams(b, tc, iteration));
+    if (b->use_graph) {
+        if (!b->graph_ready) {
+            B_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            int st = enqueue_step(b, b->stream);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            if (st != GS_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return st;
+            }
+            if (e != hipSuccess) return gs_hip_status(e, "hipStreamEndCapture");
+            b->graph.graph = graph;
+            B_HIP(hipGraphInstantiate(&b->graph.exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            b->graph_ready = true;
+        }
+        B_HIP(hipGraphLaunch(b->graph.exec, b->stream), "hipGraphLaunch");
+    } else {
+        B_TRY(enqueue_step(b, b->stream));
+    }
+    B_HIP(hipEventRecord(b->ev_out, b->stream), "hipEventRecord");
+    B_HIP(hipStreamWaitEvent(c->stream, b->ev_out, 0), "hipStreamWaitEvent");
+    if (out) return batch_result(b, out, iteration);
+    return GS_OK;
+}
+
+// Forward only for all views (the batched render).
+extern "C" int gs_batch_render(gs_batch* b, long long* pairs) {
+    if (!b) return gs_set_error(GS_ERR_INVALID, "null batch");
+    gs_ctx* c = b->tr->ctx;
+    B_HIP(hipEventRecord(b->ev_in, c->stream), "hipEventRecord");
+    B_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0), "hipStreamWaitEvent");
+    if (b->capacity == 0) {
+        long long K;
+        B_TRY(batch_measure(b, &K));
+    }
+    B_TRY(enqueue_forward(b, b->stream));
+    B_HIP(hipEventRecord(b->ev_out, b->stream), "hipEventRecord");
+    B_HIP(hipStreamWaitEvent(c->stream, b->ev_out, 0), "hipStreamWaitEvent");
+    if (pairs) {
+        unsigned long long K = 0;
+        B_HIP(hipMemcpyAsync(&K, b->total64(), 8, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+        B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+        *pairs = (long long)K;
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_batch_result(gs_batch* b, gs_step_result* out, double* view_losses) {
+    if (!b) return gs_set_error(GS_ERR_INVALID, "null batch");
+    if (view_losses) {
+        B_HIP(hipMemcpyAsync(view_losses, b->losses.p, 8 * b->V, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+    }
+    return batch_result(b, out, 0);
+}
+
+extern "C" void* gs_batch_buffer(gs_batch* b, const char* which) {
+    if (!b || !which) return nullptr;
+    std::string w(which);
+    if (w == "image") return b->image.p;
+    if (w == "d_image") return b->d_image.p;
+    if (w == "target") return b->gt.p;
+    if (w == "T_final") return b->T_final.p;
+    if (w == "n_contrib") return b->n_contrib.p;
+    if (w == "last_contrib") return b->last.p;
+    if (w == "means2D") return b->means2D.p;
+    if (w == "radii") return b->radii.p;
+    if (w == "ranges") return b->ranges.p;
+    if (w == "pair_words") return b->words.p;
+    if (w == "losses") return b->losses.p;
+    if (w == "rg_means2D") return b->rg_means2D.p;
+    if (w == "rg_conics") return b->rg_conics.p;
+    if (w == "rg_rgb") return b->rg_rgb.p;
+    if (w == "rg_op") return b->rg_op.p;
+    return nullptr;
+}
+
+extern "C" int gs_batch_info(gs_batch* b, int* total_tiles, long long* total_pixels, long long* capacity) {
+    if (!b) return gs_set_error(GS_ERR_INVALID, "null batch");
+    if (total_tiles) *total_tiles = b->total_tiles;
+    if (total_pixels) *total_pixels = b->total_pix;
+    if (capacity) *capacity = b->capacity;
+    return GS_OK;
+}

@@ -527,6 +527,7 @@ extern "C" int gs_adam_step(gs_ctx* c, int n, int max_coeffs, float* const param
     a.eps = hp->eps;
     a.bc1 = hp->bc1;
     a.bc2 = hp->bc2;
+    a.hp = nullptr;
     GS_HIP(launch_adam(a, c->stream), "adam kernel");
     return GS_OK;
 }

@@ -242,7 +242,44 @@ struct GradArgs {
 };
 }  // namespace gsr
 
+namespace gsr {
+// A batch of views rasterized in one launch over a global tile list.
+struct MultiRaster {
+    int V, N, total_tiles;
+    const int4* views;     // per view: (W, H, tiles per row, first global tile)
+    const int* pix_off;    // per view: first pixel in the concatenated images (V + 1 entries)
+    const float2* means2D; // view-major [V][N]
+    const float* conics;   // [V][N*3]
+    const float* opac;     // [V][N]
+    const float* rgb;      // [V][N*3]
+    const uint32_t* values;
+    int vstride;
+    const uint32_t* ranges;  // [total_tiles * 2]
+    float bg0, bg1, bg2;
+    unsigned int* counter;  // persistent work queue (nullptr: one group per tile)
+};
+struct MultiPix {
+    float* img;
+    float* T_final;
+    int* n_contrib;
+    uint32_t* last_contrib;
+    const float* d_image;
+};
+struct MultiGrads {  // view-major [V][N]
+    float2* d_means2D;
+    float* d_conics;
+    float* d_rgb;
+    float* d_op;
+};
+}  // namespace gsr
+
 namespace gso {
+// Adam hyper-parameters read from device memory (updated per step outside a
+// captured graph).
+struct AdamDev {
+    float lr[5];
+    float b1, b2, eps, bc1, bc2;
+};
 struct AdamSeg {
     float* p;
     const float* g;
@@ -254,5 +291,6 @@ struct AdamSeg {
 struct AdamArgs {
     AdamSeg seg[5];
     float b1, b2, eps, bc1, bc2;
+    const AdamDev* hp;  // when set, learning rates and the rest come from device memory
 };
 }  // namespace gso

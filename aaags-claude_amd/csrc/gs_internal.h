@@ -10,8 +10,8 @@
 #include "gs_hip.h"
 
 namespace gsd { struct CamArgs; }
-namespace gsr { struct RasterArgs; struct GradArgs; }
-namespace gso { struct AdamArgs; }
+namespace gsr { struct RasterArgs; struct GradArgs; struct MultiRaster; struct MultiPix; struct MultiGrads; }
+namespace gso { struct AdamArgs; struct AdamDev; }
 
 // Small fixed scratch owned by each context.
 constexpr size_t GS_SMALL_BYTES = 4096;
@@ -83,3 +83,25 @@ hipError_t launch_l1(const float* r, const float* g, float* d, uint32_t n, doubl
 hipError_t launch_adam(const gso::AdamArgs& a, hipStream_t s);
 hipError_t launch_sgd(const gso::AdamArgs& a, hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
+
+// multi-view
+hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev, int V,
+                                       const gs_preprocessed& out, uint64_t* dkeys, uint32_t* didx, int deg,
+                                       hipStream_t s);
+hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev, int V, const int* radii,
+                                       const gs_raster_grads& rg, const gs_param_grads& out, float* const adam_p[5],
+                                       float* const adam_m[5], float* const adam_v[5], const gso::AdamDev* hp,
+                                       int sh_deg, hipStream_t s);
+hipError_t launch_gather_tiles_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
+                                     const int* tiles, int* out, hipStream_t s);
+hipError_t launch_duplicate_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
+                                  const float* means2D, const int* radii, const uint32_t* offsets, const int4* views,
+                                  uint64_t* words, uint32_t capacity, uint32_t* overflow, hipStream_t s);
+hipError_t launch_tile_ranges_multi(const uint64_t* words, uint32_t n_cap, const uint32_t* n_dev, const int4* views,
+                                    uint32_t* ranges, hipStream_t s);
+hipError_t launch_raster_fwd_multi(const gsr::MultiRaster& m, const gsr::MultiPix& o, int batch,
+                                   int persistent_groups, hipStream_t s);
+hipError_t launch_raster_bwd_multi(const gsr::MultiRaster& m, const gsr::MultiPix& o, const gsr::MultiGrads& g,
+                                   int batch, int persistent_groups, hipStream_t s);
+hipError_t launch_l1_multi(const float* r, const float* g, float* d, const int* pix_off_dev, int V, int nb,
+                           double* partial, double* loss, hipStream_t s);

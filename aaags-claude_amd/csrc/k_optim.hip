@@ -50,6 +50,45 @@ __global__ __launch_bounds__(L1_THREADS) void l1_kernel(const float* __restrict_
     }
 }
 
+// One block row per view (blockIdx.y); each view's loss is its own mean.
+__global__ __launch_bounds__(L1_THREADS) void l1_multi_kernel(const float* __restrict__ r, const float* __restrict__ g,
+                                                              float* __restrict__ d, const int* __restrict__ pix_off,
+                                                              double* __restrict__ partial, int nb) {
+    __shared__ double s_w[L1_THREADS / 64];
+    const int v = blockIdx.y;
+    const size_t e0 = (size_t)pix_off[v] * 3, e1 = (size_t)pix_off[v + 1] * 3;
+    const uint32_t n = (uint32_t)(e1 - e0);
+    const float inv_n = 1.0f / (float)n;
+    double s = 0.0;
+    for (size_t e = e0 + blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < e1; e += (size_t)gridDim.x * blockDim.x) {
+        float df = r[e] - g[e];
+        s += (double)fabsf(df);
+        d[e] = l1_grad(df, inv_n);
+    }
+    s = wave_sum_d(s);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < L1_THREADS / 64; k++) t += s_w[k];
+        partial[(size_t)v * nb + blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(256) void l1_multi_finish_kernel(const double* __restrict__ partial, int nb,
+                                                              const int* __restrict__ pix_off,
+                                                              double* __restrict__ loss) {
+    __shared__ double s_w[4];
+    const int v = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += partial[(size_t)v * nb + i];
+    s = wave_sum_d(s);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        loss[v] = (s_w[0] + s_w[1] + s_w[2] + s_w[3]) / (double)((size_t)(pix_off[v + 1] - pix_off[v]) * 3);
+}
+
 __global__ __launch_bounds__(256) void l1_finish_kernel(const double* __restrict__ partial, int nb, uint32_t n,
                                                         double* __restrict__ loss) {
     __shared__ double s_w[4];
@@ -62,7 +101,11 @@ __global__ __launch_bounds__(256) void l1_finish_kernel(const double* __restrict
 }
 
 // optimizer.cpp:32-43, same operation order (no fused multiply-add).
-__device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, float lr, const AdamArgs& a) {
+struct AdamH {
+    float b1, b2, eps, bc1, bc2;
+};
+
+__device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, float lr, const AdamH& a) {
     m = a.b1 * m + (1.0f - a.b1) * g;
     v = a.b2 * v + (1.0f - a.b2) * g * g;
     float mh = m * a.bc1;
@@ -70,8 +113,15 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, flo
     p -= lr * mh / (sqrtf(vh) + a.eps);
 }
 
-__global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
-    const AdamSeg& s = a.seg[blockIdx.y];
+__global__ __launch_bounds__(256) void adam_kernel(AdamArgs args) {
+    const AdamSeg& s = args.seg[blockIdx.y];
+    AdamH a{args.b1, args.b2, args.eps, args.bc1, args.bc2};
+    float lr = s.lr;
+    if (args.hp) {
+        const AdamDev h = *args.hp;
+        a = AdamH{h.b1, h.b2, h.eps, h.bc1, h.bc2};
+        lr = h.lr[blockIdx.y];
+    }
     const uint32_t n = s.n;
     const uint32_t n4 = n / 4;
     float4* p4 = reinterpret_cast<float4*>(s.p);
@@ -80,10 +130,10 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
     float4* v4 = reinterpret_cast<float4*>(s.v);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
         float4 p = p4[i], g = g4[i], m = m4[i], v = v4[i];
-        adam1(p.x, g.x, m.x, v.x, s.lr, a);
-        adam1(p.y, g.y, m.y, v.y, s.lr, a);
-        adam1(p.z, g.z, m.z, v.z, s.lr, a);
-        adam1(p.w, g.w, m.w, v.w, s.lr, a);
+        adam1(p.x, g.x, m.x, v.x, lr, a);
+        adam1(p.y, g.y, m.y, v.y, lr, a);
+        adam1(p.z, g.z, m.z, v.z, lr, a);
+        adam1(p.w, g.w, m.w, v.w, lr, a);
         p4[i] = p;
         m4[i] = m;
         v4[i] = v;
@@ -91,7 +141,7 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs a) {
     if (blockIdx.x == 0) {
         for (uint32_t i = n4 * 4 + threadIdx.x; i < n; i += blockDim.x) {
             float p = s.p[i], m = s.m[i], v = s.v[i];
-            adam1(p, s.g[i], m, v, s.lr, a);
+            adam1(p, s.g[i], m, v, lr, a);
             s.p[i] = p;
             s.m[i] = m;
             s.v[i] = v;
@@ -132,6 +182,14 @@ hipError_t launch_l1(const float* r, const float* g, float* d, uint32_t n, doubl
     float inv_n = 1.0f / (float)n;
     l1_kernel<<<nb, L1_THREADS, 0, s>>>(r, g, d, n, inv_n, partial);
     l1_finish_kernel<<<1, 256, 0, s>>>(partial, nb, n, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_l1_multi(const float* r, const float* g, float* d, const int* pix_off_dev, int V, int nb,
+                           double* partial, double* loss, hipStream_t s) {
+    if (V <= 0) return hipSuccess;
+    l1_multi_kernel<<<dim3(nb, V), L1_THREADS, 0, s>>>(r, g, d, pix_off_dev, partial, nb);
+    l1_multi_finish_kernel<<<V, 256, 0, s>>>(partial, nb, pix_off_dev, loss);
     return hipGetLastError();
 }
 

@@ -14,6 +14,10 @@
 // nine per-Gaussian gradient terms are summed across the wave with lane
 // shuffles, across the four waves in LDS, and land in HBM with one float
 // atomic per (tile, Gaussian, component).
+//
+// Multi-view: the same tile bodies run over a global (view, tile) list, either
+// one workgroup per global tile or a persistent grid that takes tiles from an
+// atomic counter.
 #include "gs_device.h"
 
 namespace gsr {
@@ -21,13 +25,33 @@ namespace gsr {
 using gsd::smin;
 
 template <int BATCH>
-__global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, float* __restrict__ img,
-                                                         float* __restrict__ T_final, int* __restrict__ n_contrib,
-                                                         uint32_t* __restrict__ last_contrib) {
-    __shared__ float2 s_xy[BATCH];
-    __shared__ float4 s_co[BATCH];
-    __shared__ float4 s_rgb[BATCH];
-    const int tile = blockIdx.x;
+struct FwdSmem {
+    float2 xy[BATCH];
+    float4 co[BATCH];
+    float4 rgb[BATCH];
+};
+
+constexpr int NG = 9;  // gradient terms per Gaussian: means2D 2, conic 3, rgb 3, opacity 1
+
+template <int BATCH>
+struct BwdSmem {
+    float2 xy[BATCH];
+    float4 co[BATCH];
+    float4 rgb[BATCH];
+    uint32_t id[BATCH];
+    float g[BATCH][NG];
+    uint32_t maxlast;
+};
+
+struct PixOut {
+    float* img;
+    float* T_final;
+    int* n_contrib;
+    uint32_t* last_contrib;
+};
+
+template <int BATCH>
+__device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<BATCH>& sm, const PixOut& o) {
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int t = threadIdx.x;
     const int px = tx * gsd::TILE + (t & 15), py = ty * gsd::TILE + (t >> 4);
@@ -44,17 +68,17 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, float* __
             uint32_t j = base + k;
             if (j < end) {
                 uint32_t id = a.values[(size_t)j * a.vstride];
-                s_xy[k] = a.means2D[id];
-                s_co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
-                s_rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
+                sm.xy[k] = a.means2D[id];
+                sm.co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
+                sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
             }
         }
         __syncthreads();
         const int n = (int)((end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH);
         if (!done) {
             for (int k = 0; k < n; k++) {
-                const float2 m = s_xy[k];
-                const float4 co = s_co[k];
+                const float2 m = sm.xy[k];
+                const float4 co = sm.co[k];
                 const float dx = m.x - fpx, dy = m.y - fpy;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
                 if (power > 0.0f) continue;
@@ -65,7 +89,7 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, float* __
                     done = true;
                     break;
                 }
-                const float4 c = s_rgb[k];
+                const float4 c = sm.rgb[k];
                 C0 += c.x * alpha * T;
                 C1 += c.y * alpha * T;
                 C2 += c.z * alpha * T;
@@ -77,12 +101,12 @@ __global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, float* __
     }
     if (inside) {
         const int pix = py * a.W + px;
-        img[pix * 3 + 0] = C0 + T * a.bg0;
-        img[pix * 3 + 1] = C1 + T * a.bg1;
-        img[pix * 3 + 2] = C2 + T * a.bg2;
-        if (T_final) T_final[pix] = T;
-        if (n_contrib) n_contrib[pix] = cnt;
-        if (last_contrib) last_contrib[pix] = last;
+        o.img[pix * 3 + 0] = C0 + T * a.bg0;
+        o.img[pix * 3 + 1] = C1 + T * a.bg1;
+        o.img[pix * 3 + 2] = C2 + T * a.bg2;
+        if (o.T_final) o.T_final[pix] = T;
+        if (o.n_contrib) o.n_contrib[pix] = cnt;
+        if (o.last_contrib) o.last_contrib[pix] = last;
     }
 }
 
@@ -92,20 +116,15 @@ __device__ __forceinline__ float wave_sum_f(float v) {
     return v;
 }
 
-constexpr int NG = 9;  // gradient terms per Gaussian: means2D 2, conic 3, rgb 3, opacity 1
+struct PixIn {
+    const float* T_final;
+    const uint32_t* last_contrib;
+    const float* d_image;
+};
 
 template <int BATCH>
-__global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const float* __restrict__ T_final,
-                                                         const uint32_t* __restrict__ last_contrib,
-                                                         const float* __restrict__ d_image, GradArgs g) {
-    __shared__ float2 s_xy[BATCH];
-    __shared__ float4 s_co[BATCH];
-    __shared__ float4 s_rgb[BATCH];
-    __shared__ uint32_t s_id[BATCH];
-    __shared__ float s_g[BATCH][NG];
-    __shared__ uint32_t s_maxlast;
-
-    const int tile = blockIdx.x;
+__device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<BATCH>& sm, const PixIn& in,
+                                         const GradArgs& g) {
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -116,19 +135,19 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const flo
     float T = 0.0f, dC0 = 0.0f, dC1 = 0.0f, dC2 = 0.0f;
     uint32_t last = 0;
     if (inside) {
-        T = T_final[pix];
-        last = last_contrib[pix];
-        dC0 = d_image[pix * 3];
-        dC1 = d_image[pix * 3 + 1];
-        dC2 = d_image[pix * 3 + 2];
+        T = in.T_final[pix];
+        last = in.last_contrib[pix];
+        dC0 = in.d_image[pix * 3];
+        dC1 = in.d_image[pix * 3 + 1];
+        dC2 = in.d_image[pix * 3 + 2];
     }
     float acc0 = a.bg0, acc1 = a.bg1, acc2 = a.bg2;
     const float fpx = (float)px, fpy = (float)py;
-    if (t == 0) s_maxlast = 0;
+    if (t == 0) sm.maxlast = 0;
     __syncthreads();
-    if (last > 0) atomicMax(&s_maxlast, last);
+    if (last > 0) atomicMax(&sm.maxlast, last);
     __syncthreads();
-    const uint32_t stop = s_maxlast;
+    const uint32_t stop = sm.maxlast;
 
     for (uint32_t hi = stop; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)BATCH) ? hi - BATCH : start;
@@ -137,13 +156,13 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const flo
         for (int k = t; k < BATCH; k += 256) {
             if (k < n) {
                 uint32_t id = a.values[(size_t)(lo + k) * a.vstride];
-                s_id[k] = id;
-                s_xy[k] = a.means2D[id];
-                s_co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
-                s_rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
+                sm.id[k] = id;
+                sm.xy[k] = a.means2D[id];
+                sm.co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
+                sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
             }
 #pragma unroll
-            for (int c = 0; c < NG; c++) s_g[k][c] = 0.0f;
+            for (int c = 0; c < NG; c++) sm.g[k][c] = 0.0f;
         }
         __syncthreads();
         for (int k = n - 1; k >= 0; k--) {
@@ -153,8 +172,8 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const flo
             for (int c = 0; c < NG; c++) gv[c] = 0.0f;
             bool contrib = false;
             if (j < last) {
-                const float2 m = s_xy[k];
-                const float4 co = s_co[k];
+                const float2 m = sm.xy[k];
+                const float4 co = sm.co[k];
                 const float dx = m.x - fpx, dy = m.y - fpy;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
                 if (power <= 0.0f) {
@@ -162,7 +181,7 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const flo
                     const float alpha = smin(0.99f, co.w * ep);
                     if (alpha >= 1.0f / 255.0f) {
                         contrib = true;
-                        const float4 cl = s_rgb[k];
+                        const float4 cl = sm.rgb[k];
                         const float oma = 1.0f - alpha;
                         const float Ti = (oma > 1e-6f) ? T / oma : 0.0f;
                         gv[5] = alpha * Ti * dC0;
@@ -198,25 +217,110 @@ __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, const flo
 #pragma unroll
                     for (int c = 1; c < NG; c++)
                         if (lane == c) v = gv[c];
-                    if (v != 0.0f) atomicAdd(&s_g[k][lane], v);
+                    if (v != 0.0f) atomicAdd(&sm.g[k][lane], v);
                 }
             }
         }
         __syncthreads();
         for (int k = t; k < n; k += 256) {
-            const uint32_t id = s_id[k];
+            const uint32_t id = sm.id[k];
             float v;
-            v = s_g[k][0]; if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].x, v);
-            v = s_g[k][1]; if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].y, v);
-            v = s_g[k][2]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 0], v);
-            v = s_g[k][3]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 1], v);
-            v = s_g[k][4]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 2], v);
-            v = s_g[k][5]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 0], v);
-            v = s_g[k][6]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 1], v);
-            v = s_g[k][7]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 2], v);
-            v = s_g[k][8]; if (v != 0.0f) unsafeAtomicAdd(&g.d_op[id], v);
+            v = sm.g[k][0]; if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].x, v);
+            v = sm.g[k][1]; if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].y, v);
+            v = sm.g[k][2]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 0], v);
+            v = sm.g[k][3]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 1], v);
+            v = sm.g[k][4]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 2], v);
+            v = sm.g[k][5]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 0], v);
+            v = sm.g[k][6]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 1], v);
+            v = sm.g[k][7]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 2], v);
+            v = sm.g[k][8]; if (v != 0.0f) unsafeAtomicAdd(&g.d_op[id], v);
         }
         hi = lo;
+    }
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, PixOut o) {
+    __shared__ FwdSmem<BATCH> sm;
+    fwd_tile<BATCH>(a, blockIdx.x, sm, o);
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, PixIn in, GradArgs g) {
+    __shared__ BwdSmem<BATCH> sm;
+    bwd_tile<BATCH>(a, blockIdx.x, sm, in, g);
+}
+
+// ------------------------------------------------------------ multi-view ----
+// View of a global tile (views are few; every thread of the group scans the
+// same table, so the loop is uniform).
+__device__ __forceinline__ int view_of(const MultiRaster& m, int gtile) {
+    int v = 0;
+    while (v + 1 < m.V && m.views[v + 1].w <= gtile) v++;
+    return v;
+}
+
+__device__ __forceinline__ RasterArgs view_args(const MultiRaster& m, int v) {
+    const int4 vi = m.views[v];  // (W, H, gx, first global tile)
+    RasterArgs a;
+    a.W = vi.x;
+    a.H = vi.y;
+    a.gx = vi.z;
+    a.means2D = m.means2D + (size_t)v * m.N;
+    a.conics = m.conics + (size_t)v * m.N * 3;
+    a.opac = m.opac + (size_t)v * m.N;
+    a.rgb = m.rgb + (size_t)v * m.N * 3;
+    a.values = m.values;
+    a.vstride = m.vstride;
+    a.ranges = m.ranges + 2 * (size_t)vi.w;
+    a.bg0 = m.bg0;
+    a.bg1 = m.bg1;
+    a.bg2 = m.bg2;
+    return a;
+}
+
+// Next global tile: from the atomic queue (persistent grid) or blockIdx.x.
+// Every group leaves once the counter passes the tile count, which it must,
+// since each pass through the loop takes a fresh counter value.
+__device__ __forceinline__ bool next_tile(const MultiRaster& m, int& g, int& iter, int* s_next) {
+    if (m.counter == nullptr) {
+        g = blockIdx.x;
+        return iter++ == 0 && g < m.total_tiles;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *s_next = (int)atomicAdd(m.counter, 1u);
+    __syncthreads();
+    g = *s_next;
+    return g < m.total_tiles;
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(256) void raster_fwd_multi_kernel(MultiRaster m, MultiPix o) {
+    __shared__ FwdSmem<BATCH> sm;
+    __shared__ int s_next;
+    int g, iter = 0;
+    while (next_tile(m, g, iter, &s_next)) {
+        const int v = view_of(m, g);
+        RasterArgs a = view_args(m, v);
+        const size_t p0 = (size_t)m.pix_off[v];
+        PixOut po{o.img + p0 * 3, o.T_final + p0, o.n_contrib + p0, o.last_contrib + p0};
+        fwd_tile<BATCH>(a, g - m.views[v].w, sm, po);
+    }
+}
+
+template <int BATCH>
+__global__ __launch_bounds__(256) void raster_bwd_multi_kernel(MultiRaster m, MultiPix o, MultiGrads gr) {
+    __shared__ BwdSmem<BATCH> sm;
+    __shared__ int s_next;
+    int g, iter = 0;
+    while (next_tile(m, g, iter, &s_next)) {
+        const int v = view_of(m, g);
+        RasterArgs a = view_args(m, v);
+        const size_t p0 = (size_t)m.pix_off[v];
+        PixIn pi{o.T_final + p0, o.last_contrib + p0, o.d_image + p0 * 3};
+        const size_t n0 = (size_t)v * m.N;
+        GradArgs ga{gr.d_means2D + n0, gr.d_conics + n0 * 3, gr.d_rgb + n0 * 3, gr.d_op + n0};
+        bwd_tile<BATCH>(a, g - m.views[v].w, sm, pi, ga);
     }
 }
 
@@ -227,11 +331,12 @@ using namespace gsr;
 hipError_t launch_raster_fwd(const RasterArgs& a, int num_tiles, int batch, float* img, float* T_final, int* n_contrib,
                              uint32_t* last_contrib, hipStream_t s) {
     if (num_tiles <= 0) return hipSuccess;
+    PixOut o{img, T_final, n_contrib, last_contrib};
     switch (batch) {
-        case 64: raster_fwd_kernel<64><<<num_tiles, 256, 0, s>>>(a, img, T_final, n_contrib, last_contrib); break;
-        case 128: raster_fwd_kernel<128><<<num_tiles, 256, 0, s>>>(a, img, T_final, n_contrib, last_contrib); break;
-        case 256: raster_fwd_kernel<256><<<num_tiles, 256, 0, s>>>(a, img, T_final, n_contrib, last_contrib); break;
-        case 512: raster_fwd_kernel<512><<<num_tiles, 256, 0, s>>>(a, img, T_final, n_contrib, last_contrib); break;
+        case 64: raster_fwd_kernel<64><<<num_tiles, 256, 0, s>>>(a, o); break;
+        case 128: raster_fwd_kernel<128><<<num_tiles, 256, 0, s>>>(a, o); break;
+        case 256: raster_fwd_kernel<256><<<num_tiles, 256, 0, s>>>(a, o); break;
+        case 512: raster_fwd_kernel<512><<<num_tiles, 256, 0, s>>>(a, o); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -240,10 +345,40 @@ hipError_t launch_raster_fwd(const RasterArgs& a, int num_tiles, int batch, floa
 hipError_t launch_raster_bwd(const RasterArgs& a, int num_tiles, int batch, const float* T_final,
                              const uint32_t* last_contrib, const float* d_image, const GradArgs& g, hipStream_t s) {
     if (num_tiles <= 0) return hipSuccess;
+    PixIn in{T_final, last_contrib, d_image};
     switch (batch) {
-        case 64: raster_bwd_kernel<64><<<num_tiles, 256, 0, s>>>(a, T_final, last_contrib, d_image, g); break;
-        case 128: raster_bwd_kernel<128><<<num_tiles, 256, 0, s>>>(a, T_final, last_contrib, d_image, g); break;
-        case 256: raster_bwd_kernel<256><<<num_tiles, 256, 0, s>>>(a, T_final, last_contrib, d_image, g); break;
+        case 64: raster_bwd_kernel<64><<<num_tiles, 256, 0, s>>>(a, in, g); break;
+        case 128: raster_bwd_kernel<128><<<num_tiles, 256, 0, s>>>(a, in, g); break;
+        case 256: raster_bwd_kernel<256><<<num_tiles, 256, 0, s>>>(a, in, g); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// grid: one group per global tile, or `persistent_groups` groups pulling
+// tiles from m.counter (which must be zero at launch).
+hipError_t launch_raster_fwd_multi(const MultiRaster& m, const MultiPix& o, int batch, int persistent_groups,
+                                   hipStream_t s) {
+    if (m.total_tiles <= 0) return hipSuccess;
+    int grid = m.counter ? persistent_groups : m.total_tiles;
+    switch (batch) {
+        case 64: raster_fwd_multi_kernel<64><<<grid, 256, 0, s>>>(m, o); break;
+        case 128: raster_fwd_multi_kernel<128><<<grid, 256, 0, s>>>(m, o); break;
+        case 256: raster_fwd_multi_kernel<256><<<grid, 256, 0, s>>>(m, o); break;
+        case 512: raster_fwd_multi_kernel<512><<<grid, 256, 0, s>>>(m, o); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_bwd_multi(const MultiRaster& m, const MultiPix& o, const MultiGrads& g, int batch,
+                                   int persistent_groups, hipStream_t s) {
+    if (m.total_tiles <= 0) return hipSuccess;
+    int grid = m.counter ? persistent_groups : m.total_tiles;
+    switch (batch) {
+        case 64: raster_bwd_multi_kernel<64><<<grid, 256, 0, s>>>(m, o, g); break;
+        case 128: raster_bwd_multi_kernel<128><<<grid, 256, 0, s>>>(m, o, g); break;
+        case 256: raster_bwd_multi_kernel<256><<<grid, 256, 0, s>>>(m, o, g); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

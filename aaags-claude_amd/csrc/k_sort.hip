@@ -68,6 +68,16 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T* total) {
     return wbase + inc - v;
 }
 
+// n_dev (optional): the item count lives on the device (graph capture); the
+// launch is sized for n (the capacity) and the kernels use min(*n_dev, n).
+__device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* n_dev) {
+    if (n_dev) {
+        uint32_t d = *n_dev;
+        return d < n ? d : n;
+    }
+    return n;
+}
+
 // ---------------------------------------------------------------- scan ----
 // Phase 1: per-block sums of `in` (int32 or uint32 values treated as uint64).
 template <typename TIn>
@@ -230,6 +240,73 @@ __global__ __launch_bounds__(256) void duplicate_ranked_kernel(int n, const uint
         }
 }
 
+// Multi-view: the depth sort runs over keys view << 32 | depth bits, so the
+// sorted order is (view, depth, index).  Pair words carry the view too:
+// view << 48 | local tile << 32 | index; the tile sort keys on the local tile
+// bits only, which leaves each (view, tile) run contiguous and in depth order.
+__global__ __launch_bounds__(256) void gather_tiles_multi_kernel(uint32_t total, int N,
+                                                                 const uint64_t* __restrict__ skeys,
+                                                                 const uint32_t* __restrict__ sidx,
+                                                                 const int* __restrict__ tiles, int* __restrict__ out) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= total) return;
+    uint32_t v = (uint32_t)(skeys[r] >> 32);
+    out[r] = tiles[(size_t)v * N + sidx[r]];
+}
+
+__global__ __launch_bounds__(256) void duplicate_multi_kernel(uint32_t total, int N, const uint64_t* __restrict__ skeys,
+                                                              const uint32_t* __restrict__ sidx,
+                                                              const float2* __restrict__ means2D,
+                                                              const int* __restrict__ radii,
+                                                              const uint32_t* __restrict__ offsets_incl,
+                                                              const int4* __restrict__ views,
+                                                              uint64_t* __restrict__ words, uint32_t capacity,
+                                                              uint32_t* __restrict__ overflow) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= total) return;
+    uint32_t v = (uint32_t)(skeys[r] >> 32);
+    uint32_t i = sidx[r];
+    size_t o = (size_t)v * N + i;
+    int rad = radii[o];
+    if (rad <= 0) return;
+    int4 vi = views[v];
+    int gx = vi.z, gy = (vi.y + 15) / 16;
+    float2 m = means2D[o];
+    int mn[2], mx[2];
+    gsd::get_rect(m.x, m.y, rad, gx, gy, mn, mx);
+    uint32_t cnt = (uint32_t)((mx[1] - mn[1]) * (mx[0] - mn[0]));
+    uint32_t off = offsets_incl[r] - cnt;
+    const uint64_t hi = (uint64_t)v << 48;
+    for (int y = mn[1]; y < mx[1]; y++)
+        for (int x = mn[0]; x < mx[0]; x++) {
+            if (off < capacity) words[off] = hi | ((uint64_t)(uint32_t)(y * gx + x) << 32) | i;
+            else if (overflow) *overflow = 1u;
+            off++;
+        }
+}
+
+// Ranges per global tile (first global tile of the view + local tile).
+__global__ __launch_bounds__(256) void tile_ranges_multi_kernel(const uint64_t* __restrict__ words, uint32_t n_cap,
+                                                                const uint32_t* __restrict__ n_dev,
+                                                                const int4* __restrict__ views,
+                                                                uint32_t* __restrict__ ranges) {
+    const uint32_t n = live_count(n_cap, n_dev);
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint64_t w = words[j];
+    uint32_t g = (uint32_t)views[w >> 48].w + (uint32_t)((w >> 32) & 0xFFFFu);
+    if (j == 0) ranges[g * 2] = 0;
+    else {
+        uint64_t wp = words[j - 1];
+        uint32_t gp = (uint32_t)views[wp >> 48].w + (uint32_t)((wp >> 32) & 0xFFFFu);
+        if (g != gp) {
+            ranges[gp * 2 + 1] = j;
+            ranges[g * 2] = j;
+        }
+    }
+    if (j == n - 1) ranges[g * 2 + 1] = n;
+}
+
 // ---------------------------------------------------------- radix sort ----
 template <int BITS>
 __device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) {
@@ -246,16 +323,6 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
         peers &= ((d >> b) & 1u) ? bal : ~bal;
     }
     return peers;
-}
-
-// n_dev (optional): the item count lives on the device (graph capture); the
-// launch is sized for n (the capacity) and the kernels use min(*n_dev, n).
-__device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* n_dev) {
-    if (n_dev) {
-        uint32_t d = *n_dev;
-        return d < n ? d : n;
-    }
-    return n;
 }
 
 template <int BITS>
@@ -544,6 +611,30 @@ hipError_t launch_duplicate_ranked(int n, const uint32_t* order, const float* me
     if (n <= 0) return hipSuccess;
     duplicate_ranked_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, order, reinterpret_cast<const float2*>(means2D), radii,
                                                             offsets, gx, gy, words, capacity, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_tiles_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
+                                     const int* tiles, int* out, hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    gather_tiles_multi_kernel<<<(total + 255) / 256, 256, 0, s>>>(total, N, skeys, sidx, tiles, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_duplicate_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
+                                  const float* means2D, const int* radii, const uint32_t* offsets, const int4* views,
+                                  uint64_t* words, uint32_t capacity, uint32_t* overflow, hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    duplicate_multi_kernel<<<(total + 255) / 256, 256, 0, s>>>(total, N, skeys, sidx,
+                                                               reinterpret_cast<const float2*>(means2D), radii,
+                                                               offsets, views, words, capacity, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_ranges_multi(const uint64_t* words, uint32_t n_cap, const uint32_t* n_dev, const int4* views,
+                                    uint32_t* ranges, hipStream_t s) {
+    if (n_cap == 0) return hipSuccess;
+    tile_ranges_multi_kernel<<<(n_cap + 255) / 256, 256, 0, s>>>(words, n_cap, n_dev, views, ranges);
     return hipGetLastError();
 }
 

@@ -450,3 +450,83 @@ class GsTrainer:
         self.hip._check(self.lib.gs_trainer_get_params(self.tr, *[a.ctypes.data_as(_vp) for a in out], 1),
                         "gs_trainer_get_params")
         return out
+
+
+class GsBatch:
+    """Batched multi-view training step over a GsTrainer's parameters
+    (gs_batch_*).  Views are fixed for the batch's lifetime."""
+
+    def __init__(self, trainer: GsTrainer, cams, cfg, targets, persistent=True, fused_adam=True, use_graph=True):
+        self.tr = trainer
+        self.hip = trainer.hip
+        self.lib = trainer.lib
+        self.cams = list(cams)
+        self.V = len(self.cams)
+        self.b = _vp()
+        carr = (GsCamera * self.V)(*[c.to_c() for c in self.cams])
+        f = cfg.to_c()
+        self.hip._check(self.lib.gs_batch_create(trainer.tr, self.V, carr, ctypes.byref(f), ctypes.byref(self.b)),
+                        "gs_batch_create")
+        self.lib.gs_batch_buffer.restype = _vp
+        self.set_options(persistent, fused_adam, use_graph)
+        arrs = [np.ascontiguousarray(t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else t, np.float32)
+                for t in targets]
+        ptrs = (_vp * self.V)(*[a.ctypes.data_as(_vp) for a in arrs])
+        self.hip._check(self.lib.gs_batch_set_targets(self.b, ptrs, 1), "gs_batch_set_targets")
+        self.pix_off = [0]
+        for c in self.cams:
+            self.pix_off.append(self.pix_off[-1] + c.width * c.height)
+
+    def __del__(self):
+        try:
+            if self.b:
+                self.lib.gs_batch_destroy(self.b)
+        except Exception:
+            pass
+
+    def set_options(self, persistent=True, fused_adam=True, use_graph=True):
+        self.hip._check(self.lib.gs_batch_set_options(self.b, int(persistent), int(fused_adam), int(use_graph)),
+                        "gs_batch_set_options")
+
+    def step(self, tc, iteration, sync=False):
+        res = _StepResult()
+        st = self.lib.gs_batch_step(self.b, ctypes.byref(tc), iteration, ctypes.byref(res) if sync else None)
+        self.hip._check(st, "gs_batch_step")
+        return res if sync else None
+
+    def result(self):
+        res = _StepResult()
+        losses = np.zeros(self.V, np.float64)
+        self.hip._check(self.lib.gs_batch_result(self.b, ctypes.byref(res), losses.ctypes.data_as(_vp)),
+                        "gs_batch_result")
+        return res, losses
+
+    def render(self):
+        k = ctypes.c_longlong()
+        self.hip._check(self.lib.gs_batch_render(self.b, ctypes.byref(k)), "gs_batch_render")
+        return int(k.value)
+
+    def info(self):
+        t, p, c = _i32(), ctypes.c_longlong(), ctypes.c_longlong()
+        self.hip._check(self.lib.gs_batch_info(self.b, ctypes.byref(t), ctypes.byref(p), ctypes.byref(c)),
+                        "gs_batch_info")
+        return int(t.value), int(p.value), int(c.value)
+
+    def _buffer(self, name, numel, dtype):
+        p = self.lib.gs_batch_buffer(self.b, name.encode())
+        if not p:
+            raise GsError(f"no batch buffer {name}")
+        out = torch.empty(numel, dtype=dtype, device=self.hip.dev)
+        torch.cuda.synchronize(self.hip.dev)
+        hipMemcpy = ctypes.CDLL("libamdhip64.so").hipMemcpy
+        hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, _i32]
+        st = hipMemcpy(_vp(out.data_ptr()), _vp(p), numel * out.element_size(), 3)
+        if st != 0:
+            raise GsError(f"hipMemcpy failed ({st})")
+        return out
+
+    def images(self):
+        """Per-view rendered images (copies), HWC."""
+        flat = self._buffer("image", self.pix_off[-1] * 3, torch.float32)
+        return [flat[self.pix_off[v] * 3:self.pix_off[v + 1] * 3].view(c.height, c.width, 3)
+                for v, c in enumerate(self.cams)]

@@ -30,44 +30,43 @@ def log(*a):
 
 
 def stage_breakdown(hip, p, cam, cfg, gt, reps):
-    """Per-stage device time (ms) of the single-view step, HIP events on the
-    library's stream (torch's current stream)."""
+    """Per-stage device time (ms) of the single-view step on the driver's path
+    (depth-ordered binning), from HIP events recorded on the library's stream
+    (torch's current stream)."""
     import torch
 
-    names = ["preprocess", "scan", "duplicate", "sort", "raster_fwd", "l1", "zero_grads", "raster_bwd",
-             "preprocess_bwd", "adam"]
+    names = ["preprocess", "binning", "raster_fwd", "l1", "zero_grads", "raster_bwd", "preprocess_bwd", "adam"]
     acc = {k: 0.0 for k in names}
     n, kc = p["n"], p["max_coeffs"]
     m = hip.alloc_pgrads(n, kc)
     v = hip.alloc_pgrads(n, kc)
     K = 0
-    for r in range(reps):
+    cap = None
+    for r in range(reps + 1):  # the first repetition sizes the pair buffer and is not counted
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
         ev[0].record()
         pre = hip.preprocess(p, cam, cfg)
         ev[1].record()
-        offsets, K = hip.scan_tiles(pre["tiles"])  # includes the pair-count readback
+        words, ranges, K = hip.bin_sorted_words(pre, cam.width, cam.height, capacity=cap)
+        cap = max(cap or 0, int(K * 1.1) + 1024)
         ev[2].record()
-        keys, vals = hip.duplicate(pre, offsets, K, cam.width, cam.height)
+        img, px = hip.raster_fwd_words(pre, words, ranges, cam.width, cam.height, cfg.bg_color)
         ev[3].record()
-        ks, vs, ranges = hip.sort_pairs(keys, vals, cam.num_tiles)
-        ev[4].record()
-        img, px = hip.raster_fwd(pre, vs, ranges, cam.width, cam.height, cfg.bg_color)
-        ev[5].record()
         loss, d_img = hip.l1_loss(img, gt)
-        ev[6].record()
+        ev[4].record()
         rg = hip.alloc_rgrads(n)
-        ev[7].record()
-        rg = hip.raster_bwd(pre, vs, ranges, cam.width, cam.height, px, d_img, cfg.bg_color, rg=rg)
-        ev[8].record()
+        ev[5].record()
+        rg = hip.raster_bwd_words(pre, words, ranges, cam.width, cam.height, px, d_img, cfg.bg_color, rg=rg)
+        ev[6].record()
         grads = hip.preprocess_bwd(p, cam, cfg, pre["radii"], rg)
-        ev[9].record()
+        ev[7].record()
         hip.adam(p, grads, m, v, [1e-3] * 5, 0.9, 0.999, 1e-8, r + 1)
-        ev[10].record()
+        ev[8].record()
         torch.cuda.synchronize()
-        for i, k in enumerate(names):
-            acc[k] += ev[i].elapsed_time(ev[i + 1])
-        del keys, vals, ks, vs, ranges, img, px, d_img, rg, grads, pre
+        if r > 0:
+            for i, k in enumerate(names):
+                acc[k] += ev[i].elapsed_time(ev[i + 1])
+        del words, ranges, img, px, d_img, rg, grads, pre
     return {k: acc[k] / reps for k in names}, K
 
 
@@ -85,12 +84,71 @@ def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles):
         "scan": N * 8,
         "duplicate": N * 20 + K * 12,
         "sort": K * (24 * p8 + 8),
+        # depth-ordered binning: depth keys + 4 depth passes (32 B/G each) + ranked gather/scan (16 B/G),
+        # pair words written (8 B/pair), 2 tile passes (upsweep 8 + downsweep 16 B/pair), ranges (8 B/pair)
+        "binning": N * (16 + 4 * 32 + 16) + K * (8 + 2 * 24 + 8),
         "raster_fwd": K * 40 + npix * 20,
         "l1": npix * 36,
         "raster_bwd": K * 40 + npix * 20 + N * 72,
         "preprocess_bwd": N * 580,
         "adam": N * params_per_g * 28,
     }.get(stage)
+
+
+def bench_batched(hip, cfg, args):
+    """16 views of uneven resolution over the same 1M-Gaussian scene: the plain
+    single-view path (one gs_trainer_step per view, pair count read back per
+    step) against the batched, graph-captured step (gs_batch_step, all views
+    in one launch per stage, Adam fused), both in views trained per second."""
+    import torch
+    from gs_hip import GsBatch, GsTrainer, train_cfg
+    from gs_scenes import make_workload
+
+    scene, cams, gts = make_workload(args.batched_workload)
+    V = len(cams)
+    pix = sum(c.width * c.height for c in cams)
+    tc = train_cfg(lr=1e-3)
+    gts_t = [torch.tensor(g, device=hip.dev) for g in gts]
+    # plain single-view path over the same views
+    tr = GsTrainer(hip, scene)
+    for v in range(V):
+        tr.step(cams[v], cfg, gts_t[v], tc, v)
+    torch.cuda.synchronize()
+    rounds = max(1, args.steps // 10)
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        for v in range(V):
+            tr.step(cams[v], cfg, gts_t[v], tc, V + r * V + v)
+    torch.cuda.synchronize()
+    plain = rounds * V / (time.perf_counter() - t0)
+    del tr
+    # batched, graph-captured step
+    tr = GsTrainer(hip, scene)
+    b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=True, use_graph=True)
+    for i in range(2):
+        b.step(tc, i, sync=True)
+    steps = max(2, args.steps // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        b.step(tc, 2 + i)
+    res, losses = b.result()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    batched = steps * V / dt
+    # batched forward only
+    b.render()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        b.render()
+    torch.cuda.synchronize()
+    fwd_mpix = pix * steps / (time.perf_counter() - t0) / 1e6
+    tiles, _, cap = b.info()
+    return {"workload": args.batched_workload, "views": V, "pixels": pix, "pairs": int(res.pairs),
+            "views_per_s": round(batched, 2), "ms_per_batch_step": round(1e3 * dt / steps, 3),
+            "plain_views_per_s": round(plain, 2), "batched_vs_single": round(batched / plain, 3),
+            "fwd_mpix_s": round(fwd_mpix, 1), "loss": round(float(res.loss), 6)}
 
 
 def cpu_baseline(scene, cam, cfg, gt):
@@ -122,6 +180,8 @@ def main():
     ap.add_argument("--workload", default="c3_1m_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown-reps", type=int, default=3)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-workload", default="c4_1m_16views")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -173,12 +233,17 @@ def main():
     stages, K = stage_breakdown(hip, p, cam, cfg, gt, args.breakdown_reps)
     log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
     npix = cam.width * cam.height
-    dom = max((k for k in stages if k != "scan"), key=lambda k: stages[k])
+    dom = max(stages, key=lambda k: stages[k])
     byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
     achieved = byts / (stages[dom] * 1e-3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes": int(byts), "ms": round(stages[dom], 4)}
+
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(hip, cfg, args)
+        log(f"batched: {batched}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -201,6 +266,7 @@ def main():
         "config": {"workload": args.workload, "gaussians": scene.n, "sh_degree": scene.sh_degree,
                    "width": cam.width, "height": cam.height, "pairs": int(K), "optimizer": "adam lr 1e-3"},
         "fwd_mpix_s": round(mpix_s, 2),
+        "batched": batched,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -301,6 +301,38 @@ int gs_trainer_step(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* c
  * only, image into the trainer's "image" buffer. */
 int gs_trainer_render(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, long long* pairs);
 
+/* ------------------------------------------------------------------------
+ * Batched multi-view step (no counterpart in the reference, which trains one
+ * camera per Trainer::step; the gradients are the sum of the per-view
+ * Trainer::step gradients and the loss the mean of the per-view losses).
+ * One launch per stage over all views: multi-view preprocess reading the
+ * parameters once, joint depth sort and tile sort, forward/backward raster
+ * over the global (view, tile) list (persistent work queue), multi-view
+ * preprocess backward with Adam fused in, optionally captured in a hipGraph.
+ * ---------------------------------------------------------------------- */
+typedef struct gs_batch gs_batch;
+
+/* Views are fixed for the batch's lifetime (a new camera set = a new batch). */
+int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cams, const gs_render_cfg* cfg,
+                    gs_batch** out);
+int gs_batch_destroy(gs_batch* b);
+/* persistent: raster over an atomic tile queue; fused_adam: the preprocess
+ * backward applies Adam in place (else gradients go to the trainer's grad
+ * buffers and a separate Adam kernel runs); use_graph: capture once, replay. */
+int gs_batch_set_options(gs_batch* b, int persistent, int fused_adam, int use_graph);
+/* Per-view HWC targets (host or device arrays, one per view). */
+int gs_batch_set_targets(gs_batch* b, const float* const* targets, int from_host);
+/* One batched step with Adam.  out == NULL: no host synchronisation (results
+ * and the capacity check are read by a later gs_batch_result / step with out).
+ * Returns GS_ERR_CAPACITY (and regrows) if any step since the last check had
+ * more pairs than the pre-sized buffers. */
+int gs_batch_step(gs_batch* b, const gs_train_cfg* tc, int iteration, gs_step_result* out);
+int gs_batch_result(gs_batch* b, gs_step_result* out, double* view_losses);
+/* Forward only for all views into the batch's concatenated image buffer. */
+int gs_batch_render(gs_batch* b, long long* pairs);
+void* gs_batch_buffer(gs_batch* b, const char* which);
+int gs_batch_info(gs_batch* b, int* total_tiles, long long* total_pixels, long long* capacity);
+
 #ifdef __cplusplus
 }
 #endif
