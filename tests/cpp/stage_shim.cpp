@@ -1,0 +1,185 @@
+// stage_shim.cpp -- test-only extern "C" shim over the C++ drop-in classes
+// (gs3d_hip.h), so the tests can run the reference-shaped stage objects on
+// host arrays and compare them with the oracle.  Built into
+// tests/cpp/_build/libgs_stage_shim.so, linked against libgs_hip.so.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "gs3d_hip.h"
+
+static Camera cam_of(const gs_camera* c) {
+    Camera k;
+    std::memcpy(&k, c, sizeof(Camera));
+    return k;
+}
+static RenderConfig cfg_of(const gs_render_cfg* c) {
+    RenderConfig r;
+    for (int i = 0; i < 3; i++) r.bg_color[i] = c->bg_color[i];
+    r.scale_modifier = c->scale_modifier;
+    r.antialiasing = c->antialiasing != 0;
+    r.training = c->training != 0;
+    r.sh_degree = c->sh_degree;
+    r.tile_w = c->tile_w;
+    r.tile_h = c->tile_h;
+    return r;
+}
+static GaussianData gd(int n, int deg, int mc, float* pos, float* sh, float* sc, float* rot, float* op) {
+    GaussianData g;
+    g.count = n; g.sh_degree = deg; g.max_coeffs = mc;
+    g.positions = pos; g.sh_coeffs = sh; g.scales = sc; g.rotations = rot; g.opacities = op; g.filter_3D = nullptr;
+    return g;
+}
+
+static std::string g_err;
+
+extern "C" const char* shim_error() { return g_err.c_str(); }
+
+#define SHIM_TRY(...)                   \
+    try {                               \
+        __VA_ARGS__;                    \
+        return 0;                       \
+    } catch (const std::exception& e) { \
+        g_err = e.what();               \
+        return 1;                       \
+    }
+
+// Renderer(PreprocessorHIP, TileBinnerHIP, SorterHIP, RasterizerHIP)::render on activated host arrays.
+extern "C" int shim_render(int n, int deg, int mc, float* pos, float* sh, float* sc, float* rot, float* op,
+                           const gs_camera* cam, const gs_render_cfg* cfg, float* img) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        Renderer r(std::unique_ptr<Preprocessor>(new PreprocessorHIP(ctx)),
+                   std::unique_ptr<TileBinner>(new TileBinnerHIP(ctx)), std::unique_ptr<Sorter>(new SorterHIP(ctx)),
+                   std::unique_ptr<Rasterizer>(new RasterizerHIP(ctx)), 512ull << 20);
+        r.render(gd(n, deg, mc, pos, sh, sc, rot, op), cam_of(cam), cfg_of(cfg), img);
+    })
+}
+
+// Each stage object in turn; host copies of every stage output.  keys/values
+// buffers hold `cap` pairs; *K receives the pair count.
+extern "C" int shim_stages(int n, int deg, int mc, float* pos, float* sh, float* sc, float* rot, float* op,
+                           const gs_camera* cam, const gs_render_cfg* cfg, float* means2D, float* depths,
+                           float* conics, float* op2d, float* rgb, int* radii, int* tiles, long long cap,
+                           long long* K, uint64_t* keys_sorted, uint32_t* values_sorted, uint32_t* ranges,
+                           float* img, float* T_final, int* n_contrib) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        FrameAllocator alloc(1ull << 30);
+        PreprocessorHIP pp(ctx);
+        TileBinnerHIP tb(ctx);
+        tb.fill_host_outputs = true;
+        SorterHIP so(ctx);
+        RasterizerHIP rs(ctx);
+        Camera c = cam_of(cam);
+        RenderConfig f = cfg_of(cfg);
+        PreprocessOutput pre = pp.process(gd(n, deg, mc, pos, sh, sc, rot, op), c, f, alloc);
+        std::memcpy(means2D, pre.means2D, (size_t)n * 8);
+        std::memcpy(depths, pre.depths, (size_t)n * 4);
+        std::memcpy(conics, pre.conics, (size_t)n * 12);
+        std::memcpy(op2d, pre.opacities_2d, (size_t)n * 4);
+        std::memcpy(rgb, pre.rgb, (size_t)n * 12);
+        std::memcpy(radii, pre.radii, (size_t)n * 4);
+        std::memcpy(tiles, pre.tiles_touched, (size_t)n * 4);
+        BinningOutput bin = tb.bin(pre, n, c, f, alloc);
+        *K = bin.total_pairs;
+        if (bin.total_pairs > cap) throw std::runtime_error("shim: pair buffer too small");
+        if (bin.total_pairs > 0) so.sort(bin, alloc);
+        if (bin.total_pairs > 0) {
+            std::memcpy(keys_sorted, bin.keys_sorted, (size_t)bin.total_pairs * 8);
+            std::memcpy(values_sorted, bin.values_sorted, (size_t)bin.total_pairs * 4);
+        }
+        std::memcpy(ranges, bin.tile_ranges, (size_t)bin.num_tiles * 8);
+        ForwardCache cache{};
+        rs.rasterize(pre, bin, c, f, img, nullptr, &cache, &alloc);
+        std::memcpy(T_final, cache.T_final, (size_t)c.width * c.height * 4);
+        std::memcpy(n_contrib, cache.n_contrib, (size_t)c.width * c.height * 4);
+    })
+}
+
+// Host-produced stage inputs (no device_data): the binner, sorter and
+// rasterizer upload them.  pre arrays as from a CPU preprocessor.
+extern "C" int shim_from_host(int n, float* means2D, float* depths, float* conics, float* op2d, float* rgb,
+                              int* radii, int* tiles, const gs_camera* cam, const gs_render_cfg* cfg, float* img) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        FrameAllocator alloc(1ull << 30);
+        PreprocessOutput pre{};
+        pre.means2D = means2D; pre.depths = depths; pre.conics = conics; pre.opacities_2d = op2d; pre.rgb = rgb;
+        pre.radii = radii; pre.tiles_touched = tiles;
+        Camera c = cam_of(cam);
+        RenderConfig f = cfg_of(cfg);
+        TileBinnerHIP tb(ctx);
+        tb.fill_host_outputs = true;
+        SorterHIP so(ctx);
+        RasterizerHIP rs(ctx);
+        BinningOutput bin = tb.bin(pre, n, c, f, alloc);
+        // drop the device data: the sorter and rasterizer must work from host arrays
+        bin.device_data = nullptr;
+        if (bin.total_pairs > 0) so.sort(bin, alloc);
+        bin.device_data = nullptr;
+        rs.rasterize(pre, bin, c, f, img);
+    })
+}
+
+// RasterizerBackwardHIP + PreprocessorBackwardHIP after PreprocessorHIP /
+// TileBinnerHIP / SorterHIP, with the caller's d_image; raw parameters in.
+extern "C" int shim_backward(int n, int deg, int mc, float* raw_pos, float* raw_sc, float* raw_rot, float* raw_sh,
+                             float* raw_op, const gs_camera* cam, const gs_render_cfg* cfg, const float* d_image,
+                             float* d_means2D, float* d_conics, float* d_rgb, float* d_op2d, float* g_pos,
+                             float* g_sc, float* g_rot, float* g_sh, float* g_op) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        FrameAllocator alloc(1ull << 30);
+        RawGaussianParams raw{n, deg, mc, raw_pos, raw_sc, raw_rot, raw_sh, raw_op};
+        std::vector<float> ap(n * 3), as(n * 3), ar(n * 4), ash((size_t)n * mc * 3), ao(n);
+        GaussianData g = gd(n, deg, mc, ap.data(), ash.data(), as.data(), ar.data(), ao.data());
+        raw.activate(g);
+        Camera c = cam_of(cam);
+        RenderConfig f = cfg_of(cfg);
+        PreprocessorHIP pp(ctx);
+        TileBinnerHIP tb(ctx);
+        SorterHIP so(ctx);
+        PreprocessOutput pre = pp.process(g, c, f, alloc);
+        BinningOutput bin = tb.bin(pre, n, c, f, alloc);
+        if (bin.total_pairs > 0) so.sort(bin, alloc);
+        ForwardCache cache{};
+        cache.pre = &pre;
+        cache.bin = &bin;
+        RasterGradOutput rg{d_means2D, d_conics, d_rgb, d_op2d};
+        RasterizerBackwardHIP rb(ctx);
+        rb.backward(pre, bin, c, f, cache, d_image, n, rg);
+        GradientOutput go{g_pos, g_sc, g_rot, g_sh, g_op};
+        PreprocessorBackwardHIP pb(ctx);
+        pb.backward(g, c, f, cache, rg, raw, go);
+    })
+}
+
+// TrainerHIP::step `steps` times on host raw parameters (updated in place).
+extern "C" int shim_trainer(int n, int deg, int mc, float* raw_pos, float* raw_sc, float* raw_rot, float* raw_sh,
+                            float* raw_op, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                            int use_adam, int steps, float* losses, float* img) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        TrainerHIP tr(ctx);
+        RawGaussianParams raw{n, deg, mc, raw_pos, raw_sc, raw_rot, raw_sh, raw_op};
+        TrainConfig tc;
+        tc.use_adam = use_adam != 0;
+        for (int s = 0; s < steps; s++) losses[s] = tr.step(raw, cam_of(cam), gt, cfg_of(cfg), tc, s).loss;
+        std::memcpy(img, tr.last_rendered(), (size_t)cam->width * cam->height * 12);
+    })
+}
+
+// AdamOptimizerHIP::step on host arrays.
+extern "C" int shim_adam(int n, int deg, int mc, float* p_pos, float* p_sc, float* p_rot, float* p_sh, float* p_op,
+                         float* g_pos, float* g_sc, float* g_rot, float* g_sh, float* g_op, int steps) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        AdamOptimizerHIP adam(ctx);
+        RawGaussianParams p{n, deg, mc, p_pos, p_sc, p_rot, p_sh, p_op};
+        GradientOutput g{g_pos, g_sc, g_rot, g_sh, g_op};
+        TrainConfig tc;
+        tc.use_adam = true;
+        for (int s = 0; s < steps; s++) adam.step(p, g, tc, s);
+    })
+}
