@@ -1,9 +1,11 @@
 // gs_batch.cpp -- batched multi-view training step.
 //
 // One step over V views of uneven resolution: the Gaussians' parameters are
-// read once per step by a multi-view preprocess, all views are binned and
-// sorted together (depth sort on view << 32 | depth bits, tile sort on the
-// local tile bits of view << 48 | tile << 32 | index words), the forward and
+// read once per step by a multi-view preprocess, all views are binned
+// together (depth sort on view << 32 | depth bits, then direct placement of
+// each Gaussian index into its tiles' lists, k_bin.hip; views with more than
+// GS_BIN_MAX_TILES tiles use a tile sort of view << 48 | tile << 32 | index
+// words instead), the forward and
 // backward raster each run as one launch over the global (view, tile) list
 // (optionally a persistent grid pulling tiles from an atomic counter), and the
 // multi-view preprocess backward sums the views' gradients in registers and
@@ -55,6 +57,10 @@ struct gs_batch {
     int persistent = 1, fused_adam = 1, use_graph = 1;
     int persistent_groups = 0;
     uint32_t capacity = 0;
+    bool direct = true;            // direct-placement binning (else the word sort)
+    int bin_G = 0, bin_C = 0;      // Gaussians per chunk, chunks per view
+    size_t bin_L = 0;              // (view, tile, chunk) table entries + 1
+    int max_tiles = 0;
 
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -66,6 +72,7 @@ struct gs_batch {
     DevBuf means2D, depths, conics, op2d, rgb, radii, tiles;
     DevBuf dkeys, didx, dkeys_s, didx_s, rtiles, roff, scan_partial, dsort_ws;
     DevBuf words, words_tmp, tsort_ws, ranges;
+    DevBuf values, bviews_dev, bin_table, bin_scanned, bin_partial;
     DevBuf image, d_image, gt, T_final, n_contrib, last;
     DevBuf rg_means2D, rg_conics, rg_rgb, rg_op;
     DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16
@@ -86,7 +93,8 @@ gs_batch::~gs_batch() {
     graph.release();
     DevBuf* all[] = {&cams_dev, &views_dev, &pixoff_dev, &hp_dev, &counters, &means2D, &depths, &conics, &op2d, &rgb,
                      &radii, &tiles, &dkeys, &didx, &dkeys_s, &didx_s, &rtiles, &roff, &scan_partial, &dsort_ws,
-                     &words, &words_tmp, &tsort_ws, &ranges, &image, &d_image, &gt, &T_final, &n_contrib, &last,
+                     &words, &words_tmp, &tsort_ws, &ranges, &values, &bviews_dev, &bin_table, &bin_scanned,
+                     &bin_partial, &image, &d_image, &gt, &T_final, &n_contrib, &last,
                      &rg_means2D, &rg_conics, &rg_rgb, &rg_op, &losses, &l1_partial, &small};
     for (DevBuf* b : all) b->release();
     if (ev_in) (void)hipEventDestroy(ev_in);
@@ -113,12 +121,16 @@ static int bits_for(long long n) {
 
 // Pair-count-sized buffers for `cap` pairs.
 static int batch_size_pairs(gs_batch* b, uint32_t cap) {
-    if (cap <= b->capacity && b->words.p) return GS_OK;
+    if (cap <= b->capacity && (b->words.p || b->values.p)) return GS_OK;
     b->graph.release();
     b->graph_ready = false;
-    B_TRY(b->words.ensure((size_t)cap * 8));
-    B_TRY(b->words_tmp.ensure((size_t)cap * 8));
-    B_TRY(b->tsort_ws.ensure(radix_workspace_bytes(cap, b->radix_bits)));
+    if (b->direct) {
+        B_TRY(b->values.ensure((size_t)cap * 4));
+    } else {
+        B_TRY(b->words.ensure((size_t)cap * 8));
+        B_TRY(b->words_tmp.ensure((size_t)cap * 8));
+        B_TRY(b->tsort_ws.ensure(radix_workspace_bytes(cap, b->radix_bits)));
+    }
     b->capacity = cap;
     return GS_OK;
 }
@@ -155,6 +167,14 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
     b->total_pix = b->pix_off.back();
     b->tile_bits = std::max(1, bits_for(max_tiles));
     b->view_bits = bits_for(n_views);
+    b->max_tiles = max_tiles;
+    b->direct = max_tiles <= GS_BIN_MAX_TILES;
+    // chunks of the depth-ordered list: larger with more views, so that the
+    // (view, tile, chunk) table stays a small fraction of the pair array
+    b->bin_G = std::min(65536, tr->ctx->bin_chunk * (n_views >= 8 ? 4 : n_views >= 2 ? 2 : 1));
+    b->bin_C = (b->N + b->bin_G - 1) / b->bin_G;
+    b->bin_L = (size_t)b->bin_C * (size_t)b->total_tiles + 1;
+    if (b->direct && b->bin_L >= 0xFFFFFFFFull) b->direct = false;
     int st = GS_OK;
     auto fail = [&](int s) {
         delete b;
@@ -178,7 +198,6 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
         (st = b->conics.ensure(VN * 12)) || (st = b->op2d.ensure(VN * 4)) || (st = b->rgb.ensure(VN * 12)) ||
         (st = b->radii.ensure(VN * 4)) || (st = b->tiles.ensure(VN * 4)) || (st = b->dkeys.ensure(VN * 8)) ||
         (st = b->didx.ensure(VN * 4)) || (st = b->dkeys_s.ensure(VN * 8)) || (st = b->didx_s.ensure(VN * 4)) ||
-        (st = b->rtiles.ensure(VN * 4)) || (st = b->roff.ensure(VN * 4)) ||
         (st = b->scan_partial.ensure(((VN + SCAN_BLOCK - 1) / SCAN_BLOCK + 1) * 8)) ||
         (st = b->dsort_ws.ensure(radix_workspace_bytes((uint32_t)VN, b->radix_bits))) ||
         (st = b->ranges.ensure((size_t)b->total_tiles * 8)) || (st = b->image.ensure(P * 12)) ||
@@ -191,7 +210,25 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
     for (int v = 0; v < n_views; v++) max_pix = std::max(max_pix, b->pix_off[v + 1] - b->pix_off[v]);
     b->l1_nb = std::max(1, std::min(512, (max_pix * 3 + 255) / 256 / 4));
     if ((st = b->l1_partial.ensure((size_t)n_views * b->l1_nb * 8))) return fail(st);
+    std::vector<int4> bviews(n_views);
+    if (b->direct) {
+        size_t tab = 0;
+        for (int v = 0; v < n_views; v++) {
+            const int gx = b->views[v].z, gy = (b->views[v].y + 15) / 16;
+            bviews[v] = int4{gx, gy, (int)(uint32_t)tab, b->views[v].w};
+            tab += (size_t)gx * gy * b->bin_C;
+        }
+        if ((st = b->bviews_dev.ensure(sizeof(int4) * n_views)) || (st = b->bin_table.ensure(b->bin_L * 4)) ||
+            (st = b->bin_scanned.ensure(b->bin_L * 4)) ||
+            (st = b->bin_partial.ensure(((b->bin_L + SCAN_BLOCK - 1) / SCAN_BLOCK + 1) * 8)))
+            return fail(st);
+    } else if ((st = b->rtiles.ensure(VN * 4)) || (st = b->roff.ensure(VN * 4))) {
+        return fail(st);
+    }
     hipStream_t s = b->stream;
+    if (b->direct && hipMemcpyAsync(b->bviews_dev.p, bviews.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, s) !=
+                         hipSuccess)
+        return fail(gs_set_error(GS_ERR_HIP, "gs_batch_create: upload failed"));
     if (hipMemcpyAsync(b->cams_dev.p, ca.data(), sizeof(gsd::CamArgs) * n_views, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b->views_dev.p, b->views.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b->pixoff_dev.p, b->pix_off.data(), sizeof(int) * (n_views + 1), hipMemcpyHostToDevice, s) !=
@@ -246,6 +283,31 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
                                  static_cast<uint64_t*>(b->dkeys_s.p), b->didx_s.u(), VN, 0, 32 + b->view_bits,
                                  b->radix_bits, b->dsort_ws.p, s, nullptr),
           "depth sort");
+    if (b->direct) {
+        gsb::BinArgs a{};
+        a.V = b->V;
+        a.N = b->N;
+        a.C = b->bin_C;
+        a.G = b->bin_G;
+        a.sidx = b->didx_s.u();
+        a.means2D = reinterpret_cast<const float2*>(pre.means2D);
+        a.radii = pre.radii;
+        a.views = static_cast<const int4*>(b->bviews_dev.p);
+        a.table = b->bin_table.u();
+        a.table_len = b->bin_L;
+        a.out = b->values.u();
+        a.capacity = b->capacity;
+        a.overflow = b->overflow();
+        B_HIP(launch_bin_count(a, b->max_tiles, s), "binning count");
+        B_HIP(launch_scan_u32_exclusive(b->bin_table.u(), (uint32_t)b->bin_L, b->bin_scanned.u(),
+                                        static_cast<unsigned long long*>(b->bin_partial.p), s, b->total32(),
+                                        b->total64()),
+              "binning scan");
+        a.table = b->bin_scanned.u();
+        B_HIP(launch_bin_place(a, b->max_tiles, s), "binning place");
+        B_HIP(launch_bin_ranges(a, b->bin_scanned.u(), (uint32_t)b->total_tiles, b->ranges.u(), s), "binning ranges");
+        return GS_OK;
+    }
     B_HIP(launch_gather_tiles_multi(VN, b->N, static_cast<const uint64_t*>(b->dkeys_s.p), b->didx_s.u(), pre.tiles_touched,
                                     b->rtiles.i(), s),
           "gather tiles");
@@ -278,8 +340,8 @@ static gsr::MultiRaster multi_raster(gs_batch* b, unsigned* counter) {
     m.conics = b->conics.f();
     m.opac = b->op2d.f();
     m.rgb = b->rgb.f();
-    m.values = b->words.u();
-    m.vstride = 2;
+    m.values = b->direct ? b->values.u() : b->words.u();
+    m.vstride = b->direct ? 1 : 2;
     m.ranges = b->ranges.u();
     m.bg0 = b->cfg.bg_color[0];
     m.bg1 = b->cfg.bg_color[1];
@@ -494,7 +556,7 @@ extern "C" void* gs_batch_buffer(gs_batch* b, const char* which) {
     if (w == "means2D") return b->means2D.p;
     if (w == "radii") return b->radii.p;
     if (w == "ranges") return b->ranges.p;
-    if (w == "pair_words") return b->words.p;
+    if (w == "pairs") return b->direct ? b->values.p : b->words.p;  // uint32 indices or 64-bit words
     if (w == "losses") return b->losses.p;
     if (w == "rg_means2D") return b->rg_means2D.p;
     if (w == "rg_conics") return b->rg_conics.p;

@@ -8,6 +8,7 @@
 // and the loss readback of StepResult.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -224,14 +225,38 @@ extern "C" void* gs_trainer_buffer(gs_trainer* t, const char* which) {
     if (w == "radii") return v.radii.p;
     if (w == "tiles_touched") return v.tiles.p;
     if (w == "ranges") return v.ranges.p;
-    if (w == "pair_words") return t->keys_sorted.p;
+    if (w == "pairs") return t->keys_sorted.p;  // uint32 indices (pair_stride 1) or 64-bit words (2)
     if (w == "loss") return v.loss.p;
     return nullptr;
 }
 
+// Binning of one view into the trainer's pair buffer: direct placement
+// (gs_bin_tiles, 4-byte Gaussian indices) when the view has at most
+// GS_BIN_MAX_TILES tiles, else the depth-ordered word sort
+// (gs_bin_sorted_words, 8-byte words).  The buffer grows when a view needs more
+// pairs than it holds.  Sets t->pair_stride (uint32 entries per pair).
+static int tr_bin(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs_preprocessed& pre, long long* K) {
+    gs_ctx* c = t->ctx;
+    const int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
+    const bool words = tiles > GS_BIN_MAX_TILES;
+    const size_t per = words ? 8 : 4;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        const uint32_t cap = (uint32_t)std::min<size_t>(t->keys_sorted.bytes / per, 0xFFFFFFFFu);
+        int st = words ? gs_bin_sorted_words(c, t->n, &pre, cam->width, cam->height,
+                                             static_cast<uint64_t*>(t->keys_sorted.p), cap, v.ranges.u(), K)
+                       : gs_bin_tiles(c, t->n, &pre, cam->width, cam->height, t->keys_sorted.u(), cap, v.ranges.u(), K);
+        if (st == GS_ERR_CAPACITY && *K > (long long)cap && attempt == 0) {
+            TR_TRY(t->keys_sorted.ensure((size_t)(*K + *K / 8 + 1024) * per));
+            continue;
+        }
+        TR_TRY(st);
+        t->pair_stride = words ? 2 : 1;
+        return GS_OK;
+    }
+    return gs_set_error(GS_ERR_CAPACITY, "trainer binning: pair buffer did not grow");
+}
+
 // Forward of one view into view buffers; returns the pair count via *pairs.
-// Binning is the depth-ordered path (gs_bin_sorted_words); the pair buffer
-// grows when a view needs more pairs than it holds.
 static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs_render_cfg* cfg,
                       long long* pairs) {
     gs_ctx* c = t->ctx;
@@ -240,19 +265,14 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
     gs_preprocessed pre = v.pre();
     TR_TRY(gs_preprocess_fwd(c, &g, cam, cfg, &pre));
     long long K = 0;
-    uint32_t cap = (uint32_t)(t->keys_sorted.bytes / 8);
-    int st = gs_bin_sorted_words(c, t->n, &pre, cam->width, cam->height, static_cast<uint64_t*>(t->keys_sorted.p), cap,
-                                 v.ranges.u(), &K);
-    if (st == GS_ERR_CAPACITY && K > (long long)cap) {
-        TR_TRY(t->keys_sorted.ensure((size_t)(K + K / 8 + 1024) * 8));
-        cap = (uint32_t)(t->keys_sorted.bytes / 8);
-        st = gs_bin_sorted_words(c, t->n, &pre, cam->width, cam->height, static_cast<uint64_t*>(t->keys_sorted.p), cap,
-                                 v.ranges.u(), &K);
-    }
-    TR_TRY(st);
+    TR_TRY(tr_bin(t, v, cam, pre, &K));
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
-    TR_TRY(gs_raster_fwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
-                               v.ranges.u(), cfg->bg_color, v.image.f(), &px));
+    if (t->pair_stride == 1)
+        TR_TRY(gs_raster_fwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color,
+                             v.image.f(), &px));
+    else
+        TR_TRY(gs_raster_fwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
+                                   v.ranges.u(), cfg->bg_color, v.image.f(), &px));
     *pairs = K;
     return GS_OK;
 }
@@ -302,8 +322,12 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
     gs_preprocessed pre = v.pre();
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
     gs_raster_grads rg = v.rgrads();
-    TR_TRY(gs_raster_bwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
-                               v.ranges.u(), cfg->bg_color, &px, v.d_image.f(), &rg));
+    if (t->pair_stride == 1)
+        TR_TRY(gs_raster_bwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color, &px,
+                             v.d_image.f(), &rg));
+    else
+        TR_TRY(gs_raster_bwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
+                                   v.ranges.u(), cfg->bg_color, &px, v.d_image.f(), &rg));
     gs_gaussians g = t->gaussians();
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
     TR_TRY(gs_preprocess_bwd(c, &g, cam, cfg, pre.radii, &rg, &pg, 0));

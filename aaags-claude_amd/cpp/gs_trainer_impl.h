@@ -53,6 +53,7 @@ struct gs_trainer {
     gs::DevBuf param[5], grad[5], adam_m[5], adam_v[5];  // positions, scales, rotations, sh, opacities
     std::vector<gs::ViewBufs> views;
     gs::DevBuf keys, vals, keys_sorted, vals_sorted;
+    int pair_stride = 1;  // uint32 entries per pair in keys_sorted: 1 (indices) or 2 (words)
     std::vector<gs::GraphSlot> graphs;
     gs_gaussians gaussians() const;
     size_t tensor_elems(int k) const;

@@ -396,6 +396,95 @@ extern "C" int gs_bin_sorted_words(gs_ctx* c, int n, const gs_preprocessed* pre,
     return GS_OK;
 }
 
+// Chunk size of the direct-placement binning: Gaussians per chunk.  Larger
+// chunks shrink the (tile, chunk) table; smaller ones shorten each block's walk.
+extern "C" int gs_ctx_set_bin_chunk(gs_ctx* c, int gaussians_per_chunk) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(gaussians_per_chunk >= 256 && gaussians_per_chunk <= 65536 &&
+                     (gaussians_per_chunk & (gaussians_per_chunk - 1)) == 0,
+                 "chunk size must be a power of two in [256, 65536]");
+    c->bin_chunk = gaussians_per_chunk;
+    return GS_OK;
+}
+
+extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
+                            uint32_t capacity, uint32_t* ranges, long long* total_host) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
+    GS_CHECK_ARG(n >= 0, "negative count");
+    GS_CHECK_ARG(ranges != nullptr, "null tile ranges");
+    int st = check_pre(pre, true);
+    if (st) return st;
+    const int gx = (width + 15) / 16, gy = (height + 15) / 16;
+    const int num_tiles = gx * gy;
+    if (num_tiles > GS_BIN_MAX_TILES)
+        return gs_set_error(GS_ERR_UNSUPPORTED, "gs_bin_tiles: %d tiles exceed %d (use gs_bin_sorted_words)",
+                            num_tiles, GS_BIN_MAX_TILES);
+    if (n == 0) {
+        GS_HIP(launch_zero(ranges, (size_t)num_tiles * 8, c->stream), "zero kernel");
+        if (total_host) *total_host = 0;
+        return GS_OK;
+    }
+    void* sm;
+    if ((st = gs_small_get(c, &sm))) return st;
+    uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_OVERFLOW);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t un = (size_t)n;
+    const int G = c->bin_chunk;
+    const int C = (n + G - 1) / G;
+    const size_t L = (size_t)C * num_tiles + 1;
+    GS_CHECK_ARG(L < 0xFFFFFFFFull, "binning table too large");
+    const size_t np = scan_partials_needed((uint32_t)L) + 1;
+    void* ws2;
+    if ((st = gs_ws2_get(c, 2 * al(un * 8) + 2 * al(un * 4) + radix_workspace_bytes((uint32_t)n, 8) + 2 * al(L * 4) +
+                                al(np * 8),
+                         &ws2)))
+        return st;
+    char* w = static_cast<char*>(ws2);
+    uint64_t* dkeys = reinterpret_cast<uint64_t*>(w); w += al(un * 8);
+    uint64_t* dkeys_s = reinterpret_cast<uint64_t*>(w); w += al(un * 8);
+    uint32_t* didx = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
+    uint32_t* order = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
+    uint32_t* table = reinterpret_cast<uint32_t*>(w); w += al(L * 4);
+    uint32_t* scanned = reinterpret_cast<uint32_t*>(w); w += al(L * 4);
+    unsigned long long* partial = reinterpret_cast<unsigned long long*>(w); w += al(np * 8);
+    void* dsort_ws = w;
+    GS_HIP(launch_depth_keys(n, pre->depths, pre->radii, dkeys, didx, c->stream), "depth keys kernel");
+    GS_HIP(launch_radix_sort_bits(dkeys, didx, dkeys_s, order, (uint32_t)n, 0, 32, 8, dsort_ws, c->stream, nullptr),
+           "depth sort");
+    gsb::BinArgs a{};
+    a.V = 1;
+    a.N = n;
+    a.C = C;
+    a.G = G;
+    a.sidx = order;
+    a.means2D = reinterpret_cast<const float2*>(pre->means2D);
+    a.radii = pre->radii;
+    a.views = nullptr;
+    a.view0 = make_int4(gx, gy, 0, 0);
+    a.table = table;
+    a.table_len = L;
+    a.out = values;
+    a.capacity = capacity;
+    a.overflow = ovf;
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
+    GS_HIP(launch_bin_count(a, num_tiles, c->stream), "binning count kernel");
+    GS_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr, total64),
+           "binning scan");
+    unsigned long long K = 0;
+    GS_HIP(hipMemcpyAsync(&K, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
+    GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (total_host) *total_host = (long long)K;
+    if (K > capacity) return gs_set_error(GS_ERR_CAPACITY, "gs_bin_tiles: %llu pairs exceed capacity %u", K, capacity);
+    if (K > 0) {
+        GS_CHECK_ARG(values != nullptr, "null values buffer");
+        a.table = scanned;
+        GS_HIP(launch_bin_place(a, num_tiles, c->stream), "binning place kernel");
+    }
+    GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
+    return GS_OK;
+}
+
 static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed* pre, const uint32_t* values,
                                    const uint32_t* ranges, const float bg[3], int vstride = 1) {
     gsr::RasterArgs a;

@@ -273,6 +273,24 @@ struct MultiGrads {  // view-major [V][N]
 };
 }  // namespace gsr
 
+namespace gsb {
+// Direct-placement binning over V views of N Gaussians each (k_bin.hip).
+struct BinArgs {
+    int V, N;
+    int C, G;               // chunks per view, Gaussians per chunk
+    const uint32_t* sidx;   // [V*N]: each view's Gaussian indices in depth order
+    const float2* means2D;  // view-major [V][N]
+    const int* radii;       // [V][N]
+    const int4* views;      // per view: (tiles per row, tile rows, table offset, first global tile)
+    int4 view0;             // used instead when views == nullptr (one view)
+    uint32_t* table;        // count: per-(view, tile, chunk) counts; place: their exclusive scan
+    size_t table_len;       // sum over views of tiles * C, plus one trailing entry
+    uint32_t* out;          // place: pair values (Gaussian index)
+    uint32_t capacity;
+    uint32_t* overflow;
+};
+}  // namespace gsb
+
 namespace gso {
 // Adam hyper-parameters read from device memory (updated per step outside a
 // captured graph).

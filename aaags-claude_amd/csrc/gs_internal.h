@@ -12,6 +12,7 @@
 namespace gsd { struct CamArgs; }
 namespace gsr { struct RasterArgs; struct GradArgs; struct MultiRaster; struct MultiPix; struct MultiGrads; }
 namespace gso { struct AdamArgs; struct AdamDev; }
+namespace gsb { struct BinArgs; }
 
 // Small fixed scratch owned by each context.
 constexpr size_t GS_SMALL_BYTES = 4096;
@@ -30,6 +31,7 @@ struct gs_ctx {
     int radix_bits = 8;
     int raster_batch = 256;
     int raster_bwd_batch = 256;
+    int bin_chunk = 1024;  // Gaussians per chunk in gs_bin_tiles
 };
 
 int gs_set_error(int code, const char* fmt, ...);
@@ -42,6 +44,7 @@ int gs_tile_key_bits(int num_tiles);
 
 extern "C" int gs_ctx_set_tuning(gs_ctx* c, int radix_bits, int raster_batch, int raster_bwd_batch);
 extern "C" int gs_ctx_reserve(gs_ctx* c, size_t bytes);
+extern "C" int gs_ctx_set_bin_chunk(gs_ctx* c, int gaussians_per_chunk);
 extern "C" int gs_check_overflow(gs_ctx* c, int* overflowed);
 
 // kernel launchers
@@ -97,6 +100,16 @@ hipError_t launch_gather_tiles_multi(uint32_t total, int N, const uint64_t* skey
 hipError_t launch_duplicate_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
                                   const float* means2D, const int* radii, const uint32_t* offsets, const int4* views,
                                   uint64_t* words, uint32_t capacity, uint32_t* overflow, hipStream_t s);
+// direct-placement binning (k_bin.hip); views with more tiles are rejected
+constexpr int GS_BIN_MAX_TILES = 15000;
+size_t bin_lds_bytes(int max_tiles);
+hipError_t launch_bin_count(const gsb::BinArgs& a, int max_tiles, hipStream_t s);
+hipError_t launch_bin_place(const gsb::BinArgs& a, int max_tiles, hipStream_t s);
+hipError_t launch_bin_ranges(const gsb::BinArgs& a, const uint32_t* scanned, uint32_t total_tiles, uint32_t* ranges,
+                             hipStream_t s);
+// total32 / total64 (optional, device): the sum of all n inputs
+hipError_t launch_scan_u32_exclusive(const uint32_t* in, uint32_t n, uint32_t* out, unsigned long long* partial,
+                                     hipStream_t s, uint32_t* total32 = nullptr, unsigned long long* total64 = nullptr);
 hipError_t launch_tile_ranges_multi(const uint64_t* words, uint32_t n_cap, const uint32_t* n_dev, const int4* views,
                                     uint32_t* ranges, hipStream_t s);
 hipError_t launch_raster_fwd_multi(const gsr::MultiRaster& m, const gsr::MultiPix& o, int batch,

@@ -11,8 +11,9 @@
 // The backward walks the same list back to front from the largest last
 // contributor in the tile.  Each pixel recovers T_i = T/(1-alpha) exactly as
 // the reference CPU backward (cpu/rasterizer_backward_cpu.cpp:69-134); the
-// nine per-Gaussian gradient terms are summed across the wave with lane
-// shuffles, across the four waves in LDS, and land in HBM with one float
+// nine per-Gaussian gradient terms are summed across the wave with packed
+// permlane-swap/DPP steps (no LDS traffic), across the four waves with LDS
+// atomics, and land in HBM with one float
 // atomic per (tile, Gaussian, component).
 //
 // Multi-view: the same tile bodies run over a global (view, tile) list, either
@@ -110,10 +111,70 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
     }
 }
 
-__device__ __forceinline__ float wave_sum_f(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// ---- packed wave reduction of the nine gradient terms ----------------------
+// Summing nine values over 64 lanes one by one costs 9 x 6 cross-lane steps.
+// Packed, two values share each exchange: v_permlane32_swap leaves value a
+// summed over lane pairs {l, l+32} in lanes 0-31 and value b in lanes 32-63;
+// v_permlane16_swap does the same between 16-lane rows; DPP row rotations
+// finish each row.  Afterwards row r (lanes 16r..16r+15) holds, in every lane,
+//   k0 = term A[r], k1 = term 4 + A[r], with A = {0, 2, 1, 3},
+// and k2 = term 8 everywhere.
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 16 lanes of each row (row_ror:8, 4, 2, 1).
+__device__ __forceinline__ float row_sum(float v) {
+    v += dpp_f<0x128>(v);
+    v += dpp_f<0x124>(v);
+    v += dpp_f<0x122>(v);
+    v += dpp_f<0x121>(v);
     return v;
+}
+
+// Reduces gv[0..8] over the wave; lanes {0,1,16,17,32,33,48,49,2} return
+// true with one distinct term each in (*term, *val).
+__device__ __forceinline__ bool wave_reduce9(const float (&gv)[NG], int lane, int* term, float* val) {
+    const float h0 = swap32_sum(gv[0], gv[1]);
+    const float h1 = swap32_sum(gv[2], gv[3]);
+    const float h2 = swap32_sum(gv[4], gv[5]);
+    const float h3 = swap32_sum(gv[6], gv[7]);
+    const float h4 = swap32_sum(gv[8], gv[8]);
+    float k0 = swap16_sum(h0, h1);
+    float k1 = swap16_sum(h2, h3);
+    float k2 = swap16_sum(h4, h4);
+    k0 = row_sum(k0);
+    k1 = row_sum(k1);
+    k2 = row_sum(k2);
+    const int r = lane >> 4, p = lane & 15;
+    const int a = ((r & 1) << 1) | (r >> 1);
+    if (p == 0) {
+        *term = a;
+        *val = k0;
+        return true;
+    }
+    if (p == 1) {
+        *term = 4 + a;
+        *val = k1;
+        return true;
+    }
+    if (lane == 2) {
+        *term = 8;
+        *val = k2;
+        return true;
+    }
+    return false;
 }
 
 struct PixIn {
@@ -210,15 +271,9 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 }
             }
             if (__ballot(contrib) != 0ull) {
-#pragma unroll
-                for (int c = 0; c < NG; c++) gv[c] = wave_sum_f(gv[c]);
-                if (lane < NG) {
-                    float v = gv[0];
-#pragma unroll
-                    for (int c = 1; c < NG; c++)
-                        if (lane == c) v = gv[c];
-                    if (v != 0.0f) atomicAdd(&sm.g[k][lane], v);
-                }
+                int term;
+                float v;
+                if (wave_reduce9(gv, lane, &term, &v) && v != 0.0f) atomicAdd(&sm.g[k][term], v);
             }
         }
         __syncthreads();

@@ -9,7 +9,7 @@
 //             stably with wave ballots (match-any on the digit bits), stages
 //             the block's keys in LDS in digit order and writes them out in
 //             runs, so global writes are contiguous within each digit bucket.
-//             4096 keys per 256-thread block; 48 KB of LDS per block.
+//             1024 or 4096 keys per 256-thread block (by sort size).
 #include "gs_device.h"
 
 namespace gsk {
@@ -20,9 +20,10 @@ constexpr int SCAN_BLOCK = SCAN_THREADS * SCAN_ITEMS;  // 4096
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / 64;
-constexpr int RS_ROUNDS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;  // 4096 keys per block
-constexpr int RS_WAVE_SPAN = RS_ROUNDS * 64;     // 1024 contiguous keys per wave
+// Keys per block = 256 threads x ROUNDS: 4096 for large sorts (long output
+// runs per digit), 1024 below 4M keys so that a sort of ~1M keys still spreads
+// over ~1000 blocks instead of ~250.
+constexpr uint32_t RS_SMALL_LIMIT = 1u << 22;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
@@ -325,18 +326,19 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
     return peers;
 }
 
-template <int BITS>
+template <int BITS, int ROUNDS>
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
                                                                 const uint32_t* __restrict__ n_dev, int shift,
                                                                 uint32_t nb, uint32_t* __restrict__ hist) {
     constexpr int RADIX = 1 << BITS;
+    constexpr int RS_TILE = RS_THREADS * ROUNDS, RS_WAVE_SPAN = ROUNDS * 64;
     const uint32_t n = live_count(n_cap, n_dev);
     __shared__ uint32_t s_h[RS_WAVES][RADIX];
     const int w = threadIdx.x >> 6, l = lane_id();
     for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_h[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t wbase = blockIdx.x * RS_TILE + w * RS_WAVE_SPAN;
-    for (int r = 0; r < RS_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         uint32_t i = wbase + r * 64 + l;
         bool valid = i < n;
         uint64_t active = __ballot(valid);
@@ -355,7 +357,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* 
 }
 
 // VALS = false sorts self-contained 64-bit words (no separate value array).
-template <int BITS, bool VALS>
+template <int BITS, bool VALS, int ROUNDS>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t* __restrict__ keys_in,
                                                                   const uint32_t* __restrict__ vals_in, uint32_t n_cap,
                                                                   const uint32_t* __restrict__ n_dev, int shift,
@@ -363,6 +365,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
                                                                   uint64_t* __restrict__ keys_out,
                                                                   uint32_t* __restrict__ vals_out) {
     constexpr int RADIX = 1 << BITS;
+    constexpr int RS_TILE = RS_THREADS * ROUNDS, RS_WAVE_SPAN = ROUNDS * 64;
     const uint32_t n = live_count(n_cap, n_dev);
     if (blockIdx.x * (uint32_t)RS_TILE >= n) return;  // whole block past the live count
     __shared__ uint64_t s_k[RS_TILE];
@@ -379,13 +382,13 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
 
     const uint32_t bbase = blockIdx.x * RS_TILE;
     const uint32_t wbase = bbase + w * RS_WAVE_SPAN;
-    uint64_t k_r[RS_ROUNDS];
-    uint32_t v_r[VALS ? RS_ROUNDS : 1];
-    uint32_t rank_r[RS_ROUNDS];
-    uint32_t d_r[RS_ROUNDS];
+    uint64_t k_r[ROUNDS];
+    uint32_t v_r[VALS ? ROUNDS : 1];
+    uint32_t rank_r[ROUNDS];
+    uint32_t d_r[ROUNDS];
     // 1. per-wave stable ranks: rounds in key order, lanes in key order.
 #pragma unroll
-    for (int r = 0; r < RS_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         uint32_t i = wbase + r * 64 + l;
         bool valid = i < n;
         uint64_t active = __ballot(valid);
@@ -427,7 +430,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
     __syncthreads();
     // 3. stage the block's pairs in LDS in (digit, input order) order.
 #pragma unroll
-    for (int r = 0; r < RS_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         uint32_t i = wbase + r * 64 + l;
         if (i < n) {
             uint32_t p = s_dstart[d_r[r]] + s_wh[w][d_r[r]] + rank_r[r];
@@ -489,12 +492,13 @@ hipError_t launch_scan_tiles(const int* in, uint32_t n, uint32_t* out, unsigned 
     return hipGetLastError();
 }
 
-// Exclusive scan of uint32 counts in place-compatible buffers (in != out).
-static hipError_t launch_scan_u32_exclusive(const uint32_t* in, uint32_t n, uint32_t* out,
-                                            unsigned long long* partial, hipStream_t s) {
+// Exclusive scan of uint32 counts (in != out).
+hipError_t launch_scan_u32_exclusive(const uint32_t* in, uint32_t n, uint32_t* out,
+                                            unsigned long long* partial, hipStream_t s, uint32_t* total32,
+                                            unsigned long long* total64) {
     uint32_t nb = scan_partials_needed(n);
     scan_reduce_kernel<uint32_t><<<nb, SCAN_THREADS, 0, s>>>(in, n, partial);
-    scan_partials_kernel<<<1, 1024, 0, s>>>(partial, nb, nullptr, nullptr);
+    scan_partials_kernel<<<1, 1024, 0, s>>>(partial, nb, total32, total64);
     scan_downsweep_kernel<uint32_t, false><<<nb, SCAN_THREADS, 0, s>>>(in, n, partial, out);
     return hipGetLastError();
 }
@@ -508,7 +512,11 @@ hipError_t launch_duplicate(int n, const float* means2D, const float* depths, co
     return hipGetLastError();
 }
 
-uint32_t radix_blocks(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+static int radix_rounds(uint32_t n) { return n >= RS_SMALL_LIMIT ? 16 : 4; }
+uint32_t radix_blocks(uint32_t n) {
+    uint32_t tile = (uint32_t)(RS_THREADS * radix_rounds(n));
+    return (n + tile - 1) / tile;
+}
 
 // Workspace layout for one sort of n items with the given radix:
 //   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][partials]
@@ -521,17 +529,29 @@ size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
     return al((size_t)n * 8) + al((size_t)n * 4) + 2 * al(tab * 4) + al(np * 8);
 }
 
+template <int BITS, int ROUNDS>
+static hipError_t sort_pass_r(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n,
+                              const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff,
+                              unsigned long long* partial, hipStream_t s) {
+    uint32_t nb = radix_blocks(n);
+    rs_upsweep_kernel<BITS, ROUNDS><<<nb, RS_THREADS, 0, s>>>(ki, n, n_dev, shift, nb, hist);
+    hipError_t e =
+        launch_scan_u32_exclusive(hist, (uint32_t)((1u << BITS) * nb), goff, partial, s, nullptr, nullptr);
+    if (e != hipSuccess) return e;
+    if (vi)
+        rs_downsweep_kernel<BITS, true, ROUNDS><<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, nb, goff, ko, vo);
+    else
+        rs_downsweep_kernel<BITS, false, ROUNDS>
+            <<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, nb, goff, ko, nullptr);
+    return hipGetLastError();
+}
+
 template <int BITS>
 static hipError_t sort_pass(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n,
                             const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff,
                             unsigned long long* partial, hipStream_t s) {
-    uint32_t nb = radix_blocks(n);
-    rs_upsweep_kernel<BITS><<<nb, RS_THREADS, 0, s>>>(ki, n, n_dev, shift, nb, hist);
-    hipError_t e = launch_scan_u32_exclusive(hist, (uint32_t)((1u << BITS) * nb), goff, partial, s);
-    if (e != hipSuccess) return e;
-    if (vi) rs_downsweep_kernel<BITS, true><<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, nb, goff, ko, vo);
-    else rs_downsweep_kernel<BITS, false><<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, nb, goff, ko, nullptr);
-    return hipGetLastError();
+    if (radix_rounds(n) == 16) return sort_pass_r<BITS, 16>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s);
+    return sort_pass_r<BITS, 4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s);
 }
 
 // Stable LSD sort of n keys (with optional uint32 values) on key bits

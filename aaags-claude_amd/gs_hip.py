@@ -77,6 +77,8 @@ def load_library(path: str = LIB_PATH):
         _LIB.gs_duplicate_keys.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _u32]
         _LIB.gs_scan_tiles.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
         _LIB.gs_bin_sorted_words.argtypes = [_vp, _i32, _vp, _i32, _i32, _vp, _u32, _vp, _vp]
+        _LIB.gs_bin_tiles.argtypes = [_vp, _i32, _vp, _i32, _i32, _vp, _u32, _vp, _vp]
+        _LIB.gs_ctx_set_bin_chunk.argtypes = [_vp, _i32]
     return _LIB
 
 
@@ -117,6 +119,9 @@ class GsHip:
     def set_tuning(self, radix_bits=0, raster_batch=0, raster_bwd_batch=0):
         self._check(self.lib.gs_ctx_set_tuning(self.ctx, _i32(radix_bits), _i32(raster_batch), _i32(raster_bwd_batch)),
                     "gs_ctx_set_tuning")
+
+    def set_bin_chunk(self, gaussians_per_chunk):
+        self._check(self.lib.gs_ctx_set_bin_chunk(self.ctx, _i32(gaussians_per_chunk)), "gs_ctx_set_bin_chunk")
 
     def sync(self):
         self._check(self.lib.gs_ctx_synchronize(self.ctx), "gs_ctx_synchronize")
@@ -234,6 +239,24 @@ class GsHip:
                     "gs_bin_sorted_words")
         return words[:K.value], ranges, int(K.value)
 
+    def bin_tiles(self, pre, width, height, capacity=None):
+        """Direct-placement binning: (values[K] sorted per tile, ranges, K)."""
+        n = pre["radii"].numel()
+        num_tiles = ((width + 15) // 16) * ((height + 15) // 16)
+        ranges = torch.empty(num_tiles * 2, dtype=torch.int32, device=self.dev)
+        K = ctypes.c_longlong()
+        if capacity is None:
+            st = self.lib.gs_bin_tiles(self.ctx, n, ctypes.byref(self._pre_struct(pre)), width, height, None, _u32(0),
+                                       _vp(ptr(ranges)), ctypes.byref(K))
+            if st not in (0, 3):
+                self._check(st, "gs_bin_tiles")
+            capacity = max(int(K.value), 1)
+        values = torch.empty(capacity, dtype=torch.int32, device=self.dev)
+        self._check(self.lib.gs_bin_tiles(self.ctx, n, ctypes.byref(self._pre_struct(pre)), width, height,
+                                          _vp(ptr(values)), _u32(capacity), _vp(ptr(ranges)), ctypes.byref(K)),
+                    "gs_bin_tiles")
+        return values[:K.value], ranges, int(K.value)
+
     def raster_fwd_words(self, pre, words, ranges, width, height, bg=(0.0, 0.0, 0.0)):
         d = self.dev
         img = torch.empty((height, width, 3), device=d)
@@ -344,7 +367,14 @@ class GsHip:
                     px=px)
 
     def forward_fast(self, p, cam, cfg):
-        """Single-view forward with depth-ordered binning (the trainer's path)."""
+        """Single-view forward with direct-placement binning (the trainer's path)."""
+        pre = self.preprocess(p, cam, cfg)
+        vals, ranges, K = self.bin_tiles(pre, cam.width, cam.height)
+        img, px = self.raster_fwd(pre, vals, ranges, cam.width, cam.height, cfg.bg_color)
+        return dict(pre=pre, K=K, values_sorted=vals, ranges=ranges, img=img, px=px)
+
+    def forward_words(self, p, cam, cfg):
+        """Single-view forward with the depth-ordered word sort."""
         pre = self.preprocess(p, cam, cfg)
         words, ranges, K = self.bin_sorted_words(pre, cam.width, cam.height)
         img, px = self.raster_fwd_words(pre, words, ranges, cam.width, cam.height, cfg.bg_color)
@@ -353,8 +383,8 @@ class GsHip:
     def forward_backward_fast(self, p, cam, cfg, gt):
         f = self.forward_fast(p, cam, cfg)
         loss, d_img = self.l1_loss(f["img"], gt)
-        rg = self.raster_bwd_words(f["pre"], f["words"], f["ranges"], cam.width, cam.height, f["px"], d_img,
-                                   cfg.bg_color)
+        rg = self.raster_bwd(f["pre"], f["values_sorted"], f["ranges"], cam.width, cam.height, f["px"], d_img,
+                             cfg.bg_color)
         grads = self.preprocess_bwd(p, cam, cfg, f["pre"]["radii"], rg)
         f.update(loss=loss, d_image=d_img, rgrad=rg, grads=grads)
         return f

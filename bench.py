@@ -31,8 +31,8 @@ def log(*a):
 
 def stage_breakdown(hip, p, cam, cfg, gt, reps):
     """Per-stage device time (ms) of the single-view step on the driver's path
-    (depth-ordered binning), from HIP events recorded on the library's stream
-    (torch's current stream)."""
+    (direct-placement binning), from HIP events recorded on the library's
+    stream (torch's current stream)."""
     import torch
 
     names = ["preprocess", "binning", "raster_fwd", "l1", "zero_grads", "raster_bwd", "preprocess_bwd", "adam"]
@@ -47,16 +47,16 @@ def stage_breakdown(hip, p, cam, cfg, gt, reps):
         ev[0].record()
         pre = hip.preprocess(p, cam, cfg)
         ev[1].record()
-        words, ranges, K = hip.bin_sorted_words(pre, cam.width, cam.height, capacity=cap)
+        vals, ranges, K = hip.bin_tiles(pre, cam.width, cam.height, capacity=cap)
         cap = max(cap or 0, int(K * 1.1) + 1024)
         ev[2].record()
-        img, px = hip.raster_fwd_words(pre, words, ranges, cam.width, cam.height, cfg.bg_color)
+        img, px = hip.raster_fwd(pre, vals, ranges, cam.width, cam.height, cfg.bg_color)
         ev[3].record()
         loss, d_img = hip.l1_loss(img, gt)
         ev[4].record()
         rg = hip.alloc_rgrads(n)
         ev[5].record()
-        rg = hip.raster_bwd_words(pre, words, ranges, cam.width, cam.height, px, d_img, cfg.bg_color, rg=rg)
+        rg = hip.raster_bwd(pre, vals, ranges, cam.width, cam.height, px, d_img, cfg.bg_color, rg=rg)
         ev[6].record()
         grads = hip.preprocess_bwd(p, cam, cfg, pre["radii"], rg)
         ev[7].record()
@@ -66,11 +66,11 @@ def stage_breakdown(hip, p, cam, cfg, gt, reps):
         if r > 0:
             for i, k in enumerate(names):
                 acc[k] += ev[i].elapsed_time(ev[i + 1])
-        del words, ranges, img, px, d_img, rg, grads, pre
+        del vals, ranges, img, px, d_img, rg, grads, pre
     return {k: acc[k] / reps for k in names}, K
 
 
-def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles):
+def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles, table_entries=0):
     """Bytes each stage must move at minimum (per launch), from the per-unit
     figures of the hot-path analysis: preprocess 236 B/G read (SH3) + 48 B/G
     written; sort K*(24*P8 + 8), P8 = ceil((ceil(log2 tiles) + 32) / 8);
@@ -84,9 +84,10 @@ def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles):
         "scan": N * 8,
         "duplicate": N * 20 + K * 12,
         "sort": K * (24 * p8 + 8),
-        # depth-ordered binning: depth keys + 4 depth passes (32 B/G each) + ranked gather/scan (16 B/G),
-        # pair words written (8 B/pair), 2 tile passes (upsweep 8 + downsweep 16 B/pair), ranges (8 B/pair)
-        "binning": N * (16 + 4 * 32 + 16) + K * (8 + 2 * 24 + 8),
+        # direct-placement binning: depth keys (16 B/G) + 4 depth-sort passes (32 B/G each), two walks
+        # over the depth-ordered Gaussians (16 B/G each), the (tile, chunk) table written, scanned
+        # (read + write) and read (16 B/entry), each pair's index written once (4 B/pair)
+        "binning": N * (16 + 4 * 32 + 2 * 16) + table_entries * 16 + K * 4,
         "raster_fwd": K * 40 + npix * 20,
         "l1": npix * 36,
         "raster_bwd": K * 40 + npix * 20 + N * 72,
@@ -234,7 +235,8 @@ def main():
     log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
     npix = cam.width * cam.height
     dom = max(stages, key=lambda k: stages[k])
-    byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
+    table = ((scene.n + 1023) // 1024) * cam.num_tiles + 1  # (tile, chunk) entries at 1024 Gaussians per chunk
+    byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles, table)
     achieved = byts / (stages[dom] * 1e-3) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

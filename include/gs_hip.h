@@ -187,6 +187,18 @@ int gs_sort_pairs(gs_ctx* ctx, uint32_t count, const uint64_t* keys_in, const ui
 int gs_bin_sorted_words(gs_ctx* ctx, int n, const gs_preprocessed* pre, int width, int height,
                         uint64_t* words, uint32_t capacity, uint32_t* ranges, long long* total_host);
 
+/* Binning by direct placement, the trainer's path: replaces TileBinner::bin +
+ * Sorter::sort (tile_binner.h:7-12, sorter.h:7; tile_binner_cpu.cpp:59-99,
+ * sorter_cpu.cpp:6-44).  values[0..K) and ranges are exactly the values_out
+ * and ranges of gs_sort_pairs on the duplicate-with-keys pairs; the keys are
+ * not produced.  The Gaussians are stably sorted by depth bits, then each
+ * index is written once, straight to its slot: per-tile counts of depth-ordered
+ * chunks of Gaussians, scanned tile-major, give every pair's position.  K is
+ * read back and must not exceed `capacity`.  Up to 15000 tiles per image
+ * (GS_ERR_UNSUPPORTED beyond). */
+int gs_bin_tiles(gs_ctx* ctx, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
+                 uint32_t capacity, uint32_t* ranges, long long* total_host);
+
 /* Forward rasterization: Rasterizer::rasterize (rasterizer.h:7-15), the
  * rasterize kernel (gpu/kernels/rasterize.cl:10-274, 2D lines) and
  * RasterizerCPU::rasterize (cpu/rasterizer_cpu.cpp:137-198).  One workgroup

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 import torch
 
+from gs_hip import GsError
 from gs_scenes import RenderCfg, make_workload, random_small_scene
 from helpers import (IMAGE_TOL, assert_grad_close, assert_image_close, load_golden, rel_err, scene_dirs, u32)
 
@@ -117,6 +118,59 @@ def test_depth_ordered_binning_bitexact(hip, oracle, n, deg, seed, radix_bits):
     assert np.array_equal((w & np.uint64(0xFFFFFFFF)).astype(np.uint32), vs_ref)
     assert np.array_equal((w >> np.uint64(32)).astype(np.uint32), (ks_ref >> np.uint64(32)).astype(np.uint32))
     assert np.array_equal(u32(ranges), rg_ref)
+
+
+@pytest.mark.parametrize("n,deg,seed", SMALL_CASES[1:] + [(10000, 0, 0)])
+@pytest.mark.parametrize("chunk", [256, 1024])
+def test_direct_binning_bitexact(hip, oracle, n, deg, seed, chunk):
+    """gs_bin_tiles: values and ranges equal to the reference's
+    duplicate-with-keys + stable 64-bit sort, bit for bit, with several
+    chunks per view (chunk 256) and with one (chunk 1024 on small scenes)."""
+    _say("direct binning", n, deg, seed, chunk)
+    if n == 10000:
+        scene, cams, _ = make_workload("c1_10k_256")
+        cam = cams[0]
+    else:
+        scene, cam, _ = random_small_scene(n, deg, seed, width=160, height=120)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    g = oracle.activate(scene)
+    ref = oracle.preprocess(g, scene.n, scene.sh_degree, scene.max_coeffs, cam, cfg, cache=False)
+    if n == 200:  # force depth ties: equal depths must keep index order
+        ref["depths"][::3] = ref["depths"][0]
+    keys_ref, vals_ref = oracle.duplicate(ref, scene.n, cam.width, cam.height)
+    ks_ref, vs_ref, rg_ref = oracle.sort_pairs(keys_ref, vals_ref, cam.num_tiles)
+    pre = _to_dev(hip, {k: ref[k] for k in ["means2D", "depths", "conics", "op2d", "rgb", "radii", "tiles"]})
+    hip.set_bin_chunk(chunk)
+    try:
+        vals, ranges, K = hip.bin_tiles(pre, cam.width, cam.height)
+    finally:
+        hip.set_bin_chunk(1024)
+    assert K == len(keys_ref)
+    assert np.array_equal(u32(vals), vs_ref)
+    assert np.array_equal(u32(ranges), rg_ref)
+
+
+def test_direct_binning_capacity_and_empty(hip, oracle):
+    """Capacity errors report K without writing; an all-culled scene gives
+    K = 0 and zero ranges."""
+    _say("direct binning capacity/empty")
+    scene, cam, _ = random_small_scene(500, 1, 88, width=96, height=80)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    g = oracle.activate(scene)
+    ref = oracle.preprocess(g, scene.n, scene.sh_degree, scene.max_coeffs, cam, cfg, cache=False)
+    pre = _to_dev(hip, {k: ref[k] for k in ["means2D", "depths", "conics", "op2d", "rgb", "radii", "tiles"]})
+    K_ref = int(ref["tiles"].astype(np.int64).sum())
+    with pytest.raises(GsError):
+        hip.bin_tiles(pre, cam.width, cam.height, capacity=max(K_ref - 1, 1))
+    vals, ranges, K = hip.bin_tiles(pre, cam.width, cam.height, capacity=K_ref)
+    assert K == K_ref
+    culled = dict(ref)
+    culled["radii"] = np.zeros_like(ref["radii"])
+    culled["tiles"] = np.zeros_like(ref["tiles"])
+    pre0 = _to_dev(hip, {k: culled[k] for k in ["means2D", "depths", "conics", "op2d", "rgb", "radii", "tiles"]})
+    vals, ranges, K = hip.bin_tiles(pre0, cam.width, cam.height)
+    assert K == 0
+    assert not u32(ranges).any()
 
 
 @pytest.mark.parametrize("radix_bits", [4, 5, 6, 7, 8])
