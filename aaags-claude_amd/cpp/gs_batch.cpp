@@ -4,8 +4,8 @@
 // read once per step by a multi-view preprocess, all views are binned
 // together (depth sort on view << 32 | depth bits, then direct placement of
 // each Gaussian index into its tiles' lists, k_bin.hip; views with more than
-// GS_BIN_MAX_TILES tiles use a tile sort of view << 48 | tile << 32 | index
-// words instead), the forward and
+// GS_BIN_MAX_TILES_XY tiles per side use a tile sort of view << 48 | tile << 32
+// | index words instead), the forward and
 // backward raster each run as one launch over the global (view, tile) list
 // (optionally a persistent grid pulling tiles from an atomic counter), and the
 // multi-view preprocess backward sums the views' gradients in registers and
@@ -58,9 +58,10 @@ struct gs_batch {
     int persistent_groups = 0;
     uint32_t capacity = 0;
     bool direct = true;            // direct-placement binning (else the word sort)
-    int bin_G = 0, bin_C = 0;      // Gaussians per chunk, chunks per view
-    size_t bin_L = 0;              // (view, tile, chunk) table entries + 1
-    int max_tiles = 0;
+    int bin_G = 0, bin_C = 0;      // Gaussians per group, groups per view
+    size_t bin_L = 0;              // (view, tile, group) table entries + 1
+    int bin_blocks = 0, bin_tpl = 2;
+    int num_cus = 0;
 
     hipStream_t stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -72,7 +73,7 @@ struct gs_batch {
     DevBuf means2D, depths, conics, op2d, rgb, radii, tiles;
     DevBuf dkeys, didx, dkeys_s, didx_s, rtiles, roff, scan_partial, dsort_ws;
     DevBuf words, words_tmp, tsort_ws, ranges;
-    DevBuf values, bviews_dev, bin_table, bin_scanned, bin_partial;
+    DevBuf values, bviews_dev, bvblock_dev, bin_table, bin_scanned, bin_partial, bin_rects;
     DevBuf image, d_image, gt, T_final, n_contrib, last;
     DevBuf rg_means2D, rg_conics, rg_rgb, rg_op;
     DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16
@@ -93,8 +94,8 @@ gs_batch::~gs_batch() {
     graph.release();
     DevBuf* all[] = {&cams_dev, &views_dev, &pixoff_dev, &hp_dev, &counters, &means2D, &depths, &conics, &op2d, &rgb,
                      &radii, &tiles, &dkeys, &didx, &dkeys_s, &didx_s, &rtiles, &roff, &scan_partial, &dsort_ws,
-                     &words, &words_tmp, &tsort_ws, &ranges, &values, &bviews_dev, &bin_table, &bin_scanned,
-                     &bin_partial, &image, &d_image, &gt, &T_final, &n_contrib, &last,
+                     &words, &words_tmp, &tsort_ws, &ranges, &values, &bviews_dev, &bvblock_dev, &bin_table, &bin_scanned,
+                     &bin_partial, &bin_rects, &image, &d_image, &gt, &T_final, &n_contrib, &last,
                      &rg_means2D, &rg_conics, &rg_rgb, &rg_op, &losses, &l1_partial, &small};
     for (DevBuf* b : all) b->release();
     if (ev_in) (void)hipEventDestroy(ev_in);
@@ -135,6 +136,42 @@ static int batch_size_pairs(gs_batch* b, uint32_t cap) {
     return GS_OK;
 }
 
+// Direct-placement binning layout for the current group size / bin shape:
+// groups per view, the (view, tile, group) table and each view's first block.
+static int batch_setup_binning(gs_batch* b, int group, int tpl) {
+    b->bin_G = group;
+    b->bin_tpl = tpl;
+    b->bin_C = (b->N + b->bin_G - 1) / b->bin_G;
+    b->bin_L = (size_t)b->bin_C * (size_t)b->total_tiles + 1;
+    if (b->bin_L >= 0xFFFFFFFFull) return gs_set_error(GS_ERR_INVALID, "binning table too large");
+    const int V = b->V;
+    std::vector<int4> bviews(V);
+    std::vector<int> vblock0(V + 1, 0);
+    size_t tab = 0;
+    for (int v = 0; v < V; v++) {
+        const int gx = b->views[v].z, gy = (b->views[v].y + 15) / 16;
+        bviews[v] = int4{gx, gy, (int)(uint32_t)tab, b->views[v].w};
+        tab += (size_t)gx * gy * b->bin_C;
+        vblock0[v + 1] = vblock0[v] + bin_blocks(b->bin_C, gx, gy, b->bin_tpl);
+    }
+    b->bin_blocks = vblock0[V];
+    const size_t VN = (size_t)V * b->N;
+    B_TRY(b->bviews_dev.ensure(sizeof(int4) * V));
+    B_TRY(b->bvblock_dev.ensure(sizeof(int) * (V + 1)));
+    B_TRY(b->bin_table.ensure(b->bin_L * 4));
+    B_TRY(b->bin_scanned.ensure(b->bin_L * 4));
+    B_TRY(b->bin_rects.ensure(VN * 8));
+    B_TRY(b->bin_partial.ensure(((b->bin_L + SCAN_BLOCK - 1) / SCAN_BLOCK + 1) * 8));
+    B_HIP(hipMemcpyAsync(b->bviews_dev.p, bviews.data(), sizeof(int4) * V, hipMemcpyHostToDevice, b->stream),
+          "hipMemcpyAsync(views)");
+    B_HIP(hipMemcpyAsync(b->bvblock_dev.p, vblock0.data(), sizeof(int) * (V + 1), hipMemcpyHostToDevice, b->stream),
+          "hipMemcpyAsync(blocks)");
+    B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");  // the host vectors die here
+    b->graph.release();
+    b->graph_ready = false;
+    return GS_OK;
+}
+
 extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cams, const gs_render_cfg* cfg,
                                gs_batch** out) {
     if (!tr || !cams || !cfg || !out || n_views <= 0)
@@ -150,7 +187,7 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
     b->radix_bits = tr->ctx->radix_bits;
     b->raster_batch = tr->ctx->raster_batch;
     b->raster_bwd_batch = tr->ctx->raster_bwd_batch;
-    int max_tiles = 0;
+    int max_tiles = 0, max_side = 0;
     b->pix_off.push_back(0);
     for (int v = 0; v < n_views; v++) {
         int W = cams[v].width, H = cams[v].height;
@@ -162,19 +199,13 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
         b->views.push_back(int4{W, H, gx, b->total_tiles});
         b->total_tiles += gx * gy;
         max_tiles = std::max(max_tiles, gx * gy);
+        max_side = std::max(max_side, std::max(gx, gy));
         b->pix_off.push_back(b->pix_off.back() + W * H);
     }
     b->total_pix = b->pix_off.back();
     b->tile_bits = std::max(1, bits_for(max_tiles));
     b->view_bits = bits_for(n_views);
-    b->max_tiles = max_tiles;
-    b->direct = max_tiles <= GS_BIN_MAX_TILES;
-    // chunks of the depth-ordered list: larger with more views, so that the
-    // (view, tile, chunk) table stays a small fraction of the pair array
-    b->bin_G = std::min(65536, tr->ctx->bin_chunk * (n_views >= 8 ? 4 : n_views >= 2 ? 2 : 1));
-    b->bin_C = (b->N + b->bin_G - 1) / b->bin_G;
-    b->bin_L = (size_t)b->bin_C * (size_t)b->total_tiles + 1;
-    if (b->direct && b->bin_L >= 0xFFFFFFFFull) b->direct = false;
+    b->direct = max_side <= GS_BIN_MAX_TILES_XY;
     int st = GS_OK;
     auto fail = [&](int s) {
         delete b;
@@ -182,7 +213,9 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
     };
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, tr->ctx->device) != hipSuccess) return fail(gs_set_error(GS_ERR_HIP, "props"));
-    b->persistent_groups = prop.multiProcessorCount * 6;
+    b->num_cus = prop.multiProcessorCount;
+    b->persistent_groups = b->num_cus * std::max(1, tr->ctx->persistent_per_cu);
+    if (tr->ctx->persistent_per_cu == 0) b->persistent = 0;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
@@ -210,25 +243,12 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
     for (int v = 0; v < n_views; v++) max_pix = std::max(max_pix, b->pix_off[v + 1] - b->pix_off[v]);
     b->l1_nb = std::max(1, std::min(512, (max_pix * 3 + 255) / 256 / 4));
     if ((st = b->l1_partial.ensure((size_t)n_views * b->l1_nb * 8))) return fail(st);
-    std::vector<int4> bviews(n_views);
     if (b->direct) {
-        size_t tab = 0;
-        for (int v = 0; v < n_views; v++) {
-            const int gx = b->views[v].z, gy = (b->views[v].y + 15) / 16;
-            bviews[v] = int4{gx, gy, (int)(uint32_t)tab, b->views[v].w};
-            tab += (size_t)gx * gy * b->bin_C;
-        }
-        if ((st = b->bviews_dev.ensure(sizeof(int4) * n_views)) || (st = b->bin_table.ensure(b->bin_L * 4)) ||
-            (st = b->bin_scanned.ensure(b->bin_L * 4)) ||
-            (st = b->bin_partial.ensure(((b->bin_L + SCAN_BLOCK - 1) / SCAN_BLOCK + 1) * 8)))
-            return fail(st);
+        if ((st = batch_setup_binning(b, tr->ctx->bin_chunk, tr->ctx->bin_tpl))) return fail(st);
     } else if ((st = b->rtiles.ensure(VN * 4)) || (st = b->roff.ensure(VN * 4))) {
         return fail(st);
     }
     hipStream_t s = b->stream;
-    if (b->direct && hipMemcpyAsync(b->bviews_dev.p, bviews.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, s) !=
-                         hipSuccess)
-        return fail(gs_set_error(GS_ERR_HIP, "gs_batch_create: upload failed"));
     if (hipMemcpyAsync(b->cams_dev.p, ca.data(), sizeof(gsd::CamArgs) * n_views, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b->views_dev.p, b->views.data(), sizeof(int4) * n_views, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(b->pixoff_dev.p, b->pix_off.data(), sizeof(int) * (n_views + 1), hipMemcpyHostToDevice, s) !=
@@ -293,18 +313,20 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
         a.means2D = reinterpret_cast<const float2*>(pre.means2D);
         a.radii = pre.radii;
         a.views = static_cast<const int4*>(b->bviews_dev.p);
+        a.vblock0 = b->bvblock_dev.i();
         a.table = b->bin_table.u();
         a.table_len = b->bin_L;
+        a.rects = static_cast<uint2*>(b->bin_rects.p);
         a.out = b->values.u();
         a.capacity = b->capacity;
         a.overflow = b->overflow();
-        B_HIP(launch_bin_count(a, b->max_tiles, s), "binning count");
+        B_HIP(launch_bin_count(a, b->bin_blocks, b->bin_tpl, s), "binning count");
         B_HIP(launch_scan_u32_exclusive(b->bin_table.u(), (uint32_t)b->bin_L, b->bin_scanned.u(),
                                         static_cast<unsigned long long*>(b->bin_partial.p), s, b->total32(),
                                         b->total64()),
               "binning scan");
         a.table = b->bin_scanned.u();
-        B_HIP(launch_bin_place(a, b->max_tiles, s), "binning place");
+        B_HIP(launch_bin_place(a, b->bin_blocks, b->bin_tpl, s), "binning place");
         B_HIP(launch_bin_ranges(a, b->bin_scanned.u(), (uint32_t)b->total_tiles, b->ranges.u(), s), "binning ranges");
         return GS_OK;
     }
@@ -350,35 +372,51 @@ static gsr::MultiRaster multi_raster(gs_batch* b, unsigned* counter) {
     return m;
 }
 
-static int enqueue_forward(gs_batch* b, hipStream_t s) {
-    B_TRY(enqueue_binning(b, s));
+static int enqueue_raster_fwd(gs_batch* b, hipStream_t s) {
     unsigned* ctr = b->persistent ? static_cast<unsigned*>(b->counters.p) : nullptr;
-    if (ctr) B_HIP(launch_zero(ctr, 4, s), "zero counter");
+    if (ctr) B_HIP(launch_zero(ctr, 32, s), "zero queue counters");
     gsr::MultiRaster m = multi_raster(b, ctr);
     gsr::MultiPix px{b->image.f(), b->T_final.f(), b->n_contrib.i(), b->last.u(), b->d_image.f()};
     B_HIP(launch_raster_fwd_multi(m, px, b->raster_batch, b->persistent_groups, s), "multi-view raster forward");
     return GS_OK;
 }
 
-static int enqueue_step(gs_batch* b, hipStream_t s) {
-    gs_trainer* t = b->tr;
-    const size_t VN = (size_t)b->V * b->N;
-    B_TRY(enqueue_forward(b, s));
+static int enqueue_forward(gs_batch* b, hipStream_t s) {
+    B_TRY(enqueue_binning(b, s));
+    return enqueue_raster_fwd(b, s);
+}
+
+static int enqueue_l1(gs_batch* b, hipStream_t s) {
     B_HIP(launch_l1_multi(b->image.f(), b->gt.f(), b->d_image.f(), b->pixoff_dev.i(), b->V, b->l1_nb,
                           static_cast<double*>(b->l1_partial.p), static_cast<double*>(b->losses.p), s),
           "multi-view l1");
+    return GS_OK;
+}
+
+// Zeroes the screen-space gradients and runs the multi-view backward raster.
+static int enqueue_raster_bwd(gs_batch* b, hipStream_t s) {
+    const size_t VN = (size_t)b->V * b->N;
     B_HIP(launch_zero(b->rg_means2D.p, VN * 8, s), "zero");
     B_HIP(launch_zero(b->rg_conics.p, VN * 12, s), "zero");
     B_HIP(launch_zero(b->rg_rgb.p, VN * 12, s), "zero");
     B_HIP(launch_zero(b->rg_op.p, VN * 4, s), "zero");
     unsigned* ctr = b->persistent ? static_cast<unsigned*>(b->counters.p) + 16 : nullptr;
-    if (ctr) B_HIP(launch_zero(ctr, 4, s), "zero counter");
+    if (ctr) B_HIP(launch_zero(ctr, 32, s), "zero queue counters");
     gsr::MultiRaster m = multi_raster(b, ctr);
     gsr::MultiPix px{b->image.f(), b->T_final.f(), b->n_contrib.i(), b->last.u(), b->d_image.f()};
     gs_raster_grads rg = b->rgrads();
     gsr::MultiGrads mg{reinterpret_cast<float2*>(rg.d_means2D), rg.d_conics, rg.d_rgb, rg.d_opacities_2d};
     B_HIP(launch_raster_bwd_multi(m, px, mg, b->raster_bwd_batch, b->persistent_groups, s),
           "multi-view raster backward");
+    return GS_OK;
+}
+
+static int enqueue_step(gs_batch* b, hipStream_t s) {
+    gs_trainer* t = b->tr;
+    B_TRY(enqueue_forward(b, s));
+    B_TRY(enqueue_l1(b, s));
+    B_TRY(enqueue_raster_bwd(b, s));
+    gs_raster_grads rg = b->rgrads();
     gs_gaussians g = t->gaussians();
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
     float* P[5];
@@ -570,5 +608,122 @@ extern "C" int gs_batch_info(gs_batch* b, int* total_tiles, long long* total_pix
     if (total_tiles) *total_tiles = b->total_tiles;
     if (total_pixels) *total_pixels = b->total_pix;
     if (capacity) *capacity = b->capacity;
+    return GS_OK;
+}
+
+// ------------------------------------------------------------ autotuning ----
+// Coordinate descent over the batch's knobs, each timed on its own stage with
+// HIP events on the batch's stream (median of `reps` runs after one untimed
+// run): binning bin shape and group size (binning pass), persistent groups per
+// CU (forward + backward raster; 0 = one group per tile) and the LDS batch
+// depths of the two rasterizers.  Parameters and optimizer state are not
+// touched: only binning, forward, loss and backward raster run.
+namespace {
+
+struct BatchTimer {
+    hipEvent_t a = nullptr, e = nullptr;
+    ~BatchTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
+template <typename F>
+int batch_time(gs_batch* b, BatchTimer& tm, int reps, F&& f, float* ms) {
+    B_TRY(f());
+    std::vector<float> t;
+    for (int r = 0; r < reps; r++) {
+        B_HIP(hipEventRecord(tm.a, b->stream), "hipEventRecord");
+        B_TRY(f());
+        B_HIP(hipEventRecord(tm.e, b->stream), "hipEventRecord");
+        B_HIP(hipEventSynchronize(tm.e), "hipEventSynchronize");
+        float x = 0.0f;
+        B_HIP(hipEventElapsedTime(&x, tm.a, tm.e), "hipEventElapsedTime");
+        t.push_back(x);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return GS_OK;
+}
+
+template <typename Set, typename Run>
+int batch_pick(gs_batch* b, BatchTimer& tm, int reps, const std::vector<int>& cands, Set&& set, Run&& run, int* best,
+               float* best_ms) {
+    float bm = 0.0f;
+    int bv = cands.front();
+    bool have = false;
+    for (int v : cands) {
+        B_TRY(set(v));
+        float ms = 0.0f;
+        B_TRY(batch_time(b, tm, reps, run, &ms));
+        if (!have || ms < bm) {
+            bm = ms;
+            bv = v;
+            have = true;
+        }
+    }
+    B_TRY(set(bv));
+    *best = bv;
+    *best_ms = bm;
+    return GS_OK;
+}
+
+}  // namespace
+
+extern "C" int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out) {
+    if (!b) return gs_set_error(GS_ERR_INVALID, "null batch");
+    if (reps < 1) reps = 1;
+    gs_ctx* c = b->tr->ctx;
+    B_HIP(hipEventRecord(b->ev_in, c->stream), "hipEventRecord");
+    B_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0), "hipStreamWaitEvent");
+    if (b->capacity == 0) {
+        long long K;
+        B_TRY(batch_measure(b, &K));
+    }
+    BatchTimer tm;
+    B_HIP(hipEventCreate(&tm.a), "hipEventCreate");
+    B_HIP(hipEventCreate(&tm.e), "hipEventCreate");
+    gs_tuning res{};
+    res.radix_bits = b->radix_bits;
+    auto bin = [&]() { return enqueue_binning(b, b->stream); };
+    if (b->direct) {
+        B_TRY(batch_pick(b, tm, reps, {1, 2, 4}, [&](int x) { return batch_setup_binning(b, b->bin_G, x); }, bin,
+                         &res.bin_tiles_per_lane, &res.binning_ms));
+        B_TRY(batch_pick(b, tm, reps, {8192, 16384, 32768, 65536},
+                         [&](int x) { return batch_setup_binning(b, x, b->bin_tpl); }, bin, &res.bin_group,
+                         &res.binning_ms));
+    } else {
+        B_TRY(batch_time(b, tm, reps, bin, &res.binning_ms));
+    }
+    B_TRY(bin());
+    // persistent groups per CU, on forward + loss + backward raster
+    const int persistent0 = b->persistent;
+    auto set_ppc = [&](int x) {
+        b->persistent = x > 0 ? 1 : 0;
+        b->persistent_groups = b->num_cus * (x > 0 ? x : 1);
+        return GS_OK;
+    };
+    auto fwd_bwd = [&]() {
+        B_TRY(enqueue_raster_fwd(b, b->stream));
+        B_TRY(enqueue_l1(b, b->stream));
+        return enqueue_raster_bwd(b, b->stream);
+    };
+    float ms_fb = 0.0f;
+    std::vector<int> ppc = {2, 4, 6, 8};
+    if (!persistent0) ppc = {0};
+    else ppc.insert(ppc.begin(), 0);
+    B_TRY(batch_pick(b, tm, reps, ppc, set_ppc, fwd_bwd, &res.persistent_per_cu, &ms_fb));
+    auto fwd = [&]() { return enqueue_raster_fwd(b, b->stream); };
+    B_TRY(batch_pick(b, tm, reps, {128, 256, 512}, [&](int x) { b->raster_batch = x; return GS_OK; }, fwd,
+                     &res.raster_batch, &res.raster_fwd_ms));
+    B_TRY(fwd());
+    B_TRY(enqueue_l1(b, b->stream));
+    auto bwd = [&]() { return enqueue_raster_bwd(b, b->stream); };
+    B_TRY(batch_pick(b, tm, reps, {128, 256}, [&](int x) { b->raster_bwd_batch = x; return GS_OK; }, bwd,
+                     &res.raster_bwd_batch, &res.raster_bwd_ms));
+    B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    b->graph.release();
+    b->graph_ready = false;
+    if (out) *out = res;
     return GS_OK;
 }

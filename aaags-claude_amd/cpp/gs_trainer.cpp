@@ -232,13 +232,12 @@ extern "C" void* gs_trainer_buffer(gs_trainer* t, const char* which) {
 
 // Binning of one view into the trainer's pair buffer: direct placement
 // (gs_bin_tiles, 4-byte Gaussian indices) when the view has at most
-// GS_BIN_MAX_TILES tiles, else the depth-ordered word sort
+// GS_BIN_MAX_TILES_XY tiles per side, else the depth-ordered word sort
 // (gs_bin_sorted_words, 8-byte words).  The buffer grows when a view needs more
 // pairs than it holds.  Sets t->pair_stride (uint32 entries per pair).
 static int tr_bin(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs_preprocessed& pre, long long* K) {
     gs_ctx* c = t->ctx;
-    const int tiles = ((cam->width + 15) / 16) * ((cam->height + 15) / 16);
-    const bool words = tiles > GS_BIN_MAX_TILES;
+    const bool words = (cam->width + 15) / 16 > GS_BIN_MAX_TILES_XY || (cam->height + 15) / 16 > GS_BIN_MAX_TILES_XY;
     const size_t per = words ? 8 : 4;
     for (int attempt = 0; attempt < 2; attempt++) {
         const uint32_t cap = (uint32_t)std::min<size_t>(t->keys_sorted.bytes / per, 0xFFFFFFFFu);

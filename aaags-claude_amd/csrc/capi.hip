@@ -396,14 +396,45 @@ extern "C" int gs_bin_sorted_words(gs_ctx* c, int n, const gs_preprocessed* pre,
     return GS_OK;
 }
 
-// Chunk size of the direct-placement binning: Gaussians per chunk.  Larger
-// chunks shrink the (tile, chunk) table; smaller ones shorten each block's walk.
-extern "C" int gs_ctx_set_bin_chunk(gs_ctx* c, int gaussians_per_chunk) {
+// Direct-placement binning knobs (0 keeps a value): Gaussians per group
+// (larger groups shrink the (tile, group) table and lengthen each block's walk
+// and its contiguous output runs) and tiles per lane (bins of 8 x 8*tpl tiles;
+// larger bins mean fewer walks over the records, more work per visit).
+extern "C" int gs_ctx_set_binning(gs_ctx* c, int gaussians_per_group, int tiles_per_lane) {
     GS_CHECK_ARG(c != nullptr, "null context");
-    GS_CHECK_ARG(gaussians_per_chunk >= 256 && gaussians_per_chunk <= 65536 &&
-                     (gaussians_per_chunk & (gaussians_per_chunk - 1)) == 0,
-                 "chunk size must be a power of two in [256, 65536]");
-    c->bin_chunk = gaussians_per_chunk;
+    if (gaussians_per_group) {
+        GS_CHECK_ARG(gaussians_per_group >= 256 && gaussians_per_group <= 65536 &&
+                         (gaussians_per_group & (gaussians_per_group - 1)) == 0,
+                     "group size must be a power of two in [256, 65536]");
+        c->bin_chunk = gaussians_per_group;
+    }
+    if (tiles_per_lane) {
+        GS_CHECK_ARG(tiles_per_lane == 1 || tiles_per_lane == 2 || tiles_per_lane == 4,
+                     "tiles per lane must be 1, 2 or 4");
+        c->bin_tpl = tiles_per_lane;
+    }
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_get_tuning(gs_ctx* c, gs_tuning* out) {
+    GS_CHECK_ARG(c && out, "null argument");
+    *out = gs_tuning{};
+    out->radix_bits = c->radix_bits;
+    out->raster_batch = c->raster_batch;
+    out->raster_bwd_batch = c->raster_bwd_batch;
+    out->bin_group = c->bin_chunk;
+    out->bin_tiles_per_lane = c->bin_tpl;
+    out->persistent_per_cu = c->persistent_per_cu;
+    return GS_OK;
+}
+
+extern "C" int gs_ctx_apply_tuning(gs_ctx* c, const gs_tuning* t) {
+    GS_CHECK_ARG(c && t, "null argument");
+    GS_CHECK_ARG(t->persistent_per_cu >= 0 && t->persistent_per_cu <= 64, "persistent_per_cu must be in [0, 64]");
+    int st = gs_ctx_set_tuning(c, t->radix_bits, t->raster_batch, t->raster_bwd_batch);
+    if (st) return st;
+    if ((st = gs_ctx_set_binning(c, t->bin_group, t->bin_tiles_per_lane))) return st;
+    c->persistent_per_cu = t->persistent_per_cu;
     return GS_OK;
 }
 
@@ -417,9 +448,10 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     if (st) return st;
     const int gx = (width + 15) / 16, gy = (height + 15) / 16;
     const int num_tiles = gx * gy;
-    if (num_tiles > GS_BIN_MAX_TILES)
-        return gs_set_error(GS_ERR_UNSUPPORTED, "gs_bin_tiles: %d tiles exceed %d (use gs_bin_sorted_words)",
-                            num_tiles, GS_BIN_MAX_TILES);
+    if (gx > GS_BIN_MAX_TILES_XY || gy > GS_BIN_MAX_TILES_XY)
+        return gs_set_error(GS_ERR_UNSUPPORTED, "gs_bin_tiles: %d x %d tiles exceed %d per side (use gs_bin_sorted_words)",
+                            gx, gy, GS_BIN_MAX_TILES_XY);
+    GS_CHECK_ARG(aligned(values, 16) || values == nullptr, "values must be 16-byte aligned");
     if (n == 0) {
         GS_HIP(launch_zero(ranges, (size_t)num_tiles * 8, c->stream), "zero kernel");
         if (total_host) *total_host = 0;
@@ -436,8 +468,8 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     GS_CHECK_ARG(L < 0xFFFFFFFFull, "binning table too large");
     const size_t np = scan_partials_needed((uint32_t)L) + 1;
     void* ws2;
-    if ((st = gs_ws2_get(c, 2 * al(un * 8) + 2 * al(un * 4) + radix_workspace_bytes((uint32_t)n, 8) + 2 * al(L * 4) +
-                                al(np * 8),
+    if ((st = gs_ws2_get(c, 2 * al(un * 8) + 2 * al(un * 4) + al(un * 8) + radix_workspace_bytes((uint32_t)n, 8) +
+                                2 * al(L * 4) + al(np * 8),
                          &ws2)))
         return st;
     char* w = static_cast<char*>(ws2);
@@ -445,12 +477,14 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     uint64_t* dkeys_s = reinterpret_cast<uint64_t*>(w); w += al(un * 8);
     uint32_t* didx = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
     uint32_t* order = reinterpret_cast<uint32_t*>(w); w += al(un * 4);
+    uint2* rects = reinterpret_cast<uint2*>(w); w += al(un * 8);
     uint32_t* table = reinterpret_cast<uint32_t*>(w); w += al(L * 4);
     uint32_t* scanned = reinterpret_cast<uint32_t*>(w); w += al(L * 4);
     unsigned long long* partial = reinterpret_cast<unsigned long long*>(w); w += al(np * 8);
     void* dsort_ws = w;
     GS_HIP(launch_depth_keys(n, pre->depths, pre->radii, dkeys, didx, c->stream), "depth keys kernel");
-    GS_HIP(launch_radix_sort_bits(dkeys, didx, dkeys_s, order, (uint32_t)n, 0, 32, 8, dsort_ws, c->stream, nullptr),
+    GS_HIP(launch_radix_sort_bits(dkeys, didx, dkeys_s, order, (uint32_t)n, 0, 32, c->radix_bits, dsort_ws, c->stream,
+                                  nullptr),
            "depth sort");
     gsb::BinArgs a{};
     a.V = 1;
@@ -462,13 +496,16 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     a.radii = pre->radii;
     a.views = nullptr;
     a.view0 = make_int4(gx, gy, 0, 0);
+    a.vblock0 = nullptr;
     a.table = table;
     a.table_len = L;
+    a.rects = rects;
     a.out = values;
     a.capacity = capacity;
     a.overflow = ovf;
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
-    GS_HIP(launch_bin_count(a, num_tiles, c->stream), "binning count kernel");
+    const int blocks = bin_blocks(C, gx, gy, c->bin_tpl);
+    GS_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");
     GS_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr, total64),
            "binning scan");
     unsigned long long K = 0;
@@ -479,7 +516,7 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     if (K > 0) {
         GS_CHECK_ARG(values != nullptr, "null values buffer");
         a.table = scanned;
-        GS_HIP(launch_bin_place(a, num_tiles, c->stream), "binning place kernel");
+        GS_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
     }
     GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
     return GS_OK;

@@ -221,6 +221,19 @@ struct CamArgs {
 
 }  // namespace gsd
 
+namespace gsd {
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8) and each XCD has its own L2.  This bijection on [0, nb) gives
+// every XCD one contiguous run of work items, so items that share data
+// (neighbouring tiles, neighbouring chunks) share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    constexpr int XCD = 8;
+    const int k = nb / XCD, r = nb % XCD;
+    const int x = b % XCD, q = b / XCD;
+    return x * k + (x < r ? x : r) + q;
+}
+}  // namespace gsd
+
 // Kernel argument blocks shared by the launchers and the C ABI layer.
 namespace gsr {
 struct RasterArgs {
@@ -277,15 +290,17 @@ namespace gsb {
 // Direct-placement binning over V views of N Gaussians each (k_bin.hip).
 struct BinArgs {
     int V, N;
-    int C, G;               // chunks per view, Gaussians per chunk
+    int C, G;               // groups per view, Gaussians per group
     const uint32_t* sidx;   // [V*N]: each view's Gaussian indices in depth order
     const float2* means2D;  // view-major [V][N]
     const int* radii;       // [V][N]
     const int4* views;      // per view: (tiles per row, tile rows, table offset, first global tile)
     int4 view0;             // used instead when views == nullptr (one view)
-    uint32_t* table;        // count: per-(view, tile, chunk) counts; place: their exclusive scan
+    const int* vblock0;     // per view: first block (V + 1 entries); nullptr for one view
+    uint32_t* table;        // count: per-(view, tile, group) counts; place: their exclusive scan
     size_t table_len;       // sum over views of tiles * C, plus one trailing entry
-    uint32_t* out;          // place: pair values (Gaussian index)
+    uint2* rects;           // [V*N] scratch: depth-ordered (x0|x1<<8|y0<<16|y1<<24, index)
+    uint32_t* out;          // place: pair values (Gaussian index); 16-byte aligned
     uint32_t capacity;
     uint32_t* overflow;
 };

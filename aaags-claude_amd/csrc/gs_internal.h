@@ -31,7 +31,9 @@ struct gs_ctx {
     int radix_bits = 8;
     int raster_batch = 256;
     int raster_bwd_batch = 256;
-    int bin_chunk = 1024;  // Gaussians per chunk in gs_bin_tiles
+    int bin_chunk = 16384;  // Gaussians per group in direct-placement binning
+    int bin_tpl = 2;        // tiles per lane (bin = 8 x 8*tpl tiles)
+    int persistent_per_cu = 6;  // batched raster: persistent groups per CU (0: one group per tile)
 };
 
 int gs_set_error(int code, const char* fmt, ...);
@@ -44,7 +46,7 @@ int gs_tile_key_bits(int num_tiles);
 
 extern "C" int gs_ctx_set_tuning(gs_ctx* c, int radix_bits, int raster_batch, int raster_bwd_batch);
 extern "C" int gs_ctx_reserve(gs_ctx* c, size_t bytes);
-extern "C" int gs_ctx_set_bin_chunk(gs_ctx* c, int gaussians_per_chunk);
+extern "C" int gs_ctx_set_binning(gs_ctx* c, int gaussians_per_group, int tiles_per_lane);
 extern "C" int gs_check_overflow(gs_ctx* c, int* overflowed);
 
 // kernel launchers
@@ -100,11 +102,11 @@ hipError_t launch_gather_tiles_multi(uint32_t total, int N, const uint64_t* skey
 hipError_t launch_duplicate_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
                                   const float* means2D, const int* radii, const uint32_t* offsets, const int4* views,
                                   uint64_t* words, uint32_t capacity, uint32_t* overflow, hipStream_t s);
-// direct-placement binning (k_bin.hip); views with more tiles are rejected
-constexpr int GS_BIN_MAX_TILES = 15000;
-size_t bin_lds_bytes(int max_tiles);
-hipError_t launch_bin_count(const gsb::BinArgs& a, int max_tiles, hipStream_t s);
-hipError_t launch_bin_place(const gsb::BinArgs& a, int max_tiles, hipStream_t s);
+// direct-placement binning (k_bin.hip); larger views use the word sort
+constexpr int GS_BIN_MAX_TILES_XY = 255;  // tiles per row and per column (4080 pixels)
+int bin_blocks(int C, int gx, int gy, int tpl);
+hipError_t launch_bin_count(const gsb::BinArgs& a, int blocks, int tpl, hipStream_t s);
+hipError_t launch_bin_place(const gsb::BinArgs& a, int blocks, int tpl, hipStream_t s);
 hipError_t launch_bin_ranges(const gsb::BinArgs& a, const uint32_t* scanned, uint32_t total_tiles, uint32_t* ranges,
                              hipStream_t s);
 // total32 / total64 (optional, device): the sum of all n inputs

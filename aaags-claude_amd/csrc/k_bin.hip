@@ -8,103 +8,187 @@
 //
 // Input is each view's Gaussians in depth order: a stable sort by the float
 // bits of depth, so equal depths keep index order, with culled Gaussians last.
-// The list is cut into chunks of G Gaussians.
-//   count:  per chunk, LDS counters (one per tile of the view) count how many
-//           of its Gaussians cover each tile; the counts go to a table laid out
-//           tile-major, (view, tile, chunk), plus one trailing entry.
-//   scan:   one exclusive scan of the table gives, for every (tile, chunk), the
-//           output slot of the chunk's first pair in that tile (the tile-major
+// That list is cut into groups of G Gaussians, and each view's tiles into bins
+// of 8 x (8 * TPL) tiles.  One single-wave block handles one (group, bin), and
+// lane l owns the TPL tiles (l % 8, l / 8 + 8 k) of the bin: a lane's counters
+// and cursors live in its registers, so nothing is shared and no atomics are
+// needed, and the order in which the wave visits Gaussians is the order each
+// tile sees them.
+//   rects:  the depth-ordered Gaussians as packed tile rectangles (8 bytes).
+//   count:  each block walks its group in depth order; every lane counts the
+//           Gaussians whose rectangle covers each of its tiles.  The counts go
+//           to a table laid out tile-major, (view, tile, group), plus one
+//           trailing entry.
+//   scan:   one exclusive scan of the table gives, for every (tile, group), the
+//           output slot of the group's first pair in that tile (the tile-major
 //           layout makes the tile starts part of the same scan), and the total.
-//   place:  each chunk walks its Gaussians again in depth order with the
-//           counters set from the scanned table and writes each Gaussian index
-//           at counter++ -- its stable rank in the tile.
-//   ranges: (start, end) of a tile are the scanned entries of its first chunk
-//           and of the next tile's first chunk.
-// The four waves of a block split the view's tile rows into bands, so every
-// counter has a single owner wave and each tile sees its Gaussians in order.
-// HBM traffic: 4 B per pair written once, ~16 B per Gaussian read twice, and the
-// table (4 B per tile and chunk) written, scanned and read -- no pass over the
-// pairs is repeated.
+//   place:  the same walk again with each lane's cursors set from the scanned
+//           table; each Gaussian index goes to slot cursor++ -- its stable rank
+//           in the tile.  A lane collects its tile's slots in a 16-entry
+//           (64-byte) LDS line buffer and writes the line with one store when
+//           it fills, so HBM sees whole-line writes except at the two ends of
+//           each (tile, group) run.
+//   ranges: (start, end) of a tile are the scanned entries of its first group
+//           and of the next tile's first group.
+// A ballot over 64 rectangles at a time skips the Gaussians that miss the bin,
+// so a block's cost is its group's length / 64 plus the Gaussians that touch
+// its bin.  HBM traffic: 4 B per pair written once, the 8-byte records (read
+// once per bin, mostly from cache), and the table (4 B per tile and group).
 #include "gs_device.h"
 
 namespace gsb {
 
-constexpr int BIN_THREADS = 256;
-constexpr int BIN_WAVES = BIN_THREADS / 64;
-constexpr int BIN_STAGE = BIN_THREADS;  // Gaussian rectangles staged per round
+constexpr int LINE = 16;  // pair slots per 64-byte output line
 
-template <bool PLACE>
-__global__ __launch_bounds__(BIN_THREADS) void bin_chunk_kernel(BinArgs a) {
-    extern __shared__ uint32_t cnt[];  // one counter per tile of the view
-    __shared__ int4 s_rect[BIN_STAGE];  // x0, y0, x1, y1 (empty when x0 == x1)
-    __shared__ uint32_t s_id[BIN_STAGE];
-
-    const int q = blockIdx.x;
-    const int v = q / a.C, c = q - v * a.C;
-    const int4 vi = a.views ? a.views[v] : a.view0;  // (tiles per row, tile rows, table offset, first tile)
-    const int gx = vi.x, gy = vi.y, T = gx * gy;
-    const size_t tab = (size_t)(uint32_t)vi.z;
-    if (!PLACE && q == 0 && threadIdx.x == 0) a.table[a.table_len - 1] = 0u;
-    for (int t = threadIdx.x; t < T; t += BIN_THREADS) cnt[t] = PLACE ? a.table[tab + (size_t)t * a.C + c] : 0u;
-
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int rb = (gy * w) / BIN_WAVES, re = (gy * (w + 1)) / BIN_WAVES;  // this wave's tile rows
+// Depth-ordered rectangle records, one per (view, rank): (x0 | x1 << 8 |
+// y0 << 16 | y1 << 24, Gaussian index); culled Gaussians get an empty
+// rectangle.  Views are at most 255 x 255 tiles.
+__global__ __launch_bounds__(256) void bin_rects_kernel(BinArgs a) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (size_t)a.V * a.N) return;
+    const int v = (int)(r / (size_t)a.N);
+    const int4 vi = a.views ? a.views[v] : a.view0;
     const size_t seg = (size_t)v * a.N;
-    const uint32_t r0 = (uint32_t)c * (uint32_t)a.G;
-    const uint32_t r1 = (r0 + (uint32_t)a.G < (uint32_t)a.N) ? r0 + (uint32_t)a.G : (uint32_t)a.N;
+    const uint32_t id = a.sidx[r];
+    const int rad = a.radii[seg + id];
+    uint2 rec = make_uint2(0u, id);
+    if (rad > 0) {
+        const float2 m = a.means2D[seg + id];
+        int mn[2], mx[2];
+        gsd::get_rect(m.x, m.y, rad, vi.x, vi.y, mn, mx);
+        rec.x = (uint32_t)mn[0] | ((uint32_t)mx[0] << 8) | ((uint32_t)mn[1] << 16) | ((uint32_t)mx[1] << 24);
+    }
+    a.rects[r] = rec;
+}
 
-    for (uint32_t base = r0; base < r1; base += BIN_STAGE) {
-        __syncthreads();  // counters initialised / previous stage consumed
-        const uint32_t r = base + threadIdx.x;
-        int4 rect = make_int4(0, 0, 0, 0);
-        uint32_t id = 0u;
-        if (r < r1) {
-            id = a.sidx[seg + r];
-            const int rad = a.radii[seg + id];
-            if (rad > 0) {
-                const float2 m = a.means2D[seg + id];
-                int mn[2], mx[2];
-                gsd::get_rect(m.x, m.y, rad, gx, gy, mn, mx);
-                rect = make_int4(mn[0], mn[1], mx[0], mx[1]);
-            }
-        }
-        s_rect[threadIdx.x] = rect;
-        s_id[threadIdx.x] = id;
-        __syncthreads();
-        const int nst = (int)((r1 - base) < (uint32_t)BIN_STAGE ? (r1 - base) : (uint32_t)BIN_STAGE);
-        for (int j = 0; j < nst; j++) {
-            const int4 R = s_rect[j];
-            const int x0 = __builtin_amdgcn_readfirstlane(R.x);
-            const int x1 = __builtin_amdgcn_readfirstlane(R.z);
-            int y0 = __builtin_amdgcn_readfirstlane(R.y);
-            int y1 = __builtin_amdgcn_readfirstlane(R.w);
-            y0 = y0 > rb ? y0 : rb;
-            y1 = y1 < re ? y1 : re;
-            if (x0 >= x1 || y0 >= y1) continue;
-            const int wd = x1 - x0;
-            const int np = wd * (y1 - y0);
-            const uint32_t gid = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_id[j]);
-            const float inv = 1.0f / (float)wd;
-            for (int e = lane; e < np; e += 64) {
-                // row of pair e inside the band rectangle: float estimate, then
-                // one correction step (the estimate is off by at most one)
-                int yy = (int)((float)e * inv);
-                if (yy * wd > e) yy--;
-                else if ((yy + 1) * wd <= e) yy++;
-                const int t = (y0 + yy) * gx + x0 + (e - yy * wd);
-                if (PLACE) {
-                    const uint32_t p = atomicAdd(&cnt[t], 1u);
-                    if (p < a.capacity) a.out[p] = gid;
-                    else *a.overflow = 1u;
-                } else {
-                    atomicAdd(&cnt[t], 1u);
-                }
+// The (view, group, bin) of this block.  Blocks of one view are numbered
+// group-major, so the blocks of one XCD (after the remap) share a few groups
+// and their records stay in that XCD's L2.
+template <int TPL>
+struct BinBlock {
+    int v, c, bx0, by0;
+    int4 vi;  // (tiles per row, tile rows, table offset, first global tile)
+};
+
+template <int TPL>
+__device__ __forceinline__ BinBlock<TPL> bin_block(const BinArgs& a) {
+    const int q = gsd::xcd_remap(blockIdx.x, gridDim.x);
+    BinBlock<TPL> b;
+    b.v = 0;
+    if (a.vblock0)
+        while (b.v + 1 < a.V && a.vblock0[b.v + 1] <= q) b.v++;
+    const int local = q - (a.vblock0 ? a.vblock0[b.v] : 0);
+    b.vi = a.views ? a.views[b.v] : a.view0;
+    const int nbx = (b.vi.x + 7) / 8, nby = (b.vi.y + 8 * TPL - 1) / (8 * TPL);
+    b.c = local / (nbx * nby);
+    const int bin = local - b.c * (nbx * nby);
+    b.bx0 = (bin % nbx) * 8;
+    b.by0 = (bin / nbx) * (8 * TPL);
+    return b;
+}
+
+// Walks the block's group in depth order and calls f(k, gid) on every lane
+// whose k-th tile is covered by the Gaussian's rectangle.  Rectangles are read
+// 64 at a time, one per lane (the next 64 prefetched); the ones that meet the
+// bin are visited lowest lane first, i.e. in depth order.
+template <int TPL, typename F>
+__device__ __forceinline__ void walk_bin(const BinArgs& a, const BinBlock<TPL>& b, int tx, int ty, int lane, F&& f) {
+    const uint2* rects = a.rects + (size_t)b.v * a.N;
+    const uint32_t r0 = (uint32_t)b.c * (uint32_t)a.G;
+    const uint32_t r1 = (r0 + (uint32_t)a.G < (uint32_t)a.N) ? r0 + (uint32_t)a.G : (uint32_t)a.N;
+    const int bx1 = b.bx0 + 8, by1 = b.by0 + 8 * TPL;
+    uint2 nxt = make_uint2(0u, 0u);
+    if (r0 + (uint32_t)lane < r1) nxt = rects[r0 + lane];
+    for (uint32_t base = r0; base < r1; base += 64) {
+        const uint2 rec = nxt;
+        const uint32_t rn = base + 64u + (uint32_t)lane;
+        if (rn < r1) nxt = rects[rn];
+        const int x0 = (int)(rec.x & 0xFFu), x1 = (int)((rec.x >> 8) & 0xFFu);
+        const int y0 = (int)((rec.x >> 16) & 0xFFu), y1 = (int)(rec.x >> 24);
+        const bool live = base + (uint32_t)lane < r1;
+        uint64_t todo = __ballot(live && x0 < x1 && y0 < y1 && x0 < bx1 && x1 > b.bx0 && y0 < by1 && y1 > b.by0);
+        while (todo) {
+            const int j = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const uint32_t R = (uint32_t)__builtin_amdgcn_readlane((int)rec.x, j);
+            const uint32_t gid = (uint32_t)__builtin_amdgcn_readlane((int)rec.y, j);
+            const int jx0 = (int)(R & 0xFFu), jx1 = (int)((R >> 8) & 0xFFu);
+            const int jy0 = (int)((R >> 16) & 0xFFu), jy1 = (int)(R >> 24);
+            const bool inx = tx >= jx0 && tx < jx1;
+#pragma unroll
+            for (int k = 0; k < TPL; k++) {
+                const int y = ty + 8 * k;
+                if (inx && y >= jy0 && y < jy1) f(k, gid);
             }
         }
     }
-    if (!PLACE) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < T; t += BIN_THREADS) a.table[tab + (size_t)t * a.C + c] = cnt[t];
+}
+
+template <int TPL>
+__global__ __launch_bounds__(64) void bin_count_kernel(BinArgs a) {
+    const BinBlock<TPL> b = bin_block<TPL>(a);
+    const int lane = threadIdx.x;
+    if (blockIdx.x == 0 && lane == 0) a.table[a.table_len - 1] = 0u;
+    const int tx = b.bx0 + (lane & 7), ty = b.by0 + (lane >> 3);
+    uint32_t cnt[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; k++) cnt[k] = 0u;
+    walk_bin<TPL>(a, b, tx, ty, lane, [&](int k, uint32_t) { cnt[k]++; });
+    const int gx = b.vi.x, gy = b.vi.y;
+    const size_t tab = (size_t)(uint32_t)b.vi.z;
+#pragma unroll
+    for (int k = 0; k < TPL; k++) {
+        const int y = ty + 8 * k;
+        if (tx < gx && y < gy) a.table[tab + (size_t)(y * gx + tx) * a.C + b.c] = cnt[k];
+    }
+}
+
+// Writes slots [lo, hi] of one tile's line buffer: one 64-byte store when the
+// whole line is this block's and below capacity, else slot by slot.
+__device__ __forceinline__ void flush_line(const BinArgs& a, const uint32_t* line, uint32_t lo, uint32_t hi) {
+    if (lo + (LINE - 1) == hi && hi < a.capacity) {
+        const uint4* src = reinterpret_cast<const uint4*>(line);
+        uint4* dst = reinterpret_cast<uint4*>(a.out + lo);
+#pragma unroll
+        for (int k = 0; k < LINE / 4; k++) dst[k] = src[k];
+        return;
+    }
+    for (uint32_t k = lo; k <= hi; k++) {
+        if (k < a.capacity) a.out[k] = line[k & (LINE - 1)];
+        else *a.overflow = 1u;
+    }
+}
+
+template <int TPL>
+__global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
+    __shared__ __align__(16) uint32_t lines[TPL * 64 * LINE];  // per (tile slot, lane): one line
+    const BinBlock<TPL> b = bin_block<TPL>(a);
+    const int lane = threadIdx.x;
+    const int tx = b.bx0 + (lane & 7), ty = b.by0 + (lane >> 3);
+    const int gx = b.vi.x, gy = b.vi.y;
+    const size_t tab = (size_t)(uint32_t)b.vi.z;
+    uint32_t cur[TPL], beg[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; k++) {
+        const int y = ty + 8 * k;
+        cur[k] = (tx < gx && y < gy) ? a.table[tab + (size_t)(y * gx + tx) * a.C + b.c] : 0u;
+        beg[k] = cur[k];
+    }
+    walk_bin<TPL>(a, b, tx, ty, lane, [&](int k, uint32_t gid) {
+        uint32_t* line = lines + (k * 64 + lane) * LINE;
+        const uint32_t p = cur[k]++;
+        line[p & (LINE - 1)] = gid;
+        if ((p & (LINE - 1)) == LINE - 1) {
+            const uint32_t lo = p - (LINE - 1);
+            flush_line(a, line, lo > beg[k] ? lo : beg[k], p);
+        }
+    });
+    // the last, partly filled line of each tile
+#pragma unroll
+    for (int k = 0; k < TPL; k++) {
+        const uint32_t end = cur[k], lo = end & ~(uint32_t)(LINE - 1);
+        const uint32_t from = lo > beg[k] ? lo : beg[k];
+        if (from < end) flush_line(a, lines + (k * 64 + lane) * LINE, from, end - 1u);
     }
 }
 
@@ -131,19 +215,29 @@ __global__ __launch_bounds__(256) void bin_ranges_kernel(const uint32_t* __restr
 
 using namespace gsb;
 
-size_t bin_lds_bytes(int max_tiles) { return (size_t)max_tiles * 4; }
+int bin_blocks(int C, int gx, int gy, int tpl) { return C * ((gx + 7) / 8) * ((gy + 8 * tpl - 1) / (8 * tpl)); }
 
-hipError_t launch_bin_count(const BinArgs& a, int max_tiles, hipStream_t s) {
-    const int blocks = a.V * a.C;
+hipError_t launch_bin_count(const BinArgs& a, int blocks, int tpl, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    bin_chunk_kernel<false><<<blocks, BIN_THREADS, bin_lds_bytes(max_tiles), s>>>(a);
+    const size_t VN = (size_t)a.V * a.N;
+    bin_rects_kernel<<<(unsigned)((VN + 255) / 256), 256, 0, s>>>(a);
+    switch (tpl) {
+        case 1: bin_count_kernel<1><<<blocks, 64, 0, s>>>(a); break;
+        case 2: bin_count_kernel<2><<<blocks, 64, 0, s>>>(a); break;
+        case 4: bin_count_kernel<4><<<blocks, 64, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_bin_place(const BinArgs& a, int max_tiles, hipStream_t s) {
-    const int blocks = a.V * a.C;
+hipError_t launch_bin_place(const BinArgs& a, int blocks, int tpl, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    bin_chunk_kernel<true><<<blocks, BIN_THREADS, bin_lds_bytes(max_tiles), s>>>(a);
+    switch (tpl) {
+        case 1: bin_place_kernel<1><<<blocks, 64, 0, s>>>(a); break;
+        case 2: bin_place_kernel<2><<<blocks, 64, 0, s>>>(a); break;
+        case 4: bin_place_kernel<4><<<blocks, 64, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -13,12 +13,12 @@
 // the reference CPU backward (cpu/rasterizer_backward_cpu.cpp:69-134); the
 // nine per-Gaussian gradient terms are summed across the wave with packed
 // permlane-swap/DPP steps (no LDS traffic), across the four waves with LDS
-// atomics, and land in HBM with one float
-// atomic per (tile, Gaussian, component).
+// atomics, and land in HBM with one float atomic per (tile, Gaussian,
+// component).
 //
 // Multi-view: the same tile bodies run over a global (view, tile) list, either
-// one workgroup per global tile or a persistent grid that takes tiles from an
-// atomic counter.
+// one workgroup per global tile or a persistent grid that takes tiles from
+// per-XCD atomic queues.  Workgroup-to-tile maps are XCD-aware throughout.
 #include "gs_device.h"
 
 namespace gsr {
@@ -297,13 +297,13 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
 template <int BATCH>
 __global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, PixOut o) {
     __shared__ FwdSmem<BATCH> sm;
-    fwd_tile<BATCH>(a, blockIdx.x, sm, o);
+    fwd_tile<BATCH>(a, gsd::xcd_remap(blockIdx.x, gridDim.x), sm, o);
 }
 
 template <int BATCH>
 __global__ __launch_bounds__(256) void raster_bwd_kernel(RasterArgs a, PixIn in, GradArgs g) {
     __shared__ BwdSmem<BATCH> sm;
-    bwd_tile<BATCH>(a, blockIdx.x, sm, in, g);
+    bwd_tile<BATCH>(a, gsd::xcd_remap(blockIdx.x, gridDim.x), sm, in, g);
 }
 
 // ------------------------------------------------------------ multi-view ----
@@ -334,27 +334,52 @@ __device__ __forceinline__ RasterArgs view_args(const MultiRaster& m, int v) {
     return a;
 }
 
-// Next global tile: from the atomic queue (persistent grid) or blockIdx.x.
-// Every group leaves once the counter passes the tile count, which it must,
-// since each pass through the loop takes a fresh counter value.
-__device__ __forceinline__ bool next_tile(const MultiRaster& m, int& g, int& iter, int* s_next) {
+// Next global tile: blockIdx.x (one group per tile, XCD-remapped), or from the
+// persistent work queues.  The queues are 8 counters over 8 contiguous ranges of
+// global tiles; a group starts on the queue of its own XCD (so neighbouring
+// tiles share an L2) and moves to the next queue when one runs dry.  Every
+// pass takes a fresh counter value, so each group leaves after at most
+// total_tiles + 8 passes.
+constexpr int NQ = 8;
+
+struct QueueState {
+    int q;     // queues tried so far
+    int tile;  // broadcast slot
+};
+
+__device__ __forceinline__ bool next_tile(const MultiRaster& m, int& g, int& iter, QueueState* s) {
     if (m.counter == nullptr) {
-        g = blockIdx.x;
-        return iter++ == 0 && g < m.total_tiles;
+        g = gsd::xcd_remap(blockIdx.x, gridDim.x);
+        return iter++ == 0 && blockIdx.x < (unsigned)m.total_tiles;
     }
     __syncthreads();
-    if (threadIdx.x == 0) *s_next = (int)atomicAdd(m.counter, 1u);
+    if (threadIdx.x == 0) {
+        if (iter++ == 0) s->q = 0;
+        const int k = m.total_tiles / NQ, r = m.total_tiles % NQ;
+        int tile = -1;
+        while (s->q < NQ) {
+            const int qi = ((int)(blockIdx.x % NQ) + s->q) % NQ;
+            const int len = k + (qi < r ? 1 : 0);
+            const int i = (int)atomicAdd(&m.counter[qi], 1u);
+            if (i < len) {
+                tile = qi * k + (qi < r ? qi : r) + i;
+                break;
+            }
+            s->q++;
+        }
+        s->tile = tile;
+    }
     __syncthreads();
-    g = *s_next;
-    return g < m.total_tiles;
+    g = s->tile;
+    return g >= 0;
 }
 
 template <int BATCH>
 __global__ __launch_bounds__(256) void raster_fwd_multi_kernel(MultiRaster m, MultiPix o) {
     __shared__ FwdSmem<BATCH> sm;
-    __shared__ int s_next;
+    __shared__ QueueState qs;
     int g, iter = 0;
-    while (next_tile(m, g, iter, &s_next)) {
+    while (next_tile(m, g, iter, &qs)) {
         const int v = view_of(m, g);
         RasterArgs a = view_args(m, v);
         const size_t p0 = (size_t)m.pix_off[v];
@@ -366,9 +391,9 @@ __global__ __launch_bounds__(256) void raster_fwd_multi_kernel(MultiRaster m, Mu
 template <int BATCH>
 __global__ __launch_bounds__(256) void raster_bwd_multi_kernel(MultiRaster m, MultiPix o, MultiGrads gr) {
     __shared__ BwdSmem<BATCH> sm;
-    __shared__ int s_next;
+    __shared__ QueueState qs;
     int g, iter = 0;
-    while (next_tile(m, g, iter, &s_next)) {
+    while (next_tile(m, g, iter, &qs)) {
         const int v = view_of(m, g);
         RasterArgs a = view_args(m, v);
         const size_t p0 = (size_t)m.pix_off[v];
@@ -411,7 +436,7 @@ hipError_t launch_raster_bwd(const RasterArgs& a, int num_tiles, int batch, cons
 }
 
 // grid: one group per global tile, or `persistent_groups` groups pulling
-// tiles from m.counter (which must be zero at launch).
+// tiles from the 8 queue counters at m.counter (zero at launch).
 hipError_t launch_raster_fwd_multi(const MultiRaster& m, const MultiPix& o, int batch, int persistent_groups,
                                    hipStream_t s) {
     if (m.total_tiles <= 0) return hipSuccess;

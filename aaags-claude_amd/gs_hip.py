@@ -51,6 +51,16 @@ class _AdamHP(ctypes.Structure):
                 ("eps", ctypes.c_float), ("bc1", ctypes.c_float), ("bc2", ctypes.c_float)]
 
 
+class _Tuning(ctypes.Structure):
+    _fields_ = [("radix_bits", _i32), ("raster_batch", _i32), ("raster_bwd_batch", _i32), ("bin_group", _i32),
+                ("bin_tiles_per_lane", _i32), ("persistent_per_cu", _i32), ("binning_ms", ctypes.c_float),
+                ("raster_fwd_ms", ctypes.c_float), ("raster_bwd_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {k: (round(getattr(self, k), 4) if isinstance(getattr(self, k), float) else getattr(self, k))
+                for k, _ in self._fields_}
+
+
 class GsError(RuntimeError):
     pass
 
@@ -78,7 +88,7 @@ def load_library(path: str = LIB_PATH):
         _LIB.gs_scan_tiles.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp]
         _LIB.gs_bin_sorted_words.argtypes = [_vp, _i32, _vp, _i32, _i32, _vp, _u32, _vp, _vp]
         _LIB.gs_bin_tiles.argtypes = [_vp, _i32, _vp, _i32, _i32, _vp, _u32, _vp, _vp]
-        _LIB.gs_ctx_set_bin_chunk.argtypes = [_vp, _i32]
+        _LIB.gs_ctx_set_binning.argtypes = [_vp, _i32, _i32]
     return _LIB
 
 
@@ -120,8 +130,18 @@ class GsHip:
         self._check(self.lib.gs_ctx_set_tuning(self.ctx, _i32(radix_bits), _i32(raster_batch), _i32(raster_bwd_batch)),
                     "gs_ctx_set_tuning")
 
-    def set_bin_chunk(self, gaussians_per_chunk):
-        self._check(self.lib.gs_ctx_set_bin_chunk(self.ctx, _i32(gaussians_per_chunk)), "gs_ctx_set_bin_chunk")
+    def tuning(self) -> dict:
+        t = _Tuning()
+        self._check(self.lib.gs_ctx_get_tuning(self.ctx, ctypes.byref(t)), "gs_ctx_get_tuning")
+        return t.as_dict()
+
+    def apply_tuning(self, d: dict):
+        t = _Tuning(**{k: d[k] for k, _ in _Tuning._fields_ if k in d})
+        self._check(self.lib.gs_ctx_apply_tuning(self.ctx, ctypes.byref(t)), "gs_ctx_apply_tuning")
+
+    def set_binning(self, group=0, tiles_per_lane=0):
+        """Direct-placement binning knobs (0 keeps the current value)."""
+        self._check(self.lib.gs_ctx_set_binning(self.ctx, _i32(group), _i32(tiles_per_lane)), "gs_ctx_set_binning")
 
     def sync(self):
         self._check(self.lib.gs_ctx_synchronize(self.ctx), "gs_ctx_synchronize")
@@ -452,6 +472,14 @@ class GsTrainer:
                                                  ctypes.byref(tc), iteration, ctypes.byref(res)), "gs_trainer_step")
         return res
 
+    def autotune(self, cam, cfg, gt, reps=3) -> dict:
+        """gs_autotune on this trainer's parameters and one view (no parameter update)."""
+        c, f = cam.to_c(), cfg.to_c()
+        t = _Tuning()
+        self.hip._check(self.lib.gs_autotune(self.tr, ctypes.byref(c), ctypes.byref(f), _vp(ptr(gt)), _i32(reps),
+                                             ctypes.byref(t)), "gs_autotune")
+        return t.as_dict()
+
     def render(self, cam, cfg):
         k = ctypes.c_longlong()
         c, f = cam.to_c(), cfg.to_c()
@@ -513,6 +541,12 @@ class GsBatch:
                 self.lib.gs_batch_destroy(self.b)
         except Exception:
             pass
+
+    def autotune(self, reps=3) -> dict:
+        """gs_batch_autotune on the batch's views and targets (no parameter update)."""
+        t = _Tuning()
+        self.hip._check(self.lib.gs_batch_autotune(self.b, _i32(reps), ctypes.byref(t)), "gs_batch_autotune")
+        return t.as_dict()
 
     def set_options(self, persistent=True, fused_adam=True, use_graph=True):
         self.hip._check(self.lib.gs_batch_set_options(self.b, int(persistent), int(fused_adam), int(use_graph)),

@@ -126,6 +126,7 @@ def bench_batched(hip, cfg, args):
     # batched, graph-captured step
     tr = GsTrainer(hip, scene)
     b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=True, use_graph=True)
+    tuning = None if args.no_autotune else b.autotune(reps=2)
     for i in range(2):
         b.step(tc, i, sync=True)
     steps = max(2, args.steps // 2)
@@ -149,7 +150,40 @@ def bench_batched(hip, cfg, args):
     return {"workload": args.batched_workload, "views": V, "pixels": pix, "pairs": int(res.pairs),
             "views_per_s": round(batched, 2), "ms_per_batch_step": round(1e3 * dt / steps, 3),
             "plain_views_per_s": round(plain, 2), "batched_vs_single": round(batched / plain, 3),
-            "fwd_mpix_s": round(fwd_mpix, 1), "loss": round(float(res.loss), 6)}
+            "fwd_mpix_s": round(fwd_mpix, 1), "loss": round(float(res.loss), 6), "tuning": tuning}
+
+
+def bench_large(hip, cfg, args):
+    """The large batched step: 6M Gaussians in 32 clusters, 8 views at 1080p,
+    ~10^8 pairs, graph-captured with Adam fused.  Reports views/s, the pair
+    count and the device memory the trainer and batch hold (hipMemGetInfo
+    before and after)."""
+    import torch
+    from gs_hip import GsBatch, GsTrainer, train_cfg
+    from gs_scenes import make_workload
+
+    scene, cams, gts = make_workload(args.large_workload)
+    torch.cuda.synchronize()
+    free0, total = torch.cuda.mem_get_info()
+    tr = GsTrainer(hip, scene)
+    b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=True, use_graph=True)
+    tc = train_cfg(lr=1e-3)
+    for i in range(2):
+        b.step(tc, i, sync=True)
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    steps = 3
+    t0 = time.perf_counter()
+    for i in range(steps):
+        b.step(tc, 2 + i)
+    res, _ = b.result()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"workload": args.large_workload, "gaussians": scene.n, "views": len(cams), "pairs": int(res.pairs),
+           "views_per_s": round(steps * len(cams) / dt, 2), "ms_per_batch_step": round(1e3 * dt / steps, 3),
+           "device_gb_held": round((free0 - free1) / 1e9, 2), "device_gb_total": round(total / 1e9, 1)}
+    del b, tr
+    return out
 
 
 def cpu_baseline(scene, cam, cfg, gt):
@@ -183,6 +217,9 @@ def main():
     ap.add_argument("--breakdown-reps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-workload", default="c4_1m_16views")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default knobs")
+    ap.add_argument("--no-large", action="store_true", help="skip the 6M-Gaussian 8-view batched leg")
+    ap.add_argument("--large-workload", default="c5_6m_8x1080p")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -207,6 +244,11 @@ def main():
     gt = torch.tensor(gt_np, device=hip.dev)
     tr = GsTrainer(hip, scene)
 
+    tuning = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = tr.autotune(cam, cfg, gt, reps=3)
+        log(f"autotune ({time.perf_counter() - t0:.1f}s): {tuning}")
     for i in range(args.warmup):
         r = tr.step(cam, cfg, gt, tc, i)
     torch.cuda.synchronize()
@@ -247,6 +289,11 @@ def main():
         batched = bench_batched(hip, cfg, args)
         log(f"batched: {batched}")
 
+    large = None
+    if not args.no_large:
+        large = bench_large(hip, cfg, args)
+        log(f"large: {large}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, cam, cfg, gt_np)
@@ -269,7 +316,9 @@ def main():
                    "width": cam.width, "height": cam.height, "pairs": int(K), "optimizer": "adam lr 1e-3"},
         "fwd_mpix_s": round(mpix_s, 2),
         "batched": batched,
+        "large": large,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

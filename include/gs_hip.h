@@ -192,10 +192,10 @@ int gs_bin_sorted_words(gs_ctx* ctx, int n, const gs_preprocessed* pre, int widt
  * sorter_cpu.cpp:6-44).  values[0..K) and ranges are exactly the values_out
  * and ranges of gs_sort_pairs on the duplicate-with-keys pairs; the keys are
  * not produced.  The Gaussians are stably sorted by depth bits, then each
- * index is written once, straight to its slot: per-tile counts of depth-ordered
- * chunks of Gaussians, scanned tile-major, give every pair's position.  K is
- * read back and must not exceed `capacity`.  Up to 15000 tiles per image
- * (GS_ERR_UNSUPPORTED beyond). */
+ * index is written once, straight to its slot: per-tile counts over groups of
+ * depth-ordered Gaussians, scanned tile-major, give every pair's position.
+ * K is read back and must not exceed `capacity`; `values` must be 16-byte
+ * aligned.  Up to 255 x 255 tiles (GS_ERR_UNSUPPORTED beyond). */
 int gs_bin_tiles(gs_ctx* ctx, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
                  uint32_t capacity, uint32_t* ranges, long long* total_host);
 
@@ -344,6 +344,41 @@ int gs_batch_result(gs_batch* b, gs_step_result* out, double* view_losses);
 int gs_batch_render(gs_batch* b, long long* pairs);
 void* gs_batch_buffer(gs_batch* b, const char* which);
 int gs_batch_info(gs_batch* b, int* total_tiles, long long* total_pixels, long long* capacity);
+
+/* ------------------------------------------------------------------------
+ * Autotuning (no counterpart in the reference, whose kernels use fixed
+ * work-group sizes and a fixed 4-bit radix).  The tuner times the candidates
+ * of each knob on the caller's own workload with HIP events on the library's
+ * stream, one knob at a time (each knob only affects its own stage), and
+ * leaves the fastest setting on the context.  It runs forward, loss and
+ * backward stages only: parameters and optimizer state are not modified.
+ * The tile size is not a knob: tile lists and ranges are defined on the
+ * reference's 16 x 16 tiles, which parity requires.
+ * ---------------------------------------------------------------------- */
+typedef struct gs_tuning {
+    int radix_bits;         /* depth-sort digit width: 4..8 */
+    int raster_batch;       /* forward LDS batch depth: 64, 128, 256, 512 */
+    int raster_bwd_batch;   /* backward LDS batch depth: 64, 128, 256 */
+    int bin_group;          /* Gaussians per binning group: power of two, 256..65536 */
+    int bin_tiles_per_lane; /* binning bin shape: 1, 2 or 4 (bin = 8 x 8*tpl tiles) */
+    int persistent_per_cu;  /* batched raster: persistent groups per CU (0: one group per tile) */
+    float binning_ms, raster_fwd_ms, raster_bwd_ms; /* best times measured */
+} gs_tuning;
+
+/* Current knobs of a context (times zero) / set all knobs at once (for a
+ * caller that stores a tuning and applies it later). */
+int gs_ctx_get_tuning(gs_ctx* ctx, gs_tuning* out);
+int gs_ctx_apply_tuning(gs_ctx* ctx, const gs_tuning* t);
+
+/* Single-view tuning on the trainer's parameters for one camera; gt is the
+ * device target image used to obtain an image gradient for the backward. */
+int gs_autotune(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt, int reps,
+                gs_tuning* out);
+
+/* Batched tuning: binning knobs and the persistent grid / LDS batch depths of
+ * the multi-view rasterizers, on the batch's views and targets.  The batch's
+ * captured graph, if any, is rebuilt on the next step. */
+int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out);
 
 #ifdef __cplusplus
 }

@@ -121,12 +121,12 @@ def test_depth_ordered_binning_bitexact(hip, oracle, n, deg, seed, radix_bits):
 
 
 @pytest.mark.parametrize("n,deg,seed", SMALL_CASES[1:] + [(10000, 0, 0)])
-@pytest.mark.parametrize("chunk", [256, 1024])
-def test_direct_binning_bitexact(hip, oracle, n, deg, seed, chunk):
+@pytest.mark.parametrize("group,tpl", [(256, 1), (256, 2), (1024, 4), (16384, 2)])
+def test_direct_binning_bitexact(hip, oracle, n, deg, seed, group, tpl):
     """gs_bin_tiles: values and ranges equal to the reference's
     duplicate-with-keys + stable 64-bit sort, bit for bit, with several
-    chunks per view (chunk 256) and with one (chunk 1024 on small scenes)."""
-    _say("direct binning", n, deg, seed, chunk)
+    groups per view (256) and with one (16384), for every bin shape."""
+    _say("direct binning", n, deg, seed, group, tpl)
     if n == 10000:
         scene, cams, _ = make_workload("c1_10k_256")
         cam = cams[0]
@@ -140,11 +140,11 @@ def test_direct_binning_bitexact(hip, oracle, n, deg, seed, chunk):
     keys_ref, vals_ref = oracle.duplicate(ref, scene.n, cam.width, cam.height)
     ks_ref, vs_ref, rg_ref = oracle.sort_pairs(keys_ref, vals_ref, cam.num_tiles)
     pre = _to_dev(hip, {k: ref[k] for k in ["means2D", "depths", "conics", "op2d", "rgb", "radii", "tiles"]})
-    hip.set_bin_chunk(chunk)
+    hip.set_binning(group, tpl)
     try:
         vals, ranges, K = hip.bin_tiles(pre, cam.width, cam.height)
     finally:
-        hip.set_bin_chunk(1024)
+        hip.set_binning(16384, 2)
     assert K == len(keys_ref)
     assert np.array_equal(u32(vals), vs_ref)
     assert np.array_equal(u32(ranges), rg_ref)
