@@ -12,8 +12,11 @@
 #include <cstring>
 #include <string>
 
+#include "gs_device.h"
 #include "gs_internal.h"
 #include "gs_trainer_impl.h"
+
+gsd::CamArgs make_cam_args(const gs_camera* cam, const gs_render_cfg* cfg);
 
 using gs::DevBuf;
 
@@ -154,6 +157,8 @@ gs_trainer::~gs_trainer() {
     vals.release();
     keys_sorted.release();
     vals_sorted.release();
+    cam_dev.release();
+    hp_dev.release();
     for (auto& g : graphs) g.release();
 }
 
@@ -303,6 +308,51 @@ static int tr_optimizer(gs_trainer* t, const gs_train_cfg* tc, int iteration) {
     return gs_sgd_step(c, t->n, t->max_coeffs, P, G, lrs);
 }
 
+// Preprocess backward with Adam applied in place from the gradients held in
+// registers (the multi-view kernel with one view): the same arithmetic as
+// gs_preprocess_bwd followed by gs_adam_step, without writing the gradients.
+static int tr_fused_backward_adam(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg,
+                                  const gs_train_cfg* tc, int iteration, const int* radii, const gs_raster_grads& rg) {
+    gs_ctx* c = t->ctx;
+    TR_TRY(t->cam_dev.ensure(sizeof(gsd::CamArgs)));
+    TR_TRY(t->hp_dev.ensure(sizeof(gso::AdamDev)));
+    const gsd::CamArgs ca = make_cam_args(cam, cfg);
+    gso::AdamDev h{};
+    h.lr[0] = gs_lr_schedule(tc->lr_position_init, tc->lr_position_final, iteration, tc->max_steps);
+    h.lr[1] = tc->lr_scaling;
+    h.lr[2] = tc->lr_rotation;
+    h.lr[3] = tc->lr_feature;
+    h.lr[4] = tc->lr_opacity;
+    h.b1 = tc->adam_beta1;
+    h.b2 = tc->adam_beta2;
+    h.eps = tc->adam_eps;
+    t->adam_steps++;  // AdamOptimizer::step_count_ (optimizer.cpp:72)
+    gs_adam_bias_corrections(h.b1, h.b2, t->adam_steps, &h.bc1, &h.bc2);
+    // pageable sources: the runtime stages them before returning
+    hipError_t e = hipMemcpyAsync(t->cam_dev.p, &ca, sizeof(ca), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(t->hp_dev.p, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(camera/hparams)");
+    float* P[5];
+    float* M[5];
+    float* V[5];
+    for (int k = 0; k < 5; k++) {
+        P[k] = t->param[k].f();
+        M[k] = t->adam_m[k].f();
+        V[k] = t->adam_v[k].f();
+    }
+    gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
+    e = launch_preprocess_bwd_multi(t->gaussians(), static_cast<const gsd::CamArgs*>(t->cam_dev.p), 1, radii, rg, pg, P,
+                                    M, V, static_cast<const gso::AdamDev*>(t->hp_dev.p), t->sh_degree, c->stream);
+    if (e != hipSuccess) return gs_hip_status(e, "preprocess backward + adam kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_set_options(gs_trainer* t, int fused_adam) {
+    if (!t) return gs_set_error(GS_ERR_INVALID, "null trainer");
+    t->fused_adam = fused_adam ? 1 : 0;
+    return GS_OK;
+}
+
 extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
                                const gs_train_cfg* tc, int iteration, gs_step_result* out) {
     if (!t || !cam || !cfg || !gt || !tc) return gs_set_error(GS_ERR_INVALID, "gs_trainer_step: null argument");
@@ -329,8 +379,12 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
                                    v.ranges.u(), cfg->bg_color, &px, v.d_image.f(), &rg));
     gs_gaussians g = t->gaussians();
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
-    TR_TRY(gs_preprocess_bwd(c, &g, cam, cfg, pre.radii, &rg, &pg, 0));
-    TR_TRY(tr_optimizer(t, tc, iteration));
+    if (t->fused_adam && tc->use_adam) {
+        TR_TRY(tr_fused_backward_adam(t, cam, cfg, tc, iteration, pre.radii, rg));
+    } else {
+        TR_TRY(gs_preprocess_bwd(c, &g, cam, cfg, pre.radii, &rg, &pg, 0));
+        TR_TRY(tr_optimizer(t, tc, iteration));
+    }
     double loss = 0.0;
     hipError_t e = hipMemcpyAsync(&loss, v.loss.p, 8, hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(loss)");

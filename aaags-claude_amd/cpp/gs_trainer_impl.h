@@ -54,6 +54,8 @@ struct gs_trainer {
     std::vector<gs::ViewBufs> views;
     gs::DevBuf keys, vals, keys_sorted, vals_sorted;
     int pair_stride = 1;  // uint32 entries per pair in keys_sorted: 1 (indices) or 2 (words)
+    int fused_adam = 0;   // Adam applied inside the preprocess backward (gradients not written)
+    gs::DevBuf cam_dev, hp_dev;  // device copies of the camera and Adam hyper-parameters (fused step)
     std::vector<gs::GraphSlot> graphs;
     gs_gaussians gaussians() const;
     size_t tensor_elems(int k) const;

@@ -3,8 +3,8 @@
 // Every function keeps the floating-point operation order of the reference's
 // CPU path (harmonyos_3dgs/src/math_utils.cpp, sh_eval.cpp,
 // cpu/preprocessor_cpu.cpp) so that, with -ffp-contract=off and IEEE division
-// and square root, the GPU reproduces the CPU results bit for bit; only expf
-// (device libm, <= 1 ulp) can differ in the last bit.
+// and square root, the GPU reproduces the CPU results bit for bit.  The
+// exponential the CPU path calls (glibc's expf) is restated as exp_ref.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -12,6 +12,44 @@
 #include "gs_hip.h"
 
 namespace gsd {
+
+// expf exactly as the reference's CPU path evaluates it: glibc's expf, the
+// build with fused multiply-add that x86-64 hosts with FMA select (2.35,
+// sysdeps/ieee754/flt-32/e_expf.c; the published algorithm of Szabolcs
+// Nagy's optimized-routines).  exp(x) = 2^(k/32) * 2^(r/32): with
+// z = x * 32/ln2 in double, k = round(z) (via the 1.5 * 2^52 shift), r = z - k,
+// 2^(k/32) from a table of 2^(i/32) and the exponent bits of k, and 2^(r/32)
+// from a cubic; the double result is rounded to float once.  The table is
+// tab[i] = bits(2^(i/32) rounded to double) - (i << 47).  Equal to glibc's
+// expf on all 2^32 float inputs (checked exhaustively on the CPU).
+static __constant__ uint64_t kExp2Tab[32] = {
+    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
+    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
+    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
+    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
+    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
+    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
+    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
+    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
+
+__device__ __forceinline__ float exp_ref(float x) {
+    if (!(x <= 0x1.62e42ep6f)) return x != x ? x + x : __builtin_inff();  // NaN, overflow
+    if (x < -0x1.9fe368p6f) return 0.0f;                                   // below 2^-150
+    constexpr double kInvLn2N = 0x1.71547652b82fep+0 * 32.0, kShift = 0x1.8p+52;
+    constexpr double kC0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32, kC1 = 0x1.ebfce50fac4f3p-3 / 32 / 32,
+                     kC2 = 0x1.62e42ff0c52d6p-1 / 32;
+    const double xd = (double)x;
+    double kd = __builtin_fma(kInvLn2N, xd, kShift);
+    const uint64_t ki = (uint64_t)__double_as_longlong(kd);
+    kd -= kShift;
+    const double r = __builtin_fma(kInvLn2N, xd, -kd);
+    const double sc = __longlong_as_double((long long)(kExp2Tab[ki & 31u] + (ki << 47)));
+    const double z = __builtin_fma(kC0, r, kC1);
+    const double r2 = r * r;
+    double y = __builtin_fma(kC2, r, 1.0);
+    y = __builtin_fma(z, r2, y);
+    return (float)(y * sc);
+}
 
 // std::min / std::max semantics: the first argument wins unless strictly beaten.
 __device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
@@ -198,7 +236,7 @@ __device__ __forceinline__ float sh_raw_channel_rt(int deg, const ShDir& d, cons
 
 // Activation of one Gaussian's raw parameters (train_types.cpp:6-36).
 __device__ __forceinline__ void activate_scale(const float rs[3], float s[3]) {
-    s[0] = expf(rs[0]); s[1] = expf(rs[1]); s[2] = expf(rs[2]);
+    s[0] = exp_ref(rs[0]); s[1] = exp_ref(rs[1]); s[2] = exp_ref(rs[2]);
 }
 __device__ __forceinline__ void activate_rot(const float q[4], float o[4]) {
     float len = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
@@ -206,7 +244,24 @@ __device__ __forceinline__ void activate_rot(const float q[4], float o[4]) {
     float inv = 1.0f / len;
     o[0] = q[0] * inv; o[1] = q[1] * inv; o[2] = q[2] * inv; o[3] = q[3] * inv;
 }
-__device__ __forceinline__ float activate_opacity(float l) { return 1.0f / (1.0f + expf(-l)); }
+__device__ __forceinline__ float activate_opacity(float l) { return 1.0f / (1.0f + exp_ref(-l)); }
+
+// alpha = min(0.99, o * exp(power)) of rasterizer_cpu.cpp:164-166 and the
+// 1/255 cut that follows.  The device expf (<= 1 ulp) is used, and where
+// that alpha lies within 2^-18 (relative) of the cut -- where a 1-ulp
+// difference could decide it -- the alpha is recomputed with exp_ref, so
+// every include/skip decision is the CPU path's.
+__device__ __forceinline__ float raster_alpha(float o, float power, float* ep) {
+    float e = expf(power);
+    float a = smin(0.99f, o * e);
+    constexpr float kCut = 1.0f / 255.0f;
+    if (fabsf(a - kCut) <= kCut * 0x1p-18f) {
+        e = exp_ref(power);
+        a = smin(0.99f, o * e);
+    }
+    *ep = e;
+    return a;
+}
 
 // Camera constants shared by the per-Gaussian kernels, passed by value.
 struct CamArgs {

@@ -20,6 +20,10 @@ __global__ __launch_bounds__(256) void preprocess_fwd_kernel(gs_gaussians G, Cam
     constexpr int NB = (DEG + 1) * (DEG + 1);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G.count) return;
+    // the SH block (most of the bytes) is requested first, so its loads are in
+    // flight while the geometry is computed; culled Gaussians do not use it
+    float sh[NB * 3];
+    load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
     Params P;
     load_params(G, i, G.raw != 0, P);
     Geom g;
@@ -29,8 +33,6 @@ __global__ __launch_bounds__(256) void preprocess_fwd_kernel(gs_gaussians G, Cam
     float depth = 0.0f, op2d = 0.0f;
     int radius = 0, tiles = 0;
     if (vis) {
-        float sh[NB * 3];
-        load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
         sh_color_from<DEG>(sh, sh_dir(P.p, c.campos), c.training != 0, rgb);
         m2 = make_float2(g.px, g.py);
         con[0] = g.conic[0]; con[1] = g.conic[1]; con[2] = g.conic[2];
@@ -189,78 +191,150 @@ struct AdamState {
     const gso::AdamDev* hp;
 };
 
-__device__ __forceinline__ void adam_elem(float* p, float* m, float* v, size_t k, float g, float lr,
-                                          const gso::AdamDev& h) {
-    float mm = h.b1 * m[k] + (1.0f - h.b1) * g;
-    float vv = h.b2 * v[k] + (1.0f - h.b2) * g * g;
-    m[k] = mm;
-    v[k] = vv;
+// One Adam element (AdamOptimizer::adam_update, optimizer.cpp:60-101) on
+// values already loaded; returns the new parameter and updates *m, *v.
+__device__ __forceinline__ float adam_math(float p, float* m, float* v, float g, float lr, const gso::AdamDev& h) {
+    float mm = h.b1 * *m + (1.0f - h.b1) * g;
+    float vv = h.b2 * *v + (1.0f - h.b2) * g * g;
+    *m = mm;
+    *v = vv;
     float mh = mm * h.bc1;
     float vh = vv * h.bc2;
-    p[k] -= lr * mh / (sqrtf(vh) + h.eps);
+    return p - lr * mh / (sqrtf(vh) + h.eps);
+}
+
+// Adam on K consecutive elements starting at k0: every load is issued before
+// any store (the arrays are distinct), so the K updates overlap in memory.
+template <int K>
+__device__ __forceinline__ void adam_run(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                                         size_t k0, const float* g, float lr, const gso::AdamDev& h) {
+    float pp[K], mm[K], vv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        pp[k] = p[k0 + k];
+        mm[k] = m[k0 + k];
+        vv[k] = v[k0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        pp[k] = adam_math(pp[k], &mm[k], &vv[k], g[k], lr, h);
+        p[k0 + k] = pp[k];
+        m[k0 + k] = mm[k];
+        v[k0 + k] = vv[k];
+    }
 }
 
 // Backward over V views, the views' gradients summed in view order in
 // registers.  FUSED_ADAM applies the optimizer in place (parameters, m, v);
 // otherwise the summed gradients are written (overwriting `out`).
+//
+// With FUSED_ADAM the SH part of the update (MAXC*3 floats per Gaussian, most
+// of the parameter bytes) runs cooperatively: each wave stores its 64
+// Gaussians' SH gradients in LDS in the global layout (rows padded by one
+// float so that the per-thread stores hit distinct banks), then sweeps the
+// wave's contiguous SH block with one float per lane per step, so every load
+// and store of p, m and v is coalesced.  The few per-Gaussian scalars
+// (position, scale, rotation, opacity) are updated by their own thread.
 template <int MAXC, int DEG, bool FUSED_ADAM>
 __global__ __launch_bounds__(256) void preprocess_bwd_multi_kernel(gs_gaussians G, const CamArgs* __restrict__ cams,
                                                                    int V, const int* __restrict__ radii,
                                                                    gs_raster_grads rg, gs_param_grads out,
                                                                    AdamState ad) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
+    constexpr int E = MAXC * 3;  // SH floats per Gaussian
+    constexpr int ROW = E + 1;   // padded LDS row
+    __shared__ float s_sh[FUSED_ADAM ? 4 * 64 * ROW : 1];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = G.count;
-    if (i >= N) return;
-    Params P;
-    load_params(G, i, true, P);
-    float sh[NB * 3];
-    load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
-    float tpos[3] = {0.0f, 0.0f, 0.0f}, tsc[3] = {0.0f, 0.0f, 0.0f}, trot[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float top = 0.0f;
-    float tsh[MAXC * 3];
+    const bool valid = i < N;
+    float tsh[E];
 #pragma unroll
-    for (int k = 0; k < MAXC * 3; k++) tsh[k] = 0.0f;
-    for (int v = 0; v < V; v++) {
-        const size_t o = (size_t)v * N + i;
-        if (radii[o] <= 0) continue;
-        const CamArgs& c = cams[v];
-        Geom g;
-        preprocess_geom(P.p, P.s, P.q, c, g);
-        float gpos[3] = {0.0f, 0.0f, 0.0f};
-        float gsh[MAXC * 3];
-        const float drgb[3] = {rg.d_rgb[o * 3], rg.d_rgb[o * 3 + 1], rg.d_rgb[o * 3 + 2]};
-        sh_backward_from<MAXC, DEG>(sh, sh_dir(P.p, c.campos), drgb, gsh, gpos);
-        RGrad r{rg.d_means2D[o * 2], rg.d_means2D[o * 2 + 1], rg.d_conics[o * 3], rg.d_conics[o * 3 + 1],
-                rg.d_conics[o * 3 + 2], rg.d_opacities_2d[o]};
-        float gsc[3], grot[4], gop;
-        geom_backward(P, g, c, r, gpos, gsc, grot, gop);
+    for (int k = 0; k < E; k++) tsh[k] = 0.0f;
+    if (valid) {
+        Params P;
+        load_params(G, i, true, P);
+        float sh[NB * 3];
+        load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
+        float tpos[3] = {0.0f, 0.0f, 0.0f}, tsc[3] = {0.0f, 0.0f, 0.0f}, trot[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float top = 0.0f;
+        for (int v = 0; v < V; v++) {
+            const size_t o = (size_t)v * N + i;
+            if (radii[o] <= 0) continue;
+            const CamArgs& c = cams[v];
+            Geom g;
+            preprocess_geom(P.p, P.s, P.q, c, g);
+            float gpos[3] = {0.0f, 0.0f, 0.0f};
+            float gsh[E];
+            const float drgb[3] = {rg.d_rgb[o * 3], rg.d_rgb[o * 3 + 1], rg.d_rgb[o * 3 + 2]};
+            sh_backward_from<MAXC, DEG>(sh, sh_dir(P.p, c.campos), drgb, gsh, gpos);
+            RGrad r{rg.d_means2D[o * 2], rg.d_means2D[o * 2 + 1], rg.d_conics[o * 3], rg.d_conics[o * 3 + 1],
+                    rg.d_conics[o * 3 + 2], rg.d_opacities_2d[o]};
+            float gsc[3], grot[4], gop;
+            geom_backward(P, g, c, r, gpos, gsc, grot, gop);
 #pragma unroll
-        for (int k = 0; k < 3; k++) { tpos[k] += gpos[k]; tsc[k] += gsc[k]; }
+            for (int k = 0; k < 3; k++) { tpos[k] += gpos[k]; tsc[k] += gsc[k]; }
 #pragma unroll
-        for (int k = 0; k < 4; k++) trot[k] += grot[k];
-        top += gop;
+            for (int k = 0; k < 4; k++) trot[k] += grot[k];
+            top += gop;
 #pragma unroll
-        for (int k = 0; k < MAXC * 3; k++) tsh[k] += gsh[k];
+            for (int k = 0; k < E; k++) tsh[k] += gsh[k];
+        }
+        if constexpr (FUSED_ADAM) {
+            const gso::AdamDev h = *ad.hp;
+            adam_run<3>(ad.p[0], ad.m[0], ad.v[0], (size_t)i * 3, tpos, h.lr[0], h);
+            adam_run<3>(ad.p[1], ad.m[1], ad.v[1], (size_t)i * 3, tsc, h.lr[1], h);
+            adam_run<4>(ad.p[2], ad.m[2], ad.v[2], (size_t)i * 4, trot, h.lr[2], h);
+            adam_run<1>(ad.p[4], ad.m[4], ad.v[4], (size_t)i, &top, h.lr[4], h);
+        } else {
+            out.d_positions[i * 3] = tpos[0]; out.d_positions[i * 3 + 1] = tpos[1];
+            out.d_positions[i * 3 + 2] = tpos[2];
+            out.d_scales[i * 3] = tsc[0]; out.d_scales[i * 3 + 1] = tsc[1]; out.d_scales[i * 3 + 2] = tsc[2];
+            reinterpret_cast<float4*>(out.d_rotations)[i] = make_float4(trot[0], trot[1], trot[2], trot[3]);
+            store_sh<MAXC>(out.d_sh_coeffs, i, tsh, false);
+            out.d_opacities[i] = top;
+        }
     }
     if constexpr (FUSED_ADAM) {
-        const gso::AdamDev h = *ad.hp;
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        float* row = s_sh + (w * 64 + lane) * ROW;
 #pragma unroll
-        for (int k = 0; k < 3; k++) adam_elem(ad.p[0], ad.m[0], ad.v[0], (size_t)i * 3 + k, tpos[k], h.lr[0], h);
+        for (int k = 0; k < E; k++) row[k] = tsh[k];
+        __syncthreads();
+        const int g0 = blockIdx.x * blockDim.x + w * 64;  // first Gaussian of this wave
+        const int cnt = N - g0 < 64 ? N - g0 : 64;
+        if (cnt > 0) {
+            const gso::AdamDev h = *ad.hp;
+            float* __restrict__ p = ad.p[3] + (size_t)g0 * E;
+            float* __restrict__ m = ad.m[3] + (size_t)g0 * E;
+            float* __restrict__ v = ad.v[3] + (size_t)g0 * E;
+            const float* wsh = s_sh + w * 64 * ROW;
+            const int total = cnt * E;
+            constexpr int U = 8;  // elements per lane in flight
+            for (int j0 = lane; j0 < total; j0 += 64 * U) {
+                float pp[U], mm[U], vv[U], gg[U];
 #pragma unroll
-        for (int k = 0; k < 3; k++) adam_elem(ad.p[1], ad.m[1], ad.v[1], (size_t)i * 3 + k, tsc[k], h.lr[1], h);
+                for (int u = 0; u < U; u++) {
+                    const int j = j0 + 64 * u;
+                    if (j < total) {
+                        const int gl = j / E;
+                        gg[u] = wsh[gl * ROW + (j - gl * E)];
+                        pp[u] = p[j];
+                        mm[u] = m[j];
+                        vv[u] = v[j];
+                    }
+                }
 #pragma unroll
-        for (int k = 0; k < 4; k++) adam_elem(ad.p[2], ad.m[2], ad.v[2], (size_t)i * 4 + k, trot[k], h.lr[2], h);
-#pragma unroll
-        for (int k = 0; k < MAXC * 3; k++)
-            adam_elem(ad.p[3], ad.m[3], ad.v[3], (size_t)i * MAXC * 3 + k, tsh[k], h.lr[3], h);
-        adam_elem(ad.p[4], ad.m[4], ad.v[4], (size_t)i, top, h.lr[4], h);
-    } else {
-        out.d_positions[i * 3] = tpos[0]; out.d_positions[i * 3 + 1] = tpos[1]; out.d_positions[i * 3 + 2] = tpos[2];
-        out.d_scales[i * 3] = tsc[0]; out.d_scales[i * 3 + 1] = tsc[1]; out.d_scales[i * 3 + 2] = tsc[2];
-        reinterpret_cast<float4*>(out.d_rotations)[i] = make_float4(trot[0], trot[1], trot[2], trot[3]);
-        store_sh<MAXC>(out.d_sh_coeffs, i, tsh, false);
-        out.d_opacities[i] = top;
+                for (int u = 0; u < U; u++) {
+                    const int j = j0 + 64 * u;
+                    if (j < total) {
+                        pp[u] = adam_math(pp[u], &mm[u], &vv[u], gg[u], h.lr[3], h);
+                        p[j] = pp[u];
+                        m[j] = mm[u];
+                        v[j] = vv[u];
+                    }
+                }
+            }
+        }
     }
 }
 

@@ -83,7 +83,8 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 const float dx = m.x - fpx, dy = m.y - fpy;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
                 if (power > 0.0f) continue;
-                const float alpha = smin(0.99f, co.w * expf(power));
+                float ep;
+                const float alpha = gsd::raster_alpha(co.w, power, &ep);
                 if (alpha < 1.0f / 255.0f) continue;
                 const float test_T = T * (1.0f - alpha);
                 if (test_T < 0.0001f) {
@@ -207,6 +208,14 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
     if (t == 0) sm.maxlast = 0;
     __syncthreads();
     if (last > 0) atomicMax(&sm.maxlast, last);
+    // this wave's own largest last contributor: entries past it touch none of
+    // its pixels, so the wave skips them
+    uint32_t wlast = last;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t u = (uint32_t)__shfl_xor((int)wlast, o, 64);
+        wlast = u > wlast ? u : wlast;
+    }
     __syncthreads();
     const uint32_t stop = sm.maxlast;
 
@@ -226,7 +235,8 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
             for (int c = 0; c < NG; c++) sm.g[k][c] = 0.0f;
         }
         __syncthreads();
-        for (int k = n - 1; k >= 0; k--) {
+        const int kmax = wlast <= lo ? -1 : ((int)(wlast - lo) < n ? (int)(wlast - lo) : n) - 1;
+        for (int k = kmax; k >= 0; k--) {
             const uint32_t j = lo + (uint32_t)k;
             float gv[NG];
 #pragma unroll
@@ -238,13 +248,16 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 const float dx = m.x - fpx, dy = m.y - fpy;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
                 if (power <= 0.0f) {
-                    const float ep = expf(power);
-                    const float alpha = smin(0.99f, co.w * ep);
+                    float ep;
+                    const float alpha = gsd::raster_alpha(co.w, power, &ep);
                     if (alpha >= 1.0f / 255.0f) {
                         contrib = true;
                         const float4 cl = sm.rgb[k];
                         const float oma = 1.0f - alpha;
-                        const float Ti = (oma > 1e-6f) ? T / oma : 0.0f;
+                        // T_i = T / (1 - alpha) as the reference recovers it
+                        // (rasterizer_backward_cpu.cpp:85), with the hardware
+                        // reciprocal (1 ulp): gradients need 1e-3, not bits
+                        const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
                         gv[5] = alpha * Ti * dC0;
                         gv[6] = alpha * Ti * dC1;
                         gv[7] = alpha * Ti * dC2;

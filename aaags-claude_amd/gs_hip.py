@@ -472,6 +472,10 @@ class GsTrainer:
                                                  ctypes.byref(tc), iteration, ctypes.byref(res)), "gs_trainer_step")
         return res
 
+    def set_options(self, fused_adam=False):
+        """fused_adam: Adam inside the preprocess backward (gradient buffers not written)."""
+        self.hip._check(self.lib.gs_trainer_set_options(self.tr, _i32(int(fused_adam))), "gs_trainer_set_options")
+
     def autotune(self, cam, cfg, gt, reps=3) -> dict:
         """gs_autotune on this trainer's parameters and one view (no parameter update)."""
         c, f = cam.to_c(), cfg.to_c()

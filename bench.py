@@ -207,6 +207,35 @@ def cpu_baseline(scene, cam, cfg, gt):
             "seconds": round(dt, 2)}
 
 
+def traffic_probe(args):
+    """Dispatches of each single-view stage kernel at the headline workload,
+    plus device copies of a known size, for hardware-counter runs
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the copy's counter values per
+    byte calibrate the stage kernels' values into HBM bytes per launch."""
+    import torch
+    from gs_hip import GsHip
+    from gs_scenes import RenderCfg, make_workload
+
+    scene, cams, gts = make_workload(args.workload)
+    cam = cams[0]
+    cfg = RenderCfg(sh_degree=3, training=True)
+    hip = GsHip(0)
+    p = hip.upload_scene(scene)
+    gt = torch.tensor(gts[0], device=hip.dev)
+    m, v = hip.alloc_pgrads(scene.n, scene.max_coeffs), hip.alloc_pgrads(scene.n, scene.max_coeffs)
+    for r in range(2):
+        f = hip.forward_backward_fast(p, cam, cfg, gt)
+        hip.adam(p, f["grads"], m, v, [1e-3] * 5, 0.9, 0.999, 1e-8, r + 1)
+    torch.cuda.synchronize()
+    copy_bytes = 1 << 30
+    src = torch.ones(copy_bytes // 4, device=hip.dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"traffic probe done: pairs {f['K']}, copy {copy_bytes} B x 3")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,9 +248,15 @@ def main():
     ap.add_argument("--batched-workload", default="c4_1m_16views")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default knobs")
     ap.add_argument("--no-large", action="store_true", help="skip the 6M-Gaussian 8-view batched leg")
+    ap.add_argument("--no-fused-adam", action="store_true", help="single-view step with the separate Adam kernel")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run each stage kernel and a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--large-workload", default="c5_6m_8x1080p")
     args = ap.parse_args()
 
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is design-only",
               file=sys.stderr)
@@ -243,6 +278,7 @@ def main():
     hip = GsHip(0)
     gt = torch.tensor(gt_np, device=hip.dev)
     tr = GsTrainer(hip, scene)
+    tr.set_options(fused_adam=not args.no_fused_adam)  # the step at this size specifies a fused Adam
 
     tuning = None
     if not args.no_autotune:
@@ -313,7 +349,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Philox scene and target image)",
         "config": {"workload": args.workload, "gaussians": scene.n, "sh_degree": scene.sh_degree,
-                   "width": cam.width, "height": cam.height, "pairs": int(K), "optimizer": "adam lr 1e-3"},
+                   "width": cam.width, "height": cam.height, "pairs": int(K),
+                   "optimizer": "adam lr 1e-3" + ("" if args.no_fused_adam else " (fused into the backward)")},
         "fwd_mpix_s": round(mpix_s, 2),
         "batched": batched,
         "large": large,

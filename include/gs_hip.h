@@ -309,6 +309,13 @@ void* gs_trainer_buffer(gs_trainer* tr, const char* which);
 int gs_trainer_step(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
                     const gs_train_cfg* tc, int iteration, gs_step_result* out);
 
+/* fused_adam != 0: with use_adam, the step applies Adam inside the
+ * preprocess backward from the gradients held in registers (the same
+ * arithmetic as gs_preprocess_bwd + gs_adam_step; the gradient buffers are
+ * then not written).  Off by default, as Trainer::step writes the gradients
+ * (trainer.cpp:44-62). */
+int gs_trainer_set_options(gs_trainer* tr, int fused_adam);
+
 /* Renderer::render (renderer.h:17-21) on the trainer's parameters: forward
  * only, image into the trainer's "image" buffer. */
 int gs_trainer_render(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, long long* pairs);

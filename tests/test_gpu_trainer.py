@@ -63,3 +63,31 @@ def test_trainer_loss_decreases_on_c1_workload(hip):
     tc = train_cfg(lr=1e-2)
     losses = [tr.step(cams[0], cfg, gt, tc, i).loss for i in range(30)]
     assert losses[-1] < losses[0] * 0.97, losses
+
+
+def test_trainer_fused_adam_matches_separate_adam(hip):
+    """gs_trainer_set_options(fused_adam): one step with Adam inside the
+    preprocess backward against the separate kernels.  The screen-space
+    gradients are summed with float atomics, so two runs differ in the last
+    bits; where a near-zero gradient changes sign, Adam's first step moves the
+    parameter by +-lr instead of -+lr; the rest is bit-identical almost
+    everywhere."""
+    from gs_hip import GsTrainer, train_cfg
+    print("[case] trainer fused adam", flush=True)
+    scene, cam, gt = random_small_scene(2500, 3, 33, width=140, height=100)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    gt_t = torch.tensor(gt, device=hip.dev)
+    lr = 1e-3
+    out = []
+    for fused in (False, True):
+        tr = GsTrainer(hip, scene)
+        tr.set_options(fused_adam=fused)
+        loss = tr.step(cam, cfg, gt_t, train_cfg(lr=lr), 0).loss
+        out.append((tr.params(), loss))
+    assert out[0][1] == out[1][1]  # same forward
+    total = same = 0
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.abs(a - b).max() <= 2 * lr * 1.0001
+        same += int((a.view(np.uint32) == b.view(np.uint32)).sum())
+        total += a.size
+    assert same >= 0.99 * total, (same, total)
