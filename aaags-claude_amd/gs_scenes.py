@@ -249,11 +249,11 @@ def target_image(width, height, seed) -> np.ndarray:
 # Workloads: (scene builder, cameras builder).  Seeds 0..4, camera seeds +1000,
 # target images +2000 (+ view index).
 WORKLOADS = {
-    "c1_10k_256": dict(seed=0, n=10_000, degree=0, kind="cube", views=[("fixed", 256, 256, 60.0, 3.0)]),
-    "c2_200k_720p": dict(seed=1, n=200_000, degree=3, kind="normal", views=[("sphere", 1280, 720, 50.0, 4.0)]),
-    "c3_1m_1080p": dict(seed=2, n=1_000_000, degree=3, kind="cube", views=[("fixed", 1920, 1080, 60.0, 3.0)]),
-    "c4_1m_16views": dict(seed=3, n=1_000_000, degree=3, kind="cube", views=[("mixed16", 0, 0, 60.0, 4.0)]),
-    "c5_6m_8x1080p": dict(seed=4, n=6_000_000, degree=3, kind="clusters", views=[("sphere8", 1920, 1080, 60.0, 8.0)]),
+    "10k_256": dict(seed=0, n=10_000, degree=0, kind="cube", views=[("fixed", 256, 256, 60.0, 3.0)]),
+    "200k_720p": dict(seed=1, n=200_000, degree=3, kind="normal", views=[("sphere", 1280, 720, 50.0, 4.0)]),
+    "1m_1080p": dict(seed=2, n=1_000_000, degree=3, kind="cube", views=[("fixed", 1920, 1080, 60.0, 3.0)]),
+    "1m_16views": dict(seed=3, n=1_000_000, degree=3, kind="cube", views=[("mixed16", 0, 0, 60.0, 4.0)]),
+    "6m_8x1080p": dict(seed=4, n=6_000_000, degree=3, kind="clusters", views=[("sphere8", 1920, 1080, 60.0, 8.0)]),
 }
 
 MIXED_RES = [(640, 480), (1280, 720), (1600, 900), (1920, 1080)]

@@ -70,6 +70,20 @@ def stage_breakdown(hip, p, cam, cfg, gt, reps):
     return {k: acc[k] / reps for k in names}, K
 
 
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 2
+
+
+def load_traffic(workload):
+    """Per-stage HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (calibrated
+    with a 1 GiB device copy) and VALU wave-instruction counts, from the committed summary for this
+    workload; {} when none is committed."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("stages", {})
+
+
 def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles, table_entries=0):
     """Bytes each stage must move at minimum (per launch), from the per-unit
     figures of the hot-path analysis: preprocess 236 B/G read (SH3) + 48 B/G
@@ -241,17 +255,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c3_1m_1080p")
+    ap.add_argument("--workload", default="1m_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown-reps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--batched-workload", default="c4_1m_16views")
+    ap.add_argument("--batched-workload", default="1m_16views")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default knobs")
     ap.add_argument("--no-large", action="store_true", help="skip the 6M-Gaussian 8-view batched leg")
     ap.add_argument("--no-fused-adam", action="store_true", help="single-view step with the separate Adam kernel")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run each stage kernel and a 1 GiB device copy (for rocprofv3 --pmc runs)")
-    ap.add_argument("--large-workload", default="c5_6m_8x1080p")
+    ap.add_argument("--large-workload", default="6m_8x1080p")
     args = ap.parse_args()
 
     if args.traffic_probe:
@@ -316,9 +330,16 @@ def main():
     table = ((scene.n + 1023) // 1024) * cam.num_tiles + 1  # (tile, chunk) entries at 1024 Gaussians per chunk
     byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles, table)
     achieved = byts / (stages[dom] * 1e-3) / 1e9
+    measured = load_traffic(args.workload)
+    st_meas = measured.get(dom, {})
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": st_meas.get("traffic_bytes"),
                 "algorithmic_bytes": int(byts), "ms": round(stages[dom], 4)}
+    if "valu_wave_insts" in st_meas:
+        # wave64 FP32 VALU issue: 2 cycles per instruction per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz
+        peak_insts = VALU_WAVE_INSTS_PER_S * stages[dom] * 1e-3
+        roofline["valu_issue_frac"] = round(st_meas["valu_wave_insts"] / peak_insts, 4)
 
     batched = None
     if not args.no_batched:

@@ -17,14 +17,14 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def c3():
-    scene, cams, gts = make_workload("c3_1m_1080p")
+def scene_1m_1080p():
+    scene, cams, gts = make_workload("1m_1080p")
     return scene, cams[0], gts[0]
 
 
-def test_c3_binning_paths_agree(hip, c3):
-    print("[case] c3 binning paths", flush=True)
-    scene, cam, _ = c3
+def test_1m_1080p_binning_paths_agree(hip, scene_1m_1080p):
+    print("[case] 1m 1080p binning paths", flush=True)
+    scene, cam, _ = scene_1m_1080p
     cfg = RenderCfg(sh_degree=3, training=True)
     p = hip.upload_scene(scene)
     pre = hip.preprocess(p, cam, cfg)
@@ -41,23 +41,23 @@ def test_c3_binning_paths_agree(hip, c3):
     assert (k[1:] >= k[:-1]).all()
 
 
-def test_c3_forward_image_matches_oracle(hip, oracle, c3):
-    print("[case] c3 forward vs oracle", flush=True)
-    scene, cam, _ = c3
+def test_1m_1080p_forward_image_matches_oracle(hip, oracle, scene_1m_1080p):
+    print("[case] 1m 1080p forward vs oracle", flush=True)
+    scene, cam, _ = scene_1m_1080p
     cfg = RenderCfg(sh_degree=3, training=True)
     f = hip.forward_fast(hip.upload_scene(scene), cam, cfg)
     torch.cuda.synchronize()
     ref = oracle.forward(scene, cam, cfg, backward_cache=False)
     assert f["K"] == len(ref["keys"])
-    assert_image_close(f["img"].cpu().numpy(), ref["img"], "c3 image")
+    assert_image_close(f["img"].cpu().numpy(), ref["img"], "1m 1080p image")
     nc = f["px"]["n_contrib"].cpu().numpy()
     assert np.abs(nc.astype(np.int64) - ref["n_contrib"].astype(np.int64)).max() <= 1
 
 
-def test_c4_batched_render_equals_single_views(hip):
+def test_1m_16views_batched_render_equals_single_views(hip):
     from gs_hip import GsBatch, GsTrainer
-    print("[case] c4 batched vs single renders", flush=True)
-    scene, cams, gts = make_workload("c4_1m_16views")
+    print("[case] 1m 16 views batched vs single renders", flush=True)
+    scene, cams, gts = make_workload("1m_16views")
     cfg = RenderCfg(sh_degree=3, training=True)
     tr = GsTrainer(hip, scene)
     b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=True, use_graph=False)

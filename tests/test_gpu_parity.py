@@ -90,7 +90,7 @@ def test_binning_and_sort_bitexact(hip, oracle, n, deg, seed):
     """Identical preprocess outputs in -> identical keys, values, ranges out."""
     _say("binning+sort", n, deg, seed)
     if n == 10000:
-        scene, cams, _ = make_workload("c1_10k_256")
+        scene, cams, _ = make_workload("10k_256")
         cam = cams[0]
     else:
         scene, cam, _ = random_small_scene(n, deg, seed, width=160, height=120)
@@ -121,7 +121,7 @@ def test_depth_ordered_binning_bitexact(hip, oracle, n, deg, seed, radix_bits):
     duplicate-with-keys + stable 64-bit sort, bit for bit."""
     _say("depth-ordered binning", n, deg, seed, radix_bits)
     if n == 10000:
-        scene, cams, _ = make_workload("c1_10k_256")
+        scene, cams, _ = make_workload("10k_256")
         cam = cams[0]
     else:
         scene, cam, _ = random_small_scene(n, deg, seed, width=160, height=120)
@@ -151,7 +151,7 @@ def test_direct_binning_bitexact(hip, oracle, n, deg, seed, group, tpl):
     groups per view (256) and with one (16384), for every bin shape."""
     _say("direct binning", n, deg, seed, group, tpl)
     if n == 10000:
-        scene, cams, _ = make_workload("c1_10k_256")
+        scene, cams, _ = make_workload("10k_256")
         cam = cams[0]
     else:
         scene, cam, _ = random_small_scene(n, deg, seed, width=160, height=120)
@@ -404,7 +404,7 @@ def test_full_path_matches_oracle(hip, oracle, n, deg, seed, fast):
 def test_c1_workload_full_path(hip, oracle):
     """10k Gaussians, 256x256, SH0: the reference's own CPU-runnable workload."""
     _say("c1 workload")
-    scene, cams, gts = make_workload("c1_10k_256")
+    scene, cams, gts = make_workload("10k_256")
     cam, gt = cams[0], gts[0]
     cfg = RenderCfg(sh_degree=3, training=True)
     ref = oracle.forward_backward(scene, cam, cfg, gt)

@@ -56,7 +56,7 @@ def test_cpp_renderer_matches_oracle(shim, oracle, n, deg, seed):
 
 def test_cpp_stages_outputs_match_oracle(shim, oracle):
     print("[case] cpp stages", flush=True)
-    scene, cams, _ = make_workload("c1_10k_256")
+    scene, cams, _ = make_workload("10k_256")
     cam = cams[0]
     n = scene.n
     cfg = RenderCfg(sh_degree=3, training=True)

@@ -56,7 +56,7 @@ def test_trainer_loss_decreases_on_c1_workload(hip):
     (10k Gaussians, 256x256, Adam)."""
     from gs_hip import GsTrainer, train_cfg
     print("[case] trainer c1 convergence", flush=True)
-    scene, cams, gts = make_workload("c1_10k_256")
+    scene, cams, gts = make_workload("10k_256")
     cfg = RenderCfg(sh_degree=3, training=True)
     tr = GsTrainer(hip, scene)
     gt = torch.tensor(gts[0], device=hip.dev)
