@@ -32,7 +32,8 @@ struct FwdSmem {
     float4 rgb[BATCH];
 };
 
-constexpr int NG = 9;  // gradient terms per Gaussian: means2D 2, conic 3, rgb 3, opacity 1
+// gradient sums per Gaussian: dpow * (dx, dy, dx^2, dx dy, dy^2), rgb 3, opacity 1
+constexpr int NG = 9;
 
 template <int BATCH>
 struct BwdSmem {
@@ -274,12 +275,16 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                             dpow = dal * co.w * ep;
                             dop = dal * ep;
                         }
+                        // raw moments of dpow; the conic transform of the mean
+                        // gradient and the conic-gradient constants are applied
+                        // once per Gaussian when the sums leave LDS
                         gv[8] = dop;
-                        gv[0] = dpow * (-co.x * dx - co.y * dy);
-                        gv[1] = dpow * (-co.z * dy - co.y * dx);
-                        gv[2] = dpow * (-0.5f * dx * dx);
-                        gv[3] = dpow * (-dx * dy);
-                        gv[4] = dpow * (-0.5f * dy * dy);
+                        const float pdx = dpow * dx, pdy = dpow * dy;
+                        gv[0] = pdx;
+                        gv[1] = pdy;
+                        gv[2] = pdx * dx;
+                        gv[3] = pdx * dy;
+                        gv[4] = pdy * dy;
                     }
                 }
             }
@@ -292,12 +297,17 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
         __syncthreads();
         for (int k = t; k < n; k += 256) {
             const uint32_t id = sm.id[k];
+            // sums of dpow * (dx, dy, dx^2, dx*dy, dy^2) -> the reference's terms
+            // (rasterizer_backward_cpu.cpp): d_mean = -(a Sx + b Sy, c Sy + b Sx),
+            // d_conic = (-Sxx / 2, -Sxy, -Syy / 2)
+            const float4 co = sm.co[k];
+            const float sx = sm.g[k][0], sy = sm.g[k][1];
             float v;
-            v = sm.g[k][0]; if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].x, v);
-            v = sm.g[k][1]; if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].y, v);
-            v = sm.g[k][2]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 0], v);
-            v = sm.g[k][3]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 1], v);
-            v = sm.g[k][4]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 2], v);
+            v = -(co.x * sx + co.y * sy); if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].x, v);
+            v = -(co.z * sy + co.y * sx); if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].y, v);
+            v = -0.5f * sm.g[k][2]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 0], v);
+            v = -sm.g[k][3]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 1], v);
+            v = -0.5f * sm.g[k][4]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 2], v);
             v = sm.g[k][5]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 0], v);
             v = sm.g[k][6]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 1], v);
             v = sm.g[k][7]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 2], v);
