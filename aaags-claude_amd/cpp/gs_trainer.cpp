@@ -159,6 +159,8 @@ gs_trainer::~gs_trainer() {
     vals_sorted.release();
     cam_dev.release();
     hp_dev.release();
+    if (k_event) (void)hipEventDestroy(k_event);
+    if (k_pinned) (void)hipHostFree(k_pinned);
     for (auto& g : graphs) g.release();
 }
 
@@ -260,16 +262,35 @@ static int tr_bin(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs
     return gs_set_error(GS_ERR_CAPACITY, "trainer binning: pair buffer did not grow");
 }
 
-// Forward of one view into view buffers; returns the pair count via *pairs.
+// Forward of one view into view buffers.  When the direct-placement path has a
+// pair buffer already (a previous step sized it), binning does not wait for
+// the pair count: it is copied to pinned memory and *pending is set, and the
+// caller checks it with tr_pairs() before the results are used for an update;
+// otherwise the count is read back first and *pairs is set.
 static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs_render_cfg* cfg,
-                      long long* pairs) {
+                      long long* pairs, bool* pending) {
     gs_ctx* c = t->ctx;
     TR_TRY(tr_alloc_view(t, v, cam->width, cam->height));
     gs_gaussians g = t->gaussians();
     gs_preprocessed pre = v.pre();
     TR_TRY(gs_preprocess_fwd(c, &g, cam, cfg, &pre));
     long long K = 0;
-    TR_TRY(tr_bin(t, v, cam, pre, &K));
+    *pending = false;
+    const bool direct = (cam->width + 15) / 16 <= GS_BIN_MAX_TILES_XY && (cam->height + 15) / 16 <= GS_BIN_MAX_TILES_XY;
+    if (direct && t->keys_sorted.bytes >= 4 && t->allow_async) {
+        if (!t->k_pinned) {
+            hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&t->k_pinned), 8, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&t->k_event, hipEventDisableTiming);
+            if (e != hipSuccess) return gs_hip_status(e, "pinned pair count");
+        }
+        const uint32_t cap = (uint32_t)std::min<size_t>(t->keys_sorted.bytes / 4, 0xFFFFFFFFu);
+        TR_TRY(gs_bin_tiles_impl(c, t->n, &pre, cam->width, cam->height, t->keys_sorted.u(), cap, v.ranges.u(), nullptr,
+                                 t->k_pinned, t->k_event));
+        t->pair_stride = 1;
+        *pending = true;
+    } else {
+        TR_TRY(tr_bin(t, v, cam, pre, &K));
+    }
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
     if (t->pair_stride == 1)
         TR_TRY(gs_raster_fwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color,
@@ -278,6 +299,25 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
         TR_TRY(gs_raster_fwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
                                    v.ranges.u(), cfg->bg_color, v.image.f(), &px));
     *pairs = K;
+    return GS_OK;
+}
+
+// The pair count of a pending binning (waits for its copy only); *fits is
+// false when the pairs overflowed the buffer, which is then grown.
+static int tr_pairs(gs_trainer* t, long long* K, bool* fits) {
+    hipError_t e = hipEventSynchronize(t->k_event);
+    if (e != hipSuccess) return gs_hip_status(e, "hipEventSynchronize");
+    *K = (long long)*t->k_pinned;
+    const size_t cap = t->keys_sorted.bytes / 4;
+    *fits = (size_t)*K <= cap;
+    if (!*fits) {
+        // the dropped pairs raised the context's overflow flag: clear it, grow
+        void* sm = nullptr;
+        TR_TRY(gs_small_get(t->ctx, &sm));
+        e = launch_zero(static_cast<char*>(sm) + GS_SMALL_OVERFLOW, 4, t->ctx->stream);
+        if (e != hipSuccess) return gs_hip_status(e, "zero kernel");
+        TR_TRY(t->keys_sorted.ensure((size_t)(*K + *K / 8 + 1024) * 4));
+    }
     return GS_OK;
 }
 
@@ -361,22 +401,43 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
     if (t->views.empty()) t->views.resize(1);
     gs::ViewBufs& v = t->views[0];
     long long K = 0;
-    TR_TRY(tr_forward(t, v, cam, cfg, &K));
-    TR_TRY(gs_l1_loss(c, cam->width, cam->height, v.image.f(), gt, v.d_image.f(), static_cast<double*>(v.loss.p)));
-    size_t n = (size_t)t->n;
-    TR_TRY(gs_zero(c, v.rg_means2D.p, n * 8));
-    TR_TRY(gs_zero(c, v.rg_conics.p, n * 12));
-    TR_TRY(gs_zero(c, v.rg_rgb.p, n * 12));
-    TR_TRY(gs_zero(c, v.rg_op.p, n * 4));
+    // forward, loss and raster backward; with a pending pair count the host
+    // checks it here, while the rasterizers run, and redoes the three stages
+    // in the rare case that the pairs outgrew the buffer
+    auto fwd_bwd = [&](bool* pending) -> int {
+        TR_TRY(tr_forward(t, v, cam, cfg, &K, pending));
+        TR_TRY(gs_l1_loss(c, cam->width, cam->height, v.image.f(), gt, v.d_image.f(), static_cast<double*>(v.loss.p)));
+        const size_t n = (size_t)t->n;
+        TR_TRY(gs_zero(c, v.rg_means2D.p, n * 8));
+        TR_TRY(gs_zero(c, v.rg_conics.p, n * 12));
+        TR_TRY(gs_zero(c, v.rg_rgb.p, n * 12));
+        TR_TRY(gs_zero(c, v.rg_op.p, n * 4));
+        gs_preprocessed pre = v.pre();
+        gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
+        gs_raster_grads rg = v.rgrads();
+        if (t->pair_stride == 1)
+            TR_TRY(gs_raster_bwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color,
+                                 &px, v.d_image.f(), &rg));
+        else
+            TR_TRY(gs_raster_bwd_words(c, cam->width, cam->height, &pre,
+                                       static_cast<const uint64_t*>(t->keys_sorted.p), v.ranges.u(), cfg->bg_color,
+                                       &px, v.d_image.f(), &rg));
+        return GS_OK;
+    };
+    bool pending = false;
+    TR_TRY(fwd_bwd(&pending));
+    if (pending) {
+        bool fits = true;
+        TR_TRY(tr_pairs(t, &K, &fits));
+        if (!fits) {
+            t->allow_async = false;
+            const int st = fwd_bwd(&pending);
+            t->allow_async = true;
+            TR_TRY(st);
+        }
+    }
     gs_preprocessed pre = v.pre();
-    gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
     gs_raster_grads rg = v.rgrads();
-    if (t->pair_stride == 1)
-        TR_TRY(gs_raster_bwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color, &px,
-                             v.d_image.f(), &rg));
-    else
-        TR_TRY(gs_raster_bwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
-                                   v.ranges.u(), cfg->bg_color, &px, v.d_image.f(), &rg));
     gs_gaussians g = t->gaussians();
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
     if (t->fused_adam && tc->use_adam) {
@@ -404,7 +465,18 @@ extern "C" int gs_trainer_render(gs_trainer* t, const gs_camera* cam, const gs_r
     TR_TRY(check_cam_cfg(cam, cfg));
     if (t->views.empty()) t->views.resize(1);
     long long K = 0;
-    TR_TRY(tr_forward(t, t->views[0], cam, cfg, &K));
+    bool pending = false;
+    TR_TRY(tr_forward(t, t->views[0], cam, cfg, &K, &pending));
+    if (pending) {
+        bool fits = true;
+        TR_TRY(tr_pairs(t, &K, &fits));
+        if (!fits) {
+            t->allow_async = false;
+            const int st = tr_forward(t, t->views[0], cam, cfg, &K, &pending);
+            t->allow_async = true;
+            TR_TRY(st);
+        }
+    }
     if (pairs) *pairs = K;
     return GS_OK;
 }

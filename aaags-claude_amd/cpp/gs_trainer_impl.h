@@ -56,6 +56,9 @@ struct gs_trainer {
     int pair_stride = 1;  // uint32 entries per pair in keys_sorted: 1 (indices) or 2 (words)
     int fused_adam = 0;   // Adam applied inside the preprocess backward (gradients not written)
     gs::DevBuf cam_dev, hp_dev;  // device copies of the camera and Adam hyper-parameters (fused step)
+    unsigned long long* k_pinned = nullptr;  // pair count of a pending binning (pinned host memory)
+    hipEvent_t k_event = nullptr;             // recorded after its copy
+    bool allow_async = true;
     std::vector<gs::GraphSlot> graphs;
     gs_gaussians gaussians() const;
     size_t tensor_elems(int k) const;

@@ -438,8 +438,15 @@ extern "C" int gs_ctx_apply_tuning(gs_ctx* c, const gs_tuning* t) {
     return GS_OK;
 }
 
-extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
-                            uint32_t capacity, uint32_t* ranges, long long* total_host) {
+// gs_bin_tiles, and its form for drivers that keep the GPU busy: with
+// total_async (pinned host memory) the pair count is copied there without a
+// host wait (`ev_total` is recorded after the copy), the placement runs with
+// the given capacity whatever the count, pairs past it are dropped (overflow
+// flag) and the ranges are clamped to it; the caller compares the count with
+// the capacity before using the results.
+int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
+                      uint32_t capacity, uint32_t* ranges, long long* total_host, unsigned long long* total_async,
+                      hipEvent_t ev_total) {
     GS_CHECK_ARG(c != nullptr, "null context");
     GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
     GS_CHECK_ARG(n >= 0, "negative count");
@@ -508,6 +515,15 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     GS_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");
     GS_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr, total64),
            "binning scan");
+    a.table = scanned;
+    if (total_async) {
+        GS_CHECK_ARG(values != nullptr && capacity > 0, "null values buffer");
+        GS_HIP(hipMemcpyAsync(total_async, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
+        if (ev_total) GS_HIP(hipEventRecord(ev_total, c->stream), "hipEventRecord");
+        GS_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
+        GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
+        return GS_OK;
+    }
     unsigned long long K = 0;
     GS_HIP(hipMemcpyAsync(&K, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
     GS_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -515,11 +531,15 @@ extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int wi
     if (K > capacity) return gs_set_error(GS_ERR_CAPACITY, "gs_bin_tiles: %llu pairs exceed capacity %u", K, capacity);
     if (K > 0) {
         GS_CHECK_ARG(values != nullptr, "null values buffer");
-        a.table = scanned;
         GS_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
     }
     GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
     return GS_OK;
+}
+
+extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
+                            uint32_t capacity, uint32_t* ranges, long long* total_host) {
+    return gs_bin_tiles_impl(c, n, pre, width, height, values, capacity, ranges, total_host, nullptr, nullptr);
 }
 
 static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed* pre, const uint32_t* values,

@@ -47,6 +47,9 @@ int gs_tile_key_bits(int num_tiles);
 extern "C" int gs_ctx_set_tuning(gs_ctx* c, int radix_bits, int raster_batch, int raster_bwd_batch);
 extern "C" int gs_ctx_reserve(gs_ctx* c, size_t bytes);
 extern "C" int gs_ctx_set_binning(gs_ctx* c, int gaussians_per_group, int tiles_per_lane);
+int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
+                      uint32_t capacity, uint32_t* ranges, long long* total_host, unsigned long long* total_async,
+                      hipEvent_t ev_total);
 extern "C" int gs_check_overflow(gs_ctx* c, int* overflowed);
 
 // kernel launchers

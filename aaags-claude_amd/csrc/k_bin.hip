@@ -25,9 +25,9 @@
 //   place:  the same walk again with each lane's cursors set from the scanned
 //           table; each Gaussian index goes to slot cursor++ -- its stable rank
 //           in the tile.  A lane collects its tile's slots in a 16-entry
-//           (64-byte) LDS line buffer and writes the line with one store when
-//           it fills, so HBM sees whole-line writes except at the two ends of
-//           each (tile, group) run.
+//           (64-byte) LDS line buffer; when lines fill, the wave writes them
+//           out one by one, 16 lanes per line, so HBM sees whole-line writes
+//           except at the two ends of each (tile, group) run.
 //   ranges: (start, end) of a tile are the scanned entries of its first group
 //           and of the next tile's first group.
 // A ballot over 64 rectangles at a time skips the Gaussians that miss the bin,
@@ -87,8 +87,9 @@ __device__ __forceinline__ BinBlock<TPL> bin_block(const BinArgs& a) {
     return b;
 }
 
-// Walks the block's group in depth order and calls f(k, gid) on every lane
-// whose k-th tile is covered by the Gaussian's rectangle.  Rectangles are read
+// Walks the block's group in depth order and, for every Gaussian that meets the
+// bin, calls f(in, gid) on all lanes with in[k] set where the lane's k-th tile
+// is covered by the Gaussian's rectangle.  Rectangles are read
 // 64 at a time, one per lane (the next 64 prefetched); the ones that meet the
 // bin are visited lowest lane first, i.e. in depth order.
 template <int TPL, typename F>
@@ -115,11 +116,13 @@ __device__ __forceinline__ void walk_bin(const BinArgs& a, const BinBlock<TPL>& 
             const int jx0 = (int)(R & 0xFFu), jx1 = (int)((R >> 8) & 0xFFu);
             const int jy0 = (int)((R >> 16) & 0xFFu), jy1 = (int)(R >> 24);
             const bool inx = tx >= jx0 && tx < jx1;
+            bool in[TPL];
 #pragma unroll
             for (int k = 0; k < TPL; k++) {
                 const int y = ty + 8 * k;
-                if (inx && y >= jy0 && y < jy1) f(k, gid);
+                in[k] = inx && y >= jy0 && y < jy1;
             }
+            f(in, gid);
         }
     }
 }
@@ -133,7 +136,10 @@ __global__ __launch_bounds__(64) void bin_count_kernel(BinArgs a) {
     uint32_t cnt[TPL];
 #pragma unroll
     for (int k = 0; k < TPL; k++) cnt[k] = 0u;
-    walk_bin<TPL>(a, b, tx, ty, lane, [&](int k, uint32_t) { cnt[k]++; });
+    walk_bin<TPL>(a, b, tx, ty, lane, [&](const bool (&in)[TPL], uint32_t) {
+#pragma unroll
+        for (int k = 0; k < TPL; k++) cnt[k] += in[k] ? 1u : 0u;
+    });
     const int gx = b.vi.x, gy = b.vi.y;
     const size_t tab = (size_t)(uint32_t)b.vi.z;
 #pragma unroll
@@ -143,18 +149,16 @@ __global__ __launch_bounds__(64) void bin_count_kernel(BinArgs a) {
     }
 }
 
-// Writes slots [lo, hi] of one tile's line buffer: one 64-byte store when the
-// whole line is this block's and below capacity, else slot by slot.
-__device__ __forceinline__ void flush_line(const BinArgs& a, const uint32_t* line, uint32_t lo, uint32_t hi) {
-    if (lo + (LINE - 1) == hi && hi < a.capacity) {
-        const uint4* src = reinterpret_cast<const uint4*>(line);
-        uint4* dst = reinterpret_cast<uint4*>(a.out + lo);
-#pragma unroll
-        for (int k = 0; k < LINE / 4; k++) dst[k] = src[k];
-        return;
-    }
-    for (uint32_t k = lo; k <= hi; k++) {
-        if (k < a.capacity) a.out[k] = line[k & (LINE - 1)];
+// Writes slots [lo, hi] (one line, hi & 15 the line's last slot or its
+// current end) of lane j's k-th line buffer with the whole wave: lane l < 16
+// moves slot l, so the line leaves as one coalesced 64-byte store (fewer lanes
+// for the partial lines at the ends of a run).  Slots past the capacity are
+// dropped and raise the overflow flag.
+__device__ __forceinline__ void flush_line_wave(const BinArgs& a, const uint32_t* line, uint32_t lo, uint32_t hi,
+                                                int lane) {
+    const uint32_t idx = (hi & ~(uint32_t)(LINE - 1)) + (uint32_t)lane;
+    if (lane < LINE && idx >= lo && idx <= hi) {
+        if (idx < a.capacity) a.out[idx] = line[lane];
         else *a.overflow = 1u;
     }
 }
@@ -174,13 +178,26 @@ __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
         cur[k] = (tx < gx && y < gy) ? a.table[tab + (size_t)(y * gx + tx) * a.C + b.c] : 0u;
         beg[k] = cur[k];
     }
-    walk_bin<TPL>(a, b, tx, ty, lane, [&](int k, uint32_t gid) {
-        uint32_t* line = lines + (k * 64 + lane) * LINE;
-        const uint32_t p = cur[k]++;
-        line[p & (LINE - 1)] = gid;
-        if ((p & (LINE - 1)) == LINE - 1) {
-            const uint32_t lo = p - (LINE - 1);
-            flush_line(a, line, lo > beg[k] ? lo : beg[k], p);
+    walk_bin<TPL>(a, b, tx, ty, lane, [&](const bool (&in)[TPL], uint32_t gid) {
+#pragma unroll
+        for (int k = 0; k < TPL; k++) {
+            uint32_t p = 0u;
+            bool full = false;
+            if (in[k]) {
+                p = cur[k]++;
+                lines[(k * 64 + lane) * LINE + (p & (LINE - 1))] = gid;
+                full = (p & (LINE - 1)) == LINE - 1;
+            }
+            // lines that just filled leave one at a time, each written by the wave
+            uint64_t fm = __ballot(full);
+            while (fm) {
+                const int j = __builtin_ctzll(fm);
+                fm &= fm - 1ull;
+                const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)p, j);
+                const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)beg[k], j);
+                const uint32_t lo = pj - (LINE - 1);
+                flush_line_wave(a, lines + (k * 64 + j) * LINE, lo > bj ? lo : bj, pj, lane);
+            }
         }
     });
     // the last, partly filled line of each tile
@@ -188,14 +205,24 @@ __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
     for (int k = 0; k < TPL; k++) {
         const uint32_t end = cur[k], lo = end & ~(uint32_t)(LINE - 1);
         const uint32_t from = lo > beg[k] ? lo : beg[k];
-        if (from < end) flush_line(a, lines + (k * 64 + lane) * LINE, from, end - 1u);
+        uint64_t fm = __ballot(from < end);
+        while (fm) {
+            const int j = __builtin_ctzll(fm);
+            fm &= fm - 1ull;
+            const uint32_t fj = (uint32_t)__builtin_amdgcn_readlane((int)from, j);
+            const uint32_t ej = (uint32_t)__builtin_amdgcn_readlane((int)end, j);
+            flush_line_wave(a, lines + (k * 64 + j) * LINE, fj, ej - 1u, lane);
+        }
     }
 }
 
 // One thread per global tile.
+// Ranges are clamped to the pair capacity, so that after an overflow (pairs
+// dropped, flag raised) the rasterizers still read inside the buffer.
 __global__ __launch_bounds__(256) void bin_ranges_kernel(const uint32_t* __restrict__ scanned,
                                                          const int4* __restrict__ views, int4 view0, int V, int C,
-                                                         uint32_t total_tiles, uint32_t* __restrict__ ranges) {
+                                                         uint32_t total_tiles, uint32_t capacity,
+                                                         uint32_t* __restrict__ ranges) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= total_tiles) return;
     int v = 0;
@@ -206,6 +233,8 @@ __global__ __launch_bounds__(256) void bin_ranges_kernel(const uint32_t* __restr
     const size_t tab = (size_t)(uint32_t)vi.z;
     uint32_t s = scanned[tab + t * C];
     uint32_t e = scanned[tab + (t + 1) * C];
+    s = s < capacity ? s : capacity;
+    e = e < capacity ? e : capacity;
     if (s == e) s = e = 0u;
     ranges[2 * (size_t)g] = s;
     ranges[2 * (size_t)g + 1] = e;
@@ -245,6 +274,6 @@ hipError_t launch_bin_ranges(const BinArgs& a, const uint32_t* scanned, uint32_t
                              hipStream_t s) {
     if (total_tiles == 0) return hipSuccess;
     bin_ranges_kernel<<<(total_tiles + 255) / 256, 256, 0, s>>>(scanned, a.views, a.view0, a.V, a.C, total_tiles,
-                                                                ranges);
+                                                                a.capacity, ranges);
     return hipGetLastError();
 }

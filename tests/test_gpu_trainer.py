@@ -91,3 +91,35 @@ def test_trainer_fused_adam_matches_separate_adam(hip):
         same += int((a.view(np.uint32) == b.view(np.uint32)).sum())
         total += a.size
     assert same >= 0.99 * total, (same, total)
+
+
+def test_trainer_step_redoes_when_pairs_outgrow_buffer(hip, oracle):
+    """The pair buffer is sized by the first step (far camera); the second
+    step's view is much closer, so its binning overflows the buffer while the
+    host is not waiting for the count.  The step must detect it, grow the
+    buffer, redo the forward and backward, and give the oracle's result for
+    the parameters after the first step."""
+    from gs_hip import GsTrainer, train_cfg
+    from gs_scenes import RawScene, make_camera
+    print("[case] trainer pair-buffer overflow redo", flush=True)
+    scene, _, gt = random_small_scene(1500, 3, 34, width=128, height=96)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    far = make_camera((0.0, 0.0, -9.0), 128, 96, 60.0)
+    near = make_camera((0.0, 0.0, -2.2), 128, 96, 60.0)
+    tc = train_cfg(use_adam=False)
+    tr = GsTrainer(hip, scene)
+    gt_t = torch.tensor(gt, device=hip.dev)
+    r1 = tr.step(far, cfg, gt_t, tc, 0)
+    p1 = tr.params()
+    r2 = tr.step(near, cfg, gt_t, tc, 1)
+    torch.cuda.synchronize()
+    assert r2.pairs > (r1.pairs * 9) // 8 + 1024, (r1.pairs, r2.pairs)  # the buffer did overflow
+    mid = RawScene(p1[0], p1[1], p1[2], p1[3].reshape(scene.sh.shape), p1[4], scene.sh_degree)
+    ref = oracle.forward_backward(mid, near, cfg, gt)
+    assert abs(r2.loss - ref["loss"]) <= 1e-6
+    assert r2.pairs == len(ref["keys"])
+    assert_image_close(tr.buffer("image", (96, 128, 3)).cpu().numpy(), ref["img"])
+    n = scene.n
+    shapes = dict(pos=(n, 3), scale=(n, 3), rot=(n, 4), sh=(n, scene.max_coeffs, 3), op=(n,))
+    for k in NAMES:
+        assert_grad_close(tr.buffer("grad_" + k, shapes[k]).cpu().numpy(), ref["grads"][k], "grad_" + k)
