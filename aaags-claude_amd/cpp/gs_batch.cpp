@@ -75,14 +75,18 @@ struct gs_batch {
     DevBuf words, words_tmp, tsort_ws, ranges;
     DevBuf values, bviews_dev, bvblock_dev, bin_table, bin_scanned, bin_partial, bin_rects;
     DevBuf image, d_image, gt, T_final, n_contrib, last;
-    DevBuf rg_means2D, rg_conics, rg_rgb, rg_op;
+    DevBuf rg;  // screen-space gradients, one block of V*N: means2D (8 B), conics (12), rgb (12), opacities (4)
     DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16
     int l1_nb = 0;
 
     gs_preprocessed pre() {
         return gs_preprocessed{means2D.f(), depths.f(), conics.f(), op2d.f(), rgb.f(), radii.i(), tiles.i()};
     }
-    gs_raster_grads rgrads() { return gs_raster_grads{rg_means2D.f(), rg_conics.f(), rg_rgb.f(), rg_op.f()}; }
+    gs_raster_grads rgrads() {
+        const size_t vn = (size_t)V * N;
+        float* base = rg.f();
+        return gs_raster_grads{base, base + 2 * vn, base + 5 * vn, base + 8 * vn};
+    }
     uint32_t* total32() { return reinterpret_cast<uint32_t*>(small.p); }
     unsigned long long* total64() { return reinterpret_cast<unsigned long long*>(static_cast<char*>(small.p) + 8); }
     uint32_t* overflow() { return reinterpret_cast<uint32_t*>(static_cast<char*>(small.p) + 16); }
@@ -96,7 +100,7 @@ gs_batch::~gs_batch() {
                      &radii, &tiles, &dkeys, &didx, &dkeys_s, &didx_s, &rtiles, &roff, &scan_partial, &dsort_ws,
                      &words, &words_tmp, &tsort_ws, &ranges, &values, &bviews_dev, &bvblock_dev, &bin_table, &bin_scanned,
                      &bin_partial, &bin_rects, &image, &d_image, &gt, &T_final, &n_contrib, &last,
-                     &rg_means2D, &rg_conics, &rg_rgb, &rg_op, &losses, &l1_partial, &small};
+                     &rg, &losses, &l1_partial, &small};
     for (DevBuf* b : all) b->release();
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
@@ -235,8 +239,7 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
         (st = b->dsort_ws.ensure(radix_workspace_bytes((uint32_t)VN, b->radix_bits))) ||
         (st = b->ranges.ensure((size_t)b->total_tiles * 8)) || (st = b->image.ensure(P * 12)) ||
         (st = b->d_image.ensure(P * 12)) || (st = b->gt.ensure(P * 12)) || (st = b->T_final.ensure(P * 4)) ||
-        (st = b->n_contrib.ensure(P * 4)) || (st = b->last.ensure(P * 4)) || (st = b->rg_means2D.ensure(VN * 8)) ||
-        (st = b->rg_conics.ensure(VN * 12)) || (st = b->rg_rgb.ensure(VN * 12)) || (st = b->rg_op.ensure(VN * 4)) ||
+        (st = b->n_contrib.ensure(P * 4)) || (st = b->last.ensure(P * 4)) || (st = b->rg.ensure(VN * 36)) ||
         (st = b->losses.ensure(8 * n_views)) || (st = b->small.ensure(256)))
         return fail(st);
     int max_pix = 0;
@@ -396,10 +399,7 @@ static int enqueue_l1(gs_batch* b, hipStream_t s) {
 // Zeroes the screen-space gradients and runs the multi-view backward raster.
 static int enqueue_raster_bwd(gs_batch* b, hipStream_t s) {
     const size_t VN = (size_t)b->V * b->N;
-    B_HIP(launch_zero(b->rg_means2D.p, VN * 8, s), "zero");
-    B_HIP(launch_zero(b->rg_conics.p, VN * 12, s), "zero");
-    B_HIP(launch_zero(b->rg_rgb.p, VN * 12, s), "zero");
-    B_HIP(launch_zero(b->rg_op.p, VN * 4, s), "zero");
+    B_HIP(launch_zero(b->rg.p, VN * 36, s), "zero");
     unsigned* ctr = b->persistent ? static_cast<unsigned*>(b->counters.p) + 16 : nullptr;
     if (ctr) B_HIP(launch_zero(ctr, 32, s), "zero queue counters");
     gsr::MultiRaster m = multi_raster(b, ctr);
@@ -429,12 +429,12 @@ static int enqueue_step(gs_batch* b, hipStream_t s) {
     }
     const gso::AdamDev* hp = static_cast<const gso::AdamDev*>(b->hp_dev.p);
     if (b->fused_adam) {
-        B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), b->V, b->radii.i(), rg, pg,
-                                          P, M, Vv, hp, t->sh_degree, s),
+        B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), nullptr, b->V, b->radii.i(),
+                                          rg, pg, P, M, Vv, hp, nullptr, t->sh_degree, s),
               "multi-view preprocess backward + adam");
     } else {
-        B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), b->V, b->radii.i(), rg, pg,
-                                          nullptr, nullptr, nullptr, nullptr, t->sh_degree, s),
+        B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), nullptr, b->V, b->radii.i(),
+                                          rg, pg, nullptr, nullptr, nullptr, nullptr, nullptr, t->sh_degree, s),
               "multi-view preprocess backward");
         gso::AdamArgs a{};
         const uint32_t sizes[5] = {(uint32_t)t->n * 3u, (uint32_t)t->n * 3u, (uint32_t)t->n * 4u,
@@ -596,10 +596,11 @@ extern "C" void* gs_batch_buffer(gs_batch* b, const char* which) {
     if (w == "ranges") return b->ranges.p;
     if (w == "pairs") return b->direct ? b->values.p : b->words.p;  // uint32 indices or 64-bit words
     if (w == "losses") return b->losses.p;
-    if (w == "rg_means2D") return b->rg_means2D.p;
-    if (w == "rg_conics") return b->rg_conics.p;
-    if (w == "rg_rgb") return b->rg_rgb.p;
-    if (w == "rg_op") return b->rg_op.p;
+    const gs_raster_grads g = b->rgrads();
+    if (w == "rg_means2D") return g.d_means2D;
+    if (w == "rg_conics") return g.d_conics;
+    if (w == "rg_rgb") return g.d_rgb;
+    if (w == "rg_op") return g.d_opacities_2d;
     return nullptr;
 }
 
@@ -629,41 +630,45 @@ struct BatchTimer {
 };
 
 template <typename F>
-int batch_time(gs_batch* b, BatchTimer& tm, int reps, F&& f, float* ms) {
+int batch_time_once(gs_batch* b, BatchTimer& tm, F&& f, float* ms) {
+    B_HIP(hipEventRecord(tm.a, b->stream), "hipEventRecord");
     B_TRY(f());
-    std::vector<float> t;
-    for (int r = 0; r < reps; r++) {
-        B_HIP(hipEventRecord(tm.a, b->stream), "hipEventRecord");
-        B_TRY(f());
-        B_HIP(hipEventRecord(tm.e, b->stream), "hipEventRecord");
-        B_HIP(hipEventSynchronize(tm.e), "hipEventSynchronize");
-        float x = 0.0f;
-        B_HIP(hipEventElapsedTime(&x, tm.a, tm.e), "hipEventElapsedTime");
-        t.push_back(x);
-    }
-    std::sort(t.begin(), t.end());
-    *ms = t[t.size() / 2];
+    B_HIP(hipEventRecord(tm.e, b->stream), "hipEventRecord");
+    B_HIP(hipEventSynchronize(tm.e), "hipEventSynchronize");
+    B_HIP(hipEventElapsedTime(ms, tm.a, tm.e), "hipEventElapsedTime");
     return GS_OK;
 }
 
+// Candidates timed round-robin, `reps` runs each after one untimed run; the
+// smallest median wins and is left set.
 template <typename Set, typename Run>
 int batch_pick(gs_batch* b, BatchTimer& tm, int reps, const std::vector<int>& cands, Set&& set, Run&& run, int* best,
                float* best_ms) {
-    float bm = 0.0f;
-    int bv = cands.front();
-    bool have = false;
-    for (int v : cands) {
-        B_TRY(set(v));
-        float ms = 0.0f;
-        B_TRY(batch_time(b, tm, reps, run, &ms));
-        if (!have || ms < bm) {
-            bm = ms;
-            bv = v;
-            have = true;
+    std::vector<std::vector<float>> t(cands.size());
+    for (size_t i = 0; i < cands.size(); i++) {
+        B_TRY(set(cands[i]));
+        B_TRY(run());
+    }
+    for (int r = 0; r < reps; r++) {
+        for (size_t i = 0; i < cands.size(); i++) {
+            B_TRY(set(cands[i]));
+            float ms = 0.0f;
+            B_TRY(batch_time_once(b, tm, run, &ms));
+            t[i].push_back(ms);
         }
     }
-    B_TRY(set(bv));
-    *best = bv;
+    size_t bi = 0;
+    float bm = 0.0f;
+    for (size_t i = 0; i < cands.size(); i++) {
+        std::sort(t[i].begin(), t[i].end());
+        const float med = t[i][t[i].size() / 2];
+        if (i == 0 || med < bm) {
+            bm = med;
+            bi = i;
+        }
+    }
+    B_TRY(set(cands[bi]));
+    *best = cands[bi];
     *best_ms = bm;
     return GS_OK;
 }
@@ -693,7 +698,8 @@ extern "C" int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out) {
                          [&](int x) { return batch_setup_binning(b, x, b->bin_tpl); }, bin, &res.bin_group,
                          &res.binning_ms));
     } else {
-        B_TRY(batch_time(b, tm, reps, bin, &res.binning_ms));
+        int unused = 0;
+        B_TRY(batch_pick(b, tm, reps, {0}, [](int) { return GS_OK; }, bin, &unused, &res.binning_ms));
     }
     B_TRY(bin());
     // persistent groups per CU, on forward + loss + backward raster

@@ -70,11 +70,37 @@ static int tr_alloc_view(gs_trainer* t, gs::ViewBufs& v, int W, int H) {
     TR_TRY(v.n_contrib.ensure(npix * 4));
     TR_TRY(v.last.ensure(npix * 4));
     TR_TRY(v.loss.ensure(8));
-    TR_TRY(v.rg_means2D.ensure(n * 8));
-    TR_TRY(v.rg_conics.ensure(n * 12));
-    TR_TRY(v.rg_rgb.ensure(n * 12));
-    TR_TRY(v.rg_op.ensure(n * 4));
+    TR_TRY(v.rg.ensure(n * 36));
+    v.rg_n = n;
     return GS_OK;
+}
+
+hipError_t gs::StageTimer::collect() {
+    for (int s = 0; s < GS_NUM_STAGES; s++) {
+        if (!on(s)) continue;
+        float ms = 0.0f;
+        hipError_t e = hipEventElapsedTime(&ms, beg[s], end[s]);
+        if (e != hipSuccess) return e;
+        total_ms[s] += ms;
+    }
+    steps++;
+    return hipSuccess;
+}
+
+void gs::StageTimer::release() {
+    for (int s = 0; s < GS_NUM_STAGES; s++) {
+        if (beg[s]) (void)hipEventDestroy(beg[s]);
+        if (end[s]) (void)hipEventDestroy(end[s]);
+        beg[s] = end[s] = nullptr;
+    }
+    mask = 0;
+}
+
+// Event mark at the start (begin) or end of a timed stage.
+static int tr_mark(gs_trainer* t, int stage, bool begin) {
+    hipStream_t st = t->ctx->stream;
+    hipError_t e = begin ? t->timer.mark_begin(stage, st) : t->timer.mark_end(stage, st);
+    return e == hipSuccess ? GS_OK : gs_hip_status(e, "hipEventRecord(stage timer)");
 }
 
 gs_preprocessed gs::ViewBufs::pre() {
@@ -91,10 +117,11 @@ gs_preprocessed gs::ViewBufs::pre() {
 
 gs_raster_grads gs::ViewBufs::rgrads() {
     gs_raster_grads g;
-    g.d_means2D = rg_means2D.f();
-    g.d_conics = rg_conics.f();
-    g.d_rgb = rg_rgb.f();
-    g.d_opacities_2d = rg_op.f();
+    float* base = rg.f();
+    g.d_means2D = base;
+    g.d_conics = base + 2 * rg_n;
+    g.d_rgb = base + 5 * rg_n;
+    g.d_opacities_2d = base + 8 * rg_n;
     return g;
 }
 
@@ -157,8 +184,7 @@ gs_trainer::~gs_trainer() {
     vals.release();
     keys_sorted.release();
     vals_sorted.release();
-    cam_dev.release();
-    hp_dev.release();
+    timer.release();
     if (k_event) (void)hipEventDestroy(k_event);
     if (k_pinned) (void)hipHostFree(k_pinned);
     for (auto& g : graphs) g.release();
@@ -166,7 +192,7 @@ gs_trainer::~gs_trainer() {
 
 void gs::ViewBufs::release_all() {
     DevBuf* all[] = {&means2D, &depths, &conics, &op2d, &rgb, &radii, &tiles, &offsets, &ranges, &image, &d_image,
-                     &T_final, &n_contrib, &last, &loss, &rg_means2D, &rg_conics, &rg_rgb, &rg_op};
+                     &T_final, &n_contrib, &last, &loss, &rg};
     for (DevBuf* b : all) b->release();
 }
 
@@ -273,31 +299,40 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
     TR_TRY(tr_alloc_view(t, v, cam->width, cam->height));
     gs_gaussians g = t->gaussians();
     gs_preprocessed pre = v.pre();
+    TR_TRY(tr_mark(t, GS_STAGE_PREPROCESS, true));
     TR_TRY(gs_preprocess_fwd(c, &g, cam, cfg, &pre));
+    TR_TRY(tr_mark(t, GS_STAGE_PREPROCESS, false));
     long long K = 0;
     *pending = false;
+    TR_TRY(tr_mark(t, GS_STAGE_BINNING, true));
     const bool direct = (cam->width + 15) / 16 <= GS_BIN_MAX_TILES_XY && (cam->height + 15) / 16 <= GS_BIN_MAX_TILES_XY;
     if (direct && t->keys_sorted.bytes >= 4 && t->allow_async) {
         if (!t->k_pinned) {
-            hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&t->k_pinned), 8, hipHostMallocDefault);
+            // mapped + coherent: the scan kernel stores the count straight into it
+            hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&t->k_pinned), 8,
+                                         hipHostMallocMapped | hipHostMallocCoherent);
+            if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&t->k_pinned_dev), t->k_pinned, 0);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&t->k_event, hipEventDisableTiming);
             if (e != hipSuccess) return gs_hip_status(e, "pinned pair count");
         }
         const uint32_t cap = (uint32_t)std::min<size_t>(t->keys_sorted.bytes / 4, 0xFFFFFFFFu);
         TR_TRY(gs_bin_tiles_impl(c, t->n, &pre, cam->width, cam->height, t->keys_sorted.u(), cap, v.ranges.u(), nullptr,
-                                 t->k_pinned, t->k_event));
+                                 t->k_pinned_dev, t->k_event));
         t->pair_stride = 1;
         *pending = true;
     } else {
         TR_TRY(tr_bin(t, v, cam, pre, &K));
     }
+    TR_TRY(tr_mark(t, GS_STAGE_BINNING, false));
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
+    TR_TRY(tr_mark(t, GS_STAGE_RASTER_FWD, true));
     if (t->pair_stride == 1)
         TR_TRY(gs_raster_fwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color,
                              v.image.f(), &px));
     else
         TR_TRY(gs_raster_fwd_words(c, cam->width, cam->height, &pre, static_cast<const uint64_t*>(t->keys_sorted.p),
                                    v.ranges.u(), cfg->bg_color, v.image.f(), &px));
+    TR_TRY(tr_mark(t, GS_STAGE_RASTER_FWD, false));
     *pairs = K;
     return GS_OK;
 }
@@ -307,7 +342,7 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
 static int tr_pairs(gs_trainer* t, long long* K, bool* fits) {
     hipError_t e = hipEventSynchronize(t->k_event);
     if (e != hipSuccess) return gs_hip_status(e, "hipEventSynchronize");
-    *K = (long long)*t->k_pinned;
+    *K = (long long)*static_cast<volatile unsigned long long*>(t->k_pinned);
     const size_t cap = t->keys_sorted.bytes / 4;
     *fits = (size_t)*K <= cap;
     if (!*fits) {
@@ -354,8 +389,6 @@ static int tr_optimizer(gs_trainer* t, const gs_train_cfg* tc, int iteration) {
 static int tr_fused_backward_adam(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg,
                                   const gs_train_cfg* tc, int iteration, const int* radii, const gs_raster_grads& rg) {
     gs_ctx* c = t->ctx;
-    TR_TRY(t->cam_dev.ensure(sizeof(gsd::CamArgs)));
-    TR_TRY(t->hp_dev.ensure(sizeof(gso::AdamDev)));
     const gsd::CamArgs ca = make_cam_args(cam, cfg);
     gso::AdamDev h{};
     h.lr[0] = gs_lr_schedule(tc->lr_position_init, tc->lr_position_final, iteration, tc->max_steps);
@@ -368,10 +401,6 @@ static int tr_fused_backward_adam(gs_trainer* t, const gs_camera* cam, const gs_
     h.eps = tc->adam_eps;
     t->adam_steps++;  // AdamOptimizer::step_count_ (optimizer.cpp:72)
     gs_adam_bias_corrections(h.b1, h.b2, t->adam_steps, &h.bc1, &h.bc2);
-    // pageable sources: the runtime stages them before returning
-    hipError_t e = hipMemcpyAsync(t->cam_dev.p, &ca, sizeof(ca), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(t->hp_dev.p, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(camera/hparams)");
     float* P[5];
     float* M[5];
     float* V[5];
@@ -381,9 +410,35 @@ static int tr_fused_backward_adam(gs_trainer* t, const gs_camera* cam, const gs_
         V[k] = t->adam_v[k].f();
     }
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
-    e = launch_preprocess_bwd_multi(t->gaussians(), static_cast<const gsd::CamArgs*>(t->cam_dev.p), 1, radii, rg, pg, P,
-                                    M, V, static_cast<const gso::AdamDev*>(t->hp_dev.p), t->sh_degree, c->stream);
+    // camera and hyper-parameters travel as kernel arguments: no copies on the stream
+    hipError_t e = launch_preprocess_bwd_multi(t->gaussians(), nullptr, &ca, 1, radii, rg, pg, P, M, V, nullptr, &h,
+                                               t->sh_degree, c->stream);
     if (e != hipSuccess) return gs_hip_status(e, "preprocess backward + adam kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_set_timing(gs_trainer* t, unsigned stage_mask) {
+    if (!t) return gs_set_error(GS_ERR_INVALID, "null trainer");
+    stage_mask &= (1u << GS_NUM_STAGES) - 1u;
+    for (int s = 0; s < GS_NUM_STAGES; s++) {
+        t->timer.total_ms[s] = 0.0;
+        if (((stage_mask >> s) & 1u) && !t->timer.beg[s]) {
+            hipError_t e = hipEventCreate(&t->timer.beg[s]);
+            if (e == hipSuccess) e = hipEventCreate(&t->timer.end[s]);
+            if (e != hipSuccess) return gs_hip_status(e, "hipEventCreate(stage timer)");
+        }
+    }
+    t->timer.steps = 0;
+    t->timer.mask = stage_mask;
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_stage_times(gs_trainer* t, double* ms, int n, long long* steps) {
+    if (!t || (n > 0 && !ms)) return gs_set_error(GS_ERR_INVALID, "gs_trainer_stage_times: null argument");
+    for (int s = 0; s < n && s < GS_NUM_STAGES; s++)
+        ms[s] = (t->timer.steps > 0 && t->timer.on(s)) ? t->timer.total_ms[s] / (double)t->timer.steps : 0.0;
+    for (int s = GS_NUM_STAGES; s < n; s++) ms[s] = 0.0;
+    if (steps) *steps = t->timer.steps;
     return GS_OK;
 }
 
@@ -406,15 +461,16 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
     // in the rare case that the pairs outgrew the buffer
     auto fwd_bwd = [&](bool* pending) -> int {
         TR_TRY(tr_forward(t, v, cam, cfg, &K, pending));
+        TR_TRY(tr_mark(t, GS_STAGE_LOSS, true));
         TR_TRY(gs_l1_loss(c, cam->width, cam->height, v.image.f(), gt, v.d_image.f(), static_cast<double*>(v.loss.p)));
-        const size_t n = (size_t)t->n;
-        TR_TRY(gs_zero(c, v.rg_means2D.p, n * 8));
-        TR_TRY(gs_zero(c, v.rg_conics.p, n * 12));
-        TR_TRY(gs_zero(c, v.rg_rgb.p, n * 12));
-        TR_TRY(gs_zero(c, v.rg_op.p, n * 4));
+        TR_TRY(tr_mark(t, GS_STAGE_LOSS, false));
+        TR_TRY(tr_mark(t, GS_STAGE_CLEAR, true));
+        TR_TRY(gs_zero(c, v.rg.p, (size_t)t->n * 36));
+        TR_TRY(tr_mark(t, GS_STAGE_CLEAR, false));
         gs_preprocessed pre = v.pre();
         gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
         gs_raster_grads rg = v.rgrads();
+        TR_TRY(tr_mark(t, GS_STAGE_RASTER_BWD, true));
         if (t->pair_stride == 1)
             TR_TRY(gs_raster_bwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color,
                                  &px, v.d_image.f(), &rg));
@@ -422,6 +478,7 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
             TR_TRY(gs_raster_bwd_words(c, cam->width, cam->height, &pre,
                                        static_cast<const uint64_t*>(t->keys_sorted.p), v.ranges.u(), cfg->bg_color,
                                        &px, v.d_image.f(), &rg));
+        TR_TRY(tr_mark(t, GS_STAGE_RASTER_BWD, false));
         return GS_OK;
     };
     bool pending = false;
@@ -440,17 +497,23 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
     gs_raster_grads rg = v.rgrads();
     gs_gaussians g = t->gaussians();
     gs_param_grads pg{t->grad[0].f(), t->grad[1].f(), t->grad[2].f(), t->grad[3].f(), t->grad[4].f()};
+    TR_TRY(tr_mark(t, GS_STAGE_BACKWARD, true));
     if (t->fused_adam && tc->use_adam) {
         TR_TRY(tr_fused_backward_adam(t, cam, cfg, tc, iteration, pre.radii, rg));
     } else {
         TR_TRY(gs_preprocess_bwd(c, &g, cam, cfg, pre.radii, &rg, &pg, 0));
         TR_TRY(tr_optimizer(t, tc, iteration));
     }
+    TR_TRY(tr_mark(t, GS_STAGE_BACKWARD, false));
     double loss = 0.0;
     hipError_t e = hipMemcpyAsync(&loss, v.loss.p, 8, hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(loss)");
     e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return gs_hip_status(e, "hipStreamSynchronize");
+    if (t->timer.mask) {
+        e = t->timer.collect();
+        if (e != hipSuccess) return gs_hip_status(e, "hipEventElapsedTime(stage timer)");
+    }
     if (out) {
         out->loss = loss;
         out->iteration = iteration;

@@ -27,7 +27,8 @@ struct DevBuf {
 struct ViewBufs {
     DevBuf means2D, depths, conics, op2d, rgb, radii, tiles, offsets, ranges;
     DevBuf image, d_image, T_final, n_contrib, last, loss;
-    DevBuf rg_means2D, rg_conics, rg_rgb, rg_op;
+    DevBuf rg;  // screen-space gradients, one block: means2D (8 B/G), conics (12), rgb (12), opacities (4)
+    size_t rg_n = 0;
     gs_preprocessed pre();
     gs_raster_grads rgrads();
     void release_all();
@@ -44,6 +45,19 @@ struct GraphSlot {
     }
 };
 
+// Optional HIP-event timing of a step's stages (gs_trainer_set_timing).
+struct StageTimer {
+    unsigned mask = 0;
+    hipEvent_t beg[GS_NUM_STAGES] = {}, end[GS_NUM_STAGES] = {};
+    double total_ms[GS_NUM_STAGES] = {};
+    long long steps = 0;
+    bool on(int s) const { return (mask >> s) & 1u; }
+    hipError_t mark_begin(int s, hipStream_t st) { return on(s) ? hipEventRecord(beg[s], st) : hipSuccess; }
+    hipError_t mark_end(int s, hipStream_t st) { return on(s) ? hipEventRecord(end[s], st) : hipSuccess; }
+    hipError_t collect();  // after the step's stream synchronisation
+    void release();
+};
+
 }  // namespace gs
 
 struct gs_trainer {
@@ -55,10 +69,11 @@ struct gs_trainer {
     gs::DevBuf keys, vals, keys_sorted, vals_sorted;
     int pair_stride = 1;  // uint32 entries per pair in keys_sorted: 1 (indices) or 2 (words)
     int fused_adam = 0;   // Adam applied inside the preprocess backward (gradients not written)
-    gs::DevBuf cam_dev, hp_dev;  // device copies of the camera and Adam hyper-parameters (fused step)
-    unsigned long long* k_pinned = nullptr;  // pair count of a pending binning (pinned host memory)
+    unsigned long long* k_pinned = nullptr;      // pair count of a pending binning (mapped pinned host memory)
+    unsigned long long* k_pinned_dev = nullptr;  // its device address
     hipEvent_t k_event = nullptr;             // recorded after its copy
     bool allow_async = true;
+    gs::StageTimer timer;
     std::vector<gs::GraphSlot> graphs;
     gs_gaussians gaussians() const;
     size_t tensor_elems(int k) const;

@@ -2,8 +2,8 @@
 //
 // Each knob only changes the cost of its own stage, so the tuner runs one
 // stage at a time over that knob's candidates (coordinate descent), timing
-// each candidate with HIP events on the context's stream (median of `reps`
-// runs after one untimed run), and keeps the fastest:
+// each candidate with HIP events on the context's stream (candidates
+// interleaved, median of `reps` runs each), and keeps the fastest:
 //   binning:         tiles per lane {1, 2, 4}, then group size {4096 .. 65536},
 //                    then the depth-sort digit width {4 .. 8};
 //   raster forward:  LDS batch depth {64, 128, 256, 512};
@@ -39,44 +39,48 @@ struct Timer {
     }
 };
 
-// Median device time (ms) of `reps` runs of f() after one untimed run.
+// Device time (ms) of one run of f().
 template <typename F>
-int time_ms(gs_ctx* c, Timer& tm, int reps, F&& f, float* ms) {
+int time_once(gs_ctx* c, Timer& tm, F&& f, float* ms) {
+    if (hipEventRecord(tm.a, c->stream) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventRecord");
     TU_TRY(f());
-    std::vector<float> t;
-    for (int r = 0; r < reps; r++) {
-        if (hipEventRecord(tm.a, c->stream) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventRecord");
-        TU_TRY(f());
-        if (hipEventRecord(tm.b, c->stream) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventRecord");
-        if (hipEventSynchronize(tm.b) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventSynchronize");
-        float x = 0.0f;
-        if (hipEventElapsedTime(&x, tm.a, tm.b) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventElapsedTime");
-        t.push_back(x);
-    }
-    std::sort(t.begin(), t.end());
-    *ms = t[t.size() / 2];
+    if (hipEventRecord(tm.b, c->stream) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventRecord");
+    if (hipEventSynchronize(tm.b) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventSynchronize");
+    if (hipEventElapsedTime(ms, tm.a, tm.b) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipEventElapsedTime");
     return GS_OK;
 }
 
-// Runs `set(candidate)` + timing for each candidate, leaves the best set.
+// Times every candidate `reps` times, round-robin (one untimed run each
+// first), so that slow drifts of clocks or temperature hit all candidates
+// alike; keeps the candidate with the smallest median and leaves it set.
 template <typename Set, typename Run>
 int pick(gs_ctx* c, Timer& tm, int reps, const std::vector<int>& cands, Set&& set, Run&& run, int* best,
          float* best_ms) {
-    float bm = 0.0f;
-    int bv = cands.front();
-    bool have = false;
-    for (int v : cands) {
-        TU_TRY(set(v));
-        float ms = 0.0f;
-        TU_TRY(time_ms(c, tm, reps, run, &ms));
-        if (!have || ms < bm) {
-            bm = ms;
-            bv = v;
-            have = true;
+    std::vector<std::vector<float>> t(cands.size());
+    for (size_t i = 0; i < cands.size(); i++) {
+        TU_TRY(set(cands[i]));
+        TU_TRY(run());
+    }
+    for (int r = 0; r < reps; r++) {
+        for (size_t i = 0; i < cands.size(); i++) {
+            TU_TRY(set(cands[i]));
+            float ms = 0.0f;
+            TU_TRY(time_once(c, tm, run, &ms));
+            t[i].push_back(ms);
         }
     }
-    TU_TRY(set(bv));
-    *best = bv;
+    size_t bi = 0;
+    float bm = 0.0f;
+    for (size_t i = 0; i < cands.size(); i++) {
+        std::sort(t[i].begin(), t[i].end());
+        const float med = t[i][t[i].size() / 2];
+        if (i == 0 || med < bm) {
+            bm = med;
+            bi = i;
+        }
+    }
+    TU_TRY(set(cands[bi]));
+    *best = cands[bi];
     *best_ms = bm;
     return GS_OK;
 }

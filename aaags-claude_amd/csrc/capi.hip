@@ -439,8 +439,9 @@ extern "C" int gs_ctx_apply_tuning(gs_ctx* c, const gs_tuning* t) {
 }
 
 // gs_bin_tiles, and its form for drivers that keep the GPU busy: with
-// total_async (pinned host memory) the pair count is copied there without a
-// host wait (`ev_total` is recorded after the copy), the placement runs with
+// total_async (the device address of mapped, coherent pinned host memory) the
+// scan writes the pair count there without a host wait (`ev_total` is
+// recorded after the scan), the placement runs with
 // the given capacity whatever the count, pairs past it are dropped (overflow
 // flag) and the ranges are clamped to it; the caller compares the count with
 // the capacity before using the results.
@@ -513,12 +514,14 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
     const int blocks = bin_blocks(C, gx, gy, c->bin_tpl);
     GS_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");
-    GS_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr, total64),
+    // asynchronous: the scan writes the pair count straight into the caller's
+    // mapped host memory, and the event marks when it is there
+    GS_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr,
+                                     total_async ? total_async : total64),
            "binning scan");
     a.table = scanned;
     if (total_async) {
         GS_CHECK_ARG(values != nullptr && capacity > 0, "null values buffer");
-        GS_HIP(hipMemcpyAsync(total_async, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
         if (ev_total) GS_HIP(hipEventRecord(ev_total, c->stream), "hipEventRecord");
         GS_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
         GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");

@@ -96,9 +96,10 @@ hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev, int V,
                                        const gs_preprocessed& out, uint64_t* dkeys, uint32_t* didx, int deg,
                                        hipStream_t s);
-hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev, int V, const int* radii,
-                                       const gs_raster_grads& rg, const gs_param_grads& out, float* const adam_p[5],
-                                       float* const adam_m[5], float* const adam_v[5], const gso::AdamDev* hp,
+hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev,
+                                       const gsd::CamArgs* cam_val, int V, const int* radii, const gs_raster_grads& rg,
+                                       const gs_param_grads& out, float* const adam_p[5], float* const adam_m[5],
+                                       float* const adam_v[5], const gso::AdamDev* hp, const gso::AdamDev* hp_val,
                                        int sh_deg, hipStream_t s);
 hipError_t launch_gather_tiles_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
                                      const int* tiles, int* out, hipStream_t s);

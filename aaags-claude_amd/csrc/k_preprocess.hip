@@ -15,19 +15,59 @@
 
 namespace gsd {
 
+// When every SH float is used and rows are whole float4s (SH degree 3 with 16
+// bases, or degree 1 with 4), the workgroup reads its 256 SH rows as one
+// contiguous run: consecutive lanes load consecutive float4s (every load
+// instruction covers whole cache lines), the values go through LDS, and each
+// thread then reads its own row.  Otherwise each thread loads its row itself.
 template <int MAXC, int DEG>
 __global__ __launch_bounds__(256) void preprocess_fwd_kernel(gs_gaussians G, CamArgs c, gs_preprocessed out) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G.count) return;
-    // the SH block (most of the bytes) is requested first, so its loads are in
-    // flight while the geometry is computed; culled Gaussians do not use it
+    constexpr int E = MAXC * 3;
+    constexpr bool STAGE = (E % 4 == 0) && (NB == MAXC);
+    constexpr int Q = E / 4;  // float4s per row
+    __shared__ float s_sh[STAGE ? 256 * (E + 1) : 1];  // rows padded to E + 1 floats
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.x * 256;
+    const int i = b0 + t;
+    const bool valid = i < G.count;
+    float4 st[STAGE ? Q : 1];
+    const int rows = (G.count - b0) < 256 ? (G.count - b0) : 256;
+    if constexpr (STAGE) {
+        // issued first: in flight while the geometry is computed
+        const float4* src = reinterpret_cast<const float4*>(G.sh_coeffs + (size_t)b0 * E);
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const int k = t + 256 * j;
+            st[j] = k < rows * Q ? src[k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
     float sh[NB * 3];
-    load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
+    if constexpr (!STAGE) {
+        if (valid) load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
+    }
+    if (!valid && !STAGE) return;
     Params P;
-    load_params(G, i, G.raw != 0, P);
     Geom g;
-    bool vis = preprocess_geom(P.p, P.s, P.q, c, g);
+    bool vis = false;
+    if (valid) {
+        load_params(G, i, G.raw != 0, P);
+        vis = preprocess_geom(P.p, P.s, P.q, c, g);
+    }
+    if constexpr (STAGE) {
+#pragma unroll
+        for (int j = 0; j < Q; j++) {
+            const int k = t + 256 * j;
+            if (k < rows * Q) {
+                float* d = s_sh + (k / Q) * (E + 1) + (k % Q) * 4;
+                d[0] = st[j].x; d[1] = st[j].y; d[2] = st[j].z; d[3] = st[j].w;
+            }
+        }
+        __syncthreads();
+        if (!valid) return;
+#pragma unroll
+        for (int k = 0; k < E; k++) sh[k] = s_sh[t * (E + 1) + k];
+    }
     float2 m2 = make_float2(0.0f, 0.0f);
     float con[3] = {0.0f, 0.0f, 0.0f}, rgb[3] = {0.0f, 0.0f, 0.0f};
     float depth = 0.0f, op2d = 0.0f;
@@ -188,7 +228,8 @@ struct AdamState {
     float* p[5];
     float* m[5];
     float* v[5];
-    const gso::AdamDev* hp;
+    const gso::AdamDev* hp;  // hyper-parameters in device memory (graph replays), or
+    gso::AdamDev h0;         // by value when hp is null
 };
 
 // One Adam element (AdamOptimizer::adam_update, optimizer.cpp:60-101) on
@@ -236,8 +277,8 @@ __device__ __forceinline__ void adam_run(float* __restrict__ p, float* __restric
 // and store of p, m and v is coalesced.  The few per-Gaussian scalars
 // (position, scale, rotation, opacity) are updated by their own thread.
 template <int MAXC, int DEG, bool FUSED_ADAM>
-__global__ __launch_bounds__(256) void preprocess_bwd_multi_kernel(gs_gaussians G, const CamArgs* __restrict__ cams,
-                                                                   int V, const int* __restrict__ radii,
+__global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussians G, const CamArgs* __restrict__ cams,
+                                                                   CamArgs cam0, int V, const int* __restrict__ radii,
                                                                    gs_raster_grads rg, gs_param_grads out,
                                                                    AdamState ad) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
@@ -257,10 +298,8 @@ __global__ __launch_bounds__(256) void preprocess_bwd_multi_kernel(gs_gaussians 
         load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
         float tpos[3] = {0.0f, 0.0f, 0.0f}, tsc[3] = {0.0f, 0.0f, 0.0f}, trot[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float top = 0.0f;
-        for (int v = 0; v < V; v++) {
-            const size_t o = (size_t)v * N + i;
-            if (radii[o] <= 0) continue;
-            const CamArgs& c = cams[v];
+        // one view's gradients, summed into the totals
+        auto view = [&](const CamArgs& c, size_t o) {
             Geom g;
             preprocess_geom(P.p, P.s, P.q, c, g);
             float gpos[3] = {0.0f, 0.0f, 0.0f};
@@ -278,9 +317,17 @@ __global__ __launch_bounds__(256) void preprocess_bwd_multi_kernel(gs_gaussians 
             top += gop;
 #pragma unroll
             for (int k = 0; k < E; k++) tsh[k] += gsh[k];
+        };
+        if (cams) {
+            for (int v = 0; v < V; v++) {
+                const size_t o = (size_t)v * N + i;
+                if (radii[o] > 0) view(cams[v], o);
+            }
+        } else if (radii[i] > 0) {
+            view(cam0, (size_t)i);  // one view, camera by value
         }
         if constexpr (FUSED_ADAM) {
-            const gso::AdamDev h = *ad.hp;
+            const gso::AdamDev h = ad.hp ? *ad.hp : ad.h0;
             adam_run<3>(ad.p[0], ad.m[0], ad.v[0], (size_t)i * 3, tpos, h.lr[0], h);
             adam_run<3>(ad.p[1], ad.m[1], ad.v[1], (size_t)i * 3, tsc, h.lr[1], h);
             adam_run<4>(ad.p[2], ad.m[2], ad.v[2], (size_t)i * 4, trot, h.lr[2], h);
@@ -303,7 +350,7 @@ __global__ __launch_bounds__(256) void preprocess_bwd_multi_kernel(gs_gaussians 
         const int g0 = blockIdx.x * blockDim.x + w * 64;  // first Gaussian of this wave
         const int cnt = N - g0 < 64 ? N - g0 : 64;
         if (cnt > 0) {
-            const gso::AdamDev h = *ad.hp;
+            const gso::AdamDev h = ad.hp ? *ad.hp : ad.h0;
             float* __restrict__ p = ad.p[3] + (size_t)g0 * E;
             float* __restrict__ m = ad.m[3] + (size_t)g0 * E;
             float* __restrict__ v = ad.v[3] + (size_t)g0 * E;
@@ -396,28 +443,35 @@ hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const CamArgs* cam
     return hipGetLastError();
 }
 
-// adam_p/m/v/hp: when non-null, the summed gradients update the parameters in
-// place (G's arrays must be the same as adam_p) and `out` is unused.
-hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const CamArgs* cams_dev, int V, const int* radii,
-                                       const gs_raster_grads& rg, const gs_param_grads& out, float* const adam_p[5],
-                                       float* const adam_m[5], float* const adam_v[5], const gso::AdamDev* hp, int sh_deg,
-                                       hipStream_t s) {
+// adam_p/m/v with hp (device) or hp_val (host, passed by value): the summed
+// gradients update the parameters in place (G's arrays must be the same as
+// adam_p) and `out` is unused.  cams_dev null: one view, cam_val by value.
+hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const CamArgs* cams_dev, const CamArgs* cam_val, int V,
+                                       const int* radii, const gs_raster_grads& rg, const gs_param_grads& out,
+                                       float* const adam_p[5], float* const adam_m[5], float* const adam_v[5],
+                                       const gso::AdamDev* hp, const gso::AdamDev* hp_val, int sh_deg, hipStream_t s) {
     dim3 grid((G.count + 255) / 256), block(256);
+    CamArgs cam0{};
+    if (!cams_dev) {
+        if (!cam_val || V != 1) return hipErrorInvalidValue;
+        cam0 = *cam_val;
+    }
     AdamState ad{};
-    if (hp) {
+    if (hp || hp_val) {
         for (int k = 0; k < 5; k++) {
             ad.p[k] = adam_p[k];
             ad.m[k] = adam_m[k];
             ad.v[k] = adam_v[k];
         }
         ad.hp = hp;
+        if (!hp) ad.h0 = *hp_val;
         GS_DISPATCH_SH(G.max_coeffs, sh_deg,
-                       (preprocess_bwd_multi_kernel<MC, DG, true><<<grid, block, 0, s>>>(G, cams_dev, V, radii, rg, out,
-                                                                                        ad)));
+                       (preprocess_bwd_multi_kernel<MC, DG, true><<<grid, block, 0, s>>>(G, cams_dev, cam0, V, radii, rg,
+                                                                                        out, ad)));
     } else {
         GS_DISPATCH_SH(G.max_coeffs, sh_deg,
-                       (preprocess_bwd_multi_kernel<MC, DG, false><<<grid, block, 0, s>>>(G, cams_dev, V, radii, rg,
-                                                                                         out, ad)));
+                       (preprocess_bwd_multi_kernel<MC, DG, false><<<grid, block, 0, s>>>(G, cams_dev, cam0, V, radii,
+                                                                                         rg, out, ad)));
     }
     return hipGetLastError();
 }

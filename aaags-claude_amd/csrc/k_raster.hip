@@ -25,12 +25,73 @@ namespace gsr {
 
 using gsd::smin;
 
+// Wave w of a tile (threads 64w .. 64w+63) covers pixel rows 4w .. 4w+3, a
+// 16 x 4 strip.  Per LDS batch, mask[w][c] bit b is set when entry 64c + b can
+// pass the alpha cut somewhere in strip w (see alpha_box); each wave visits
+// only its set bits.
 template <int BATCH>
 struct FwdSmem {
     float2 xy[BATCH];
     float4 co[BATCH];
     float4 rgb[BATCH];
+    uint64_t mask[4][BATCH / 64];
 };
+
+// Pixel bounding box (x0, x1, y0, y1) of the region where a Gaussian can pass
+// the 1/255 alpha cut.  alpha = o * exp(-Q/2) with Q = d^T C d (C the conic),
+// so alpha >= 1/255 needs Q <= r2 = 2 ln(255 o): an ellipse with half-extents
+// sqrt(r2 * c / det) and sqrt(r2 * a / det).  Margins:
+//  - the rasterizer's float Q has a relative rounding error below about
+//    8 eps tr^2/det (its terms reach tr^2/det times Q); the box is bounded
+//    only where tr^2/det <= 1000 (error <= 5e-4) and r2 is widened by 0.1%,
+//    plus 1e-3 for the approximate log / rcp / sqrt used here;
+//  - the half-extents get one more pixel.
+// So a pixel centre outside the box has alpha below the cut, and skipping the
+// Gaussian there changes nothing.  o < 1/255 never passes (empty box); NaN
+// or unbounded cases give an infinite box.
+__device__ __forceinline__ float4 alpha_box(float2 m, float4 co) {
+    const float o = co.w;
+    if (!(o >= 1.0f / 255.0f)) return make_float4(1.0f, -1.0f, 1.0f, -1.0f);  // empty
+    const float det = co.x * co.z - co.y * co.y;
+    const float tr = co.x + co.z;
+    if (!(det > 0.0f) || !(tr * tr <= 1000.0f * det)) return make_float4(-3.0e38f, 3.0e38f, -3.0e38f, 3.0e38f);
+    // 2 ln(x) = 2 ln(2) log2(x)
+    const float r2 = 1.38629436f * __builtin_amdgcn_logf(255.0f * o) * 1.001f + 1e-3f;
+    const float inv = __builtin_amdgcn_rcpf(det);
+    const float ex = __builtin_amdgcn_sqrtf(r2 * co.z * inv) + 1.0f;
+    const float ey = __builtin_amdgcn_sqrtf(r2 * co.x * inv) + 1.0f;
+    return make_float4(m.x - ex, m.x + ex, m.y - ey, m.y + ey);
+}
+
+// Strip-hit bits of one staged entry: bit w set when the box meets strip w of
+// the tile whose top-left pixel is (x0, y0).  NaN box edges count as a hit.
+__device__ __forceinline__ uint32_t strip_hits(float4 b, float x0, float y0) {
+    if (b.y < x0 || b.x > x0 + 15.0f) return 0u;
+    uint32_t h = 0u;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const float s0 = y0 + (float)(4 * w);
+        if (!(b.w < s0 || b.z > s0 + 3.0f)) h |= 1u << w;
+    }
+    return h;
+}
+
+// Ballots the hit bits of 64 staged entries (one per lane of the calling wave,
+// entries 64c .. 64c+63) into the four strips' masks.
+template <int BATCH>
+__device__ __forceinline__ void store_masks(uint64_t (&mask)[4][BATCH / 64], int c, uint32_t h, int lane) {
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint64_t b = __ballot((h >> w) & 1u);
+        if (lane == 0) mask[w][c] = b;
+    }
+}
+
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 
 // gradient sums per Gaussian: dpow * (dx, dy, dx^2, dx dy, dy^2), rgb 3, opacity 1
 constexpr int NG = 9;
@@ -40,6 +101,7 @@ struct BwdSmem {
     float2 xy[BATCH];
     float4 co[BATCH];
     float4 rgb[BATCH];
+    uint64_t mask[4][BATCH / 64];
     uint32_t id[BATCH];
     float g[BATCH][NG];
     uint32_t maxlast;
@@ -60,6 +122,8 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
     const bool inside = px < a.W && py < a.H;
     const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
     const float fpx = (float)px, fpy = (float)py;
+    const float tx0 = (float)(tx * gsd::TILE), ty0 = (float)(ty * gsd::TILE);
+    const int lane = t & 63, w = t >> 6;
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
     int cnt = 0;
     uint32_t last = 0;
@@ -68,20 +132,28 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
         if (__syncthreads_count(!done) == 0) break;
         for (int k = t; k < BATCH; k += 256) {
             uint32_t j = base + k;
+            uint32_t h = 0u;
             if (j < end) {
                 uint32_t id = a.values[(size_t)j * a.vstride];
-                sm.xy[k] = a.means2D[id];
-                sm.co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
+                const float2 mm = a.means2D[id];
+                const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
+                sm.xy[k] = mm;
+                sm.co[k] = cc;
                 sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
+                h = strip_hits(alpha_box(mm, cc), tx0, ty0);
             }
+            store_masks<BATCH>(sm.mask, k >> 6, h, lane);
         }
         __syncthreads();
-        const int n = (int)((end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH);
-        if (!done) {
-            for (int k = 0; k < n; k++) {
-                const float2 m = sm.xy[k];
+        const int nc = (int)(((end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH) + 63u) >> 6;
+        for (int c = 0; c < nc && !done; c++) {
+            uint64_t m = wave_uniform(sm.mask[w][c]);
+            while (m != 0ull) {
+                const int k = (c << 6) + __builtin_ctzll(m);
+                m &= m - 1ull;
+                const float2 mm = sm.xy[k];
                 const float4 co = sm.co[k];
-                const float dx = m.x - fpx, dy = m.y - fpy;
+                const float dx = mm.x - fpx, dy = mm.y - fpy;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
                 if (power > 0.0f) continue;
                 float ep;
@@ -92,10 +164,10 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                     done = true;
                     break;
                 }
-                const float4 c = sm.rgb[k];
-                C0 += c.x * alpha * T;
-                C1 += c.y * alpha * T;
-                C2 += c.z * alpha * T;
+                const float4 cl = sm.rgb[k];
+                C0 += cl.x * alpha * T;
+                C1 += cl.y * alpha * T;
+                C2 += cl.z * alpha * T;
                 T = test_T;
                 cnt++;
                 last = base + (uint32_t)k + 1u;
@@ -206,6 +278,8 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
     }
     float acc0 = a.bg0, acc1 = a.bg1, acc2 = a.bg2;
     const float fpx = (float)px, fpy = (float)py;
+    const float tx0 = (float)(tx * gsd::TILE), ty0 = (float)(ty * gsd::TILE);
+    const int w = t >> 6;
     if (t == 0) sm.maxlast = 0;
     __syncthreads();
     if (last > 0) atomicMax(&sm.maxlast, last);
@@ -225,73 +299,85 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
         const int n = (int)(hi - lo);
         __syncthreads();
         for (int k = t; k < BATCH; k += 256) {
+            uint32_t h = 0u;
             if (k < n) {
                 uint32_t id = a.values[(size_t)(lo + k) * a.vstride];
                 sm.id[k] = id;
-                sm.xy[k] = a.means2D[id];
-                sm.co[k] = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
+                const float2 mm = a.means2D[id];
+                const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
+                sm.xy[k] = mm;
+                sm.co[k] = cc;
                 sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
+                h = strip_hits(alpha_box(mm, cc), tx0, ty0);
             }
+            store_masks<BATCH>(sm.mask, k >> 6, h, lane);
 #pragma unroll
             for (int c = 0; c < NG; c++) sm.g[k][c] = 0.0f;
         }
         __syncthreads();
+        // this wave's entries, back to front from its last contributor
         const int kmax = wlast <= lo ? -1 : ((int)(wlast - lo) < n ? (int)(wlast - lo) : n) - 1;
-        for (int k = kmax; k >= 0; k--) {
-            const uint32_t j = lo + (uint32_t)k;
-            float gv[NG];
+        for (int ch = kmax >> 6; ch >= 0; ch--) {
+            uint64_t bits = wave_uniform(sm.mask[w][ch]);
+            if (ch == (kmax >> 6)) bits &= ~0ull >> (63 - (kmax & 63));
+            while (bits != 0ull) {
+                const int k = (ch << 6) + (63 - __builtin_clzll(bits));
+                bits &= ~(1ull << (k & 63));
+                const uint32_t j = lo + (uint32_t)k;
+                float gv[NG];
 #pragma unroll
-            for (int c = 0; c < NG; c++) gv[c] = 0.0f;
-            bool contrib = false;
-            if (j < last) {
-                const float2 m = sm.xy[k];
-                const float4 co = sm.co[k];
-                const float dx = m.x - fpx, dy = m.y - fpy;
-                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-                if (power <= 0.0f) {
-                    float ep;
-                    const float alpha = gsd::raster_alpha(co.w, power, &ep);
-                    if (alpha >= 1.0f / 255.0f) {
-                        contrib = true;
-                        const float4 cl = sm.rgb[k];
-                        const float oma = 1.0f - alpha;
-                        // T_i = T / (1 - alpha) as the reference recovers it
-                        // (rasterizer_backward_cpu.cpp:85), with the hardware
-                        // reciprocal (1 ulp): gradients need 1e-3, not bits
-                        const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
-                        gv[5] = alpha * Ti * dC0;
-                        gv[6] = alpha * Ti * dC1;
-                        gv[7] = alpha * Ti * dC2;
-                        float dal = 0.0f;
-                        dal += Ti * (cl.x - acc0) * dC0;
-                        dal += Ti * (cl.y - acc1) * dC1;
-                        dal += Ti * (cl.z - acc2) * dC2;
-                        acc0 = alpha * cl.x + oma * acc0;
-                        acc1 = alpha * cl.y + oma * acc1;
-                        acc2 = alpha * cl.z + oma * acc2;
-                        T = Ti;
-                        float dpow = 0.0f, dop = 0.0f;
-                        if (alpha < 0.99f) {
-                            dpow = dal * co.w * ep;
-                            dop = dal * ep;
+                for (int c = 0; c < NG; c++) gv[c] = 0.0f;
+                bool contrib = false;
+                if (j < last) {
+                    const float2 m = sm.xy[k];
+                    const float4 co = sm.co[k];
+                    const float dx = m.x - fpx, dy = m.y - fpy;
+                    const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                    if (power <= 0.0f) {
+                        float ep;
+                        const float alpha = gsd::raster_alpha(co.w, power, &ep);
+                        if (alpha >= 1.0f / 255.0f) {
+                            contrib = true;
+                            const float4 cl = sm.rgb[k];
+                            const float oma = 1.0f - alpha;
+                            // T_i = T / (1 - alpha) as the reference recovers it
+                            // (rasterizer_backward_cpu.cpp:85), with the hardware
+                            // reciprocal (1 ulp): gradients need 1e-3, not bits
+                            const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
+                            gv[5] = alpha * Ti * dC0;
+                            gv[6] = alpha * Ti * dC1;
+                            gv[7] = alpha * Ti * dC2;
+                            float dal = 0.0f;
+                            dal += Ti * (cl.x - acc0) * dC0;
+                            dal += Ti * (cl.y - acc1) * dC1;
+                            dal += Ti * (cl.z - acc2) * dC2;
+                            acc0 = alpha * cl.x + oma * acc0;
+                            acc1 = alpha * cl.y + oma * acc1;
+                            acc2 = alpha * cl.z + oma * acc2;
+                            T = Ti;
+                            float dpow = 0.0f, dop = 0.0f;
+                            if (alpha < 0.99f) {
+                                dpow = dal * co.w * ep;
+                                dop = dal * ep;
+                            }
+                            // raw moments of dpow; the conic transform of the mean
+                            // gradient and the conic-gradient constants are applied
+                            // once per Gaussian when the sums leave LDS
+                            gv[8] = dop;
+                            const float pdx = dpow * dx, pdy = dpow * dy;
+                            gv[0] = pdx;
+                            gv[1] = pdy;
+                            gv[2] = pdx * dx;
+                            gv[3] = pdx * dy;
+                            gv[4] = pdy * dy;
                         }
-                        // raw moments of dpow; the conic transform of the mean
-                        // gradient and the conic-gradient constants are applied
-                        // once per Gaussian when the sums leave LDS
-                        gv[8] = dop;
-                        const float pdx = dpow * dx, pdy = dpow * dy;
-                        gv[0] = pdx;
-                        gv[1] = pdy;
-                        gv[2] = pdx * dx;
-                        gv[3] = pdx * dy;
-                        gv[4] = pdy * dy;
                     }
                 }
-            }
-            if (__ballot(contrib) != 0ull) {
-                int term;
-                float v;
-                if (wave_reduce9(gv, lane, &term, &v) && v != 0.0f) atomicAdd(&sm.g[k][term], v);
+                if (__ballot(contrib) != 0ull) {
+                    int term;
+                    float v;
+                    if (wave_reduce9(gv, lane, &term, &v) && v != 0.0f) atomicAdd(&sm.g[k][term], v);
+                }
             }
         }
         __syncthreads();

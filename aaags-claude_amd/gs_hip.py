@@ -476,6 +476,25 @@ class GsTrainer:
         """fused_adam: Adam inside the preprocess backward (gradient buffers not written)."""
         self.hip._check(self.lib.gs_trainer_set_options(self.tr, _i32(int(fused_adam))), "gs_trainer_set_options")
 
+    STAGES = ["preprocess", "binning", "raster_fwd", "l1", "zero_grads", "raster_bwd", "backward"]
+
+    def set_timing(self, stages=None):
+        """HIP-event timing of the named stages of each step (None: all; []: off);
+        resets the totals."""
+        names = self.STAGES if stages is None else stages
+        mask = 0
+        for nm in names:
+            mask |= 1 << self.STAGES.index(nm)
+        self.hip._check(self.lib.gs_trainer_set_timing(self.tr, ctypes.c_uint(mask)), "gs_trainer_set_timing")
+
+    def stage_times(self):
+        """Per-step mean device time (ms) of each timed stage since set_timing, and the step count."""
+        ms = (ctypes.c_double * len(self.STAGES))()
+        steps = ctypes.c_longlong()
+        self.hip._check(self.lib.gs_trainer_stage_times(self.tr, ms, _i32(len(self.STAGES)), ctypes.byref(steps)),
+                        "gs_trainer_stage_times")
+        return {nm: float(ms[i]) for i, nm in enumerate(self.STAGES)}, int(steps.value)
+
     def autotune(self, cam, cfg, gt, reps=3) -> dict:
         """gs_autotune on this trainer's parameters and one view (no parameter update)."""
         c, f = cam.to_c(), cfg.to_c()

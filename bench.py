@@ -8,7 +8,10 @@ backward -> Adam, run by the native driver (gs_trainer_step) with inputs
 resident in HBM.
 
 Prints one JSON line: train steps/s (value), forward Mpix/s, a per-stage
-breakdown, the roofline of the dominant kernel, and the reference CPU baseline.
+breakdown (HIP events around each stage of the native step), the roofline of
+the dominant kernel (its duration measured with HIP events inside the timed
+steps), the HBM throughput of preprocess and of the two sorts, the batched and
+large batched steps, and the reference CPU baseline.
 """
 import argparse
 import json
@@ -27,47 +30,6 @@ FP32_PEAK_TFLOPS = 157.3
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def stage_breakdown(hip, p, cam, cfg, gt, reps):
-    """Per-stage device time (ms) of the single-view step on the driver's path
-    (direct-placement binning), from HIP events recorded on the library's
-    stream (torch's current stream)."""
-    import torch
-
-    names = ["preprocess", "binning", "raster_fwd", "l1", "zero_grads", "raster_bwd", "preprocess_bwd", "adam"]
-    acc = {k: 0.0 for k in names}
-    n, kc = p["n"], p["max_coeffs"]
-    m = hip.alloc_pgrads(n, kc)
-    v = hip.alloc_pgrads(n, kc)
-    K = 0
-    cap = None
-    for r in range(reps + 1):  # the first repetition sizes the pair buffer and is not counted
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
-        ev[0].record()
-        pre = hip.preprocess(p, cam, cfg)
-        ev[1].record()
-        vals, ranges, K = hip.bin_tiles(pre, cam.width, cam.height, capacity=cap)
-        cap = max(cap or 0, int(K * 1.1) + 1024)
-        ev[2].record()
-        img, px = hip.raster_fwd(pre, vals, ranges, cam.width, cam.height, cfg.bg_color)
-        ev[3].record()
-        loss, d_img = hip.l1_loss(img, gt)
-        ev[4].record()
-        rg = hip.alloc_rgrads(n)
-        ev[5].record()
-        rg = hip.raster_bwd(pre, vals, ranges, cam.width, cam.height, px, d_img, cfg.bg_color, rg=rg)
-        ev[6].record()
-        grads = hip.preprocess_bwd(p, cam, cfg, pre["radii"], rg)
-        ev[7].record()
-        hip.adam(p, grads, m, v, [1e-3] * 5, 0.9, 0.999, 1e-8, r + 1)
-        ev[8].record()
-        torch.cuda.synchronize()
-        if r > 0:
-            for i, k in enumerate(names):
-                acc[k] += ev[i].elapsed_time(ev[i + 1])
-        del vals, ranges, img, px, d_img, rg, grads, pre
-    return {k: acc[k] / reps for k in names}, K
 
 
 VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 2
@@ -107,7 +69,69 @@ def algorithmic_bytes(stage, N, K, npix, max_coeffs, num_tiles, table_entries=0)
         "raster_bwd": K * 40 + npix * 20 + N * 72,
         "preprocess_bwd": N * 580,
         "adam": N * params_per_g * 28,
+        # fused preprocess backward + Adam: parameters, m and v read and written (6 x 4 B per
+        # parameter element), screen-space gradients (36 B/G) and radii (4 B/G) read
+        "backward": N * (params_per_g * 24 + 40),
     }.get(stage)
+
+
+def median_ms(f, reps=5, groups=3):
+    """Device time (ms) of one f(): reps back-to-back runs between one pair of
+    HIP events on torch's current stream (the library's stream), so launch
+    ramps overlap as in a pipeline; the median over `groups` such groups,
+    after one untimed run."""
+    import torch
+    f()
+    ts = []
+    for _ in range(groups):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            f()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b) / reps)
+    return sorted(ts)[len(ts) // 2]
+
+
+def sort_bytes(count, key_bits, radix_bits):
+    """An LSD radix sort of (key, 4-byte value) pairs moves at least one key read
+    for the digit histograms and, per pass, each 12-byte pair read and written."""
+    import math
+    return count * (24 * math.ceil(key_bits / radix_bits) + 8)
+
+
+def bench_sorts(hip, p, cam, cfg, radix_bits, reps=5):
+    """HBM throughput of the two sorts at the headline workload, timed with HIP
+    events on the library's stream (median of reps after one untimed run):
+    - the reference-compatible pair sort (gs_sort_pairs): the K (tile | depth)
+      64-bit keys and pair indices that the reference's duplicate step makes;
+    - the depth sort of the direct-placement binning (gs_radix_sort_kv on the
+      N visible Gaussians' 32-bit depth keys)."""
+    import math
+    import torch
+
+    timed = lambda f: median_ms(f, reps)
+    pre = hip.preprocess(p, cam, cfg)
+    offsets, K = hip.scan_tiles(pre["tiles"])
+    keys, vals = hip.duplicate(pre, offsets, K, cam.width, cam.height)
+    tile_bits = max(1, math.ceil(math.log2(max(cam.num_tiles, 2))))
+    ms_pairs = timed(lambda: hip.sort_pairs(keys, vals, cam.num_tiles))
+    b_pairs = sort_bytes(K, tile_bits + 32, radix_bits)
+    del keys, vals, offsets
+    vis = pre["radii"] > 0
+    dkeys = pre["depths"][vis].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    didx = torch.nonzero(vis).flatten().to(torch.int32)
+    nv = int(dkeys.numel())
+    ms_depth = timed(lambda: hip.radix_sort(dkeys, didx, key_bits=32, radix_bits=radix_bits))
+    b_depth = sort_bytes(nv, 32, radix_bits)
+    out = {}
+    for name, ms, byts, n, kb in (("pair_sort", ms_pairs, b_pairs, K, tile_bits + 32),
+                                  ("depth_sort", ms_depth, b_depth, nv, 32)):
+        gbs = byts / (ms * 1e-3) / 1e9
+        out[name] = {"count": int(n), "key_bits": kb, "radix_bits": radix_bits, "ms": round(ms, 4),
+                     "algorithmic_bytes": int(byts), "gb_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def bench_batched(hip, cfg, args):
@@ -297,49 +321,92 @@ def main():
     tuning = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        tuning = tr.autotune(cam, cfg, gt, reps=3)
+        tuning = tr.autotune(cam, cfg, gt, reps=5)
         log(f"autotune ({time.perf_counter() - t0:.1f}s): {tuning}")
     for i in range(args.warmup):
         r = tr.step(cam, cfg, gt, tc, i)
     torch.cuda.synchronize()
     log(f"warmup done: loss {r.loss:.5f} pairs {r.pairs}")
+    it = args.warmup
+    # per-stage device time of the native step: a HIP event pair around every
+    # stage on the library's stream, over a few untimed steps
+    tr.set_timing(None)
+    for i in range(args.breakdown_reps):
+        tr.step(cam, cfg, gt, tc, it)
+        it += 1
+    stages, _ = tr.stage_times()
+    log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
+    # the dominant kernel: the longest stage that is a single kernel launch
+    single = ["preprocess", "raster_fwd", "raster_bwd"] + ([] if args.no_fused_adam else ["backward"])
+    dom = max(single, key=lambda k: stages[k])
+    # timed region; the dominant kernel alone is bracketed by events in it
+    tr.set_timing([dom])
     t0 = time.perf_counter()
     for i in range(args.steps):
-        r = tr.step(cam, cfg, gt, tc, args.warmup + i)
+        r = tr.step(cam, cfg, gt, tc, it)
+        it += 1
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    dom_times, dom_steps = tr.stage_times()
+    tr.set_timing([])
     steps_s = args.steps / dt
-    log(f"train: {steps_s:.2f} steps/s ({1e3 * dt / args.steps:.3f} ms/step), last loss {r.loss:.5f}")
+    K = r.pairs
+    log(f"train: {steps_s:.2f} steps/s ({1e3 * dt / args.steps:.3f} ms/step), last loss {r.loss:.5f}; "
+        f"{dom} {dom_times[dom]:.4f} ms over {dom_steps} timed steps")
 
     for i in range(2):
         tr.render(cam, cfg)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        K = tr.render(cam, cfg)
+        tr.render(cam, cfg)
     torch.cuda.synchronize()
     dtf = time.perf_counter() - t0
     mpix_s = cam.width * cam.height * args.steps / dtf / 1e6
-    log(f"forward: {mpix_s:.1f} Mpix/s ({1e3 * dtf / args.steps:.3f} ms/frame), pairs {K}")
+    log(f"forward: {mpix_s:.1f} Mpix/s ({1e3 * dtf / args.steps:.3f} ms/frame)")
 
-    p = hip.upload_scene(scene)
-    stages, K = stage_breakdown(hip, p, cam, cfg, gt, args.breakdown_reps)
-    log("stages (ms): " + ", ".join(f"{k} {v:.3f}" for k, v in stages.items()))
     npix = cam.width * cam.height
-    dom = max(stages, key=lambda k: stages[k])
-    table = ((scene.n + 1023) // 1024) * cam.num_tiles + 1  # (tile, chunk) entries at 1024 Gaussians per chunk
-    byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles, table)
-    achieved = byts / (stages[dom] * 1e-3) / 1e9
-    measured = load_traffic(args.workload)
-    st_meas = measured.get(dom, {})
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": st_meas.get("traffic_bytes"),
-                "algorithmic_bytes": int(byts), "ms": round(stages[dom], 4)}
-    if "valu_wave_insts" in st_meas:
-        # wave64 FP32 VALU issue: 2 cycles per instruction per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz
-        peak_insts = VALU_WAVE_INSTS_PER_S * stages[dom] * 1e-3
-        roofline["valu_issue_frac"] = round(st_meas["valu_wave_insts"] / peak_insts, 4)
+    kernel_names = {"preprocess": "preprocess_fwd_kernel", "raster_fwd": "raster_fwd_kernel",
+                    "raster_bwd": "raster_bwd_kernel", "backward": "preprocess_bwd_multi_kernel (Adam fused)"}
+    dom_ms = dom_times[dom]
+    byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
+    hbm_gbs = byts / (dom_ms * 1e-3) / 1e9
+    st_meas = load_traffic(args.workload).get(dom, {})
+    if dom in ("raster_fwd", "raster_bwd") and "valu_wave_insts" in st_meas:
+        # the rasterizers are VALU-bound: wave64 FP32 VALU issue, 2 cycles per
+        # instruction per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz
+        achieved = st_meas["valu_wave_insts"] / (dom_ms * 1e-3) / 1e9
+        peak = VALU_WAVE_INSTS_PER_S / 1e9
+        roofline = {"kernel": kernel_names[dom], "bound": "valu", "achieved": round(achieved, 1),
+                    "peak": round(peak, 1), "unit": "G VALU wave-instructions/s", "frac": round(achieved / peak, 4),
+                    "traffic": st_meas.get("traffic_bytes"), "valu_wave_insts": st_meas["valu_wave_insts"],
+                    "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
+                    "algorithmic_bytes": int(byts), "ms": round(dom_ms, 4)}
+    else:
+        roofline = {"kernel": kernel_names[dom], "bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(hbm_gbs / HBM_PEAK_GBS, 4), "traffic": st_meas.get("traffic_bytes"),
+                    "algorithmic_bytes": int(byts), "ms": round(dom_ms, 4)}
+
+    # the bandwidth targets: preprocess (from the stage timing) and the two sorts
+    p = hip.upload_scene(scene)
+    rb = (tuning or {}).get("radix_bits") or 8
+    hbm_targets = bench_sorts(hip, p, cam, cfg, rb)
+    # preprocess on its own (inside the step it follows Adam, whose writes are
+    # still leaving the caches: that cost is the optimizer's, the stage timing
+    # shows it)
+    pre = hip.alloc_pre(scene.n)
+    import ctypes
+    g_c, c_c, f_c, p_c = hip._gauss(p, True), cam.to_c(), cfg.to_c(), hip._pre_struct(pre)
+    args = (hip.ctx, ctypes.byref(g_c), ctypes.byref(c_c), ctypes.byref(f_c), ctypes.byref(p_c))
+    hip._check(hip.lib.gs_preprocess_fwd(*args), "gs_preprocess_fwd")
+    ms_pre = median_ms(lambda: hip.lib.gs_preprocess_fwd(*args), reps=10)
+    pb = algorithmic_bytes("preprocess", scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
+    pg = pb / (ms_pre * 1e-3) / 1e9
+    hbm_targets["preprocess"] = {"ms": round(ms_pre, 4), "algorithmic_bytes": int(pb),
+                                 "gb_s": round(pg, 1), "frac": round(pg / HBM_PEAK_GBS, 4)}
+    del pre
+    del p
+    log(f"hbm targets: {hbm_targets}")
 
     batched = None
     if not args.no_batched:
@@ -378,6 +445,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "tuning": tuning,
         "roofline": roofline,
+        "hbm_targets": hbm_targets,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -316,6 +316,25 @@ int gs_trainer_step(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* c
  * (trainer.cpp:44-62). */
 int gs_trainer_set_options(gs_trainer* tr, int fused_adam);
 
+/* Stage timing of gs_trainer_step (no counterpart in the reference; a
+ * profiling hook).  For each stage whose bit is set in stage_mask, the step
+ * records a HIP event pair around it on the context's stream and adds the
+ * elapsed time to a running total; 0 turns timing off.  Each call resets the
+ * totals.  gs_trainer_stage_times returns the per-step mean (ms) of the first
+ * n stages and the number of steps timed. */
+enum {
+    GS_STAGE_PREPROCESS = 0, /* preprocess forward */
+    GS_STAGE_BINNING = 1,    /* depth sort + direct placement (or the word sort) + ranges */
+    GS_STAGE_RASTER_FWD = 2,
+    GS_STAGE_LOSS = 3,       /* L1 and its image gradient */
+    GS_STAGE_CLEAR = 4,      /* zeroing of the screen-space gradients */
+    GS_STAGE_RASTER_BWD = 5,
+    GS_STAGE_BACKWARD = 6,   /* preprocess backward + optimizer (fused or not) */
+    GS_NUM_STAGES = 7
+};
+int gs_trainer_set_timing(gs_trainer* tr, unsigned stage_mask);
+int gs_trainer_stage_times(gs_trainer* tr, double* ms, int n, long long* steps);
+
 /* Renderer::render (renderer.h:17-21) on the trainer's parameters: forward
  * only, image into the trainer's "image" buffer. */
 int gs_trainer_render(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, long long* pairs);

@@ -123,3 +123,33 @@ def test_trainer_step_redoes_when_pairs_outgrow_buffer(hip, oracle):
     shapes = dict(pos=(n, 3), scale=(n, 3), rot=(n, 4), sh=(n, scene.max_coeffs, 3), op=(n,))
     for k in NAMES:
         assert_grad_close(tr.buffer("grad_" + k, shapes[k]).cpu().numpy(), ref["grads"][k], "grad_" + k)
+
+
+def test_trainer_stage_timing(hip):
+    """gs_trainer_set_timing / gs_trainer_stage_times: every timed stage gets a
+    positive mean, untimed ones read 0, and the stages fit inside the step."""
+    import time
+    from gs_hip import GsTrainer, train_cfg
+    print("[case] trainer stage timing", flush=True)
+    scene, cam, gt = random_small_scene(3000, 3, 35, width=256, height=192)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    tr = GsTrainer(hip, scene)
+    tr.set_options(fused_adam=True)
+    gt_t = torch.tensor(gt, device=hip.dev)
+    tc = train_cfg()
+    tr.step(cam, cfg, gt_t, tc, 0)
+    tr.set_timing(None)
+    t0 = time.perf_counter()
+    for i in range(4):
+        tr.step(cam, cfg, gt_t, tc, 1 + i)
+    wall_ms = (time.perf_counter() - t0) * 1e3 / 4
+    ms, steps = tr.stage_times()
+    assert steps == 4
+    assert all(v > 0 for v in ms.values()), ms
+    assert sum(ms.values()) < wall_ms, (ms, wall_ms)
+    tr.set_timing(["raster_bwd"])
+    tr.step(cam, cfg, gt_t, tc, 5)
+    ms, steps = tr.stage_times()
+    assert steps == 1 and ms["raster_bwd"] > 0
+    assert all(v == 0 for k, v in ms.items() if k != "raster_bwd"), ms
+    tr.set_timing([])
