@@ -89,40 +89,57 @@ __device__ __forceinline__ BinBlock<TPL> bin_block(const BinArgs& a) {
 
 // Walks the block's group in depth order and, for every Gaussian that meets the
 // bin, calls f(in, gid) on all lanes with in[k] set where the lane's k-th tile
-// is covered by the Gaussian's rectangle.  Rectangles are read
-// 64 at a time, one per lane (the next 64 prefetched); the ones that meet the
-// bin are visited lowest lane first, i.e. in depth order.
+// is covered by the Gaussian's rectangle.  Rectangles are read WALK_DEPTH x 64
+// at a time, one per lane per 64, and the next round is prefetched, so that
+// several loads are in flight per wave; the ones that meet the bin are visited
+// lowest lane first, round by round, i.e. in depth order.
+constexpr int WALK_DEPTH = 4;
+
 template <int TPL, typename F>
 __device__ __forceinline__ void walk_bin(const BinArgs& a, const BinBlock<TPL>& b, int tx, int ty, int lane, F&& f) {
     const uint2* rects = a.rects + (size_t)b.v * a.N;
     const uint32_t r0 = (uint32_t)b.c * (uint32_t)a.G;
     const uint32_t r1 = (r0 + (uint32_t)a.G < (uint32_t)a.N) ? r0 + (uint32_t)a.G : (uint32_t)a.N;
     const int bx1 = b.bx0 + 8, by1 = b.by0 + 8 * TPL;
-    uint2 nxt = make_uint2(0u, 0u);
-    if (r0 + (uint32_t)lane < r1) nxt = rects[r0 + lane];
-    for (uint32_t base = r0; base < r1; base += 64) {
-        const uint2 rec = nxt;
-        const uint32_t rn = base + 64u + (uint32_t)lane;
-        if (rn < r1) nxt = rects[rn];
-        const int x0 = (int)(rec.x & 0xFFu), x1 = (int)((rec.x >> 8) & 0xFFu);
-        const int y0 = (int)((rec.x >> 16) & 0xFFu), y1 = (int)(rec.x >> 24);
-        const bool live = base + (uint32_t)lane < r1;
-        uint64_t todo = __ballot(live && x0 < x1 && y0 < y1 && x0 < bx1 && x1 > b.bx0 && y0 < by1 && y1 > b.by0);
-        while (todo) {
-            const int j = __builtin_ctzll(todo);
-            todo &= todo - 1ull;
-            const uint32_t R = (uint32_t)__builtin_amdgcn_readlane((int)rec.x, j);
-            const uint32_t gid = (uint32_t)__builtin_amdgcn_readlane((int)rec.y, j);
-            const int jx0 = (int)(R & 0xFFu), jx1 = (int)((R >> 8) & 0xFFu);
-            const int jy0 = (int)((R >> 16) & 0xFFu), jy1 = (int)(R >> 24);
-            const bool inx = tx >= jx0 && tx < jx1;
-            bool in[TPL];
+    constexpr uint32_t STEP = 64u * WALK_DEPTH;
+    uint2 nxt[WALK_DEPTH];
 #pragma unroll
-            for (int k = 0; k < TPL; k++) {
-                const int y = ty + 8 * k;
-                in[k] = inx && y >= jy0 && y < jy1;
+    for (int q = 0; q < WALK_DEPTH; q++) {
+        const uint32_t r = r0 + 64u * q + (uint32_t)lane;
+        nxt[q] = r < r1 ? rects[r] : make_uint2(0u, 0u);
+    }
+    for (uint32_t base = r0; base < r1; base += STEP) {
+        uint2 cur[WALK_DEPTH];
+#pragma unroll
+        for (int q = 0; q < WALK_DEPTH; q++) {
+            cur[q] = nxt[q];
+            const uint32_t r = base + STEP + 64u * q + (uint32_t)lane;
+            if (r < r1) nxt[q] = rects[r];
+        }
+#pragma unroll
+        for (int q = 0; q < WALK_DEPTH; q++) {
+            const uint2 rec = cur[q];
+            const int x0 = (int)(rec.x & 0xFFu), x1 = (int)((rec.x >> 8) & 0xFFu);
+            const int y0 = (int)((rec.x >> 16) & 0xFFu), y1 = (int)(rec.x >> 24);
+            const bool live = base + 64u * q + (uint32_t)lane < r1;
+            uint64_t todo =
+                __ballot(live && x0 < x1 && y0 < y1 && x0 < bx1 && x1 > b.bx0 && y0 < by1 && y1 > b.by0);
+            while (todo) {
+                const int j = __builtin_ctzll(todo);
+                todo &= todo - 1ull;
+                const uint32_t R = (uint32_t)__builtin_amdgcn_readlane((int)rec.x, j);
+                const uint32_t gid = (uint32_t)__builtin_amdgcn_readlane((int)rec.y, j);
+                const int jx0 = (int)(R & 0xFFu), jx1 = (int)((R >> 8) & 0xFFu);
+                const int jy0 = (int)((R >> 16) & 0xFFu), jy1 = (int)(R >> 24);
+                const bool inx = tx >= jx0 && tx < jx1;
+                bool in[TPL];
+#pragma unroll
+                for (int k = 0; k < TPL; k++) {
+                    const int y = ty + 8 * k;
+                    in[k] = inx && y >= jy0 && y < jy1;
+                }
+                f(in, gid);
             }
-            f(in, gid);
         }
     }
 }

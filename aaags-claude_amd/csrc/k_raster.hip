@@ -291,6 +291,9 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
         const uint32_t u = (uint32_t)__shfl_xor((int)wlast, o, 64);
         wlast = u > wlast ? u : wlast;
     }
+    // every lane holds the maximum; readfirstlane lets the compiler keep it,
+    // and the entry loop below, in scalar registers
+    wlast = (uint32_t)__builtin_amdgcn_readfirstlane((int)wlast);
     __syncthreads();
     const uint32_t stop = sm.maxlast;
 

@@ -246,24 +246,27 @@ def cpu_baseline(scene, cam, cfg, gt):
 
 
 def traffic_probe(args):
-    """Dispatches of each single-view stage kernel at the headline workload,
-    plus device copies of a known size, for hardware-counter runs
-    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the copy's counter values per
-    byte calibrate the stage kernels' values into HBM bytes per launch."""
+    """The headline step for hardware-counter runs (rocprofv3 --pmc ...): the
+    native trainer with Adam fused, autotuned as in the timed benchmark, then
+    two training steps; plus device copies of a known size, whose counter
+    values per byte calibrate the stage kernels' values into HBM bytes per
+    launch.  Prints the tuning, so that the counters can be matched to the
+    kernel instances (LDS batch depths) the benchmark runs."""
     import torch
-    from gs_hip import GsHip
+    from gs_hip import GsHip, GsTrainer, train_cfg
     from gs_scenes import RenderCfg, make_workload
 
     scene, cams, gts = make_workload(args.workload)
     cam = cams[0]
     cfg = RenderCfg(sh_degree=3, training=True)
     hip = GsHip(0)
-    p = hip.upload_scene(scene)
     gt = torch.tensor(gts[0], device=hip.dev)
-    m, v = hip.alloc_pgrads(scene.n, scene.max_coeffs), hip.alloc_pgrads(scene.n, scene.max_coeffs)
-    for r in range(2):
-        f = hip.forward_backward_fast(p, cam, cfg, gt)
-        hip.adam(p, f["grads"], m, v, [1e-3] * 5, 0.9, 0.999, 1e-8, r + 1)
+    tr = GsTrainer(hip, scene)
+    tr.set_options(fused_adam=True)
+    tuning = tr.autotune(cam, cfg, gt, reps=5)
+    tc = train_cfg(lr=1e-3)
+    for i in range(2):
+        r = tr.step(cam, cfg, gt, tc, i)
     torch.cuda.synchronize()
     copy_bytes = 1 << 30
     src = torch.ones(copy_bytes // 4, device=hip.dev)
@@ -271,7 +274,8 @@ def traffic_probe(args):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"traffic probe done: pairs {f['K']}, copy {copy_bytes} B x 3")
+    print(json.dumps({"traffic_probe": args.workload, "pairs": int(r.pairs), "tuning": tuning,
+                      "copy_bytes": copy_bytes, "copies": 3}), flush=True)
 
 
 def main():
@@ -372,14 +376,18 @@ def main():
     byts = algorithmic_bytes(dom, scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
     hbm_gbs = byts / (dom_ms * 1e-3) / 1e9
     st_meas = load_traffic(args.workload).get(dom, {})
-    if dom in ("raster_fwd", "raster_bwd") and "valu_wave_insts" in st_meas:
+    # VALU wave-instructions of the rasterizer instance (LDS batch depth) this run uses
+    batch = (tuning or {}).get("raster_batch" if dom == "raster_fwd" else "raster_bwd_batch")
+    by_batch = st_meas.get("valu_wave_insts_by_batch", {})
+    insts = by_batch.get(str(batch)) or (sorted(by_batch.values())[len(by_batch) // 2] if by_batch else None)
+    if dom in ("raster_fwd", "raster_bwd") and insts:
         # the rasterizers are VALU-bound: wave64 FP32 VALU issue, 2 cycles per
         # instruction per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz
-        achieved = st_meas["valu_wave_insts"] / (dom_ms * 1e-3) / 1e9
+        achieved = insts / (dom_ms * 1e-3) / 1e9
         peak = VALU_WAVE_INSTS_PER_S / 1e9
         roofline = {"kernel": kernel_names[dom], "bound": "valu", "achieved": round(achieved, 1),
                     "peak": round(peak, 1), "unit": "G VALU wave-instructions/s", "frac": round(achieved / peak, 4),
-                    "traffic": st_meas.get("traffic_bytes"), "valu_wave_insts": st_meas["valu_wave_insts"],
+                    "traffic": st_meas.get("traffic_bytes"), "valu_wave_insts": insts,
                     "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
                     "algorithmic_bytes": int(byts), "ms": round(dom_ms, 4)}
     else:
@@ -397,9 +405,9 @@ def main():
     pre = hip.alloc_pre(scene.n)
     import ctypes
     g_c, c_c, f_c, p_c = hip._gauss(p, True), cam.to_c(), cfg.to_c(), hip._pre_struct(pre)
-    args = (hip.ctx, ctypes.byref(g_c), ctypes.byref(c_c), ctypes.byref(f_c), ctypes.byref(p_c))
-    hip._check(hip.lib.gs_preprocess_fwd(*args), "gs_preprocess_fwd")
-    ms_pre = median_ms(lambda: hip.lib.gs_preprocess_fwd(*args), reps=10)
+    pre_args = (hip.ctx, ctypes.byref(g_c), ctypes.byref(c_c), ctypes.byref(f_c), ctypes.byref(p_c))
+    hip._check(hip.lib.gs_preprocess_fwd(*pre_args), "gs_preprocess_fwd")
+    ms_pre = median_ms(lambda: hip.lib.gs_preprocess_fwd(*pre_args), reps=10)
     pb = algorithmic_bytes("preprocess", scene.n, K, npix, scene.max_coeffs, cam.num_tiles)
     pg = pb / (ms_pre * 1e-3) / 1e9
     hbm_targets["preprocess"] = {"ms": round(ms_pre, 4), "algorithmic_bytes": int(pb),
