@@ -214,6 +214,23 @@ def test_radix_sort_stable_random_keys(hip, count, radix_bits):
     assert np.array_equal(u32(vo), vals[order])
 
 
+def test_radix_sort_large_many_tiles(hip):
+    """Above 4M keys (4096-key tiles, ~1300 of them): the one-sweep passes'
+    look-back crosses many tiles; 40-bit keys with heavy ties, with and
+    without values."""
+    _say("radix large")
+    rng = np.random.default_rng(11)
+    n = 5_300_001
+    keys = (rng.integers(0, 1 << 20, n, dtype=np.uint64) << np.uint64(20)) | rng.integers(0, 3, n, dtype=np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    order = np.argsort(keys, kind="stable")
+    kt = torch.tensor(keys.view(np.int64), device=hip.dev)
+    vt = torch.tensor(vals.view(np.int32), device=hip.dev)
+    ko, vo = hip.radix_sort(kt, vt, key_bits=40, radix_bits=8)
+    assert np.array_equal(u32(ko), keys[order])
+    assert np.array_equal(u32(vo), vals[order])
+
+
 def test_radix_sort_full_64bit(hip):
     _say("radix 64-bit")
     rng = np.random.default_rng(7)
