@@ -263,6 +263,24 @@ __device__ __forceinline__ float raster_alpha(float o, float power, float* ep) {
     return a;
 }
 
+// raster_alpha for code where all 64 lanes run it together: the exact
+// recompute is entered only when some lane of the wave needs it (a uniform
+// branch), so the common path carries no exec-mask bookkeeping.
+__device__ __forceinline__ float raster_alpha_wave(float o, float power, float* ep) {
+    float e = expf(power);
+    float a = smin(0.99f, o * e);
+    constexpr float kCut = 1.0f / 255.0f;
+    const bool near = fabsf(a - kCut) <= kCut * 0x1p-18f;
+    if (__ballot(near) != 0ull) {
+        if (near) {
+            e = exp_ref(power);
+            a = smin(0.99f, o * e);
+        }
+    }
+    *ep = e;
+    return a;
+}
+
 // Camera constants shared by the per-Gaussian kernels, passed by value.
 struct CamArgs {
     float view[16];

@@ -146,31 +146,35 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
         }
         __syncthreads();
         const int nc = (int)(((end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH) + 63u) >> 6;
-        for (int c = 0; c < nc && !done; c++) {
+        // predicated: every lane runs every visited entry and selects, so the
+        // only branches are uniform (the CU's scalar unit, shared by its
+        // waves, is not spent on exec-mask bookkeeping of per-pixel branches)
+        for (int c = 0; c < nc; c++) {
+            if (__ballot(!done) == 0ull) break;
             uint64_t m = wave_uniform(sm.mask[w][c]);
             while (m != 0ull) {
                 const int k = (c << 6) + __builtin_ctzll(m);
                 m &= m - 1ull;
                 const float2 mm = sm.xy[k];
                 const float4 co = sm.co[k];
+                const float4 cl = sm.rgb[k];
                 const float dx = mm.x - fpx, dy = mm.y - fpy;
                 const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-                if (power > 0.0f) continue;
                 float ep;
-                const float alpha = gsd::raster_alpha(co.w, power, &ep);
-                if (alpha < 1.0f / 255.0f) continue;
+                const float alpha = gsd::raster_alpha_wave(co.w, power, &ep);
+                // rasterizer_cpu.cpp: skip if power > 0 or alpha < 1/255; stop
+                // (without this entry) once T would fall below 1e-4
+                const bool use = !done && power <= 0.0f && alpha >= 1.0f / 255.0f;
                 const float test_T = T * (1.0f - alpha);
-                if (test_T < 0.0001f) {
-                    done = true;
-                    break;
-                }
-                const float4 cl = sm.rgb[k];
-                C0 += cl.x * alpha * T;
-                C1 += cl.y * alpha * T;
-                C2 += cl.z * alpha * T;
-                T = test_T;
-                cnt++;
-                last = base + (uint32_t)k + 1u;
+                const bool stop = use && test_T < 0.0001f;
+                const bool take = use && !stop;
+                done = done || stop;
+                C0 = take ? C0 + cl.x * alpha * T : C0;
+                C1 = take ? C1 + cl.y * alpha * T : C1;
+                C2 = take ? C2 + cl.z * alpha * T : C2;
+                T = take ? test_T : T;
+                cnt += take ? 1 : 0;
+                last = take ? base + (uint32_t)k + 1u : last;
             }
         }
     }
