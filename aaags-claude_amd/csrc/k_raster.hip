@@ -331,55 +331,46 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 const int k = (ch << 6) + (63 - __builtin_clzll(bits));
                 bits &= ~(1ull << (k & 63));
                 const uint32_t j = lo + (uint32_t)k;
+                // predicated, as the forward: every lane evaluates the entry and
+                // the contribution is selected in or out
+                const float2 m = sm.xy[k];
+                const float4 co = sm.co[k];
+                const float4 cl = sm.rgb[k];
+                const float dx = m.x - fpx, dy = m.y - fpy;
+                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                float ep;
+                const float alpha = gsd::raster_alpha_wave(co.w, power, &ep);
+                const bool contrib = j < last && power <= 0.0f && alpha >= 1.0f / 255.0f;
+                const float oma = 1.0f - alpha;
+                // T_i = T / (1 - alpha) as the reference recovers it
+                // (rasterizer_backward_cpu.cpp:85), with the hardware
+                // reciprocal (1 ulp): gradients need 1e-3, not bits
+                const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
+                float dal = 0.0f;
+                dal += Ti * (cl.x - acc0) * dC0;
+                dal += Ti * (cl.y - acc1) * dC1;
+                dal += Ti * (cl.z - acc2) * dC2;
+                const bool unsat = alpha < 0.99f;
+                const float dpow = unsat ? dal * co.w * ep : 0.0f;
+                const float dop = unsat ? dal * ep : 0.0f;
+                // raw moments of dpow; the conic transform of the mean gradient
+                // and the conic-gradient constants are applied once per
+                // Gaussian when the sums leave LDS
+                const float pdx = dpow * dx, pdy = dpow * dy;
                 float gv[NG];
-#pragma unroll
-                for (int c = 0; c < NG; c++) gv[c] = 0.0f;
-                bool contrib = false;
-                if (j < last) {
-                    const float2 m = sm.xy[k];
-                    const float4 co = sm.co[k];
-                    const float dx = m.x - fpx, dy = m.y - fpy;
-                    const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-                    if (power <= 0.0f) {
-                        float ep;
-                        const float alpha = gsd::raster_alpha(co.w, power, &ep);
-                        if (alpha >= 1.0f / 255.0f) {
-                            contrib = true;
-                            const float4 cl = sm.rgb[k];
-                            const float oma = 1.0f - alpha;
-                            // T_i = T / (1 - alpha) as the reference recovers it
-                            // (rasterizer_backward_cpu.cpp:85), with the hardware
-                            // reciprocal (1 ulp): gradients need 1e-3, not bits
-                            const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
-                            gv[5] = alpha * Ti * dC0;
-                            gv[6] = alpha * Ti * dC1;
-                            gv[7] = alpha * Ti * dC2;
-                            float dal = 0.0f;
-                            dal += Ti * (cl.x - acc0) * dC0;
-                            dal += Ti * (cl.y - acc1) * dC1;
-                            dal += Ti * (cl.z - acc2) * dC2;
-                            acc0 = alpha * cl.x + oma * acc0;
-                            acc1 = alpha * cl.y + oma * acc1;
-                            acc2 = alpha * cl.z + oma * acc2;
-                            T = Ti;
-                            float dpow = 0.0f, dop = 0.0f;
-                            if (alpha < 0.99f) {
-                                dpow = dal * co.w * ep;
-                                dop = dal * ep;
-                            }
-                            // raw moments of dpow; the conic transform of the mean
-                            // gradient and the conic-gradient constants are applied
-                            // once per Gaussian when the sums leave LDS
-                            gv[8] = dop;
-                            const float pdx = dpow * dx, pdy = dpow * dy;
-                            gv[0] = pdx;
-                            gv[1] = pdy;
-                            gv[2] = pdx * dx;
-                            gv[3] = pdx * dy;
-                            gv[4] = pdy * dy;
-                        }
-                    }
-                }
+                gv[0] = contrib ? pdx : 0.0f;
+                gv[1] = contrib ? pdy : 0.0f;
+                gv[2] = contrib ? pdx * dx : 0.0f;
+                gv[3] = contrib ? pdx * dy : 0.0f;
+                gv[4] = contrib ? pdy * dy : 0.0f;
+                gv[5] = contrib ? alpha * Ti * dC0 : 0.0f;
+                gv[6] = contrib ? alpha * Ti * dC1 : 0.0f;
+                gv[7] = contrib ? alpha * Ti * dC2 : 0.0f;
+                gv[8] = contrib ? dop : 0.0f;
+                acc0 = contrib ? alpha * cl.x + oma * acc0 : acc0;
+                acc1 = contrib ? alpha * cl.y + oma * acc1 : acc1;
+                acc2 = contrib ? alpha * cl.z + oma * acc2 : acc2;
+                T = contrib ? Ti : T;
                 if (__ballot(contrib) != 0ull) {
                     int term;
                     float v;
