@@ -182,7 +182,11 @@ __device__ __forceinline__ void flush_line_wave(const BinArgs& a, const uint32_t
 
 template <int TPL>
 __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
-    __shared__ __align__(16) uint32_t lines[TPL * 64 * LINE];  // per (tile slot, lane): one line
+    // per (tile slot, lane): one line of LINE slots plus one spare slot that
+    // takes the writes of lanes whose tile the Gaussian does not cover, so
+    // the visit has no per-lane branch
+    constexpr int LSTRIDE = LINE + 1;
+    __shared__ uint32_t lines[TPL * 64 * LSTRIDE];
     const BinBlock<TPL> b = bin_block<TPL>(a);
     const int lane = threadIdx.x;
     const int tx = b.bx0 + (lane & 7), ty = b.by0 + (lane >> 3);
@@ -198,22 +202,20 @@ __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
     walk_bin<TPL>(a, b, tx, ty, lane, [&](const bool (&in)[TPL], uint32_t gid) {
 #pragma unroll
         for (int k = 0; k < TPL; k++) {
-            uint32_t p = 0u;
-            bool full = false;
-            if (in[k]) {
-                p = cur[k]++;
-                lines[(k * 64 + lane) * LINE + (p & (LINE - 1))] = gid;
-                full = (p & (LINE - 1)) == LINE - 1;
-            }
-            // lines that just filled leave one at a time, each written by the wave
-            uint64_t fm = __ballot(full);
+            const uint32_t p = cur[k];
+            const uint32_t slot = in[k] ? (p & (LINE - 1)) : (uint32_t)LINE;
+            lines[(k * 64 + lane) * LSTRIDE + slot] = gid;
+            cur[k] = p + (in[k] ? 1u : 0u);
+            // a line fills when slot LINE - 1 was written; such lines leave one
+            // at a time, each written by the whole wave
+            uint64_t fm = __ballot(slot == LINE - 1);
             while (fm) {
                 const int j = __builtin_ctzll(fm);
                 fm &= fm - 1ull;
                 const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)p, j);
                 const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane((int)beg[k], j);
                 const uint32_t lo = pj - (LINE - 1);
-                flush_line_wave(a, lines + (k * 64 + j) * LINE, lo > bj ? lo : bj, pj, lane);
+                flush_line_wave(a, lines + (k * 64 + j) * LSTRIDE, lo > bj ? lo : bj, pj, lane);
             }
         }
     });
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
             fm &= fm - 1ull;
             const uint32_t fj = (uint32_t)__builtin_amdgcn_readlane((int)from, j);
             const uint32_t ej = (uint32_t)__builtin_amdgcn_readlane((int)end, j);
-            flush_line_wave(a, lines + (k * 64 + j) * LINE, fj, ej - 1u, lane);
+            flush_line_wave(a, lines + (k * 64 + j) * LSTRIDE, fj, ej - 1u, lane);
         }
     }
 }
