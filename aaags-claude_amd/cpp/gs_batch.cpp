@@ -60,7 +60,7 @@ struct gs_batch {
     bool direct = true;            // direct-placement binning (else the word sort)
     int bin_G = 0, bin_C = 0;      // Gaussians per group, groups per view
     size_t bin_L = 0;              // (view, tile, group) table entries + 1
-    int bin_blocks = 0, bin_tpl = 2;
+    int bin_blocks = 0, bin_tpl = 1;
     int num_cus = 0;
 
     hipStream_t stream = nullptr;
@@ -694,7 +694,7 @@ extern "C" int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out) {
     if (b->direct) {
         B_TRY(batch_pick(b, tm, reps, {1, 2, 4}, [&](int x) { return batch_setup_binning(b, b->bin_G, x); }, bin,
                          &res.bin_tiles_per_lane, &res.binning_ms));
-        B_TRY(batch_pick(b, tm, reps, {8192, 16384, 32768, 65536},
+        B_TRY(batch_pick(b, tm, reps, {2048, 4096, 8192, 16384, 32768, 65536},
                          [&](int x) { return batch_setup_binning(b, x, b->bin_tpl); }, bin, &res.bin_group,
                          &res.binning_ms));
     } else {

@@ -4,7 +4,7 @@
 // stage at a time over that knob's candidates (coordinate descent), timing
 // each candidate with HIP events on the context's stream (candidates
 // interleaved, median of `reps` runs each), and keeps the fastest:
-//   binning:         tiles per lane {1, 2, 4}, then group size {4096 .. 65536},
+//   binning:         tiles per lane {1, 2, 4}, then group size {1024 .. 65536},
 //                    then the depth-sort digit width {4 .. 8};
 //   raster forward:  LDS batch depth {64, 128, 256, 512};
 //   raster backward: LDS batch depth {64, 128, 256}.
@@ -110,7 +110,7 @@ extern "C" int gs_autotune(gs_trainer* t, const gs_camera* cam, const gs_render_
     auto bin = [&]() { return gs_bin_tiles(c, t->n, &pre, W, H, t->keys_sorted.u(), cap, v.ranges.u(), &k2); };
     TU_TRY(pick(c, tm, reps, {1, 2, 4}, [&](int x) { return gs_ctx_set_binning(c, 0, x); }, bin,
                 &res.bin_tiles_per_lane, &res.binning_ms));
-    TU_TRY(pick(c, tm, reps, {4096, 8192, 16384, 32768, 65536}, [&](int x) { return gs_ctx_set_binning(c, x, 0); },
+    TU_TRY(pick(c, tm, reps, {1024, 2048, 4096, 8192, 16384, 32768, 65536}, [&](int x) { return gs_ctx_set_binning(c, x, 0); },
                 bin, &res.bin_group, &res.binning_ms));
     TU_TRY(pick(c, tm, reps, {4, 5, 6, 7, 8}, [&](int x) { return gs_ctx_set_tuning(c, x, 0, 0); }, bin,
                 &res.radix_bits, &res.binning_ms));

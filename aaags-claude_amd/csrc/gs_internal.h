@@ -31,8 +31,8 @@ struct gs_ctx {
     int radix_bits = 8;
     int raster_batch = 256;
     int raster_bwd_batch = 256;
-    int bin_chunk = 16384;  // Gaussians per group in direct-placement binning
-    int bin_tpl = 2;        // tiles per lane (bin = 8 x 8*tpl tiles)
+    int bin_chunk = 4096;   // Gaussians per group in direct-placement binning
+    int bin_tpl = 1;        // tiles per lane (bin = 8 x 8*tpl tiles)
     int persistent_per_cu = 6;  // batched raster: persistent groups per CU (0: one group per tile)
 };
 
