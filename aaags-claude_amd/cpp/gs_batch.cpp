@@ -690,6 +690,7 @@ extern "C" int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out) {
     B_HIP(hipEventCreate(&tm.e), "hipEventCreate");
     gs_tuning res{};
     res.radix_bits = b->radix_bits;
+    res.split_groups = c->split_groups;  // a single-view knob, reported unchanged
     auto bin = [&]() { return enqueue_binning(b, b->stream); };
     if (b->direct) {
         B_TRY(batch_pick(b, tm, reps, {1, 2, 4}, [&](int x) { return batch_setup_binning(b, b->bin_G, x); }, bin,

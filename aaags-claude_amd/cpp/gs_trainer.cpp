@@ -192,7 +192,7 @@ gs_trainer::~gs_trainer() {
 
 void gs::ViewBufs::release_all() {
     DevBuf* all[] = {&means2D, &depths, &conics, &op2d, &rgb, &radii, &tiles, &offsets, &ranges, &image, &d_image,
-                     &T_final, &n_contrib, &last, &loss, &rg};
+                     &T_final, &n_contrib, &last, &loss, &rg, &split_flags, &split_rc};
     for (DevBuf* b : all) b->release();
 }
 
@@ -276,7 +276,8 @@ static int tr_bin(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, const gs
         const uint32_t cap = (uint32_t)std::min<size_t>(t->keys_sorted.bytes / per, 0xFFFFFFFFu);
         int st = words ? gs_bin_sorted_words(c, t->n, &pre, cam->width, cam->height,
                                              static_cast<uint64_t*>(t->keys_sorted.p), cap, v.ranges.u(), K)
-                       : gs_bin_tiles(c, t->n, &pre, cam->width, cam->height, t->keys_sorted.u(), cap, v.ranges.u(), K);
+                       : gs_bin_tiles_impl(c, t->n, &pre, cam->width, cam->height, t->keys_sorted.u(), cap,
+                                           v.ranges.u(), K, nullptr, nullptr, c->split_groups);
         if (st == GS_ERR_CAPACITY && *K > (long long)cap && attempt == 0) {
             TR_TRY(t->keys_sorted.ensure((size_t)(*K + *K / 8 + 1024) * per));
             continue;
@@ -317,7 +318,7 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
         }
         const uint32_t cap = (uint32_t)std::min<size_t>(t->keys_sorted.bytes / 4, 0xFFFFFFFFu);
         TR_TRY(gs_bin_tiles_impl(c, t->n, &pre, cam->width, cam->height, t->keys_sorted.u(), cap, v.ranges.u(), nullptr,
-                                 t->k_pinned_dev, t->k_event));
+                                 t->k_pinned_dev, t->k_event, c->split_groups));
         t->pair_stride = 1;
         *pending = true;
     } else {
@@ -326,7 +327,26 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
     TR_TRY(tr_mark(t, GS_STAGE_BINNING, false));
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
     TR_TRY(tr_mark(t, GS_STAGE_RASTER_FWD, true));
-    if (t->pair_stride == 1)
+    gsr::RasterSplit sp;
+    int nbins = 0;
+    TR_TRY(gs_bin_split_info(c, &sp, &nbins));
+    if (t->pair_stride == 1 && sp.round == 1) {
+        // two rounds: render what the binning placed (the nearest depth groups),
+        // place the rest for the bins with unfinished tiles, resume those tiles
+        const size_t tiles = (size_t)((cam->width + 15) / 16) * ((cam->height + 15) / 16);
+        TR_TRY(v.split_flags.ensure((tiles + (size_t)nbins) * 4));
+        TR_TRY(v.split_rc.ensure((size_t)cam->width * cam->height * 12));
+        sp.tile_flag = v.split_flags.u();
+        sp.bin_flag = v.split_flags.u() + tiles;
+        sp.rc = v.split_rc.f();
+        TR_TRY(gs_zero(c, v.split_flags.p, (tiles + (size_t)nbins) * 4));
+        TR_TRY(gs_raster_fwd_split(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(),
+                                   cfg->bg_color, v.image.f(), &px, &sp));
+        TR_TRY(gs_bin_place_rest(c, sp.bin_flag));
+        sp.round = 2;
+        TR_TRY(gs_raster_fwd_split(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(),
+                                   cfg->bg_color, v.image.f(), &px, &sp));
+    } else if (t->pair_stride == 1)
         TR_TRY(gs_raster_fwd(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(), cfg->bg_color,
                              v.image.f(), &px));
     else

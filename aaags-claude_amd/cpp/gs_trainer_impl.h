@@ -28,6 +28,7 @@ struct ViewBufs {
     DevBuf means2D, depths, conics, op2d, rgb, radii, tiles, offsets, ranges;
     DevBuf image, d_image, T_final, n_contrib, last, loss;
     DevBuf rg;  // screen-space gradients, one block: means2D (8 B/G), conics (12), rgb (12), opacities (4)
+    DevBuf split_flags, split_rc;  // two-round forward: tile and bin flags; per-pixel colour
     size_t rg_n = 0;
     gs_preprocessed pre();
     gs_raster_grads rgrads();

@@ -7,7 +7,9 @@
 //   binning:         tiles per lane {1, 2, 4}, then group size {1024 .. 65536},
 //                    then the depth-sort digit width {4 .. 8};
 //   raster forward:  LDS batch depth {64, 128, 256, 512};
-//   raster backward: LDS batch depth {64, 128, 256}.
+//   raster backward: LDS batch depth {64, 128, 256};
+//   forward:         depth groups placed before the first raster round
+//                    {0 (one round), 2, 4, 8, 16, 32}, on whole forward renders.
 // Only forward, loss and backward stages run: the trainer's parameters and
 // optimizer state are untouched.
 #include <hip/hip_runtime.h>
@@ -131,6 +133,16 @@ extern "C" int gs_autotune(gs_trainer* t, const gs_camera* cam, const gs_render_
     TU_TRY(pick(c, tm, reps, {64, 128, 256}, [&](int x) { return gs_ctx_set_tuning(c, 0, 0, x); }, bwd,
                 &res.raster_bwd_batch, &res.raster_bwd_ms));
     res.persistent_per_cu = c->persistent_per_cu;
+    // the two-round forward: groups placed before the first raster round, on
+    // the trainer's whole forward with the knobs chosen above
+    long long k3 = 0;
+    auto render = [&]() { return gs_trainer_render(t, cam, cfg, &k3); };
+    TU_TRY(pick(c, tm, reps, {0, 2, 4, 8, 16, 32},
+                [&](int x) {
+                    c->split_groups = x;
+                    return GS_OK;
+                },
+                render, &res.split_groups, &res.forward_ms));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return gs_set_error(GS_ERR_HIP, "hipStreamSynchronize");
     if (out) *out = res;
     return GS_OK;

@@ -55,12 +55,21 @@ extern "C" int gs_ctx_create(int device, void* stream, gs_ctx** out) {
     return GS_OK;
 }
 
+// The split a gs_bin_tiles_impl call left pending: the placement arguments
+// for the remaining depth groups and what the forward raster needs.
+struct BinSplit {
+    bool pending = false;
+    gsb::BinArgs a{};
+    int tpl = 1, bins = 0, C = 0, split = 0, gx = 0, gy = 0;
+};
+
 extern "C" int gs_ctx_destroy(gs_ctx* c) {
     if (!c) return GS_OK;
     (void)hipSetDevice(c->device);
     if (c->ws) (void)hipFree(c->ws);
     if (c->ws2) (void)hipFree(c->ws2);
     if (c->small) (void)hipFree(c->small);
+    delete static_cast<BinSplit*>(c->bin_split);
     delete c;
     return GS_OK;
 }
@@ -425,16 +434,19 @@ extern "C" int gs_ctx_get_tuning(gs_ctx* c, gs_tuning* out) {
     out->bin_group = c->bin_chunk;
     out->bin_tiles_per_lane = c->bin_tpl;
     out->persistent_per_cu = c->persistent_per_cu;
+    out->split_groups = c->split_groups;
     return GS_OK;
 }
 
 extern "C" int gs_ctx_apply_tuning(gs_ctx* c, const gs_tuning* t) {
     GS_CHECK_ARG(c && t, "null argument");
     GS_CHECK_ARG(t->persistent_per_cu >= 0 && t->persistent_per_cu <= 64, "persistent_per_cu must be in [0, 64]");
+    GS_CHECK_ARG(t->split_groups >= 0, "split_groups must be >= 0");
     int st = gs_ctx_set_tuning(c, t->radix_bits, t->raster_batch, t->raster_bwd_batch);
     if (st) return st;
     if ((st = gs_ctx_set_binning(c, t->bin_group, t->bin_tiles_per_lane))) return st;
     c->persistent_per_cu = t->persistent_per_cu;
+    c->split_groups = t->split_groups;
     return GS_OK;
 }
 
@@ -447,7 +459,7 @@ extern "C" int gs_ctx_apply_tuning(gs_ctx* c, const gs_tuning* t) {
 // the capacity before using the results.
 int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
                       uint32_t capacity, uint32_t* ranges, long long* total_host, unsigned long long* total_async,
-                      hipEvent_t ev_total) {
+                      hipEvent_t ev_total, int split) {
     GS_CHECK_ARG(c != nullptr, "null context");
     GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
     GS_CHECK_ARG(n >= 0, "negative count");
@@ -514,16 +526,34 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
     const int blocks = bin_blocks(C, gx, gy, c->bin_tpl);
     GS_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");
+    // with a split, only the first `split` depth groups are placed now (blocks
+    // are numbered group-major: group c's blocks are c * bins .. c * bins + bins - 1)
+    if (!c->bin_split) c->bin_split = new BinSplit();
+    BinSplit* bs = static_cast<BinSplit*>(c->bin_split);
+    bs->pending = false;
+    const int bins = blocks / C;
+    int place_blocks = blocks;
+    if (split > 0 && split < C) {
+        place_blocks = split * bins;
+        bs->pending = true;
+        bs->tpl = c->bin_tpl;
+        bs->bins = bins;
+        bs->C = C;
+        bs->split = split;
+        bs->gx = gx;
+        bs->gy = gy;
+    }
     // asynchronous: the scan writes the pair count straight into the caller's
     // mapped host memory, and the event marks when it is there
     GS_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr,
                                      total_async ? total_async : total64),
            "binning scan");
     a.table = scanned;
+    bs->a = a;
     if (total_async) {
         GS_CHECK_ARG(values != nullptr && capacity > 0, "null values buffer");
         if (ev_total) GS_HIP(hipEventRecord(ev_total, c->stream), "hipEventRecord");
-        GS_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
+        GS_HIP(launch_bin_place(a, place_blocks, c->bin_tpl, c->stream), "binning place kernel");
         GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
         return GS_OK;
     }
@@ -534,7 +564,9 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     if (K > capacity) return gs_set_error(GS_ERR_CAPACITY, "gs_bin_tiles: %llu pairs exceed capacity %u", K, capacity);
     if (K > 0) {
         GS_CHECK_ARG(values != nullptr, "null values buffer");
-        GS_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
+        GS_HIP(launch_bin_place(a, place_blocks, c->bin_tpl, c->stream), "binning place kernel");
+    } else {
+        bs->pending = false;
     }
     GS_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
     return GS_OK;
@@ -543,6 +575,31 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
 extern "C" int gs_bin_tiles(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
                             uint32_t capacity, uint32_t* ranges, long long* total_host) {
     return gs_bin_tiles_impl(c, n, pre, width, height, values, capacity, ranges, total_host, nullptr, nullptr);
+}
+
+int gs_bin_split_info(gs_ctx* c, gsr::RasterSplit* sp, int* num_bins) {
+    GS_CHECK_ARG(c && sp, "null argument");
+    *sp = gsr::RasterSplit{};
+    const BinSplit* bs = static_cast<const BinSplit*>(c->bin_split);
+    if (!bs || !bs->pending) return GS_OK;
+    sp->scanned = bs->a.table;
+    sp->C = bs->C;
+    sp->g1 = bs->split;
+    sp->nbx = (bs->gx + 7) / 8;
+    sp->bin_h = 8 * bs->tpl;
+    sp->round = 1;
+    if (num_bins) *num_bins = bs->bins;
+    return GS_OK;
+}
+
+int gs_bin_place_rest(gs_ctx* c, const uint32_t* bin_gate) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    BinSplit* bs = static_cast<BinSplit*>(c->bin_split);
+    GS_CHECK_ARG(bs && bs->pending, "no split binning pending");
+    GS_HIP(launch_bin_place(bs->a, (bs->C - bs->split) * bs->bins, bs->tpl, c->stream, bs->split * bs->bins, bin_gate),
+           "binning place kernel (remaining groups)");
+    bs->pending = false;
+    return GS_OK;
 }
 
 static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed* pre, const uint32_t* values,
@@ -566,7 +623,7 @@ static gsr::RasterArgs raster_args(int width, int height, const gs_preprocessed*
 
 static int raster_fwd_impl(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
                            int vstride, const uint32_t* ranges, const float bg[3], float* image,
-                           const gs_pixel_state* px) {
+                           const gs_pixel_state* px, const gsr::RasterSplit* sp = nullptr) {
     GS_CHECK_ARG(c != nullptr, "null context");
     GS_CHECK_ARG(width > 0 && height > 0, "bad image size");
     int st = check_pre(pre, false);
@@ -575,9 +632,16 @@ static int raster_fwd_impl(gs_ctx* c, int width, int height, const gs_preprocess
     int tiles = ((width + 15) / 16) * ((height + 15) / 16);
     gsr::RasterArgs a = raster_args(width, height, pre, values, ranges, bg, vstride);
     GS_HIP(launch_raster_fwd(a, tiles, c->raster_batch, image, px ? px->T_final : nullptr,
-                             px ? px->n_contrib : nullptr, px ? px->last_contrib : nullptr, c->stream),
+                             px ? px->n_contrib : nullptr, px ? px->last_contrib : nullptr, c->stream, sp),
            "raster forward kernel");
     return GS_OK;
+}
+
+int gs_raster_fwd_split(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                        const uint32_t* ranges, const float bg[3], float* image, const gs_pixel_state* px,
+                        const gsr::RasterSplit* sp) {
+    GS_CHECK_ARG(px && px->T_final && px->n_contrib && px->last_contrib, "a split forward needs the pixel state");
+    return raster_fwd_impl(c, width, height, pre, values, 1, ranges, bg, image, px, sp);
 }
 
 extern "C" int gs_raster_fwd(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,

@@ -10,7 +10,7 @@
 #include "gs_hip.h"
 
 namespace gsd { struct CamArgs; }
-namespace gsr { struct RasterArgs; struct GradArgs; struct MultiRaster; struct MultiPix; struct MultiGrads; }
+namespace gsr { struct RasterArgs; struct RasterSplit; struct GradArgs; struct MultiRaster; struct MultiPix; struct MultiGrads; }
 namespace gso { struct AdamArgs; struct AdamDev; }
 namespace gsb { struct BinArgs; }
 
@@ -34,6 +34,8 @@ struct gs_ctx {
     int bin_chunk = 4096;   // Gaussians per group in direct-placement binning
     int bin_tpl = 1;        // tiles per lane (bin = 8 x 8*tpl tiles)
     int persistent_per_cu = 6;  // batched raster: persistent groups per CU (0: one group per tile)
+    int split_groups = 8;       // trainer forward: depth groups placed before the first raster round (0: one round)
+    void* bin_split = nullptr;  // the split left pending by the last gs_bin_tiles_impl (BinSplit, capi.hip)
 };
 
 int gs_set_error(int code, const char* fmt, ...);
@@ -49,7 +51,17 @@ extern "C" int gs_ctx_reserve(gs_ctx* c, size_t bytes);
 extern "C" int gs_ctx_set_binning(gs_ctx* c, int gaussians_per_group, int tiles_per_lane);
 int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, int height, uint32_t* values,
                       uint32_t capacity, uint32_t* ranges, long long* total_host, unsigned long long* total_async,
-                      hipEvent_t ev_total);
+                      hipEvent_t ev_total, int split = 0);
+// After gs_bin_tiles_impl with 0 < split < groups, which placed only the
+// first `split` depth groups: the description of that split for the forward
+// raster (bin_flag, tile_flag, rc and round left to the caller), or
+// sp->round = 0 when nothing is pending; then the placement of the remaining
+// groups for the bins whose gate word is nonzero.
+int gs_bin_split_info(gs_ctx* c, gsr::RasterSplit* sp, int* num_bins);
+int gs_bin_place_rest(gs_ctx* c, const uint32_t* bin_gate);
+int gs_raster_fwd_split(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
+                        const uint32_t* ranges, const float bg[3], float* image, const gs_pixel_state* px,
+                        const gsr::RasterSplit* sp);
 extern "C" int gs_check_overflow(gs_ctx* c, int* overflowed);
 
 // kernel launchers
@@ -81,7 +93,8 @@ hipError_t launch_duplicate_ranked(int n, const uint32_t* order, const float* me
                                    const uint32_t* offsets, int gx, int gy, uint64_t* words, uint32_t capacity,
                                    uint32_t* overflow, hipStream_t s);
 hipError_t launch_raster_fwd(const gsr::RasterArgs& a, int num_tiles, int batch, float* img, float* T_final,
-                             int* n_contrib, uint32_t* last_contrib, hipStream_t s);
+                             int* n_contrib, uint32_t* last_contrib, hipStream_t s,
+                             const gsr::RasterSplit* split = nullptr);
 hipError_t launch_raster_bwd(const gsr::RasterArgs& a, int num_tiles, int batch, const float* T_final,
                              const uint32_t* last_contrib, const float* d_image, const gsr::GradArgs& g,
                              hipStream_t s);
@@ -110,7 +123,8 @@ hipError_t launch_duplicate_multi(uint32_t total, int N, const uint64_t* skeys, 
 constexpr int GS_BIN_MAX_TILES_XY = 255;  // tiles per row and per column (4080 pixels)
 int bin_blocks(int C, int gx, int gy, int tpl);
 hipError_t launch_bin_count(const gsb::BinArgs& a, int blocks, int tpl, hipStream_t s);
-hipError_t launch_bin_place(const gsb::BinArgs& a, int blocks, int tpl, hipStream_t s);
+hipError_t launch_bin_place(const gsb::BinArgs& a, int blocks, int tpl, hipStream_t s, int block0 = 0,
+                            const uint32_t* gate = nullptr);
 hipError_t launch_bin_ranges(const gsb::BinArgs& a, const uint32_t* scanned, uint32_t total_tiles, uint32_t* ranges,
                              hipStream_t s);
 // total32 / total64 (optional, device): the sum of all n inputs

@@ -66,13 +66,13 @@ __global__ __launch_bounds__(256) void bin_rects_kernel(BinArgs a) {
 // and their records stay in that XCD's L2.
 template <int TPL>
 struct BinBlock {
-    int v, c, bx0, by0;
+    int v, c, bin, bx0, by0;
     int4 vi;  // (tiles per row, tile rows, table offset, first global tile)
 };
 
 template <int TPL>
-__device__ __forceinline__ BinBlock<TPL> bin_block(const BinArgs& a) {
-    const int q = gsd::xcd_remap(blockIdx.x, gridDim.x);
+__device__ __forceinline__ BinBlock<TPL> bin_block(const BinArgs& a, int block0 = 0) {
+    const int q = block0 + gsd::xcd_remap(blockIdx.x, gridDim.x);
     BinBlock<TPL> b;
     b.v = 0;
     if (a.vblock0)
@@ -82,6 +82,7 @@ __device__ __forceinline__ BinBlock<TPL> bin_block(const BinArgs& a) {
     const int nbx = (b.vi.x + 7) / 8, nby = (b.vi.y + 8 * TPL - 1) / (8 * TPL);
     b.c = local / (nbx * nby);
     const int bin = local - b.c * (nbx * nby);
+    b.bin = bin;
     b.bx0 = (bin % nbx) * 8;
     b.by0 = (bin / nbx) * (8 * TPL);
     return b;
@@ -180,14 +181,17 @@ __device__ __forceinline__ void flush_line_wave(const BinArgs& a, const uint32_t
     }
 }
 
+// block0 / gate: the launch covers logical blocks block0 .. (a later range of
+// groups), and a block whose bin's gate word is zero does nothing (one view).
 template <int TPL>
-__global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a) {
+__global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a, int block0, const uint32_t* __restrict__ gate) {
     // per (tile slot, lane): one line of LINE slots plus one spare slot that
     // takes the writes of lanes whose tile the Gaussian does not cover, so
     // the visit has no per-lane branch
     constexpr int LSTRIDE = LINE + 1;
     __shared__ uint32_t lines[TPL * 64 * LSTRIDE];
-    const BinBlock<TPL> b = bin_block<TPL>(a);
+    const BinBlock<TPL> b = bin_block<TPL>(a, block0);
+    if (gate && gate[b.bin] == 0u) return;  // one wave, no barrier follows
     const int lane = threadIdx.x;
     const int tx = b.bx0 + (lane & 7), ty = b.by0 + (lane >> 3);
     const int gx = b.vi.x, gy = b.vi.y;
@@ -278,12 +282,12 @@ hipError_t launch_bin_count(const BinArgs& a, int blocks, int tpl, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_bin_place(const BinArgs& a, int blocks, int tpl, hipStream_t s) {
+hipError_t launch_bin_place(const BinArgs& a, int blocks, int tpl, hipStream_t s, int block0, const uint32_t* gate) {
     if (blocks <= 0) return hipSuccess;
     switch (tpl) {
-        case 1: bin_place_kernel<1><<<blocks, 64, 0, s>>>(a); break;
-        case 2: bin_place_kernel<2><<<blocks, 64, 0, s>>>(a); break;
-        case 4: bin_place_kernel<4><<<blocks, 64, 0, s>>>(a); break;
+        case 1: bin_place_kernel<1><<<blocks, 64, 0, s>>>(a, block0, gate); break;
+        case 2: bin_place_kernel<2><<<blocks, 64, 0, s>>>(a, block0, gate); break;
+        case 4: bin_place_kernel<4><<<blocks, 64, 0, s>>>(a, block0, gate); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

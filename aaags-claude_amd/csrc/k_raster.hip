@@ -115,12 +115,27 @@ struct PixOut {
 };
 
 template <int BATCH>
-__device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<BATCH>& sm, const PixOut& o) {
+__device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<BATCH>& sm, const PixOut& o,
+                                         const RasterSplit& sp) {
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int t = threadIdx.x;
     const int px = tx * gsd::TILE + (t & 15), py = ty * gsd::TILE + (t >> 4);
     const bool inside = px < a.W && py < a.H;
+    const int pix = py * a.W + px;
     const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
+    // the part of the list this pass renders: [from, to)
+    uint32_t from = start, to = end;
+    if (sp.round != 0 && end > start) {
+        const uint32_t fill = sp.scanned[(size_t)tile * sp.C + sp.g1];  // where group g1 begins
+        if (sp.round == 1) {
+            to = fill < end ? fill : end;
+        } else {
+            if (sp.tile_flag[tile] == 0u) return;  // finished in round 1 (uniform per workgroup)
+            from = fill;
+        }
+    } else if (sp.round == 2) {
+        return;
+    }
     const float fpx = (float)px, fpy = (float)py;
     const float tx0 = (float)(tx * gsd::TILE), ty0 = (float)(ty * gsd::TILE);
     const int lane = t & 63, w = t >> 6;
@@ -128,12 +143,22 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
     int cnt = 0;
     uint32_t last = 0;
     bool done = !inside;
-    for (uint32_t base = start; base < end; base += BATCH) {
+    if (sp.round == 2 && inside) {  // the state round 1 left
+        T = o.T_final[pix];
+        const uint32_t nc = (uint32_t)o.n_contrib[pix];
+        cnt = (int)(nc & 0x7FFFFFFFu);
+        done = (nc >> 31) != 0u;
+        last = o.last_contrib[pix];
+        C0 = sp.rc[pix * 3 + 0];
+        C1 = sp.rc[pix * 3 + 1];
+        C2 = sp.rc[pix * 3 + 2];
+    }
+    for (uint32_t base = from; base < to; base += BATCH) {
         if (__syncthreads_count(!done) == 0) break;
         for (int k = t; k < BATCH; k += 256) {
             uint32_t j = base + k;
             uint32_t h = 0u;
-            if (j < end) {
+            if (j < to) {
                 uint32_t id = a.values[(size_t)j * a.vstride];
                 const float2 mm = a.means2D[id];
                 const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
@@ -145,7 +170,7 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
         }
         __syncthreads();
-        const int nc = (int)(((end - base) < (uint32_t)BATCH ? (end - base) : (uint32_t)BATCH) + 63u) >> 6;
+        const int nc = (int)(((to - base) < (uint32_t)BATCH ? (to - base) : (uint32_t)BATCH) + 63u) >> 6;
         // predicated: every lane runs every visited entry and selects, so the
         // only branches are uniform (the CU's scalar unit, shared by its
         // waves, is not spent on exec-mask bookkeeping of per-pixel branches)
@@ -178,14 +203,29 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
             }
         }
     }
+    // round 1 stopped at the end of the placed part with pixels still open:
+    // flag the tile and its bin, keep the state (colour in rc, the done bit
+    // in n_contrib's top bit) for round 2
+    bool keep = false;
+    if (sp.round == 1 && to < end) {
+        keep = __syncthreads_count(!done) != 0;
+        if (keep && t == 0) {
+            sp.tile_flag[tile] = 1u;
+            atomicOr(&sp.bin_flag[(ty / sp.bin_h) * sp.nbx + tx / 8], 1u);
+        }
+    }
     if (inside) {
-        const int pix = py * a.W + px;
         o.img[pix * 3 + 0] = C0 + T * a.bg0;
         o.img[pix * 3 + 1] = C1 + T * a.bg1;
         o.img[pix * 3 + 2] = C2 + T * a.bg2;
         if (o.T_final) o.T_final[pix] = T;
-        if (o.n_contrib) o.n_contrib[pix] = cnt;
+        if (o.n_contrib) o.n_contrib[pix] = keep && done ? (int)((uint32_t)cnt | 0x80000000u) : cnt;
         if (o.last_contrib) o.last_contrib[pix] = last;
+        if (keep) {
+            sp.rc[pix * 3 + 0] = C0;
+            sp.rc[pix * 3 + 1] = C1;
+            sp.rc[pix * 3 + 2] = C2;
+        }
     }
 }
 
@@ -402,9 +442,9 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
 }
 
 template <int BATCH>
-__global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, PixOut o) {
+__global__ __launch_bounds__(256) void raster_fwd_kernel(RasterArgs a, PixOut o, RasterSplit sp) {
     __shared__ FwdSmem<BATCH> sm;
-    fwd_tile<BATCH>(a, gsd::xcd_remap(blockIdx.x, gridDim.x), sm, o);
+    fwd_tile<BATCH>(a, gsd::xcd_remap(blockIdx.x, gridDim.x), sm, o, sp);
 }
 
 template <int BATCH>
@@ -491,7 +531,7 @@ __global__ __launch_bounds__(256) void raster_fwd_multi_kernel(MultiRaster m, Mu
         RasterArgs a = view_args(m, v);
         const size_t p0 = (size_t)m.pix_off[v];
         PixOut po{o.img + p0 * 3, o.T_final + p0, o.n_contrib + p0, o.last_contrib + p0};
-        fwd_tile<BATCH>(a, g - m.views[v].w, sm, po);
+        fwd_tile<BATCH>(a, g - m.views[v].w, sm, po, RasterSplit{});
     }
 }
 
@@ -516,14 +556,16 @@ __global__ __launch_bounds__(256) void raster_bwd_multi_kernel(MultiRaster m, Mu
 using namespace gsr;
 
 hipError_t launch_raster_fwd(const RasterArgs& a, int num_tiles, int batch, float* img, float* T_final, int* n_contrib,
-                             uint32_t* last_contrib, hipStream_t s) {
+                             uint32_t* last_contrib, hipStream_t s, const RasterSplit* split) {
     if (num_tiles <= 0) return hipSuccess;
     PixOut o{img, T_final, n_contrib, last_contrib};
+    const RasterSplit sp = split ? *split : RasterSplit{};
+    if (sp.round != 0 && (!T_final || !n_contrib || !last_contrib)) return hipErrorInvalidValue;  // the round state
     switch (batch) {
-        case 64: raster_fwd_kernel<64><<<num_tiles, 256, 0, s>>>(a, o); break;
-        case 128: raster_fwd_kernel<128><<<num_tiles, 256, 0, s>>>(a, o); break;
-        case 256: raster_fwd_kernel<256><<<num_tiles, 256, 0, s>>>(a, o); break;
-        case 512: raster_fwd_kernel<512><<<num_tiles, 256, 0, s>>>(a, o); break;
+        case 64: raster_fwd_kernel<64><<<num_tiles, 256, 0, s>>>(a, o, sp); break;
+        case 128: raster_fwd_kernel<128><<<num_tiles, 256, 0, s>>>(a, o, sp); break;
+        case 256: raster_fwd_kernel<256><<<num_tiles, 256, 0, s>>>(a, o, sp); break;
+        case 512: raster_fwd_kernel<512><<<num_tiles, 256, 0, s>>>(a, o, sp); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -54,7 +54,8 @@ class _AdamHP(ctypes.Structure):
 class _Tuning(ctypes.Structure):
     _fields_ = [("radix_bits", _i32), ("raster_batch", _i32), ("raster_bwd_batch", _i32), ("bin_group", _i32),
                 ("bin_tiles_per_lane", _i32), ("persistent_per_cu", _i32), ("binning_ms", ctypes.c_float),
-                ("raster_fwd_ms", ctypes.c_float), ("raster_bwd_ms", ctypes.c_float)]
+                ("raster_fwd_ms", ctypes.c_float), ("raster_bwd_ms", ctypes.c_float), ("split_groups", _i32),
+                ("forward_ms", ctypes.c_float)]
 
     def as_dict(self):
         return {k: (round(getattr(self, k), 4) if isinstance(getattr(self, k), float) else getattr(self, k))
@@ -138,6 +139,12 @@ class GsHip:
     def apply_tuning(self, d: dict):
         t = _Tuning(**{k: d[k] for k, _ in _Tuning._fields_ if k in d})
         self._check(self.lib.gs_ctx_apply_tuning(self.ctx, ctypes.byref(t)), "gs_ctx_apply_tuning")
+
+    def set_split(self, groups):
+        """Trainer forward: depth groups placed before the first raster round (0: one round)."""
+        d = self.tuning()
+        d["split_groups"] = int(groups)
+        self.apply_tuning(d)
 
     def set_binning(self, group=0, tiles_per_lane=0):
         """Direct-placement binning knobs (0 keeps the current value)."""

@@ -67,3 +67,33 @@ def test_1m_16views_batched_render_equals_single_views(hip):
         tr.render(cam, cfg)
         single = tr.buffer("image", (cam.height, cam.width, 3))
         assert torch.equal(imgs[v], single), f"view {v}"
+
+
+def test_fullsize_two_round_forward_equals_one_round(hip):
+    """1M Gaussians, 1920x1080: the trainer's two-round forward (8 depth groups
+    first) and the one-round forward give the same image, T_final, contributor
+    counts and last contributors, bit for bit."""
+    from gs_hip import GsTrainer
+    from gs_scenes import RenderCfg, make_workload
+    print("[case] full-size two-round forward", flush=True)
+    scene, cams, _ = make_workload("1m_1080p")
+    cam = cams[0]
+    cfg = RenderCfg(sh_degree=3, training=True)
+    out = []
+    try:
+        for sp in (0, 8):
+            hip.set_split(sp)
+            tr = GsTrainer(hip, scene)
+            tr.render(cam, cfg)
+            K = tr.render(cam, cfg)
+            torch.cuda.synchronize()
+            H, W = cam.height, cam.width
+            out.append((K, [tr.buffer("image", (H, W, 3)), tr.buffer("T_final", (H, W)),
+                            tr.buffer("n_contrib", (H, W), torch.int32),
+                            tr.buffer("last_contrib", (H, W), torch.int32)]))
+            del tr
+    finally:
+        hip.set_split(8)
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1], out[1][1]):
+        assert torch.equal(a, b)

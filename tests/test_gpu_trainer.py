@@ -153,3 +153,60 @@ def test_trainer_stage_timing(hip):
     assert steps == 1 and ms["raster_bwd"] > 0
     assert all(v == 0 for k, v in ms.items() if k != "raster_bwd"), ms
     tr.set_timing([])
+
+
+def _render_state(tr, cam):
+    H, W = cam.height, cam.width
+    return [tr.buffer("image", (H, W, 3)).cpu().numpy(),
+            tr.buffer("T_final", (H, W)).cpu().numpy(),
+            tr.buffer("n_contrib", (H, W), torch.int32).cpu().numpy(),
+            tr.buffer("last_contrib", (H, W), torch.int32).cpu().numpy()]
+
+
+@pytest.mark.parametrize("split", [1, 3, 20])
+def test_trainer_two_round_forward_equals_one_round(hip, oracle, split):
+    """The trainer's two-round forward (the nearest `split` depth groups placed
+    and rendered first, the rest placed only for bins with unfinished tiles)
+    gives the one-round forward's image and pixel state bit for bit, and the
+    step then matches the oracle.  A dense scene (most tiles finish in round
+    1) with small groups (256 Gaussians, 80 groups)."""
+    from gs_hip import GsTrainer, train_cfg
+    print("[case] two-round forward", split, flush=True)
+    scene, cam, gt = random_small_scene(20000, 3, 41, width=160, height=128, logscale=(-2.0, 0.4), op=(0.6, 1.0))
+    cfg = RenderCfg(sh_degree=3, training=True)
+    gt_t = torch.tensor(gt, device=hip.dev)
+    hip.set_binning(256, 1)
+    try:
+        states = []
+        for sp in (0, split):
+            hip.set_split(sp)
+            tr = GsTrainer(hip, scene)
+            K = tr.render(cam, cfg)
+            K = tr.render(cam, cfg)  # the second render uses the asynchronous pair count
+            torch.cuda.synchronize()
+            states.append((K, _render_state(tr, cam)))
+        assert states[0][0] == states[1][0]
+        for a, b in zip(states[0][1], states[1][1]):
+            assert np.array_equal(a, b)
+        # an SGD step with the split (the second one: its pair count is
+        # asynchronous): the oracle's loss, image and gradients
+        hip.set_split(split)
+        tr2 = GsTrainer(hip, scene)
+        tr2.step(cam, cfg, gt_t, train_cfg(use_adam=False), 0)
+        torch.cuda.synchronize()
+        p1 = tr2.params()
+        res = tr2.step(cam, cfg, gt_t, train_cfg(use_adam=False), 0)
+        torch.cuda.synchronize()
+    finally:
+        hip.set_binning(4096, 1)
+        hip.set_split(8)
+    n = scene.n
+    from gs_scenes import RawScene
+    mid = RawScene(p1[0], p1[1], p1[2], p1[3].reshape(scene.sh.shape), p1[4], scene.sh_degree)
+    ref = oracle.forward_backward(mid, cam, cfg, gt)
+    assert abs(res.loss - ref["loss"]) <= 1e-6
+    assert res.pairs == len(ref["keys"])
+    assert_image_close(tr2.buffer("image", (cam.height, cam.width, 3)).cpu().numpy(), ref["img"])
+    shapes = dict(pos=(n, 3), scale=(n, 3), rot=(n, 4), sh=(n, scene.max_coeffs, 3), op=(n,))
+    for k in NAMES:
+        assert_grad_close(tr2.buffer("grad_" + k, shapes[k]).cpu().numpy(), ref["grads"][k], "grad_" + k)
