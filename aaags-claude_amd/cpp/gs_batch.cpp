@@ -61,6 +61,7 @@ struct gs_batch {
     int bin_G = 0, bin_C = 0;      // Gaussians per group, groups per view
     size_t bin_L = 0;              // (view, tile, group) table entries + 1
     int bin_blocks = 0, bin_tpl = 1;
+    int bin_diff_len = 0;  // largest view's (tiles per row + 1) * (tile rows + 1)
     int num_cus = 0;
 
     hipStream_t stream = nullptr;
@@ -152,11 +153,13 @@ static int batch_setup_binning(gs_batch* b, int group, int tpl) {
     std::vector<int4> bviews(V);
     std::vector<int> vblock0(V + 1, 0);
     size_t tab = 0;
+    b->bin_diff_len = 0;
     for (int v = 0; v < V; v++) {
         const int gx = b->views[v].z, gy = (b->views[v].y + 15) / 16;
         bviews[v] = int4{gx, gy, (int)(uint32_t)tab, b->views[v].w};
         tab += (size_t)gx * gy * b->bin_C;
         vblock0[v + 1] = vblock0[v] + bin_blocks(b->bin_C, gx, gy, b->bin_tpl);
+        b->bin_diff_len = std::max(b->bin_diff_len, (gx + 1) * (gy + 1));
     }
     b->bin_blocks = vblock0[V];
     const size_t VN = (size_t)V * b->N;
@@ -323,6 +326,7 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
         a.out = b->values.u();
         a.capacity = b->capacity;
         a.overflow = b->overflow();
+        a.diff_len = b->bin_diff_len;
         B_HIP(launch_bin_count(a, b->bin_blocks, b->bin_tpl, s), "binning count");
         B_HIP(launch_scan_u32_exclusive(b->bin_table.u(), (uint32_t)b->bin_L, b->bin_scanned.u(),
                                         static_cast<unsigned long long*>(b->bin_partial.p), s, b->total32(),

@@ -523,6 +523,7 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     a.out = values;
     a.capacity = capacity;
     a.overflow = ovf;
+    a.diff_len = (gx + 1) * (gy + 1);
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
     const int blocks = bin_blocks(C, gx, gy, c->bin_tpl);
     GS_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");

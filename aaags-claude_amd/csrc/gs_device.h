@@ -392,6 +392,7 @@ struct BinArgs {
     uint32_t* out;          // place: pair values (Gaussian index); 16-byte aligned
     uint32_t capacity;
     uint32_t* overflow;
+    int diff_len;           // max over views of (tiles per row + 1) * (tile rows + 1); 0: no LDS count
 };
 }  // namespace gsb
 

@@ -167,6 +167,65 @@ __global__ __launch_bounds__(64) void bin_count_kernel(BinArgs a) {
     }
 }
 
+// The same table from a two-dimensional difference array: one workgroup per
+// (view, group).  A rectangle [x0, x1) x [y0, y1) adds +1 at (x0, y0) and
+// (x1, y1) and -1 at (x1, y0) and (x0, y1) of a (tiles per row + 1) x (tile
+// rows + 1) array in LDS; the array's prefix sum along rows, then along
+// columns, is each tile's count.  Four LDS atomics per Gaussian whatever its
+// size, instead of one walk over the group per bin.  Used when the array of
+// the largest view fits in DIFF_MAX entries.
+constexpr int DIFF_MAX = 16384;  // 64 KB of LDS
+
+__global__ __launch_bounds__(256) void bin_count_diff_kernel(BinArgs a) {
+    extern __shared__ int dif[];
+    const int q = blockIdx.x;
+    const int v = q / a.C, c = q - v * a.C;
+    const int4 vi = a.views ? a.views[v] : a.view0;
+    const int gx = vi.x, gy = vi.y, W1 = gx + 1;
+    const int len = W1 * (gy + 1);
+    const int tid = threadIdx.x;
+    if (q == 0 && tid == 0) a.table[a.table_len - 1] = 0u;
+    for (int i = tid; i < len; i += 256) dif[i] = 0;
+    __syncthreads();
+    const uint2* rects = a.rects + (size_t)v * a.N;
+    const uint32_t r0 = (uint32_t)c * (uint32_t)a.G;
+    const uint32_t r1 = (r0 + (uint32_t)a.G < (uint32_t)a.N) ? r0 + (uint32_t)a.G : (uint32_t)a.N;
+    for (uint32_t r = r0 + (uint32_t)tid; r < r1; r += 256u) {
+        const uint32_t R = rects[r].x;
+        const int x0 = (int)(R & 0xFFu), x1 = (int)((R >> 8) & 0xFFu);
+        const int y0 = (int)((R >> 16) & 0xFFu), y1 = (int)(R >> 24);
+        if (x0 < x1 && y0 < y1) {
+            atomicAdd(&dif[y0 * W1 + x0], 1);
+            atomicAdd(&dif[y0 * W1 + x1], -1);
+            atomicAdd(&dif[y1 * W1 + x0], -1);
+            atomicAdd(&dif[y1 * W1 + x1], 1);
+        }
+    }
+    __syncthreads();
+    // rows: one per thread (a row's stride W1 is odd at the sizes that matter,
+    // so a wave's 64 rows fall in distinct banks)
+    for (int y = tid; y < gy; y += 256) {
+        int* row = dif + y * W1;
+        int s = 0;
+#pragma unroll 8
+        for (int x = 0; x < gx; x++) {
+            s += row[x];
+            row[x] = s;
+        }
+    }
+    __syncthreads();
+    // columns: one per thread; the column sums are the counts
+    const size_t tab = (size_t)(uint32_t)vi.z;
+    for (int x = tid; x < gx; x += 256) {
+        int s = 0;
+#pragma unroll 4
+        for (int y = 0; y < gy; y++) {
+            s += dif[y * W1 + x];
+            a.table[tab + (size_t)(y * gx + x) * a.C + c] = (uint32_t)s;
+        }
+    }
+}
+
 // Writes slots [lo, hi] (one line, hi & 15 the line's last slot or its
 // current end) of lane j's k-th line buffer with the whole wave: lane l < 16
 // moves slot l, so the line leaves as one coalesced 64-byte store (fewer lanes
@@ -273,6 +332,10 @@ hipError_t launch_bin_count(const BinArgs& a, int blocks, int tpl, hipStream_t s
     if (blocks <= 0) return hipSuccess;
     const size_t VN = (size_t)a.V * a.N;
     bin_rects_kernel<<<(unsigned)((VN + 255) / 256), 256, 0, s>>>(a);
+    if (a.diff_len > 0 && a.diff_len <= DIFF_MAX) {
+        bin_count_diff_kernel<<<a.V * a.C, 256, (size_t)a.diff_len * sizeof(int), s>>>(a);
+        return hipGetLastError();
+    }
     switch (tpl) {
         case 1: bin_count_kernel<1><<<blocks, 64, 0, s>>>(a); break;
         case 2: bin_count_kernel<2><<<blocks, 64, 0, s>>>(a); break;

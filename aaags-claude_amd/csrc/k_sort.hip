@@ -4,8 +4,9 @@
 //             wave-level prefix uses lane shuffles, the block level LDS.
 // duplicate:  one thread per Gaussian writes its tile|depth keys at the
 //             scanned offset (row-major over its tile rectangle).
-// radix sort: per pass an upsweep (per-block digit histogram), a scan of the
-//             digit-major histogram table, and a downsweep that ranks keys
+// radix sort: per pass an upsweep (per-block digit histogram), a scan of each
+//             digit's row of the digit-major histogram table (one wave per
+//             digit), and a downsweep that adds the digit totals' scan, ranks keys
 //             stably with wave ballots (match-any on the digit bits), stages
 //             the block's keys in LDS in digit order and writes them out in
 //             runs, so global writes are contiguous within each digit bucket.
@@ -356,12 +357,41 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* 
     }
 }
 
+// One wave per digit: exclusive scan of the digit's row of the digit-major
+// count table (its count in every block, in block order) and the digit's
+// total.  The downsweep adds the exclusive scan of the digit totals, so a pass
+// needs no separate three-launch scan of the whole table.
+__global__ __launch_bounds__(64) void rs_digit_scan_kernel(const uint32_t* __restrict__ hist, uint32_t nb,
+                                                           uint32_t* __restrict__ goff, uint32_t* __restrict__ dtot) {
+    const uint32_t* h = hist + (size_t)blockIdx.x * nb;
+    uint32_t* o = goff + (size_t)blockIdx.x * nb;
+    const int l = lane_id();
+    uint32_t carry = 0u;
+    for (uint32_t base = 0; base < nb; base += 256u) {
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = base + 64u * k + (uint32_t)l;
+            v[k] = i < nb ? h[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = base + 64u * k + (uint32_t)l;
+            const uint32_t inc = wave_inclusive_scan(v[k]);
+            if (i < nb) o[i] = carry + inc - v[k];
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        }
+    }
+    if (l == 0) dtot[blockIdx.x] = carry;
+}
+
 // VALS = false sorts self-contained 64-bit words (no separate value array).
 template <int BITS, bool VALS, int ROUNDS>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t* __restrict__ keys_in,
                                                                   const uint32_t* __restrict__ vals_in, uint32_t n_cap,
                                                                   const uint32_t* __restrict__ n_dev, int shift,
                                                                   uint32_t nb, const uint32_t* __restrict__ goff,
+                                                                  const uint32_t* __restrict__ dtot,
                                                                   uint64_t* __restrict__ keys_out,
                                                                   uint32_t* __restrict__ vals_out) {
     constexpr int RADIX = 1 << BITS;
@@ -377,7 +407,15 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
 
     const int w = threadIdx.x >> 6, l = lane_id();
     for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_wh[0][0])[i] = 0u;
-    for (int d = threadIdx.x; d < RADIX; d += RS_THREADS) s_goff[d] = goff[(size_t)d * nb + blockIdx.x];
+    {
+        // the digit's start in the output: all smaller digits, then this
+        // digit's keys in the blocks before this one
+        static_assert(RADIX <= RS_THREADS, "one digit per thread");
+        const uint32_t t = threadIdx.x < RADIX ? dtot[threadIdx.x] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(t, s_scan, &total);
+        if (threadIdx.x < RADIX) s_goff[threadIdx.x] = ex + goff[(size_t)threadIdx.x * nb + blockIdx.x];
+    }
     __syncthreads();
 
     const uint32_t bbase = blockIdx.x * RS_TILE;
@@ -519,39 +557,37 @@ uint32_t radix_blocks(uint32_t n) {
 }
 
 // Workspace layout for one sort of n items with the given radix:
-//   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][partials]
+//   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][digit totals R*4]
 size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
     size_t nb = radix_blocks(n);
     size_t R = (size_t)1 << radix_bits;
     size_t tab = R * nb;
-    size_t np = (tab + SCAN_BLOCK - 1) / SCAN_BLOCK + 1;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return al((size_t)n * 8) + al((size_t)n * 4) + 2 * al(tab * 4) + al(np * 8);
+    return al((size_t)n * 8) + al((size_t)n * 4) + 2 * al(tab * 4) + al(R * 4);
 }
 
 template <int BITS, int ROUNDS>
 static hipError_t sort_pass_r(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n,
                               const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff,
-                              unsigned long long* partial, hipStream_t s) {
+                              uint32_t* dtot, hipStream_t s) {
     uint32_t nb = radix_blocks(n);
     rs_upsweep_kernel<BITS, ROUNDS><<<nb, RS_THREADS, 0, s>>>(ki, n, n_dev, shift, nb, hist);
-    hipError_t e =
-        launch_scan_u32_exclusive(hist, (uint32_t)((1u << BITS) * nb), goff, partial, s, nullptr, nullptr);
-    if (e != hipSuccess) return e;
+    rs_digit_scan_kernel<<<1u << BITS, 64, 0, s>>>(hist, nb, goff, dtot);
     if (vi)
-        rs_downsweep_kernel<BITS, true, ROUNDS><<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, nb, goff, ko, vo);
+        rs_downsweep_kernel<BITS, true, ROUNDS>
+            <<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, nb, goff, dtot, ko, vo);
     else
         rs_downsweep_kernel<BITS, false, ROUNDS>
-            <<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, nb, goff, ko, nullptr);
+            <<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, nb, goff, dtot, ko, nullptr);
     return hipGetLastError();
 }
 
 template <int BITS>
 static hipError_t sort_pass(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n,
-                            const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff,
-                            unsigned long long* partial, hipStream_t s) {
-    if (radix_rounds(n) == 16) return sort_pass_r<BITS, 16>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s);
-    return sort_pass_r<BITS, 4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s);
+                            const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff, uint32_t* dtot,
+                            hipStream_t s) {
+    if (radix_rounds(n) == 16) return sort_pass_r<BITS, 16>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s);
+    return sort_pass_r<BITS, 4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s);
 }
 
 // Stable LSD sort of n keys (with optional uint32 values) on key bits
@@ -572,7 +608,7 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
     w += al(tab * 4);
     uint32_t* goff = reinterpret_cast<uint32_t*>(w);
     w += al(tab * 4);
-    unsigned long long* partial = reinterpret_cast<unsigned long long*>(w);
+    uint32_t* dtot = reinterpret_cast<uint32_t*>(w);
 
     const bool vals = vals_in != nullptr;
     int bits = end_bit - begin_bit;
@@ -592,11 +628,11 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
         hipError_t e;
         int shift = begin_bit + p * radix_bits;
         switch (radix_bits) {
-            case 4: e = sort_pass<4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
-            case 5: e = sort_pass<5>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
-            case 6: e = sort_pass<6>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
-            case 7: e = sort_pass<7>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
-            case 8: e = sort_pass<8>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, partial, s); break;
+            case 4: e = sort_pass<4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
+            case 5: e = sort_pass<5>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
+            case 6: e = sort_pass<6>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
+            case 7: e = sort_pass<7>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
+            case 8: e = sort_pass<8>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
             default: return hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;

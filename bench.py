@@ -263,7 +263,13 @@ def traffic_probe(args):
     gt = torch.tensor(gts[0], device=hip.dev)
     tr = GsTrainer(hip, scene)
     tr.set_options(fused_adam=True)
-    tuning = tr.autotune(cam, cfg, gt, reps=5)
+    if args.probe_tuning:
+        # fixed knobs: counter passes slow every dispatch down, so an autotuner
+        # run under the profiler may pick differently from pass to pass
+        hip.apply_tuning(json.loads(args.probe_tuning))
+        tuning = hip.tuning()
+    else:
+        tuning = tr.autotune(cam, cfg, gt, reps=5)
     tc = train_cfg(lr=1e-3)
     for i in range(2):
         r = tr.step(cam, cfg, gt, tc, i)
@@ -293,6 +299,8 @@ def main():
     ap.add_argument("--no-fused-adam", action="store_true", help="single-view step with the separate Adam kernel")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run each stage kernel and a 1 GiB device copy (for rocprofv3 --pmc runs)")
+    ap.add_argument("--probe-tuning", default="",
+                    help="with --traffic-probe: JSON knobs to apply instead of autotuning (e.g. the benchmark's)")
     ap.add_argument("--large-workload", default="6m_8x1080p")
     args = ap.parse_args()
 

@@ -173,6 +173,31 @@ def test_direct_binning_bitexact(hip, oracle, n, deg, seed, group, tpl):
     assert np.array_equal(u32(ranges), rg_ref)
 
 
+@pytest.mark.parametrize("width,height", [(2080, 2080), (2032, 2032)])
+def test_direct_binning_bitexact_large_view(hip, oracle, width, height):
+    """Both ways of counting (tile, group) pairs: 2032 x 2032 (127 x 127
+    tiles) is counted with the LDS difference array, 2080 x 2080 (130 x 130
+    tiles) is over its size and uses the per-bin walk.  Values and ranges
+    equal the reference's duplicate + stable sort."""
+    _say("direct binning large view", width, height)
+    scene, cam, _ = random_small_scene(3000, 1, 23, width=width, height=height)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    g = oracle.activate(scene)
+    ref = oracle.preprocess(g, scene.n, scene.sh_degree, scene.max_coeffs, cam, cfg, cache=False)
+    keys_ref, vals_ref = oracle.duplicate(ref, scene.n, cam.width, cam.height)
+    ks_ref, vs_ref, rg_ref = oracle.sort_pairs(keys_ref, vals_ref, cam.num_tiles)
+    assert len(keys_ref) > 10000
+    pre = _to_dev(hip, {k: ref[k] for k in ["means2D", "depths", "conics", "op2d", "rgb", "radii", "tiles"]})
+    hip.set_binning(256, 2)
+    try:
+        vals, ranges, K = hip.bin_tiles(pre, cam.width, cam.height)
+    finally:
+        hip.set_binning(4096, 1)
+    assert K == len(keys_ref)
+    assert np.array_equal(u32(vals), vs_ref)
+    assert np.array_equal(u32(ranges), rg_ref)
+
+
 def test_direct_binning_capacity_and_empty(hip, oracle):
     """Capacity errors report K without writing; an all-culled scene gives
     K = 0 and zero ranges."""
