@@ -261,9 +261,29 @@ __device__ __forceinline__ float row_sum(float v) {
     return v;
 }
 
-// Reduces gv[0..8] over the wave; lanes {0,1,16,17,32,33,48,49,2} return
-// true with one distinct term each in (*term, *val).
-__device__ __forceinline__ bool wave_reduce9(const float (&gv)[NG], int lane, int* term, float* val) {
+// Which reduced sum a lane delivers: in row r (lanes 16r .. 16r+15), lane
+// 16r delivers k0 (term A[r]) and lane 16r+1 k1 (term 4 + A[r]); lane 2
+// delivers k2 (term 8).  Fixed per lane, so it is set up once per thread and
+// the delivery below is selects, not branches.
+struct ReduceRole {
+    bool is0, is1, any;
+    int term;
+};
+
+__device__ __forceinline__ ReduceRole reduce_role(int lane) {
+    const int r = lane >> 4, p = lane & 15;
+    const int a = ((r & 1) << 1) | (r >> 1);
+    ReduceRole q;
+    q.is0 = p == 0;
+    q.is1 = p == 1;
+    q.any = q.is0 || q.is1 || lane == 2;
+    q.term = q.is0 ? a : (q.is1 ? 4 + a : 8);
+    return q;
+}
+
+// Reduces gv[0..8] over the wave; the nine lanes with q.any set return one
+// distinct term's sum each (term q.term).
+__device__ __forceinline__ float wave_reduce9(const float (&gv)[NG], const ReduceRole& q) {
     const float h0 = swap32_sum(gv[0], gv[1]);
     const float h1 = swap32_sum(gv[2], gv[3]);
     const float h2 = swap32_sum(gv[4], gv[5]);
@@ -275,24 +295,7 @@ __device__ __forceinline__ bool wave_reduce9(const float (&gv)[NG], int lane, in
     k0 = row_sum(k0);
     k1 = row_sum(k1);
     k2 = row_sum(k2);
-    const int r = lane >> 4, p = lane & 15;
-    const int a = ((r & 1) << 1) | (r >> 1);
-    if (p == 0) {
-        *term = a;
-        *val = k0;
-        return true;
-    }
-    if (p == 1) {
-        *term = 4 + a;
-        *val = k1;
-        return true;
-    }
-    if (lane == 2) {
-        *term = 8;
-        *val = k2;
-        return true;
-    }
-    return false;
+    return q.is0 ? k0 : (q.is1 ? k1 : k2);
 }
 
 struct PixIn {
@@ -324,6 +327,7 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
     const float fpx = (float)px, fpy = (float)py;
     const float tx0 = (float)(tx * gsd::TILE), ty0 = (float)(ty * gsd::TILE);
     const int w = t >> 6;
+    const ReduceRole role = reduce_role(lane);
     if (t == 0) sm.maxlast = 0;
     __syncthreads();
     if (last > 0) atomicMax(&sm.maxlast, last);
@@ -412,9 +416,8 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 acc2 = contrib ? alpha * cl.z + oma * acc2 : acc2;
                 T = contrib ? Ti : T;
                 if (__ballot(contrib) != 0ull) {
-                    int term;
-                    float v;
-                    if (wave_reduce9(gv, lane, &term, &v) && v != 0.0f) atomicAdd(&sm.g[k][term], v);
+                    const float v = wave_reduce9(gv, role);
+                    if (role.any && v != 0.0f) atomicAdd(&sm.g[k][role.term], v);
                 }
             }
         }
