@@ -27,7 +27,7 @@ using gsd::smin;
 
 // Wave w of a tile (threads 64w .. 64w+63) covers pixel rows 4w .. 4w+3, a
 // 16 x 4 strip.  Per LDS batch, mask[w][c] bit b is set when entry 64c + b can
-// pass the alpha cut somewhere in strip w (see alpha_box); each wave visits
+// pass the alpha cut somewhere in strip w (see strip_hits); each wave visits
 // only its set bits.
 template <int BATCH>
 struct FwdSmem {
@@ -37,41 +37,53 @@ struct FwdSmem {
     uint64_t mask[4][BATCH / 64];
 };
 
-// Pixel bounding box (x0, x1, y0, y1) of the region where a Gaussian can pass
-// the 1/255 alpha cut.  alpha = o * exp(-Q/2) with Q = d^T C d (C the conic),
-// so alpha >= 1/255 needs Q <= r2 = 2 ln(255 o): an ellipse with half-extents
-// sqrt(r2 * c / det) and sqrt(r2 * a / det).  Margins:
+// Strip-hit bits of one staged entry: bit w set when the Gaussian can pass the
+// 1/255 alpha cut at some pixel centre of strip w (rows y0 + 4w .. y0 + 4w + 3,
+// columns x0 .. x0 + 15) of the tile whose top-left pixel is (x0, y0).
+// alpha = o * exp(-Q/2) with Q = a u^2 + 2 b u v + c v^2 ((a, b, c) the conic,
+// (u, v) the pixel minus the mean), so passing needs Q <= r2 = 2 ln(255 o).
+// The least Q over a rectangle is 0 when the mean lies inside it, else the
+// least of its minima along the four edges; along an edge Q is a quadratic
+// in one variable whose minimum lies at its vertex clamped to the edge.
+// Margins, so that a strip without the bit has no pixel that passes:
 //  - the rasterizer's float Q has a relative rounding error below about
-//    8 eps tr^2/det (its terms reach tr^2/det times Q); the box is bounded
-//    only where tr^2/det <= 1000 (error <= 5e-4) and r2 is widened by 0.1%,
-//    plus 1e-3 for the approximate log / rcp / sqrt used here;
-//  - the half-extents get one more pixel.
-// So a pixel centre outside the box has alpha below the cut, and skipping the
-// Gaussian there changes nothing.  o < 1/255 never passes (empty box); NaN
-// or unbounded cases give an infinite box.
-__device__ __forceinline__ float4 alpha_box(float2 m, float4 co) {
-    const float o = co.w;
-    if (!(o >= 1.0f / 255.0f)) return make_float4(1.0f, -1.0f, 1.0f, -1.0f);  // empty
-    const float det = co.x * co.z - co.y * co.y;
-    const float tr = co.x + co.z;
-    if (!(det > 0.0f) || !(tr * tr <= 1000.0f * det)) return make_float4(-3.0e38f, 3.0e38f, -3.0e38f, 3.0e38f);
-    // 2 ln(x) = 2 ln(2) log2(x)
-    const float r2 = 1.38629436f * __builtin_amdgcn_logf(255.0f * o) * 1.001f + 1e-3f;
-    const float inv = __builtin_amdgcn_rcpf(det);
-    const float ex = __builtin_amdgcn_sqrtf(r2 * co.z * inv) + 1.0f;
-    const float ey = __builtin_amdgcn_sqrtf(r2 * co.x * inv) + 1.0f;
-    return make_float4(m.x - ex, m.x + ex, m.y - ey, m.y + ey);
+//    8 eps tr^2/det (its terms reach tr^2/det times Q), and so has the Q
+//    computed here; the test is exact only where tr^2/det <= 1000 (together
+//    at most 1e-3) and r2 is widened by 0.1 %, plus 1e-3 for the approximate
+//    log used here;
+//  - each rectangle is grown by half a pixel on every side.
+// o < 1/255 never passes (no bits); a degenerate or too anisotropic conic
+// gets every bit.  A NaN mean never passes in the rasterizer, whatever bits
+// it gets here.
+__device__ __forceinline__ float quad_form(float a, float b, float c, float u, float v) {
+    return (a * u * u + c * v * v) + 2.0f * b * u * v;
 }
 
-// Strip-hit bits of one staged entry: bit w set when the box meets strip w of
-// the tile whose top-left pixel is (x0, y0).  NaN box edges count as a hit.
-__device__ __forceinline__ uint32_t strip_hits(float4 b, float x0, float y0) {
-    if (b.y < x0 || b.x > x0 + 15.0f) return 0u;
+__device__ __forceinline__ uint32_t strip_hits(float2 m, float4 co, float x0, float y0) {
+    const float o = co.w;
+    if (!(o >= 1.0f / 255.0f)) return 0u;
+    const float a = co.x, b = co.y, c = co.z;
+    const float det = a * c - b * b;
+    const float tr = a + c;
+    if (!(det > 0.0f) || !(tr * tr <= 1000.0f * det)) return 0xFu;
+    // 2 ln(x) = 2 ln(2) log2(x)
+    const float r2 = 1.38629436f * __builtin_amdgcn_logf(255.0f * o) * 1.001f + 1e-3f;
+    constexpr float G = 0.5f;
+    const float u0 = x0 - G - m.x, u1 = x0 + (15.0f + G) - m.x;
+    // edge vertices: u* = -(b / a) v on the rows, v* = -(b / c) u on the columns
+    const float ka = -b * __builtin_amdgcn_rcpf(a), kc = -b * __builtin_amdgcn_rcpf(c);
+    const float vl = kc * u0, vr = kc * u1;
+    const bool in_x = u0 <= 0.0f && u1 >= 0.0f;
     uint32_t h = 0u;
 #pragma unroll
     for (int w = 0; w < 4; w++) {
-        const float s0 = y0 + (float)(4 * w);
-        if (!(b.w < s0 || b.z > s0 + 3.0f)) h |= 1u << w;
+        const float v0 = y0 + (float)(4 * w) - G - m.y, v1 = y0 + (float)(4 * w) + (3.0f + G) - m.y;
+        float q = (in_x && v0 <= 0.0f && v1 >= 0.0f) ? 0.0f : 3.0e38f;
+        q = fminf(q, quad_form(a, b, c, fminf(fmaxf(ka * v0, u0), u1), v0));
+        q = fminf(q, quad_form(a, b, c, fminf(fmaxf(ka * v1, u0), u1), v1));
+        q = fminf(q, quad_form(a, b, c, u0, fminf(fmaxf(vl, v0), v1)));
+        q = fminf(q, quad_form(a, b, c, u1, fminf(fmaxf(vr, v0), v1)));
+        h |= (q <= r2) ? (1u << w) : 0u;
     }
     return h;
 }
@@ -165,7 +177,7 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 sm.xy[k] = mm;
                 sm.co[k] = cc;
                 sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
-                h = strip_hits(alpha_box(mm, cc), tx0, ty0);
+                h = strip_hits(mm, cc, tx0, ty0);
             }
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
         }
@@ -359,7 +371,7 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 sm.xy[k] = mm;
                 sm.co[k] = cc;
                 sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
-                h = strip_hits(alpha_box(mm, cc), tx0, ty0);
+                h = strip_hits(mm, cc, tx0, ty0);
             }
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
 #pragma unroll

@@ -269,7 +269,7 @@ def traffic_probe(args):
     if args.probe_tuning:
         # fixed knobs for the measured steps: counter passes slow every
         # dispatch down, so the autotuner may pick differently from pass to pass
-        hip.apply_tuning(json.loads(args.probe_tuning))
+        hip.apply_tuning({k: int(v) for k, v in (kv.split("=") for kv in args.probe_tuning.split(","))})
         tuning = hip.tuning()
     tc = train_cfg(lr=1e-3)
     for i in range(2):
@@ -301,7 +301,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only run each stage kernel and a 1 GiB device copy (for rocprofv3 --pmc runs)")
     ap.add_argument("--probe-tuning", default="",
-                    help="with --traffic-probe: JSON knobs to apply instead of autotuning (e.g. the benchmark's)")
+                    help="with --traffic-probe: knobs for the measured steps, as key=value,key=value (every knob of "
+                         "gs_tuning: radix_bits, raster_batch, raster_bwd_batch, bin_group, bin_tiles_per_lane, "
+                         "persistent_per_cu, split_groups)")
     ap.add_argument("--large-workload", default="6m_8x1080p")
     args = ap.parse_args()
 
