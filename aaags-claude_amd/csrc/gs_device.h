@@ -280,21 +280,31 @@ __device__ __forceinline__ float raster_alpha(float o, float power, float* ep) {
     return a;
 }
 
-// raster_alpha for code where all 64 lanes run it together: the exact
-// recompute is entered only when some lane of the wave needs it (a uniform
-// branch), so the common path carries no exec-mask bookkeeping.
-__device__ __forceinline__ float raster_alpha_wave(float o, float power, float* ep) {
-    float e = exp_fast(power);
-    float a = smin(0.99f, o * e);
+// raster_alpha in two halves, for code where all 64 lanes run together:
+// alpha_fast gives the alpha from exp_fast and whether it lies near the cut;
+// where some lane of the wave is near (a uniform branch), alpha_fix
+// recomputes with exp_ref, every lane computing and the near ones taking it,
+// so the common path carries no exec-mask bookkeeping.  Callers with several
+// entries in flight evaluate all of them with alpha_fast and share one branch.
+__device__ __forceinline__ float alpha_fast(float o, float power, float* ep, bool* near) {
+    const float e = exp_fast(power);
+    const float a = smin(0.99f, o * e);
     constexpr float kCut = 1.0f / 255.0f;
-    const bool near = fabsf(a - kCut) <= kCut * 0x1p-18f;
-    if (__ballot(near) != 0ull) {
-        // every lane computes, the near ones take it: no per-lane branch
-        const float er = exp_ref(power);
-        e = near ? er : e;
-        a = smin(0.99f, o * e);
-    }
+    *near = fabsf(a - kCut) <= kCut * 0x1p-18f;
     *ep = e;
+    return a;
+}
+
+__device__ __forceinline__ void alpha_fix(float o, float power, bool near, float* alpha, float* ep) {
+    const float er = exp_ref(power);
+    *ep = near ? er : *ep;
+    *alpha = smin(0.99f, o * *ep);
+}
+
+__device__ __forceinline__ float raster_alpha_wave(float o, float power, float* ep) {
+    bool near;
+    float a = alpha_fast(o, power, ep, &near);
+    if (__ballot(near) != 0ull) alpha_fix(o, power, near, &a, ep);
     return a;
 }
 
