@@ -246,46 +246,21 @@ __device__ __forceinline__ void activate_rot(const float q[4], float o[4]) {
 }
 __device__ __forceinline__ float activate_opacity(float l) { return 1.0f / (1.0f + exp_ref(-l)); }
 
-// e^x for the rasterizers: x log2(e) split into a rounded head and an
-// FMA-recovered tail (log2(e) as 0x3fb8aa3b + 0x32a5705f), the hardware exp2
-// of the reduced argument (head minus its nearest integer, plus the tail) and
-// an ldexp by that integer.  This is the device expf's own reduction without
-// its range clamps: below x = -104 the ldexp underflows to 0 by itself, and
-// the rasterizers use the value only where x <= 0 and o e^x >= 1/255.
-__device__ __forceinline__ float exp_fast(float x) {
-    const float l2e_hi = __uint_as_float(0x3fb8aa3bu), l2e_lo = __uint_as_float(0x32a5705fu);
-    const float ph = x * l2e_hi;
-    float pl = __builtin_fmaf(x, l2e_hi, -ph);
-    pl = __builtin_fmaf(x, l2e_lo, pl);
-    const float r = __builtin_rintf(ph);
-    const float f = (ph - r) + pl;
-    return __builtin_ldexpf(__builtin_amdgcn_exp2f(f), (int)r);
-}
+// e^x for the rasterizers: the hardware exp2 of x log2(e).  The product's
+// rounding adds at most |x| ln(2) 2^-24 to the relative error.  The
+// rasterizers use the value only where o e^x >= 1/255, i.e. x > -ln(255)
+// > -5.6, where the result is within 2^-21 (relative) of e^x; alpha_fix
+// recomputes exactly wherever that could decide the 1/255 cut.
+__device__ __forceinline__ float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504f); }
 
 // alpha = min(0.99, o * exp(power)) of rasterizer_cpu.cpp:164-166 and the
-// 1/255 cut that follows.  exp_fast (the device expf's arithmetic,
-// <= 1 ulp) is used, and where
-// that alpha lies within 2^-18 (relative) of the cut -- where a 1-ulp
-// difference could decide it -- the alpha is recomputed with exp_ref, so
-// every include/skip decision is the CPU path's.
-__device__ __forceinline__ float raster_alpha(float o, float power, float* ep) {
-    float e = exp_fast(power);
-    float a = smin(0.99f, o * e);
-    constexpr float kCut = 1.0f / 255.0f;
-    if (fabsf(a - kCut) <= kCut * 0x1p-18f) {
-        e = exp_ref(power);
-        a = smin(0.99f, o * e);
-    }
-    *ep = e;
-    return a;
-}
-
-// raster_alpha in two halves, for code where all 64 lanes run together:
-// alpha_fast gives the alpha from exp_fast and whether it lies near the cut;
-// where some lane of the wave is near (a uniform branch), alpha_fix
-// recomputes with exp_ref, every lane computing and the near ones taking it,
-// so the common path carries no exec-mask bookkeeping.  Callers with several
-// entries in flight evaluate all of them with alpha_fast and share one branch.
+// 1/255 cut that follows, in two halves, for code where all 64 lanes run
+// together: alpha_fast gives the alpha from exp_fast and whether it lies
+// within 2^-18 (relative) of the cut, where exp_fast's error could decide it;
+// where some lane of the wave is that near (a uniform branch), alpha_fix
+// recomputes with exp_ref, every lane computing and the near ones taking it.
+// So every include/skip decision is the CPU path's, and the common path
+// carries no exec-mask bookkeeping.
 __device__ __forceinline__ float alpha_fast(float o, float power, float* ep, bool* near) {
     const float e = exp_fast(power);
     const float a = smin(0.99f, o * e);

@@ -53,8 +53,9 @@ struct FwdSmem {
 //    log used here;
 //  - each rectangle is grown by half a pixel on every side.
 // o < 1/255 never passes (no bits); a degenerate or too anisotropic conic
-// gets every bit.  A NaN mean never passes in the rasterizer, whatever bits
-// it gets here.
+// gets every bit.  An entry with a non-finite mean or conic never passes in
+// the rasterizer (its power is NaN) and gets no bits, so every visited entry
+// is finite.
 __device__ __forceinline__ float quad_form(float a, float b, float c, float u, float v) {
     return (a * u * u + c * v * v) + 2.0f * b * u * v;
 }
@@ -63,6 +64,9 @@ __device__ __forceinline__ uint32_t strip_hits(float2 m, float4 co, float x0, fl
     const float o = co.w;
     if (!(o >= 1.0f / 255.0f)) return 0u;
     const float a = co.x, b = co.y, c = co.z;
+    if (!(__builtin_isfinite(m.x) && __builtin_isfinite(m.y) && __builtin_isfinite(a) && __builtin_isfinite(b) &&
+          __builtin_isfinite(c) && __builtin_isfinite(o)))
+        return 0u;
     const float det = a * c - b * b;
     const float tr = a + c;
     if (!(det > 0.0f) || !(tr * tr <= 1000.0f * det)) return 0xFu;
@@ -88,6 +92,16 @@ __device__ __forceinline__ uint32_t strip_hits(float2 m, float4 co, float x0, fl
     return h;
 }
 
+// Staged conics are kept as (-a/2, -b, -c/2, o), so that the power
+// -0.5 (a dx^2 + c dy^2) - b dx dy of rasterizer_cpu.cpp:159-161 becomes
+// (a' dx^2 + c' dy^2) + b' dx dy.  Scaling by -1/2 and -1 is exact, so this is
+// the same float value (away from subnormals) in one multiply fewer.
+__device__ __forceinline__ float4 prescale(float4 co) { return make_float4(-0.5f * co.x, -co.y, -0.5f * co.z, co.w); }
+
+__device__ __forceinline__ float power_of(const float4& co, float dx, float dy) {
+    return (co.x * dx * dx + co.z * dy * dy) + co.y * dx * dy;
+}
+
 // Ballots the hit bits of 64 staged entries (one per lane of the calling wave,
 // entries 64c .. 64c+63) into the four strips' masks.
 template <int BATCH>
@@ -105,7 +119,7 @@ __device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// gradient sums per Gaussian: dpow * (dx, dy, dx^2, dx dy, dy^2), rgb 3, opacity 1
+// gradient sums per Gaussian: dop * (dx, dy, dx^2, dx dy, dy^2), rgb 3, opacity 1
 constexpr int NG = 9;
 
 template <int BATCH>
@@ -175,7 +189,7 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 const float2 mm = a.means2D[id];
                 const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
                 sm.xy[k] = mm;
-                sm.co[k] = cc;
+                sm.co[k] = prescale(cc);
                 sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
                 h = strip_hits(mm, cc, tx0, ty0);
             }
@@ -196,7 +210,7 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 const float4 co = sm.co[k];
                 const float4 cl = sm.rgb[k];
                 const float dx = mm.x - fpx, dy = mm.y - fpy;
-                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                const float power = power_of(co, dx, dy);
                 float ep;
                 const float alpha = gsd::raster_alpha_wave(co.w, power, &ep);
                 // rasterizer_cpu.cpp: skip if power > 0 or alpha < 1/255; stop
@@ -369,7 +383,7 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 const float2 mm = a.means2D[id];
                 const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
                 sm.xy[k] = mm;
-                sm.co[k] = cc;
+                sm.co[k] = prescale(cc);
                 sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
                 h = strip_hits(mm, cc, tx0, ty0);
             }
@@ -388,12 +402,14 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 bits &= ~(1ull << (k & 63));
                 const uint32_t j = lo + (uint32_t)k;
                 // predicated, as the forward: every lane evaluates the entry and
-                // the contribution is selected in or out
+                // its contribution is selected in or out, through the factors
+                // below (visited entries are finite, so a zero factor zeroes
+                // every term)
                 const float2 m = sm.xy[k];
                 const float4 co = sm.co[k];
                 const float4 cl = sm.rgb[k];
                 const float dx = m.x - fpx, dy = m.y - fpy;
-                const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+                const float power = power_of(co, dx, dy);
                 float ep;
                 const float alpha = gsd::raster_alpha_wave(co.w, power, &ep);
                 const bool contrib = j < last && power <= 0.0f && alpha >= 1.0f / 255.0f;
@@ -402,30 +418,30 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 // (rasterizer_backward_cpu.cpp:85), with the hardware
                 // reciprocal (1 ulp): gradients need 1e-3, not bits
                 const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
-                float dal = 0.0f;
-                dal += Ti * (cl.x - acc0) * dC0;
+                float dal = Ti * (cl.x - acc0) * dC0;
                 dal += Ti * (cl.y - acc1) * dC1;
                 dal += Ti * (cl.z - acc2) * dC2;
-                const bool unsat = alpha < 0.99f;
-                const float dpow = unsat ? dal * co.w * ep : 0.0f;
-                const float dop = unsat ? dal * ep : 0.0f;
-                // raw moments of dpow; the conic transform of the mean gradient
-                // and the conic-gradient constants are applied once per
-                // Gaussian when the sums leave LDS
-                const float pdx = dpow * dx, pdy = dpow * dy;
+                // dL/d(opacity * G) = dal * G; dL/dpower = opacity times that,
+                // and the opacity is applied once per Gaussian when the sums
+                // leave LDS (alpha >= 0.99 is the clamp: no gradient)
+                const float dop = (contrib && alpha < 0.99f) ? dal * ep : 0.0f;
+                const float aT = contrib ? alpha * Ti : 0.0f;
+                const float pdx = dop * dx, pdy = dop * dy;
                 float gv[NG];
-                gv[0] = contrib ? pdx : 0.0f;
-                gv[1] = contrib ? pdy : 0.0f;
-                gv[2] = contrib ? pdx * dx : 0.0f;
-                gv[3] = contrib ? pdx * dy : 0.0f;
-                gv[4] = contrib ? pdy * dy : 0.0f;
-                gv[5] = contrib ? alpha * Ti * dC0 : 0.0f;
-                gv[6] = contrib ? alpha * Ti * dC1 : 0.0f;
-                gv[7] = contrib ? alpha * Ti * dC2 : 0.0f;
-                gv[8] = contrib ? dop : 0.0f;
-                acc0 = contrib ? alpha * cl.x + oma * acc0 : acc0;
-                acc1 = contrib ? alpha * cl.y + oma * acc1 : acc1;
-                acc2 = contrib ? alpha * cl.z + oma * acc2 : acc2;
+                gv[0] = pdx;
+                gv[1] = pdy;
+                gv[2] = pdx * dx;
+                gv[3] = pdx * dy;
+                gv[4] = pdy * dy;
+                gv[5] = aT * dC0;
+                gv[6] = aT * dC1;
+                gv[7] = aT * dC2;
+                gv[8] = dop;
+                // colour behind the next entry: alpha c + (1 - alpha) acc
+                const float ac = contrib ? alpha : 0.0f;
+                acc0 += ac * (cl.x - acc0);
+                acc1 += ac * (cl.y - acc1);
+                acc2 += ac * (cl.z - acc2);
                 T = contrib ? Ti : T;
                 if (__ballot(contrib) != 0ull) {
                     const float v = wave_reduce9(gv, role);
@@ -436,17 +452,20 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
         __syncthreads();
         for (int k = t; k < n; k += 256) {
             const uint32_t id = sm.id[k];
-            // sums of dpow * (dx, dy, dx^2, dx*dy, dy^2) -> the reference's terms
+            // sums of dop * (dx, dy, dx^2, dx*dy, dy^2); times the opacity
+            // they are those of dpow, and -> the reference's terms
             // (rasterizer_backward_cpu.cpp): d_mean = -(a Sx + b Sy, c Sy + b Sx),
-            // d_conic = (-Sxx / 2, -Sxy, -Syy / 2)
-            const float4 co = sm.co[k];
-            const float sx = sm.g[k][0], sy = sm.g[k][1];
+            // d_conic = (-Sxx / 2, -Sxy, -Syy / 2); the staged conic is
+            // (-a/2, -b, -c/2, o)
+            const float4 cs = sm.co[k];
+            const float ca = -2.0f * cs.x, cb = -cs.y, cc = -2.0f * cs.z, op = cs.w;
+            const float sx = op * sm.g[k][0], sy = op * sm.g[k][1];
             float v;
-            v = -(co.x * sx + co.y * sy); if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].x, v);
-            v = -(co.z * sy + co.y * sx); if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].y, v);
-            v = -0.5f * sm.g[k][2]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 0], v);
-            v = -sm.g[k][3]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 1], v);
-            v = -0.5f * sm.g[k][4]; if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 2], v);
+            v = -(ca * sx + cb * sy); if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].x, v);
+            v = -(cc * sy + cb * sx); if (v != 0.0f) unsafeAtomicAdd(&g.d_means2D[id].y, v);
+            v = -0.5f * (op * sm.g[k][2]); if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 0], v);
+            v = -(op * sm.g[k][3]); if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 1], v);
+            v = -0.5f * (op * sm.g[k][4]); if (v != 0.0f) unsafeAtomicAdd(&g.d_conics[id * 3 + 2], v);
             v = sm.g[k][5]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 0], v);
             v = sm.g[k][6]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 1], v);
             v = sm.g[k][7]; if (v != 0.0f) unsafeAtomicAdd(&g.d_rgb[id * 3 + 2], v);
