@@ -32,7 +32,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 2
+# a wave64 FP32 VALU instruction occupies a SIMD for 4 cycles (16 lanes a
+# cycle; aaags-claude_amd/lib/valu_probe measures it): 256 CUs x 4 SIMDs x
+# 2.4 GHz / 4
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def load_traffic(workload):
@@ -392,7 +395,7 @@ def main():
     by_batch = st_meas.get("valu_wave_insts_by_batch", {})
     insts = by_batch.get(str(batch)) or (sorted(by_batch.values())[len(by_batch) // 2] if by_batch else None)
     if dom in ("raster_fwd", "raster_bwd") and insts:
-        # the rasterizers are VALU-bound: wave64 FP32 VALU issue, 2 cycles per
+        # the rasterizers are VALU-bound: wave64 FP32 VALU issue, 4 cycles per
         # instruction per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz
         achieved = insts / (dom_ms * 1e-3) / 1e9
         peak = VALU_WAVE_INSTS_PER_S / 1e9
