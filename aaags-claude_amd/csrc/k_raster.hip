@@ -29,11 +29,18 @@ using gsd::smin;
 // 16 x 4 strip.  Per LDS batch, mask[w][c] bit b is set when entry 64c + b can
 // pass the alpha cut somewhere in strip w (see strip_hits); each wave visits
 // only its set bits.
+// One staged entry: 48 bytes, read with one base address and immediate
+// offsets (mean, conic + opacity, colour).
+struct Staged {
+    float2 xy;
+    float2 pad;
+    float4 co;
+    float4 rgb;
+};
+
 template <int BATCH>
 struct FwdSmem {
-    float2 xy[BATCH];
-    float4 co[BATCH];
-    float4 rgb[BATCH];
+    Staged e[BATCH];
     uint64_t mask[4][BATCH / 64];
 };
 
@@ -124,9 +131,7 @@ constexpr int NG = 9;
 
 template <int BATCH>
 struct BwdSmem {
-    float2 xy[BATCH];
-    float4 co[BATCH];
-    float4 rgb[BATCH];
+    Staged e[BATCH];
     uint64_t mask[4][BATCH / 64];
     uint32_t id[BATCH];
     float g[BATCH][NG];
@@ -188,9 +193,9 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 uint32_t id = a.values[(size_t)j * a.vstride];
                 const float2 mm = a.means2D[id];
                 const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
-                sm.xy[k] = mm;
-                sm.co[k] = prescale(cc);
-                sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
+                sm.e[k].xy = mm;
+                sm.e[k].co = prescale(cc);
+                sm.e[k].rgb = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
                 h = strip_hits(mm, cc, tx0, ty0);
             }
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
@@ -206,9 +211,9 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
             while (m != 0ull) {
                 const int k = (c << 6) + __builtin_ctzll(m);
                 m &= m - 1ull;
-                const float2 mm = sm.xy[k];
-                const float4 co = sm.co[k];
-                const float4 cl = sm.rgb[k];
+                const float2 mm = sm.e[k].xy;
+                const float4 co = sm.e[k].co;
+                const float4 cl = sm.e[k].rgb;
                 const float dx = mm.x - fpx, dy = mm.y - fpy;
                 const float power = power_of(co, dx, dy);
                 float ep;
@@ -217,12 +222,14 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 // (without this entry) once T would fall below 1e-4
                 const bool use = !done && power <= 0.0f && alpha >= 1.0f / 255.0f;
                 const float test_T = T * (1.0f - alpha);
-                const bool stop = use && test_T < 0.0001f;
-                const bool take = use && !stop;
-                done = done || stop;
-                C0 = take ? C0 + cl.x * alpha * T : C0;
-                C1 = take ? C1 + cl.y * alpha * T : C1;
-                C2 = take ? C2 + cl.z * alpha * T : C2;
+                const bool room = test_T >= 0.0001f;  // T and alpha are finite here
+                const bool take = use && room;
+                done = done || (use && !room);
+                // colour += c alpha T, with the weight alpha T selected in or out
+                const float wt = take ? alpha * T : 0.0f;
+                C0 += cl.x * wt;
+                C1 += cl.y * wt;
+                C2 += cl.z * wt;
                 T = take ? test_T : T;
                 cnt += take ? 1 : 0;
                 last = take ? base + (uint32_t)k + 1u : last;
@@ -382,9 +389,9 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 sm.id[k] = id;
                 const float2 mm = a.means2D[id];
                 const float4 cc = make_float4(a.conics[id * 3], a.conics[id * 3 + 1], a.conics[id * 3 + 2], a.opac[id]);
-                sm.xy[k] = mm;
-                sm.co[k] = prescale(cc);
-                sm.rgb[k] = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
+                sm.e[k].xy = mm;
+                sm.e[k].co = prescale(cc);
+                sm.e[k].rgb = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
                 h = strip_hits(mm, cc, tx0, ty0);
             }
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
@@ -405,9 +412,9 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 // its contribution is selected in or out, through the factors
                 // below (visited entries are finite, so a zero factor zeroes
                 // every term)
-                const float2 m = sm.xy[k];
-                const float4 co = sm.co[k];
-                const float4 cl = sm.rgb[k];
+                const float2 m = sm.e[k].xy;
+                const float4 co = sm.e[k].co;
+                const float4 cl = sm.e[k].rgb;
                 const float dx = m.x - fpx, dy = m.y - fpy;
                 const float power = power_of(co, dx, dy);
                 float ep;
@@ -418,9 +425,7 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 // (rasterizer_backward_cpu.cpp:85), with the hardware
                 // reciprocal (1 ulp): gradients need 1e-3, not bits
                 const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
-                float dal = Ti * (cl.x - acc0) * dC0;
-                dal += Ti * (cl.y - acc1) * dC1;
-                dal += Ti * (cl.z - acc2) * dC2;
+                const float dal = Ti * ((cl.x - acc0) * dC0 + (cl.y - acc1) * dC1 + (cl.z - acc2) * dC2);
                 // dL/d(opacity * G) = dal * G; dL/dpower = opacity times that,
                 // and the opacity is applied once per Gaussian when the sums
                 // leave LDS (alpha >= 0.99 is the clamp: no gradient)
@@ -457,7 +462,7 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
             // (rasterizer_backward_cpu.cpp): d_mean = -(a Sx + b Sy, c Sy + b Sx),
             // d_conic = (-Sxx / 2, -Sxy, -Syy / 2); the staged conic is
             // (-a/2, -b, -c/2, o)
-            const float4 cs = sm.co[k];
+            const float4 cs = sm.e[k].co;
             const float ca = -2.0f * cs.x, cb = -cs.y, cc = -2.0f * cs.z, op = cs.w;
             const float sx = op * sm.g[k][0], sy = op * sm.g[k][1];
             float v;
