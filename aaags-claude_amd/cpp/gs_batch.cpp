@@ -315,6 +315,8 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
         a.N = b->N;
         a.C = b->bin_C;
         a.G = b->bin_G;
+        a.S = 0;
+        a.G1 = b->bin_G;
         a.sidx = b->didx_s.u();
         a.means2D = reinterpret_cast<const float2*>(pre.means2D);
         a.radii = pre.radii;

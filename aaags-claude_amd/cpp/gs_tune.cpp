@@ -137,7 +137,7 @@ extern "C" int gs_autotune(gs_trainer* t, const gs_camera* cam, const gs_render_
     // the trainer's whole forward with the knobs chosen above
     long long k3 = 0;
     auto render = [&]() { return gs_trainer_render(t, cam, cfg, &k3); };
-    TU_TRY(pick(c, tm, reps, {0, 2, 4, 8, 16, 32},
+    TU_TRY(pick(c, tm, reps, {0, 16, 32, 64, 128},
                 [&](int x) {
                     c->split_groups = x;
                     return GS_OK;

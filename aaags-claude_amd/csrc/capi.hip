@@ -482,8 +482,13 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_OVERFLOW);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t un = (size_t)n;
+    // with a split, the first `split` groups are small (the nearest
+    // Gaussians, placed before the first raster round: small groups give that
+    // placement more blocks), the rest have the configured size
     const int G = c->bin_chunk;
-    const int C = (n + G - 1) / G;
+    const int G1 = G < GS_SPLIT_GROUP ? G : GS_SPLIT_GROUP;
+    const int S = (split > 0 && (long long)split * G1 < n) ? split : 0;
+    const int C = S + (n - S * G1 + G - 1) / G;
     const size_t L = (size_t)C * num_tiles + 1;
     GS_CHECK_ARG(L < 0xFFFFFFFFull, "binning table too large");
     const size_t np = scan_partials_needed((uint32_t)L) + 1;
@@ -511,6 +516,8 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     a.N = n;
     a.C = C;
     a.G = G;
+    a.S = S;
+    a.G1 = G1;
     a.sidx = order;
     a.means2D = reinterpret_cast<const float2*>(pre->means2D);
     a.radii = pre->radii;
@@ -534,13 +541,13 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     bs->pending = false;
     const int bins = blocks / C;
     int place_blocks = blocks;
-    if (split > 0 && split < C) {
-        place_blocks = split * bins;
+    if (S > 0) {
+        place_blocks = S * bins;
         bs->pending = true;
         bs->tpl = c->bin_tpl;
         bs->bins = bins;
         bs->C = C;
-        bs->split = split;
+        bs->split = S;
         bs->gx = gx;
         bs->gy = gy;
     }

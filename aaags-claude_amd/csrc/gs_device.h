@@ -381,7 +381,8 @@ namespace gsb {
 // Direct-placement binning over V views of N Gaussians each (k_bin.hip).
 struct BinArgs {
     int V, N;
-    int C, G;               // groups per view, Gaussians per group
+    int C, G;               // groups per view; Gaussians per group after the first S
+    int S, G1;              // the first S groups hold G1 Gaussians each (0: none)
     const uint32_t* sidx;   // [V*N]: each view's Gaussian indices in depth order
     const float2* means2D;  // view-major [V][N]
     const int* radii;       // [V][N]

@@ -32,9 +32,11 @@ struct gs_ctx {
     int raster_batch = 256;
     int raster_bwd_batch = 256;
     int bin_chunk = 4096;   // Gaussians per group in direct-placement binning
+#define GS_SPLIT_GROUP 1024     // Gaussians per group of the part placed before the first raster round
     int bin_tpl = 1;        // tiles per lane (bin = 8 x 8*tpl tiles)
     int persistent_per_cu = 6;  // batched raster: persistent groups per CU (0: one group per tile)
-    int split_groups = 8;       // trainer forward: depth groups placed before the first raster round (0: one round)
+    int split_groups = 64;      // trainer forward: groups of GS_SPLIT_GROUP Gaussians placed before the
+                                // first raster round (0: one round)
     void* bin_split = nullptr;  // the split left pending by the last gs_bin_tiles_impl (BinSplit, capi.hip)
 };
 

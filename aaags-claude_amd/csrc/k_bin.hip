@@ -8,8 +8,9 @@
 //
 // Input is each view's Gaussians in depth order: a stable sort by the float
 // bits of depth, so equal depths keep index order, with culled Gaussians last.
-// That list is cut into groups of G Gaussians, and each view's tiles into bins
-// of 8 x (8 * TPL) tiles.  One single-wave block handles one (group, bin), and
+// That list is cut into groups (S groups of G1 Gaussians, then groups of G),
+// and each view's tiles into bins of 8 x (8 * TPL) tiles.  One single-wave
+// block handles one (group, bin), and
 // lane l owns the TPL tiles (l % 8, l / 8 + 8 k) of the bin: a lane's counters
 // and cursors live in its registers, so nothing is shared and no atomics are
 // needed, and the order in which the wave visits Gaussians is the order each
@@ -61,6 +62,20 @@ __global__ __launch_bounds__(256) void bin_rects_kernel(BinArgs a) {
     a.rects[r] = rec;
 }
 
+// Ranks [r0, r1) of group c in a view's depth order: S groups of G1 ranks,
+// then groups of G.
+__device__ __forceinline__ void group_ranks(const BinArgs& a, int c, uint32_t& r0, uint32_t& r1) {
+    uint32_t len;
+    if (c < a.S) {
+        r0 = (uint32_t)c * (uint32_t)a.G1;
+        len = (uint32_t)a.G1;
+    } else {
+        r0 = (uint32_t)a.S * (uint32_t)a.G1 + (uint32_t)(c - a.S) * (uint32_t)a.G;
+        len = (uint32_t)a.G;
+    }
+    r1 = (r0 + len < (uint32_t)a.N) ? r0 + len : (uint32_t)a.N;
+}
+
 // The (view, group, bin) of this block.  Blocks of one view are numbered
 // group-major, so the blocks of one XCD (after the remap) share a few groups
 // and their records stay in that XCD's L2.
@@ -99,8 +114,8 @@ constexpr int WALK_DEPTH = 4;
 template <int TPL, typename F>
 __device__ __forceinline__ void walk_bin(const BinArgs& a, const BinBlock<TPL>& b, int tx, int ty, int lane, F&& f) {
     const uint2* rects = a.rects + (size_t)b.v * a.N;
-    const uint32_t r0 = (uint32_t)b.c * (uint32_t)a.G;
-    const uint32_t r1 = (r0 + (uint32_t)a.G < (uint32_t)a.N) ? r0 + (uint32_t)a.G : (uint32_t)a.N;
+    uint32_t r0, r1;
+    group_ranks(a, b.c, r0, r1);
     const int bx1 = b.bx0 + 8, by1 = b.by0 + 8 * TPL;
     constexpr uint32_t STEP = 64u * WALK_DEPTH;
     uint2 nxt[WALK_DEPTH];
@@ -188,8 +203,8 @@ __global__ __launch_bounds__(256) void bin_count_diff_kernel(BinArgs a) {
     for (int i = tid; i < len; i += 256) dif[i] = 0;
     __syncthreads();
     const uint2* rects = a.rects + (size_t)v * a.N;
-    const uint32_t r0 = (uint32_t)c * (uint32_t)a.G;
-    const uint32_t r1 = (r0 + (uint32_t)a.G < (uint32_t)a.N) ? r0 + (uint32_t)a.G : (uint32_t)a.N;
+    uint32_t r0, r1;
+    group_ranks(a, c, r0, r1);
     for (uint32_t r = r0 + (uint32_t)tid; r < r1; r += 256u) {
         const uint32_t R = rects[r].x;
         const int x0 = (int)(R & 0xFFu), x1 = (int)((R >> 8) & 0xFFu);

@@ -389,8 +389,9 @@ typedef struct gs_tuning {
     int bin_tiles_per_lane; /* binning bin shape: 1, 2 or 4 (bin = 8 x 8*tpl tiles) */
     int persistent_per_cu;  /* batched raster: persistent groups per CU (0: one group per tile) */
     float binning_ms, raster_fwd_ms, raster_bwd_ms; /* best times measured */
-    int split_groups;       /* single-view forward: depth groups placed before the first raster round;
-                             * the rest is placed only for bins with unfinished tiles (0: one round) */
+    int split_groups;       /* single-view forward: groups of 1024 nearest Gaussians placed before the
+                             * first raster round; the rest is placed only for bins with unfinished
+                             * tiles (0: one round) */
     float forward_ms;       /* best forward (preprocess, binning, raster) measured with that split */
 } gs_tuning;
 

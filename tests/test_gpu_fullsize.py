@@ -70,9 +70,10 @@ def test_1m_16views_batched_render_equals_single_views(hip):
 
 
 def test_fullsize_two_round_forward_equals_one_round(hip):
-    """1M Gaussians, 1920x1080: the trainer's two-round forward (8 depth groups
-    first) and the one-round forward give the same image, T_final, contributor
-    counts and last contributors, bit for bit."""
+    """1M Gaussians, 1920x1080: the trainer's two-round forward (64 groups of
+    1024 depth-ordered Gaussians first) and the one-round forward give the
+    same image, T_final, contributor counts and last contributors, bit for
+    bit."""
     from gs_hip import GsTrainer
     from gs_scenes import RenderCfg, make_workload
     print("[case] full-size two-round forward", flush=True)
@@ -81,7 +82,7 @@ def test_fullsize_two_round_forward_equals_one_round(hip):
     cfg = RenderCfg(sh_degree=3, training=True)
     out = []
     try:
-        for sp in (0, 8):
+        for sp in (0, 64):
             hip.set_split(sp)
             tr = GsTrainer(hip, scene)
             tr.render(cam, cfg)
@@ -93,7 +94,7 @@ def test_fullsize_two_round_forward_equals_one_round(hip):
                             tr.buffer("last_contrib", (H, W), torch.int32)]))
             del tr
     finally:
-        hip.set_split(8)
+        hip.set_split(64)  # the context default
     assert out[0][0] == out[1][0]
     for a, b in zip(out[0][1], out[1][1]):
         assert torch.equal(a, b)

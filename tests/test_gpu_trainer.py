@@ -199,7 +199,7 @@ def test_trainer_two_round_forward_equals_one_round(hip, oracle, split):
         torch.cuda.synchronize()
     finally:
         hip.set_binning(4096, 1)
-        hip.set_split(8)
+        hip.set_split(64)  # the context default
     n = scene.n
     from gs_scenes import RawScene
     mid = RawScene(p1[0], p1[1], p1[2], p1[3].reshape(scene.sh.shape), p1[4], scene.sh_degree)
