@@ -328,21 +328,18 @@ static int tr_forward(gs_trainer* t, gs::ViewBufs& v, const gs_camera* cam, cons
     gs_pixel_state px{v.T_final.f(), v.n_contrib.i(), v.last.u()};
     TR_TRY(tr_mark(t, GS_STAGE_RASTER_FWD, true));
     gsr::RasterSplit sp;
-    int nbins = 0;
-    TR_TRY(gs_bin_split_info(c, &sp, &nbins));
+    TR_TRY(gs_bin_split_info(c, &sp, nullptr));
     if (t->pair_stride == 1 && sp.round == 1) {
         // two rounds: render what the binning placed (the nearest depth groups),
         // place the rest for the bins with unfinished tiles, resume those tiles
         const size_t tiles = (size_t)((cam->width + 15) / 16) * ((cam->height + 15) / 16);
-        TR_TRY(v.split_flags.ensure((tiles + (size_t)nbins) * 4));
+        TR_TRY(v.split_flags.ensure(tiles * 4));
         TR_TRY(v.split_rc.ensure((size_t)cam->width * cam->height * 12));
-        sp.tile_flag = v.split_flags.u();
-        sp.bin_flag = v.split_flags.u() + tiles;
+        sp.tile_flag = v.split_flags.u();  // written by round 1 for every tile
         sp.rc = v.split_rc.f();
-        TR_TRY(gs_zero(c, v.split_flags.p, (tiles + (size_t)nbins) * 4));
         TR_TRY(gs_raster_fwd_split(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(),
                                    cfg->bg_color, v.image.f(), &px, &sp));
-        TR_TRY(gs_bin_place_rest(c, sp.bin_flag));
+        TR_TRY(gs_bin_place_rest(c, sp.tile_flag));
         sp.round = 2;
         TR_TRY(gs_raster_fwd_split(c, cam->width, cam->height, &pre, t->keys_sorted.u(), v.ranges.u(),
                                    cfg->bg_color, v.image.f(), &px, &sp));

@@ -593,18 +593,16 @@ int gs_bin_split_info(gs_ctx* c, gsr::RasterSplit* sp, int* num_bins) {
     sp->scanned = bs->a.table;
     sp->C = bs->C;
     sp->g1 = bs->split;
-    sp->nbx = (bs->gx + 7) / 8;
-    sp->bin_h = 8 * bs->tpl;
     sp->round = 1;
     if (num_bins) *num_bins = bs->bins;
     return GS_OK;
 }
 
-int gs_bin_place_rest(gs_ctx* c, const uint32_t* bin_gate) {
+int gs_bin_place_rest(gs_ctx* c, const uint32_t* tile_gate) {
     GS_CHECK_ARG(c != nullptr, "null context");
     BinSplit* bs = static_cast<BinSplit*>(c->bin_split);
     GS_CHECK_ARG(bs && bs->pending, "no split binning pending");
-    GS_HIP(launch_bin_place(bs->a, (bs->C - bs->split) * bs->bins, bs->tpl, c->stream, bs->split * bs->bins, bin_gate),
+    GS_HIP(launch_bin_place(bs->a, (bs->C - bs->split) * bs->bins, bs->tpl, c->stream, bs->split * bs->bins, tile_gate),
            "binning place kernel (remaining groups)");
     bs->pending = false;
     return GS_OK;

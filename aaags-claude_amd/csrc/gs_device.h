@@ -324,16 +324,15 @@ struct RasterArgs {
 };
 // Forward raster in two rounds over a pair list placed in two parts (one
 // view): the binning placed only the first g1 depth groups of every tile's
-// list; round 1 renders each tile up to where group g1 begins, and a tile
-// that is not finished there raises its flag and its bin's flag and saves its
-// pixels' state; the rest of the list is then placed for flagged bins only,
-// and round 2 resumes the flagged tiles.  round 0: one pass (no split).
+// list; round 1 renders each tile up to where group g1 begins, writes every
+// tile's flag (1: not finished there) and saves the pixels' state of the
+// unfinished ones; the rest of the list is then placed for bins with a
+// flagged tile only, and round 2 resumes the flagged tiles.  round 0: one
+// pass (no split).
 struct RasterSplit {
     const uint32_t* scanned;  // the scanned binning table: slot of (tile, group)'s first pair
     int C, g1;                // groups per view; groups placed before round 1 (g1 < C)
-    int nbx, bin_h;           // bins per row; bin height in tiles (bins are 8 wide)
-    uint32_t* bin_flag;       // per bin: a tile of it is unfinished after round 1
-    uint32_t* tile_flag;      // per tile
+    uint32_t* tile_flag;      // per tile, written by round 1
     float* rc;                // per pixel: accumulated colour (3 floats) of an unfinished tile
     int round;                // 0, 1 or 2
 };

@@ -56,11 +56,11 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
                       hipEvent_t ev_total, int split = 0);
 // After gs_bin_tiles_impl with 0 < split < groups, which placed only the
 // first `split` depth groups: the description of that split for the forward
-// raster (bin_flag, tile_flag, rc and round left to the caller), or
-// sp->round = 0 when nothing is pending; then the placement of the remaining
-// groups for the bins whose gate word is nonzero.
+// raster (tile_flag and rc left to the caller), or sp->round = 0 when nothing
+// is pending; then the placement of the remaining groups for the bins with a
+// tile whose gate word is nonzero.
 int gs_bin_split_info(gs_ctx* c, gsr::RasterSplit* sp, int* num_bins);
-int gs_bin_place_rest(gs_ctx* c, const uint32_t* bin_gate);
+int gs_bin_place_rest(gs_ctx* c, const uint32_t* tile_gate);
 int gs_raster_fwd_split(gs_ctx* c, int width, int height, const gs_preprocessed* pre, const uint32_t* values,
                         const uint32_t* ranges, const float bg[3], float* image, const gs_pixel_state* px,
                         const gsr::RasterSplit* sp);

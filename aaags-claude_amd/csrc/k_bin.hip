@@ -256,7 +256,8 @@ __device__ __forceinline__ void flush_line_wave(const BinArgs& a, const uint32_t
 }
 
 // block0 / gate: the launch covers logical blocks block0 .. (a later range of
-// groups), and a block whose bin's gate word is zero does nothing (one view).
+// groups), and a block none of whose tiles has a nonzero gate word (per tile,
+// one view) does nothing.
 template <int TPL>
 __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a, int block0, const uint32_t* __restrict__ gate) {
     // per (tile slot, lane): one line of LINE slots plus one spare slot that
@@ -265,10 +266,18 @@ __global__ __launch_bounds__(64) void bin_place_kernel(BinArgs a, int block0, co
     constexpr int LSTRIDE = LINE + 1;
     __shared__ uint32_t lines[TPL * 64 * LSTRIDE];
     const BinBlock<TPL> b = bin_block<TPL>(a, block0);
-    if (gate && gate[b.bin] == 0u) return;  // one wave, no barrier follows
     const int lane = threadIdx.x;
     const int tx = b.bx0 + (lane & 7), ty = b.by0 + (lane >> 3);
     const int gx = b.vi.x, gy = b.vi.y;
+    if (gate) {
+        bool open = false;
+#pragma unroll
+        for (int k = 0; k < TPL; k++) {
+            const int y = ty + 8 * k;
+            open = open || (tx < gx && y < gy && gate[y * gx + tx] != 0u);
+        }
+        if (__ballot(open) == 0ull) return;  // one wave, no barrier follows
+    }
     const size_t tab = (size_t)(uint32_t)b.vi.z;
     uint32_t cur[TPL], beg[TPL];
 #pragma unroll

@@ -237,15 +237,13 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
         }
     }
     // round 1 stopped at the end of the placed part with pixels still open:
-    // flag the tile and its bin, keep the state (colour in rc, the done bit
-    // in n_contrib's top bit) for round 2
+    // flag the tile (every tile's flag is written, so nothing needs clearing
+    // beforehand) and keep the state (colour in rc, the done bit in
+    // n_contrib's top bit) for round 2
     bool keep = false;
-    if (sp.round == 1 && to < end) {
-        keep = __syncthreads_count(!done) != 0;
-        if (keep && t == 0) {
-            sp.tile_flag[tile] = 1u;
-            atomicOr(&sp.bin_flag[(ty / sp.bin_h) * sp.nbx + tx / 8], 1u);
-        }
+    if (sp.round == 1) {
+        if (to < end) keep = __syncthreads_count(!done) != 0;
+        if (t == 0) sp.tile_flag[tile] = keep ? 1u : 0u;
     }
     if (inside) {
         o.img[pix * 3 + 0] = C0 + T * a.bg0;
