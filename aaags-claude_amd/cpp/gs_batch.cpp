@@ -318,6 +318,7 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
         a.S = 0;
         a.G1 = b->bin_G;
         a.sidx = b->didx_s.u();
+        a.sidx_stride = 1;
         a.means2D = reinterpret_cast<const float2*>(pre.means2D);
         a.radii = pre.radii;
         a.views = static_cast<const int4*>(b->bviews_dev.p);

@@ -382,7 +382,8 @@ struct BinArgs {
     int V, N;
     int C, G;               // groups per view; Gaussians per group after the first S
     int S, G1;              // the first S groups hold G1 Gaussians each (0: none)
-    const uint32_t* sidx;   // [V*N]: each view's Gaussian indices in depth order
+    const uint32_t* sidx;   // [V*N]: each view's Gaussian indices in depth order, sidx_stride apart
+    int sidx_stride;        // 1: an index array; 2: the high words of sorted index << 32 | key words
     const float2* means2D;  // view-major [V][N]
     const int* radii;       // [V][N]
     const int4* views;      // per view: (tiles per row, tile rows, table offset, first global tile)

@@ -90,6 +90,7 @@ hipError_t launch_tile_ranges(const uint64_t* keys_sorted, uint32_t n, uint32_t 
                               hipStream_t s, const uint32_t* n_dev = nullptr);
 hipError_t launch_depth_keys(int n, const float* depths, const int* radii, uint64_t* keys, uint32_t* idx,
                              hipStream_t s);
+hipError_t launch_depth_words(int n, const float* depths, const int* radii, uint64_t* words, hipStream_t s);
 hipError_t launch_gather_tiles(int n, const uint32_t* order, const int* tiles, int* out, hipStream_t s);
 hipError_t launch_duplicate_ranked(int n, const uint32_t* order, const float* means2D, const int* radii,
                                    const uint32_t* offsets, int gx, int gy, uint64_t* words, uint32_t capacity,

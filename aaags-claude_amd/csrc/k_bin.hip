@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void bin_rects_kernel(BinArgs a) {
     const int v = (int)(r / (size_t)a.N);
     const int4 vi = a.views ? a.views[v] : a.view0;
     const size_t seg = (size_t)v * a.N;
-    const uint32_t id = a.sidx[r];
+    const uint32_t id = a.sidx[r * (size_t)a.sidx_stride];
     const int rad = a.radii[seg + id];
     uint2 rec = make_uint2(0u, id);
     if (rad > 0) {

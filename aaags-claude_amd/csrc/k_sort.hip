@@ -235,6 +235,17 @@ __global__ __launch_bounds__(256) void depth_keys_kernel(int n, const float* __r
     idx[i] = (uint32_t)i;
 }
 
+// The same keys with the index in the high word (index << 32 | key): sorted
+// on bits [0, 32) as self-contained words, the index travels with its key
+// (no separate value array through the passes).
+__global__ __launch_bounds__(256) void depth_words_kernel(int n, const float* __restrict__ depths,
+                                                          const int* __restrict__ radii, uint64_t* __restrict__ words) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t key = radii[i] > 0 ? __float_as_uint(depths[i]) : 0xFFFFFFFFu;
+    words[i] = ((uint64_t)(uint32_t)i << 32) | key;
+}
+
 __global__ __launch_bounds__(256) void gather_tiles_kernel(int n, const uint32_t* __restrict__ order,
                                                            const int* __restrict__ tiles, int* __restrict__ out) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -355,7 +366,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
 template <int BITS, int ROUNDS>
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
                                                                 const uint32_t* __restrict__ n_dev, int shift,
-                                                                uint32_t nb, uint32_t* __restrict__ hist) {
+                                                                uint32_t hs, uint32_t* __restrict__ hist) {
     constexpr int RADIX = 1 << BITS;
     constexpr int RS_TILE = RS_THREADS * ROUNDS, RS_WAVE_SPAN = ROUNDS * 64;
     const uint32_t n = live_count(n_cap, n_dev);
@@ -378,34 +389,37 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* 
         uint32_t s = 0;
 #pragma unroll
         for (int k = 0; k < RS_WAVES; k++) s += s_h[k][d];
-        hist[(size_t)d * nb + blockIdx.x] = s;
+        hist[(size_t)d * hs + blockIdx.x] = s;
     }
 }
 
 // One wave per digit: exclusive scan of the digit's row of the digit-major
-// count table (its count in every block, in block order) and the digit's
-// total.  The downsweep adds the exclusive scan of the digit totals, so a pass
-// needs no separate three-launch scan of the whole table.
+// count table (its count in every block, in block order; rows hs >= nb
+// entries apart, hs a multiple of 4) and the digit's total.  Each lane takes
+// four consecutive counts as one 16-byte load, so one wave-wide scan covers
+// 256 blocks.  The downsweep adds the exclusive scan of the digit totals, so
+// a pass needs no separate three-launch scan of the whole table.
 __global__ __launch_bounds__(64) void rs_digit_scan_kernel(const uint32_t* __restrict__ hist, uint32_t nb,
-                                                           uint32_t* __restrict__ goff, uint32_t* __restrict__ dtot) {
-    const uint32_t* h = hist + (size_t)blockIdx.x * nb;
-    uint32_t* o = goff + (size_t)blockIdx.x * nb;
+                                                           uint32_t hs, uint32_t* __restrict__ goff,
+                                                           uint32_t* __restrict__ dtot) {
+    const uint4* h = reinterpret_cast<const uint4*>(hist + (size_t)blockIdx.x * hs);
+    uint4* o = reinterpret_cast<uint4*>(goff + (size_t)blockIdx.x * hs);
+    const uint32_t nq = hs / 4u;
     const int l = lane_id();
     uint32_t carry = 0u;
-    for (uint32_t base = 0; base < nb; base += 256u) {
-        uint32_t v[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t i = base + 64u * k + (uint32_t)l;
-            v[k] = i < nb ? h[i] : 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t i = base + 64u * k + (uint32_t)l;
-            const uint32_t inc = wave_inclusive_scan(v[k]);
-            if (i < nb) o[i] = carry + inc - v[k];
-            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
-        }
+    for (uint32_t q0 = 0; q0 < nq; q0 += 64u) {
+        const uint32_t q = q0 + (uint32_t)l;
+        uint4 x = q < nq ? h[q] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t i0 = 4u * q;  // counts past the last block are padding
+        if (i0 >= nb) x.x = 0u;
+        if (i0 + 1u >= nb) x.y = 0u;
+        if (i0 + 2u >= nb) x.z = 0u;
+        if (i0 + 3u >= nb) x.w = 0u;
+        const uint32_t tot = x.x + x.y + x.z + x.w;
+        const uint32_t inc = wave_inclusive_scan(tot);
+        const uint32_t ex = carry + inc - tot;
+        if (q < nq) o[q] = make_uint4(ex, ex + x.x, ex + x.x + x.y, ex + x.x + x.y + x.z);
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
     }
     if (l == 0) dtot[blockIdx.x] = carry;
 }
@@ -415,7 +429,7 @@ template <int BITS, bool VALS, int ROUNDS>
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t* __restrict__ keys_in,
                                                                   const uint32_t* __restrict__ vals_in, uint32_t n_cap,
                                                                   const uint32_t* __restrict__ n_dev, int shift,
-                                                                  uint32_t nb, const uint32_t* __restrict__ goff,
+                                                                  uint32_t hs, const uint32_t* __restrict__ goff,
                                                                   const uint32_t* __restrict__ dtot,
                                                                   uint64_t* __restrict__ keys_out,
                                                                   uint32_t* __restrict__ vals_out) {
@@ -439,7 +453,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
         const uint32_t t = threadIdx.x < RADIX ? dtot[threadIdx.x] : 0u;
         uint32_t total;
         const uint32_t ex = block_exclusive_scan<uint32_t>(t, s_scan, &total);
-        if (threadIdx.x < RADIX) s_goff[threadIdx.x] = ex + goff[(size_t)threadIdx.x * nb + blockIdx.x];
+        if (threadIdx.x < RADIX) s_goff[threadIdx.x] = ex + goff[(size_t)threadIdx.x * hs + blockIdx.x];
     }
     __syncthreads();
 
@@ -583,8 +597,12 @@ uint32_t radix_blocks(uint32_t n) {
 
 // Workspace layout for one sort of n items with the given radix:
 //   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][digit totals R*4]
+// Row stride of the digit-major count table: the block count rounded up to a
+// multiple of 4 (16-byte rows for the digit scan).
+static uint32_t radix_row_stride(uint32_t nb) { return (nb + 3u) & ~3u; }
+
 size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
-    size_t nb = radix_blocks(n);
+    size_t nb = radix_row_stride(radix_blocks(n));
     size_t R = (size_t)1 << radix_bits;
     size_t tab = R * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -595,15 +613,15 @@ template <int BITS, int ROUNDS>
 static hipError_t sort_pass_r(const uint64_t* ki, const uint32_t* vi, uint64_t* ko, uint32_t* vo, uint32_t n,
                               const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff,
                               uint32_t* dtot, hipStream_t s) {
-    uint32_t nb = radix_blocks(n);
-    rs_upsweep_kernel<BITS, ROUNDS><<<nb, RS_THREADS, 0, s>>>(ki, n, n_dev, shift, nb, hist);
-    rs_digit_scan_kernel<<<1u << BITS, 64, 0, s>>>(hist, nb, goff, dtot);
+    const uint32_t nb = radix_blocks(n), hs = radix_row_stride(nb);
+    rs_upsweep_kernel<BITS, ROUNDS><<<nb, RS_THREADS, 0, s>>>(ki, n, n_dev, shift, hs, hist);
+    rs_digit_scan_kernel<<<1u << BITS, 64, 0, s>>>(hist, nb, hs, goff, dtot);
     if (vi)
         rs_downsweep_kernel<BITS, true, ROUNDS>
-            <<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, nb, goff, dtot, ko, vo);
+            <<<nb, RS_THREADS, 0, s>>>(ki, vi, n, n_dev, shift, hs, goff, dtot, ko, vo);
     else
         rs_downsweep_kernel<BITS, false, ROUNDS>
-            <<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, nb, goff, dtot, ko, nullptr);
+            <<<nb, RS_THREADS, 0, s>>>(ki, nullptr, n, n_dev, shift, hs, goff, dtot, ko, nullptr);
     return hipGetLastError();
 }
 
@@ -621,7 +639,7 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
                                   uint32_t* vals_out, uint32_t n, int begin_bit, int end_bit, int radix_bits,
                                   void* workspace, hipStream_t s, const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
-    size_t nb = radix_blocks(n);
+    size_t nb = radix_row_stride(radix_blocks(n));
     size_t tab = ((size_t)1 << radix_bits) * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     char* w = static_cast<char*>(workspace);
@@ -677,6 +695,12 @@ hipError_t launch_depth_keys(int n, const float* depths, const int* radii, uint6
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     depth_keys_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, depths, radii, keys, idx);
+    return hipGetLastError();
+}
+
+hipError_t launch_depth_words(int n, const float* depths, const int* radii, uint64_t* words, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    depth_words_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, depths, radii, words);
     return hipGetLastError();
 }
 
