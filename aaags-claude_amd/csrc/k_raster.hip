@@ -25,10 +25,12 @@ namespace gsr {
 
 using gsd::smin;
 
-// Wave w of a tile (threads 64w .. 64w+63) covers pixel rows 4w .. 4w+3, a
-// 16 x 4 strip.  Per LDS batch, mask[w][c] bit b is set when entry 64c + b can
-// pass the alpha cut somewhere in strip w (see strip_hits); each wave visits
-// only its set bits.
+// Wave w of a tile (threads 64w .. 64w+63) covers one block of its pixels:
+// in the forward the 8 x 8 quadrant (w & 1, w >> 1), in the backward the
+// 16 x 4 strip of rows 4w .. 4w+3 (measured: each layout is the faster one
+// for its kernel).  Per LDS batch, mask[w][c] bit b is set when entry 64c + b
+// can pass the alpha cut somewhere in block w (see strip_hits); each wave
+// visits only its set bits.
 // One staged entry: 48 bytes, read with one base address and immediate
 // offsets (mean, conic + opacity, colour).
 struct Staged {
@@ -44,9 +46,10 @@ struct FwdSmem {
     uint64_t mask[4][BATCH / 64];
 };
 
-// Strip-hit bits of one staged entry: bit w set when the Gaussian can pass the
-// 1/255 alpha cut at some pixel centre of strip w (rows y0 + 4w .. y0 + 4w + 3,
-// columns x0 .. x0 + 15) of the tile whose top-left pixel is (x0, y0).
+// Block-hit bits of one staged entry: bit w set when the Gaussian can pass the
+// 1/255 alpha cut at some pixel centre of block w of the tile whose top-left
+// pixel is (x0, y0): QUAD, the 8 x 8 quadrant (w & 1, w >> 1); else the strip
+// of rows y0 + 4w .. y0 + 4w + 3, columns x0 .. x0 + 15.
 // alpha = o * exp(-Q/2) with Q = a u^2 + 2 b u v + c v^2 ((a, b, c) the conic,
 // (u, v) the pixel minus the mean), so passing needs Q <= r2 = 2 ln(255 o).
 // The least Q over a rectangle is 0 when the mean lies inside it, else the
@@ -67,6 +70,7 @@ __device__ __forceinline__ float quad_form(float a, float b, float c, float u, f
     return (a * u * u + c * v * v) + 2.0f * b * u * v;
 }
 
+template <bool QUAD>
 __device__ __forceinline__ uint32_t strip_hits(float2 m, float4 co, float x0, float y0) {
     const float o = co.w;
     if (!(o >= 1.0f / 255.0f)) return 0u;
@@ -80,16 +84,18 @@ __device__ __forceinline__ uint32_t strip_hits(float2 m, float4 co, float x0, fl
     // 2 ln(x) = 2 ln(2) log2(x)
     const float r2 = 1.38629436f * __builtin_amdgcn_logf(255.0f * o) * 1.001f + 1e-3f;
     constexpr float G = 0.5f;
-    const float u0 = x0 - G - m.x, u1 = x0 + (15.0f + G) - m.x;
+    constexpr float BW = QUAD ? 7.0f : 15.0f, BH = QUAD ? 7.0f : 3.0f;  // block extent - 1
     // edge vertices: u* = -(b / a) v on the rows, v* = -(b / c) u on the columns
     const float ka = -b * __builtin_amdgcn_rcpf(a), kc = -b * __builtin_amdgcn_rcpf(c);
-    const float vl = kc * u0, vr = kc * u1;
-    const bool in_x = u0 <= 0.0f && u1 >= 0.0f;
     uint32_t h = 0u;
 #pragma unroll
     for (int w = 0; w < 4; w++) {
-        const float v0 = y0 + (float)(4 * w) - G - m.y, v1 = y0 + (float)(4 * w) + (3.0f + G) - m.y;
-        float q = (in_x && v0 <= 0.0f && v1 >= 0.0f) ? 0.0f : 3.0e38f;
+        const float bx = x0 + (QUAD ? (float)(8 * (w & 1)) : 0.0f);
+        const float by = y0 + (QUAD ? (float)(8 * (w >> 1)) : (float)(4 * w));
+        const float u0 = bx - G - m.x, u1 = bx + (BW + G) - m.x;
+        const float vl = kc * u0, vr = kc * u1;
+        const float v0 = by - G - m.y, v1 = by + (BH + G) - m.y;
+        float q = (u0 <= 0.0f && u1 >= 0.0f && v0 <= 0.0f && v1 >= 0.0f) ? 0.0f : 3.0e38f;
         q = fminf(q, quad_form(a, b, c, fminf(fmaxf(ka * v0, u0), u1), v0));
         q = fminf(q, quad_form(a, b, c, fminf(fmaxf(ka * v1, u0), u1), v1));
         q = fminf(q, quad_form(a, b, c, u0, fminf(fmaxf(vl, v0), v1)));
@@ -150,7 +156,8 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                                          const RasterSplit& sp) {
     const int tx = tile % a.gx, ty = tile / a.gx;
     const int t = threadIdx.x;
-    const int px = tx * gsd::TILE + (t & 15), py = ty * gsd::TILE + (t >> 4);
+    // 8 x 8 quadrant (w & 1, w >> 1) per wave
+    const int px = tx * gsd::TILE + 8 * ((t >> 6) & 1) + (t & 7), py = ty * gsd::TILE + 8 * (t >> 7) + ((t >> 3) & 7);
     const bool inside = px < a.W && py < a.H;
     const int pix = py * a.W + px;
     const uint32_t start = a.ranges[2 * tile], end = a.ranges[2 * tile + 1];
@@ -196,7 +203,7 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 sm.e[k].xy = mm;
                 sm.e[k].co = prescale(cc);
                 sm.e[k].rgb = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
-                h = strip_hits(mm, cc, tx0, ty0);
+                h = strip_hits<true>(mm, cc, tx0, ty0);
             }
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
         }
@@ -390,7 +397,7 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 sm.e[k].xy = mm;
                 sm.e[k].co = prescale(cc);
                 sm.e[k].rgb = make_float4(a.rgb[id * 3], a.rgb[id * 3 + 1], a.rgb[id * 3 + 2], 0.0f);
-                h = strip_hits(mm, cc, tx0, ty0);
+                h = strip_hits<false>(mm, cc, tx0, ty0);
             }
             store_masks<BATCH>(sm.mask, k >> 6, h, lane);
 #pragma unroll

@@ -1,9 +1,9 @@
-"""The rasterizers' strip test (k_raster.hip, strip_hits) is conservative.
+"""The rasterizers' block test (k_raster.hip, strip_hits) is conservative.
 
-For a staged Gaussian and one 16 x 4 pixel strip of a tile, the test clears
-the strip's bit only when the minimum of Q = a u^2 + 2 b u v + c v^2 over
-the strip's rectangle (grown by half a pixel) exceeds r2 = 2 ln(255 o),
-widened.  A cleared bit must mean that at every pixel centre of the strip
+For a staged Gaussian and one pixel block of a tile (a 16 x 4 strip in the
+backward, an 8 x 8 quadrant in the forward), the test clears the block's bit
+only when the minimum of Q = a u^2 + 2 b u v + c v^2 over the block's
+rectangle (grown by half a pixel) exceeds r2 = 2 ln(255 o), widened.  A cleared bit must mean that at every pixel centre of the strip
 the alpha the reference rasterizer computes (rasterizer_cpu.cpp:159-169,
 float32 without contraction) is below 1/255, so skipping the Gaussian there
 changes no result.
@@ -16,6 +16,7 @@ the tiles are placed so that strips straddle the ellipse's edge.  Three
 plausible mistakes must be caught: the factor 2 missing from the cross term,
 the two edge slopes swapped, and r2 = ln(255 o) instead of 2 ln(255 o)."""
 import numpy as np
+import pytest
 
 f32 = np.float32
 
@@ -47,13 +48,18 @@ def _quad(a, b, c, u, v, cross=2.0):
     return ((t1.astype(f32) + t2.astype(f32)).astype(f32) + t3.astype(f32)).astype(f32)
 
 
-def _strip_bits(a, b, c, o, mx, my, x0, y0, rng, cross=2.0, swap=False, r2_factor=2.0):
+def _blocks(quad):
+    """(x offset, y offset, width - 1, height - 1) of the four blocks."""
+    if quad:
+        return [(8 * (w & 1), 8 * (w >> 1), 7, 7) for w in range(4)]
+    return [(0, 4 * w, 15, 3) for w in range(4)]
+
+
+def _strip_bits(a, b, c, o, mx, my, x0, y0, rng, quad=False, cross=2.0, swap=False, r2_factor=2.0):
     """strip_hits in float32, with log2 pessimised and rcp perturbed."""
     ln2x = (np.log2(255.0 * o.astype(np.float64)) * (1 - 2e-6)).astype(f32)
     r2 = ((f32(r2_factor * np.log(2)) * ln2x).astype(f32) * f32(1.001) + f32(1e-3)).astype(f32)
     G = f32(0.5)
-    u0 = ((x0 - G).astype(f32) - mx).astype(f32)
-    u1 = ((x0 + f32(15.5)).astype(f32) - mx).astype(f32)
 
     def rcp(x):
         return (1.0 / x.astype(np.float64) * (1 + rng.choice([-3e-7, 3e-7], len(x)))).astype(f32)
@@ -62,14 +68,16 @@ def _strip_bits(a, b, c, o, mx, my, x0, y0, rng, cross=2.0, swap=False, r2_facto
     kc = (-b * rcp(c)).astype(f32)
     if swap:
         ka, kc = kc, ka
-    vl, vr = (kc * u0).astype(f32), (kc * u1).astype(f32)
-    in_x = (u0 <= 0) & (u1 >= 0)
     bits = np.zeros((len(a), 4), bool)
-    for w in range(4):
-        base = (y0 + f32(4 * w)).astype(f32)
-        v0 = ((base - G).astype(f32) - my).astype(f32)
-        v1 = ((base + f32(3.5)).astype(f32) - my).astype(f32)
-        q = np.where(in_x & (v0 <= 0) & (v1 >= 0), f32(0), f32(3.0e38))
+    for w, (ox, oy, bw, bh) in enumerate(_blocks(quad)):
+        bx = (x0 + f32(ox)).astype(f32)
+        by = (y0 + f32(oy)).astype(f32)
+        u0 = ((bx - G).astype(f32) - mx).astype(f32)
+        u1 = ((bx + f32(bw + 0.5)).astype(f32) - mx).astype(f32)
+        vl, vr = (kc * u0).astype(f32), (kc * u1).astype(f32)
+        v0 = ((by - G).astype(f32) - my).astype(f32)
+        v1 = ((by + f32(bh + 0.5)).astype(f32) - my).astype(f32)
+        q = np.where((u0 <= 0) & (u1 >= 0) & (v0 <= 0) & (v1 >= 0), f32(0), f32(3.0e38))
         q = np.minimum(q, _quad(a, b, c, np.minimum(np.maximum((ka * v0).astype(f32), u0), u1), v0, cross))
         q = np.minimum(q, _quad(a, b, c, np.minimum(np.maximum((ka * v1).astype(f32), u0), u1), v1, cross))
         q = np.minimum(q, _quad(a, b, c, u0, np.minimum(np.maximum(vl, v0), v1), cross))
@@ -78,14 +86,15 @@ def _strip_bits(a, b, c, o, mx, my, x0, y0, rng, cross=2.0, swap=False, r2_facto
     return bits
 
 
-def _passing_in_cleared(a, b, c, o, mx, my, x0, y0, bits):
-    """Pixel centres of cleared strips where the reference alpha passes the cut."""
+def _passing_in_cleared(a, b, c, o, mx, my, x0, y0, bits, quad=False):
+    """Pixel centres of cleared blocks where the reference alpha passes the cut."""
     bad = 0
     A, B, C, O = a[:, None], b[:, None], c[:, None], o[:, None]
-    for w in range(4):
+    for w, (ox, oy, bw, bh) in enumerate(_blocks(quad)):
         cleared = ~bits[:, w]
-        px = (x0[:, None] + np.arange(16, dtype=f32)[None, :].repeat(4, 0).reshape(1, 64)).astype(f32)
-        py = (y0[:, None] + f32(4 * w) + np.repeat(np.arange(4, dtype=f32), 16)[None, :]).astype(f32)
+        W, H = bw + 1, bh + 1
+        px = (x0[:, None] + f32(ox) + np.tile(np.arange(W, dtype=f32), H)[None, :]).astype(f32)
+        py = (y0[:, None] + f32(oy) + np.repeat(np.arange(H, dtype=f32), W)[None, :]).astype(f32)
         dx = (mx[:, None] - px).astype(f32)
         dy = (my[:, None] - py).astype(f32)
         quad = ((A * dx * dx).astype(f32) + (C * dy * dy).astype(f32)).astype(f32)
@@ -96,7 +105,8 @@ def _passing_in_cleared(a, b, c, o, mx, my, x0, y0, bits):
     return bad
 
 
-def test_strip_test_is_conservative():
+@pytest.mark.parametrize("quad", [False, True])
+def test_strip_test_is_conservative(quad):
     rng = np.random.default_rng(12)
     a, b, c, o, ext = _conics(rng, 40000)
     assert len(a) > 35000
@@ -107,11 +117,11 @@ def test_strip_test_is_conservative():
     reach = (2.5 * ext + 12).astype(np.float64)
     mx = (x0 + 8 + rng.uniform(-1, 1, len(a)) * reach).astype(f32)
     my = (y0 + 8 + rng.uniform(-1, 1, len(a)) * reach).astype(f32)
-    bits = _strip_bits(a, b, c, o, mx, my, x0, y0, rng)
+    bits = _strip_bits(a, b, c, o, mx, my, x0, y0, rng, quad=quad)
     cleared = int((~bits).sum())
     assert cleared > 20000 and bits.sum() > 20000  # both outcomes well represented
-    assert _passing_in_cleared(a, b, c, o, mx, my, x0, y0, bits) == 0
+    assert _passing_in_cleared(a, b, c, o, mx, my, x0, y0, bits, quad) == 0
     # the same check fails for plausible mistakes
     for kw in ({"cross": 1.0}, {"swap": True}, {"r2_factor": 1.0}):
-        wrong = _strip_bits(a, b, c, o, mx, my, x0, y0, rng, **kw)
-        assert _passing_in_cleared(a, b, c, o, mx, my, x0, y0, wrong) > 0, kw
+        wrong = _strip_bits(a, b, c, o, mx, my, x0, y0, rng, quad=quad, **kw)
+        assert _passing_in_cleared(a, b, c, o, mx, my, x0, y0, wrong, quad) > 0, kw
