@@ -284,7 +284,11 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
     constexpr int NB = (DEG + 1) * (DEG + 1);
     constexpr int E = MAXC * 3;  // SH floats per Gaussian
     constexpr int ROW = E + 1;   // padded LDS row
-    __shared__ float s_sh[FUSED_ADAM ? 4 * 64 * ROW : 1];
+    // per wave: 64 parameter rows (the SH read once, by the owner thread),
+    // then 32 gradient rows for the half of the wave's Gaussians being swept
+    constexpr int WROWS = 96;
+    constexpr bool KEEP_P = FUSED_ADAM && (NB == MAXC);
+    __shared__ float s_sh[FUSED_ADAM ? 4 * WROWS * ROW : 1];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = G.count;
     const bool valid = i < N;
@@ -296,6 +300,11 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
         load_params(G, i, true, P);
         float sh[NB * 3];
         load_sh<MAXC, NB>(G.sh_coeffs, i, sh);
+        if constexpr (KEEP_P) {
+            float* prow = s_sh + ((threadIdx.x >> 6) * WROWS + (threadIdx.x & 63)) * ROW;
+#pragma unroll
+            for (int k = 0; k < E; k++) prow[k] = sh[k];
+        }
         float tpos[3] = {0.0f, 0.0f, 0.0f}, tsc[3] = {0.0f, 0.0f, 0.0f}, trot[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         float top = 0.0f;
         // one view's gradients, summed into the totals
@@ -341,9 +350,62 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
             out.d_opacities[i] = top;
         }
     }
-    if constexpr (FUSED_ADAM) {
+    if constexpr (KEEP_P) {
+        // two sweeps of 32 whole rows each (contiguous in memory): the
+        // parameters come from the rows the owners wrote, the gradients from
+        // the 32 gradient rows, which each half's owners fill in turn
         const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        float* row = s_sh + (w * 64 + lane) * ROW;
+        float* wrows = s_sh + w * WROWS * ROW;
+        const int g0 = blockIdx.x * blockDim.x + w * 64;  // first Gaussian of this wave
+        const gso::AdamDev h = ad.hp ? *ad.hp : ad.h0;
+#pragma unroll 1
+        for (int half = 0; half < 2; half++) {
+            if ((lane >> 5) == half) {
+                float* grow = wrows + (64 + (lane & 31)) * ROW;
+#pragma unroll
+                for (int k = 0; k < E; k++) grow[k] = tsh[k];
+            }
+            __syncthreads();
+            const int gh = g0 + 32 * half;
+            const int cnt = N - gh < 32 ? N - gh : 32;
+            if (cnt > 0) {
+                float* __restrict__ p = ad.p[3] + (size_t)gh * E;
+                float* __restrict__ m = ad.m[3] + (size_t)gh * E;
+                float* __restrict__ v = ad.v[3] + (size_t)gh * E;
+                const float* prows = wrows + 32 * half * ROW;
+                const float* grows = wrows + 64 * ROW;
+                const int total = cnt * E;
+                constexpr int U = 8;  // elements per lane in flight
+                for (int j0 = lane; j0 < total; j0 += 64 * U) {
+                    float pp[U], mm[U], vv[U], gg[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int j = j0 + 64 * u;
+                        if (j < total) {
+                            const int gl = j / E, k = j - gl * E;
+                            pp[u] = prows[gl * ROW + k];
+                            gg[u] = grows[gl * ROW + k];
+                            mm[u] = m[j];
+                            vv[u] = v[j];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const int j = j0 + 64 * u;
+                        if (j < total) {
+                            pp[u] = adam_math(pp[u], &mm[u], &vv[u], gg[u], h.lr[3], h);
+                            p[j] = pp[u];
+                            m[j] = mm[u];
+                            v[j] = vv[u];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    } else if constexpr (FUSED_ADAM) {
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        float* row = s_sh + (w * WROWS + lane) * ROW;
 #pragma unroll
         for (int k = 0; k < E; k++) row[k] = tsh[k];
         __syncthreads();
@@ -354,7 +416,7 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
             float* __restrict__ p = ad.p[3] + (size_t)g0 * E;
             float* __restrict__ m = ad.m[3] + (size_t)g0 * E;
             float* __restrict__ v = ad.v[3] + (size_t)g0 * E;
-            const float* wsh = s_sh + w * 64 * ROW;
+            const float* wsh = s_sh + w * WROWS * ROW;
             const int total = cnt * E;
             constexpr int U = 8;  // elements per lane in flight
             for (int j0 = lane; j0 < total; j0 += 64 * U) {
