@@ -33,6 +33,10 @@ class _Pre(ctypes.Structure):
                 ("radii", _vp), ("tiles_touched", _vp)]
 
 
+class _Pre3D(ctypes.Structure):
+    _fields_ = [("gauss2screen", _vp), ("cov3D_inv", _vp), ("mean_offset", _vp)]
+
+
 class _RGrads(ctypes.Structure):
     _fields_ = [("d_means2D", _vp), ("d_conics", _vp), ("d_rgb", _vp), ("d_opacities_2d", _vp)]
 
@@ -323,6 +327,65 @@ class GsHip:
         self._check(self.lib.gs_raster_fwd(self.ctx, width, height, ctypes.byref(ps), _vp(ptr(vs)), _vp(ptr(ranges)),
                                            bgv, _vp(ptr(img)), ctypes.byref(st)), "gs_raster_fwd")
         return img, px
+
+    # ------------------------------------------------ eval_3D forward --
+    def alloc_pre3d(self, n):
+        d = self.dev
+        return dict(g2s=torch.empty((n, 16), device=d), cov3D_inv=torch.empty((n, 6), device=d),
+                    mean_offset=torch.empty((n, 3), device=d))
+
+    @staticmethod
+    def _pre3d_struct(p3) -> _Pre3D:
+        return _Pre3D(ptr(p3["g2s"]), ptr(p3["cov3D_inv"]), ptr(p3["mean_offset"]))
+
+    def preprocess_3d(self, p, cam, cfg, raw=True, filter3d=None, pre=None, pre3d=None):
+        pre = pre if pre is not None else self.alloc_pre(p["n"])
+        pre3d = pre3d if pre3d is not None else self.alloc_pre3d(p["n"])
+        g = self._gauss(p, raw)
+        c, f = cam.to_c(), cfg.to_c()
+        ps, p3 = self._pre_struct(pre), self._pre3d_struct(pre3d)
+        self._check(self.lib.gs_preprocess_fwd_3d(self.ctx, ctypes.byref(g), _vp(ptr(filter3d)), ctypes.byref(c),
+                                                  ctypes.byref(f), ctypes.byref(ps), ctypes.byref(p3)),
+                    "gs_preprocess_fwd_3d")
+        return pre, pre3d
+
+    def duplicate_3d(self, pre, pre3d, offsets, K, cam, capacity=None):
+        n = pre["radii"].numel()
+        cap = K if capacity is None else capacity
+        keys = torch.empty(max(cap, 1), dtype=torch.int64, device=self.dev)
+        vals = torch.empty(max(cap, 1), dtype=torch.int32, device=self.dev)
+        ps, p3, c = self._pre_struct(pre), self._pre3d_struct(pre3d), cam.to_c()
+        self._check(self.lib.gs_duplicate_keys_3d(self.ctx, n, ctypes.byref(ps), ctypes.byref(p3), _vp(ptr(offsets)),
+                                                  ctypes.byref(c), 16, 16, _vp(ptr(keys)), _vp(ptr(vals)),
+                                                  ctypes.c_uint32(cap)), "gs_duplicate_keys_3d")
+        return keys[:K], vals[:K]
+
+    def raster_fwd_3d(self, pre, pre3d, values_sorted, ranges, cam, bg=(0.0, 0.0, 0.0), img=None, T_final=None,
+                      n_contrib=None):
+        d = self.dev
+        H, W = cam.height, cam.width
+        img = img if img is not None else torch.empty((H, W, 3), device=d)
+        T_final = T_final if T_final is not None else torch.empty((H, W), device=d)
+        n_contrib = n_contrib if n_contrib is not None else torch.empty((H, W), dtype=torch.int32, device=d)
+        ps, p3, c = self._pre_struct(pre), self._pre3d_struct(pre3d), cam.to_c()
+        bgv = (ctypes.c_float * 3)(*bg)
+        vs = values_sorted if values_sorted.numel() else torch.zeros(1, dtype=torch.int32, device=d)
+        self._check(self.lib.gs_raster_fwd_3d(self.ctx, ctypes.byref(c), ctypes.byref(ps), ctypes.byref(p3),
+                                              _vp(ptr(vs)), _vp(ptr(ranges)), bgv, _vp(ptr(img)), _vp(ptr(T_final)),
+                                              _vp(ptr(n_contrib))), "gs_raster_fwd_3d")
+        return img, T_final, n_contrib
+
+    def render_3d(self, p, cam, cfg, raw=True, filter3d=None):
+        """The eval_3D forward stage by stage: preprocess, scan (K read back),
+        per-tile depth keys, stable pair sort, k-buffer raster."""
+        pre, pre3d = self.preprocess_3d(p, cam, cfg, raw, filter3d)
+        offsets, K = self.scan_tiles(pre["tiles"])
+        keys, vals = self.duplicate_3d(pre, pre3d, offsets, K, cam)
+        gx, gy = (cam.width + 15) // 16, (cam.height + 15) // 16
+        ks, vs, ranges = self.sort_pairs(keys, vals, gx * gy)
+        img, T, nc = self.raster_fwd_3d(pre, pre3d, vs, ranges, cam, tuple(cfg.bg_color))
+        return dict(pre=pre, pre3d=pre3d, keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges,
+                    img=img, T_final=T, n_contrib=nc, K=K)
 
     def l1_loss(self, img, gt):
         H, W = img.shape[0], img.shape[1]

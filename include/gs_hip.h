@@ -36,7 +36,7 @@ extern "C" {
 #define GS_ERR_INVALID 1     /* bad argument (null pointer, bad size, unsupported mode) */
 #define GS_ERR_HIP 2         /* a HIP runtime call failed */
 #define GS_ERR_CAPACITY 3    /* a pre-sized buffer was too small (pairs > capacity) */
-#define GS_ERR_UNSUPPORTED 4 /* feature outside this path (eval_3D) */
+#define GS_ERR_UNSUPPORTED 4 /* feature outside this path */
 
 /* Camera: same fields and order as the reference Camera (types.h:48-54).
  * Matrices are column-major. */
@@ -49,7 +49,7 @@ typedef struct gs_camera {
 } gs_camera;
 
 /* The 2D-path fields of the reference RenderConfig (types.h:57-66).
- * eval_3D is not part of this path. */
+ * eval_3D is requested by calling the *_3d entry points below. */
 typedef struct gs_render_cfg {
     float bg_color[3];
     float scale_modifier;
@@ -85,6 +85,15 @@ typedef struct gs_preprocessed {
     int* radii;       /* [N]   */
     int* tiles_touched; /* [N] */
 } gs_preprocessed;
+
+/* The eval_3D arrays of PreprocessOutput (types.h:77-79): the per-Gaussian
+ * gauss-to-screen matrix (row-major 4x4), the upper triangle of the inverse
+ * dilated 3D covariance, and position - camera position. */
+typedef struct gs_preprocessed_3d {
+    float* gauss2screen; /* [N*16], 16-byte aligned */
+    float* cov3D_inv;    /* [N*6]  */
+    float* mean_offset;  /* [N*3]  */
+} gs_preprocessed_3d;
 
 /* Screen-space gradients (RasterGradOutput, train_types.h:25-31). */
 typedef struct gs_raster_grads {
@@ -211,6 +220,41 @@ int gs_raster_fwd(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre
 int gs_raster_fwd_words(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre,
                         const uint64_t* words, const uint32_t* ranges, const float bg[3], float* image,
                         const gs_pixel_state* px);
+
+/* ---- eval_3D forward (AAA-Gaussians 3D evaluation, StopThePop k-buffer) ----
+ * The stage-by-stage path of Renderer::render with RenderConfig::eval_3D
+ * set (types.h:61): preprocess -> gs_scan_tiles -> gs_duplicate_keys_3d ->
+ * gs_sort_pairs -> gs_raster_fwd_3d.  Forward only, as in the reference.
+ *
+ * Preprocess, eval_3D branch: PreprocessorCPU::process
+ * (cpu/preprocessor_cpu.cpp:64-187; math_utils.cpp:474-583 computeGauss2Screen,
+ * :253-338 maxContribGaussianFrustum3D, :342-472 computeAABBScreen/View,
+ * :590-613 computeCov3DInv) and the preprocess kernel's 3D branch
+ * (gpu/kernels/preprocess.cl:394-582).  filter_3d (GaussianData::filter_3D,
+ * types.h:44) may be NULL.  out->conics is written with zeros; culled
+ * Gaussians get radius 0 and zeros elsewhere. */
+int gs_preprocess_fwd_3d(gs_ctx* ctx, const gs_gaussians* g, const float* filter_3d, const gs_camera* cam,
+                         const gs_render_cfg* cfg, const gs_preprocessed* out, const gs_preprocessed_3d* out3d);
+
+/* Duplicate-with-keys, eval_3D: TileBinnerCPU::bin (cpu/tile_binner_cpu.cpp:14-20,
+ * 59-99) and scatter.cl:118-145.  The low key bits are the depth along the
+ * ray through the tile's centre (depthAlongRay, math_utils.cpp:616-630) where
+ * that is > 0, else the view depth.  offsets as for gs_duplicate_keys. */
+int gs_duplicate_keys_3d(gs_ctx* ctx, int n, const gs_preprocessed* pre, const gs_preprocessed_3d* pre3d,
+                         const uint32_t* offsets, const gs_camera* cam, int tile_w, int tile_h,
+                         uint64_t* keys, uint32_t* values, uint32_t capacity);
+
+/* Raster forward, eval_3D: RasterizerCPU::rasterize's k-buffer branch
+ * (cpu/rasterizer_cpu.cpp:16-21,61-135,185-193; rasterize.cl:148-222):
+ * per pixel, the ray's Mahalanobis minimum gives alpha and the 16 nearest
+ * entries by per-pixel depth are kept sorted and blended front to back.
+ * T_final and n_contrib (ForwardCache, types.h:97-99) may be NULL.  Where
+ * the view-projection matrix is singular (the reference then reads an
+ * uninitialised inverse) the view depth orders the k-buffer, as it orders
+ * the binner's keys. */
+int gs_raster_fwd_3d(gs_ctx* ctx, const gs_camera* cam, const gs_preprocessed* pre,
+                     const gs_preprocessed_3d* pre3d, const uint32_t* values_sorted, const uint32_t* ranges,
+                     const float bg[3], float* image, float* T_final, int* n_contrib);
 
 /* L1 loss and its image gradient: l1_loss / l1_loss_double (loss.cpp:4-30).
  * loss_dev: device double, the mean absolute difference (sum in FP64). */

@@ -846,3 +846,591 @@ void PFX(adam_bias_corrections)(float b1, float b2, int t, float* bc1, float* bc
     *bc1 = 1.0f / (1.0f - powf(b1, (float)t));
     *bc2 = 1.0f / (1.0f - powf(b2, (float)t));
 }
+
+/* =====================================================================
+ * eval_3D forward (AAA-Gaussians 3D evaluation + StopThePop k-buffer),
+ * float build only.  Restates, in the reference's operation order:
+ *   preprocess3d ... cpu/preprocessor_cpu.cpp:64-187 (eval_3D branch)
+ *                    + math_utils.cpp:158-651 (gauss2screen, frustum cull,
+ *                    screen / view AABB, inverse covariance)
+ *   duplicate3d .... cpu/tile_binner_cpu.cpp:14-20,59-99 (per-tile depth key)
+ *   rasterize3d .... cpu/rasterizer_cpu.cpp:16-21,61-135,185-193 (K = 16)
+ *   invert4 ........ math_utils.cpp:653-696
+ * Every `x = 0; x += ...` accumulation starts from +0 as the reference's loops
+ * do (0 + -0 is +0).  Transcendentals: logf, atan2f, tanf, fmodf as the
+ * reference's float overloads of std::log/atan2/tan/fmod.
+ * Pinned in tests/ against the reference compiled from source (oracle/_ref,
+ * ref_render_3d), and by closed forms: the ray contribution against a dense
+ * scan of the Mahalanobis distance along the ray, invert4 against numpy.
+ * ===================================================================== */
+#ifndef GS_ORACLE_DOUBLE
+#define A3_ALPHA (1.0f / 255.0f)
+#define A3_PI ((float)3.14159265358979323846)
+
+static float a3_sq(float x) { return x * x; }
+
+/* math_utils.cpp:158-163 (same matrix as rot_matrix) */
+static void a3_quat2mat(const float* q, float R[3][3]) { rot_matrix(q, R); }
+
+/* math_utils.cpp:166-174: out = A*B, column-major */
+static void a3_mat4mul(const float* A, const float* B, float* out) {
+    for (int col = 0; col < 4; col++)
+        for (int row = 0; row < 4; row++) {
+            float sum = 0;
+            for (int k = 0; k < 4; k++) sum += A[k * 4 + row] * B[col * 4 + k];
+            out[col * 4 + row] = sum;
+        }
+}
+
+/* math_utils.cpp:186-206 */
+static float a3_ray(const float* pa, const float* pb, float* mp) {
+    float dx = pa[1] * pb[2] - pa[2] * pb[1];
+    float dy = pa[2] * pb[0] - pa[0] * pb[2];
+    float dz = pa[0] * pb[1] - pa[1] * pb[0];
+    float mx = pa[3] * pb[0] - pa[0] * pb[3];
+    float my = pa[3] * pb[1] - pa[1] * pb[3];
+    float mz = pa[3] * pb[2] - pa[2] * pb[3];
+    float dd = dx * dx + dy * dy + dz * dz;
+    float q[3] = {mx / dd, my / dd, mz / dd};
+    mp[0] = dy * q[2] - dz * q[1];
+    mp[1] = dz * q[0] - dx * q[2];
+    mp[2] = dx * q[1] - dy * q[0];
+    mp[3] = 1.0f;
+    return mx * q[0] + my * q[1] + mz * q[2];
+}
+float gso_ray_contrib(const float* pa, const float* pb, float* mp) { return a3_ray(pa, pb, mp); }
+
+/* transpose(g2s) * v for row-major g2s (math_utils.cpp:214-219, 241-246) */
+static void a3_to_screen(const float* g2s, const float* v, float* o) {
+    for (int i = 0; i < 4; i++) {
+        o[i] = 0;
+        for (int j = 0; j < 4; j++) o[i] += g2s[j * 4 + i] * v[j];
+    }
+}
+
+/* math_utils.cpp:210-221 */
+static float a3_plane(const float* pl, const float* g2s, float* ps) {
+    float norm = pl[3] / (pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+    float g[4] = {-pl[0] * norm, -pl[1] * norm, -pl[2] * norm, -pl[3] * norm};
+    a3_to_screen(g2s, g, ps);
+    return -g[3];
+}
+
+/* math_utils.cpp:225-235 */
+static int a3_in(const float* ps, const float* from, const float* ext, int dim, float* d) {
+    *d = ps[dim] - from[dim] * ps[3];
+    return *d > 0.0f && *d < ext[dim] * ps[3];
+}
+static int a3_in2(const float* ps, const float* from, const float* ext, int dim) {
+    float d;
+    return a3_in(ps, from, ext, dim, &d);
+}
+
+/* math_utils.cpp:237-249 */
+static float a3_ray_screen(const float* pa, const float* pb, const float* g2s, float* ps) {
+    float g[4];
+    float c = a3_ray(pa, pb, g);
+    a3_to_screen(g2s, g, ps);
+    return c;
+}
+
+/* math_utils.cpp:253-338 */
+float gso_frustum_contrib(float x0, float y0, float x1, float y1, const float* g2s, float* depth) {
+    float best = 1e30f;
+    float bp[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float from[3] = {x0, y0, -1.0f};
+    float ext[3] = {x1 - x0, y1 - y0, 1.0f - -1.0f};
+    float mean[4] = {g2s[3], g2s[7], g2s[11], g2s[15]};
+    float d[3];
+    int bx = a3_in(mean, from, ext, 0, &d[0]);
+    int by = a3_in(mean, from, ext, 1, &d[1]);
+    int bz = a3_in(mean, from, ext, 2, &d[2]);
+    if (bx && by && bz) {
+        for (int i = 0; i < 4; i++) bp[i] = mean[i];
+        best = 0.0f;
+    } else {
+        float sx = (d[0] - (ext[0] * 0.5f * mean[3])) >= 0 ? ext[0] * 0.5f : -ext[0] * 0.5f;
+        float sy = (d[1] - (ext[1] * 0.5f * mean[3])) >= 0 ? ext[1] * 0.5f : -ext[1] * 0.5f;
+        float cx = from[0] + ext[0] * 0.5f + sx;
+        float cy = from[1] + ext[1] * 0.5f + sy;
+        float px[4], py[4], ps[4], c;
+        for (int i = 0; i < 4; i++) {
+            px[i] = g2s[i] - g2s[12 + i] * cx;
+            py[i] = g2s[4 + i] - g2s[12 + i] * cy;
+        }
+        c = a3_plane(px, g2s, ps);
+        if (c < best && a3_in2(ps, from, ext, 1) && a3_in2(ps, from, ext, 2)) { best = c; memcpy(bp, ps, 16); }
+        c = a3_plane(py, g2s, ps);
+        if (c < best && a3_in2(ps, from, ext, 0) && a3_in2(ps, from, ext, 2)) { best = c; memcpy(bp, ps, 16); }
+        c = a3_ray_screen(px, py, g2s, ps);
+        if (c < best && a3_in2(ps, from, ext, 2)) { best = c; memcpy(bp, ps, 16); }
+        float oy[4], ox[4];
+        float ocy = from[1] + ext[1] * 0.5f - sy;
+        for (int i = 0; i < 4; i++) oy[i] = g2s[4 + i] - g2s[12 + i] * ocy;
+        c = a3_ray_screen(px, oy, g2s, ps);
+        if (c < best && a3_in2(ps, from, ext, 2)) { best = c; memcpy(bp, ps, 16); }
+        float ocx = from[0] + ext[0] * 0.5f - sx;
+        for (int i = 0; i < 4; i++) ox[i] = g2s[i] - g2s[12 + i] * ocx;
+        c = a3_ray_screen(ox, py, g2s, ps);
+        if (c < best && a3_in2(ps, from, ext, 2)) { best = c; memcpy(bp, ps, 16); }
+    }
+    *depth = bp[2] / bp[3];
+    return 0.5f * best;
+}
+
+/* math_utils.cpp:342-389 */
+int gso_aabb_screen(const float* g2s, float cutoff, float* mean2d, float* ext2d) {
+    float t[4] = {cutoff, cutoff, cutoff, -1.0f};
+    const float *r0 = g2s, *r1 = g2s + 4, *r2 = g2s + 8, *r3 = g2s + 12;
+    float s = 0;
+    for (int k = 0; k < 4; k++) s += t[k] * r3[k] * r3[k];
+    if (s >= 0.0f) return 0;
+    float f[4];
+    for (int k = 0; k < 4; k++) f[k] = t[k] / s;
+    float p[3] = {0, 0, 0};
+    for (int k = 0; k < 4; k++) {
+        p[0] += f[k] * r0[k] * r3[k];
+        p[1] += f[k] * r1[k] * r3[k];
+        p[2] += f[k] * r2[k] * r3[k];
+    }
+    float h[3] = {p[0] * p[0], p[1] * p[1], p[2] * p[2]};
+    for (int k = 0; k < 4; k++) {
+        h[0] -= f[k] * r0[k] * r0[k];
+        h[1] -= f[k] * r1[k] * r1[k];
+        h[2] -= f[k] * r2[k] * r2[k];
+    }
+    float ez = sqrtf(SMAX(0.0f, h[2]));
+    float lo = p[2] - ez, hi = p[2] + ez;
+    if (lo < -1.0f || hi > 1.0f) return 0;
+    mean2d[0] = p[0];
+    mean2d[1] = p[1];
+    ext2d[0] = sqrtf(SMAX(0.0f, h[0]));
+    ext2d[1] = sqrtf(SMAX(0.0f, h[1]));
+    return 1;
+}
+
+/* math_utils.cpp:391-396 */
+static float a3_norm_angle(float th) {
+    th = fmodf(th, 2.0f * A3_PI);
+    if (th > A3_PI) th -= 2.0f * A3_PI;
+    if (th <= -A3_PI) th += 2.0f * A3_PI;
+    return th;
+}
+
+/* math_utils.cpp:413-458: the tangents of the two bounding angles about one axis */
+static void a3_theta(const float* g2v, const float* vd, int axis, const float* t, float* tlo, float* thi) {
+    float mu = atan2f(axis == 0 ? vd[0] : vd[1], vd[2]);
+    const float* Ta = g2v + axis * 4;
+    const float* T2 = g2v + 8;
+    float sa = 0, sz = 0, mid = 0;
+    for (int i = 0; i < 4; i++) {
+        sa += t[i] * Ta[i] * Ta[i];
+        sz += t[i] * T2[i] * T2[i];
+        mid += t[i] * Ta[i] * T2[i];
+    }
+    float in = mid * mid - sz * sa;
+    float lo = -(A3_PI / 2.0f - 1e-5f);
+    float hi = A3_PI / 2.0f - 1e-5f;
+    if (in > 0.0f) {
+        float sq = sqrtf(in);
+        float a0 = atan2f(-(mid + sq), -sz);
+        float a1 = atan2f(-(mid - sq), -sz);
+        while (a0 > mu) a0 -= A3_PI;
+        while (a0 < (mu - A3_PI)) a0 += A3_PI;
+        while (a1 < mu) a1 += A3_PI;
+        while (a1 > (mu + A3_PI)) a1 -= A3_PI;
+        float n0 = a3_norm_angle(a0), n1 = a3_norm_angle(a1);
+        if (mu < 0.0f && fabsf(n0) < fabsf(n1)) {
+            a0 += 2.0f * A3_PI;
+            a1 += 2.0f * A3_PI;
+        } else if (mu > 0.0f && fabsf(n1) < fabsf(n0)) {
+            a0 -= 2.0f * A3_PI;
+            a1 -= 2.0f * A3_PI;
+        }
+        lo = SMAX(lo, a0);
+        hi = SMIN(hi, a1);
+    }
+    *tlo = tanf(lo);
+    *thi = tanf(hi);
+}
+
+/* math_utils.cpp:398-472 */
+int gso_aabb_view(const float* g2v, const float* mv, float fx, float fy, float W, float H, float cutoff,
+                  float* mean2d, float* ext2d) {
+    float t[4] = {cutoff, cutoff, cutoff, -1.0f};
+    float vlen = sqrtf(mv[0] * mv[0] + mv[1] * mv[1] + mv[2] * mv[2]);
+    float vd[3] = {mv[0] / vlen, mv[1] / vlen, mv[2] / vlen};
+    float bxl, bxh, byl, byh;
+    a3_theta(g2v, vd, 0, t, &bxl, &bxh);
+    a3_theta(g2v, vd, 1, t, &byl, &byh);
+    float xl = W / 2.0f + fx * bxl, xh = W / 2.0f + fx * bxh;
+    float yl = H / 2.0f + fy * byl, yh = H / 2.0f + fy * byh;
+    mean2d[0] = (xh + xl) / 2.0f;
+    mean2d[1] = (yh + yl) / 2.0f;
+    ext2d[0] = (xh - xl) / 2.0f;
+    ext2d[1] = (yh - yl) / 2.0f;
+    return 1;
+}
+
+/* math_utils.cpp:474-583; returns the dilation factor */
+float gso_gauss2screen(const float* mean, const float* scale, const float* rot, float smod, const float* campos,
+                       const float* vp, const float* view, float focal, float ksize, float filt, int W, int H,
+                       float* g2s, float* g2v) {
+    float R[3][3];
+    a3_quat2mat(rot, R);
+    float sdl[3] = {scale[0], scale[1], scale[2]};
+    float dil = 1.0f;
+    if (ksize > 0.0f) {
+        float vr[3] = {mean[0] - campos[0], mean[1] - campos[1], mean[2] - campos[2]};
+        float vl = sqrtf(vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]);
+        vr[0] /= vl; vr[1] /= vl; vr[2] /= vl;
+        float vg[3];
+        for (int i = 0; i < 3; i++) vg[i] = R[i][0] * vr[0] + R[i][1] * vr[1] + R[i][2] * vr[2];
+        float r2[3] = {vg[0] * vg[0], vg[1] * vg[1], vg[2] * vg[2]};
+        float mv[3];
+        xform43(mean, view, mv);
+        float mip = a3_sq(mv[2] / focal) * ksize;
+        mip = SMAX(a3_sq(filt), mip);
+        for (int j = 0; j < 3; j++) sdl[j] = sqrtf(scale[j] * scale[j] + mip);
+        float s[3] = {scale[0] * scale[0], scale[1] * scale[1], scale[2] * scale[2]};
+        float sd2[3] = {sdl[0] * sdl[0], sdl[1] * sdl[1], sdl[2] * sdl[2]};
+        float dm = r2[0] * (s[1] * s[2]) + r2[1] * (s[2] * s[0]) + r2[2] * (s[0] * s[1]);
+        float dmd = r2[0] * (sd2[1] * sd2[2]) + r2[1] * (sd2[2] * sd2[0]) + r2[2] * (sd2[0] * sd2[1]);
+        dil = sqrtf(dm / dmd);
+    }
+    float sm = sqrtf(smod);
+    float sd[3] = {sdl[0] * sm, sdl[1] * sm, sdl[2] * sm};
+    float L[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) L[i][j] = sd[j] * R[j][i];
+    float g2w[16];
+    for (int col = 0; col < 3; col++) {
+        g2w[col * 4 + 0] = L[0][col];
+        g2w[col * 4 + 1] = L[1][col];
+        g2w[col * 4 + 2] = L[2][col];
+        g2w[col * 4 + 3] = 0.0f;
+    }
+    g2w[12] = mean[0]; g2w[13] = mean[1]; g2w[14] = mean[2]; g2w[15] = 1.0f;
+    float port[16];
+    memset(port, 0, sizeof(port));
+    port[0] = W / 2.0f;
+    port[5] = H / 2.0f;
+    port[10] = 1.0f;
+    port[15] = 1.0f;
+    port[12] = W / 2.0f - 0.5f;
+    port[13] = H / 2.0f - 0.5f;
+    float w2s[16], cm[16];
+    a3_mat4mul(port, vp, w2s);
+    a3_mat4mul(w2s, g2w, cm);
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) g2s[r * 4 + c] = cm[c * 4 + r];
+    a3_mat4mul(view, g2w, cm);
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) g2v[r * 4 + c] = cm[c * 4 + r];
+    return dil;
+}
+
+/* math_utils.cpp:590-613 (upper triangle, row-major) */
+static void a3_cov_inv(const float* sdl, float smod, const float* rot, float* ci) {
+    float R[3][3];
+    a3_quat2mat(rot, R);
+    float sm = sqrtf(smod);
+    float inv[3];
+    for (int j = 0; j < 3; j++) {
+        float s = sdl[j] * sm;
+        inv[j] = 1.0f / (s * s);
+    }
+    int idx = 0;
+    for (int r = 0; r < 3; r++)
+        for (int c = r; c < 3; c++) {
+            float v = 0;
+            for (int k = 0; k < 3; k++) v += R[k][r] * inv[k] * R[k][c];
+            ci[idx++] = v;
+        }
+}
+
+/* math_utils.cpp:616-630 */
+static float a3_depth_along(const float* ci, const float* mo, const float* vd) {
+    float s0 = ci[0] * vd[0] + ci[1] * vd[1] + ci[2] * vd[2];
+    float s1 = ci[1] * vd[0] + ci[3] * vd[1] + ci[4] * vd[2];
+    float s2 = ci[2] * vd[0] + ci[4] * vd[1] + ci[5] * vd[2];
+    float num = mo[0] * s0 + mo[1] * s1 + mo[2] * s2;
+    float den = vd[0] * s0 + vd[1] * s1 + vd[2] * s2;
+    if (fabsf(den) < 1e-10f) return sqrtf(mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2]);
+    return num / den;
+}
+
+/* math_utils.cpp:632-651 */
+static void a3_pixel_dir(float px, float py, int W, int H, const float* ivp, const float* campos, float* d) {
+    float nx = (2.0f * px + 1.0f) / (float)W - 1.0f;
+    float ny = (2.0f * py + 1.0f) / (float)H - 1.0f;
+    float p[4] = {nx, ny, 0.0f, 1.0f};
+    float w[4];
+    for (int i = 0; i < 4; i++) w[i] = ivp[i] * p[0] + ivp[4 + i] * p[1] + ivp[8 + i] * p[2] + ivp[12 + i] * p[3];
+    float wi = 1.0f / (w[3] + 1e-10f);
+    d[0] = w[0] * wi - campos[0];
+    d[1] = w[1] * wi - campos[1];
+    d[2] = w[2] * wi - campos[2];
+    float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (len > 1e-10f) { d[0] /= len; d[1] /= len; d[2] /= len; }
+}
+
+/* math_utils.cpp:653-696 (column-major cofactor inverse) */
+int gso_invert4(const float* m, float* inv) {
+    float a00 = m[0], a10 = m[1], a20 = m[2], a30 = m[3];
+    float a01 = m[4], a11 = m[5], a21 = m[6], a31 = m[7];
+    float a02 = m[8], a12 = m[9], a22 = m[10], a32 = m[11];
+    float a03 = m[12], a13 = m[13], a23 = m[14], a33 = m[15];
+    float b00 = a00 * a11 - a01 * a10, b01 = a00 * a12 - a02 * a10, b02 = a00 * a13 - a03 * a10;
+    float b03 = a01 * a12 - a02 * a11, b04 = a01 * a13 - a03 * a11, b05 = a02 * a13 - a03 * a12;
+    float b06 = a20 * a31 - a21 * a30, b07 = a20 * a32 - a22 * a30, b08 = a20 * a33 - a23 * a30;
+    float b09 = a21 * a32 - a22 * a31, b10 = a21 * a33 - a23 * a31, b11 = a22 * a33 - a23 * a32;
+    float det = b00 * b11 - b01 * b10 + b02 * b09 + b03 * b08 - b04 * b07 + b05 * b06;
+    if (fabsf(det) < 1e-12f) return 0;
+    float id = 1.0f / det;
+    inv[0] = (a11 * b11 - a12 * b10 + a13 * b09) * id;
+    inv[1] = (-a10 * b11 + a12 * b08 - a13 * b07) * id;
+    inv[2] = (a10 * b10 - a11 * b08 + a13 * b06) * id;
+    inv[3] = (-a10 * b09 + a11 * b07 - a12 * b06) * id;
+    inv[4] = (-a01 * b11 + a02 * b10 - a03 * b09) * id;
+    inv[5] = (a00 * b11 - a02 * b08 + a03 * b07) * id;
+    inv[6] = (-a00 * b10 + a01 * b08 - a03 * b06) * id;
+    inv[7] = (a00 * b09 - a01 * b07 + a02 * b06) * id;
+    inv[8] = (a31 * b05 - a32 * b04 + a33 * b03) * id;
+    inv[9] = (-a30 * b05 + a32 * b02 - a33 * b01) * id;
+    inv[10] = (a30 * b04 - a31 * b02 + a33 * b00) * id;
+    inv[11] = (-a30 * b03 + a31 * b01 - a32 * b00) * id;
+    inv[12] = (-a21 * b05 + a22 * b04 - a23 * b03) * id;
+    inv[13] = (a20 * b05 - a22 * b02 + a23 * b01) * id;
+    inv[14] = (-a20 * b04 + a21 * b02 - a23 * b00) * id;
+    inv[15] = (a20 * b03 - a21 * b01 + a22 * b00) * id;
+    return 1;
+}
+
+/* cpu/preprocessor_cpu.cpp:53-57,64-187 (eval_3D branch).  filter3d may be
+ * NULL (GaussianData::filter_3D absent).  Culled Gaussians get radius 0,
+ * tiles 0 and zeros in every float output. */
+void gso_preprocess3d(int N, int model_deg, int max_coeffs, const float* pos, const float* sh, const float* scale,
+                      const float* rot, const float* op, const float* filter3d, const gs_camera* cam,
+                      const gs_render_cfg* cfg, float* means2D, float* depths, float* op2d, float* rgb, int* radii,
+                      int* tiles, float* g2s_out, float* cinv_out, float* moff_out) {
+    const int W = cam->width, H = cam->height;
+    float fx = W / (2.0f * cam->tan_fovx);
+    float fy = H / (2.0f * cam->tan_fovy);
+    int gx = (W + cfg->tile_w - 1) / cfg->tile_w;
+    int gy = (H + cfg->tile_h - 1) / cfg->tile_h;
+    int deg = SMIN(cfg->sh_degree, model_deg);
+    for (long i = 0; i < N; i++) {
+        radii[i] = 0; tiles[i] = 0;
+        means2D[2 * i] = means2D[2 * i + 1] = 0; depths[i] = 0; op2d[i] = 0;
+        for (int k = 0; k < 3; k++) { rgb[3 * i + k] = 0; moff_out[3 * i + k] = 0; }
+        for (int k = 0; k < 6; k++) cinv_out[6 * i + k] = 0;
+        for (int k = 0; k < 16; k++) g2s_out[16 * i + k] = 0;
+        const float* p = &pos[3 * i];
+        const float* s = &scale[3 * i];
+        const float* q = &rot[4 * i];
+        float pv[3];
+        xform43(p, cam->view_matrix, pv);
+        if (pv[2] < 0.2f) continue;
+        float opacity = op[i];
+        float focal = SMAX(fx, fy);
+        float filt = filter3d ? filter3d[i] : 0.0f;
+        float g2s[16], g2v[16];
+        float dil = gso_gauss2screen(p, s, q, cfg->scale_modifier, cam->cam_pos, cam->viewproj_matrix,
+                                     cam->view_matrix, focal, 0.3f, filt, W, H, g2s, g2v);
+        opacity *= dil;
+        if (opacity < A3_ALPHA) continue;
+        float opt = logf(opacity / A3_ALPHA);
+        float cutoff = SMIN(11.11f, 2.0f * opt);
+        float R[3][3];
+        a3_quat2mat(q, R);
+        float sm = sqrtf(cfg->scale_modifier);
+        float sdl[3] = {s[0], s[1], s[2]};
+        {
+            float mv[3];
+            xform43(p, cam->view_matrix, mv);
+            float mip = (mv[2] / focal) * (mv[2] / focal) * 0.3f;
+            mip = SMAX(filt * filt, mip);
+            for (int j = 0; j < 3; j++) sdl[j] = sqrtf(s[j] * s[j] + mip);
+        }
+        float sd[3];
+        for (int j = 0; j < 3; j++) sd[j] = sdl[j] * sm;
+        float ci[6];
+        a3_cov_inv(sdl, cfg->scale_modifier, q, ci);
+        float mo[3] = {p[0] - cam->cam_pos[0], p[1] - cam->cam_pos[1], p[2] - cam->cam_pos[2]};
+        float df[3] = {cam->cam_pos[0] - p[0], cam->cam_pos[1] - p[1], cam->cam_pos[2] - p[2]};
+        float cg[3];
+        for (int j = 0; j < 3; j++) cg[j] = (R[j][0] * df[0] + R[j][1] * df[1] + R[j][2] * df[2]) / sd[j];
+        float dist2 = cg[0] * cg[0] + cg[1] * cg[1] + cg[2] * cg[2];
+        if (dist2 < cutoff) continue;
+        float mpd;
+        float mc = gso_frustum_contrib(0.0f, 0.0f, (float)(W - 1), (float)(H - 1), g2s, &mpd);
+        if (mc > opt) continue;
+        float ph[4];
+        xform44(p, cam->viewproj_matrix, ph);
+        float pwi = 1.0f / (ph[3] + 1e-7f);
+        float pxl = ndc2pix(ph[0] * pwi, W);
+        float pyl = ndc2pix(ph[1] * pwi, H);
+        float m2[2], e2[2];
+        if (!gso_aabb_screen(g2s, cutoff, m2, e2)) {
+            if (!gso_aabb_view(g2v, pv, fx, fy, (float)W, (float)H, cutoff, m2, e2)) continue;
+        }
+        int r = (int)ceilf(SMAX(e2[0], e2[1]));
+        r = SMAX(r, 1);
+        float pt[2] = {pxl, pyl};
+        int mn[2], mx[2];
+        get_rect(pt, r, gx, gy, cfg->tile_w, cfg->tile_h, mn, mx);
+        if ((mx[0] - mn[0]) * (mx[1] - mn[1]) == 0) continue;
+        sh_color(deg, &sh[(size_t)i * max_coeffs * 3], p, cam->cam_pos, &rgb[3 * i], cfg->training);
+        depths[i] = pv[2];
+        radii[i] = r;
+        means2D[2 * i] = pxl;
+        means2D[2 * i + 1] = pyl;
+        op2d[i] = opacity;
+        tiles[i] = (mx[1] - mn[1]) * (mx[0] - mn[0]);
+        for (int k = 0; k < 16; k++) g2s_out[16 * i + k] = g2s[k];
+        for (int k = 0; k < 6; k++) cinv_out[6 * i + k] = ci[k];
+        for (int k = 0; k < 3; k++) moff_out[3 * i + k] = mo[k];
+    }
+}
+
+/* cpu/tile_binner_cpu.cpp:14-20,59-99 with eval_3D: the low 32 key bits are
+ * the depth along the ray through the tile's centre where that is > 0, else
+ * the view depth.  Pairs are written in the reference's emission order
+ * (Gaussian, then row, then column) at the exclusive scan of tiles. */
+void gso_duplicate3d(int N, const float* means2D, const float* depths, const int* radii, const int* tiles,
+                     const float* cinv, const float* moff, const gs_camera* cam, int tile_w, int tile_h,
+                     uint64_t* keys, uint32_t* values) {
+    const int W = cam->width, H = cam->height;
+    int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+    float ivp[16];
+    int have = gso_invert4(cam->viewproj_matrix, ivp);
+    long long off = 0;
+    for (long i = 0; i < N; i++) {
+        long long o = off;
+        off += tiles[i];
+        if (radii[i] <= 0) continue;
+        float pt[2] = {means2D[2 * i], means2D[2 * i + 1]};
+        int mn[2], mx[2];
+        get_rect(pt, radii[i], gx, gy, tile_w, tile_h, mn, mx);
+        uint32_t db = depth_bits(depths[i]);
+        for (int y = mn[1]; y < mx[1]; y++)
+            for (int x = mn[0]; x < mx[0]; x++) {
+                uint32_t tb = db;
+                if (have) {
+                    float cx = (x + 0.5f) * tile_w, cy = (y + 0.5f) * tile_h;
+                    float vd[3];
+                    a3_pixel_dir(cx, cy, W, H, ivp, cam->cam_pos, vd);
+                    float ptd = a3_depth_along(&cinv[6 * i], &moff[3 * i], vd);
+                    if (ptd > 0.0f) memcpy(&tb, &ptd, 4);
+                }
+                keys[o] = ((uint64_t)(uint32_t)(y * gx + x) << 32) | tb;
+                values[o] = (uint32_t)i;
+                o++;
+            }
+    }
+}
+
+/* cpu/rasterizer_cpu.cpp:16-21,61-135,185-193: the K = 16 k-buffer blend of
+ * the 3D path, with T_final and n_contrib as the ForwardCache stores them. */
+void gso_rasterize3d(const float* depths, const float* op2d, const float* rgb, const float* g2s,
+                     const float* cinv, const float* moff, const uint32_t* values_sorted, const uint32_t* ranges,
+                     const gs_camera* cam, int tile_w, int tile_h, const float* bg, float* img, float* T_final,
+                     int* n_contrib) {
+    const int W = cam->width, H = cam->height;
+    int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+    float ivp[16];
+    /* Singular view-projection: the reference reads its inverse uninitialised
+     * (rasterizer_cpu.cpp:16-21); here the view depth orders the k-buffer, as
+     * the binner's keys do (tile_binner_cpu.cpp:80). */
+    int have = gso_invert4(cam->viewproj_matrix, ivp);
+    for (int ty = 0; ty < gy; ty++)
+        for (int tx = 0; tx < gx; tx++) {
+            int tile = ty * gx + tx;
+            uint32_t r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
+            int x1 = SMIN(tx * tile_w + tile_w, W), y1 = SMIN(ty * tile_h + tile_h, H);
+            for (int py = ty * tile_h; py < y1; py++)
+                for (int px = tx * tile_w; px < x1; px++) {
+                    float T = 1.0f, C[3] = {0, 0, 0};
+                    int cnt = 0, kc = 0, done = 0;
+                    float kd[16], ka[16];
+                    uint32_t ki[16];
+                    float rd[3] = {0, 0, 1};
+                    if (have) a3_pixel_dir((float)px + 0.5f, (float)py + 0.5f, W, H, ivp, cam->cam_pos, rd);
+                    for (uint32_t j = r0; j < r1 && !done; j++) {
+                        uint32_t id = values_sorted[j];
+                        const float* g = &g2s[16 * (size_t)id];
+                        float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+                        float plx[4], ply[4], mp[4];
+                        for (int k = 0; k < 4; k++) {
+                            plx[k] = g[k] - g[12 + k] * fx;
+                            ply[k] = g[4 + k] - g[12 + k] * fy;
+                        }
+                        float power = -0.5f * a3_ray(plx, ply, mp);
+                        if (power > 0.0f) continue;
+                        float alpha = SMIN(0.99f, op2d[id] * expf(power));
+                        if (alpha < 1.0f / 255.0f) continue;
+                        float pd = depths[id];
+                        if (have) {
+                            float ptd = a3_depth_along(&cinv[6 * (size_t)id], &moff[3 * (size_t)id], rd);
+                            if (ptd > 0.0f) pd = ptd;
+                        }
+                        if (kc >= 16) {
+                            float a = ka[0];
+                            uint32_t gid = ki[0];
+                            float tt = T * (1.0f - a);
+                            if (tt < 0.0001f) { done = 1; continue; }
+                            for (int ch = 0; ch < 3; ch++) C[ch] += rgb[gid * 3 + ch] * a * T;
+                            T = tt;
+                            cnt++;
+                            for (int k = 0; k < 15; k++) { kd[k] = kd[k + 1]; ka[k] = ka[k + 1]; ki[k] = ki[k + 1]; }
+                            kc = 15;
+                        }
+                        int at = kc;
+                        for (int k = kc - 1; k >= 0; k--) {
+                            if (kd[k] > pd) at = k; else break;
+                        }
+                        for (int k = kc; k > at; k--) { kd[k] = kd[k - 1]; ka[k] = ka[k - 1]; ki[k] = ki[k - 1]; }
+                        kd[at] = pd; ka[at] = alpha; ki[at] = id;
+                        kc++;
+                    }
+                    for (int k = 0; k < kc && !done; k++) {
+                        float a = ka[k];
+                        uint32_t gid = ki[k];
+                        float tt = T * (1.0f - a);
+                        if (tt < 0.0001f) break;
+                        for (int ch = 0; ch < 3; ch++) C[ch] += rgb[gid * 3 + ch] * a * T;
+                        T = tt;
+                        cnt++;
+                    }
+                    int pix = py * W + px;
+                    T_final[pix] = T;
+                    n_contrib[pix] = cnt;
+                    for (int ch = 0; ch < 3; ch++) img[pix * 3 + ch] = C[ch] + T * bg[ch];
+                }
+        }
+}
+#endif /* !GS_ORACLE_DOUBLE */
+
+#ifndef GS_ORACLE_DOUBLE
+/* Over every float x in [lo, hi]: *mismatch counts the x where logf(x) differs
+ * from the double logarithm rounded to float, *near those whose double
+ * logarithm lies within 2^-40 (relative) of a midpoint between adjacent
+ * floats (where a double log a few ulps off could round the other way).  The
+ * eval_3D preprocess (preprocessor_cpu.cpp:89) takes logf of values in
+ * [1, 255]; the device evaluates it as the rounded double logarithm. */
+void gso_logf_check(float lo, float hi, long long* mismatch, long long* near) {
+    long long mm = 0, nr = 0;
+    for (float x = lo; x <= hi; x = nextafterf(x, INFINITY)) {
+        double d = log((double)x);
+        float f = (float)d;
+        if (f != logf(x)) mm++;
+        double m1 = ((double)f + (double)nextafterf(f, -INFINITY)) * 0.5;
+        double m2 = ((double)f + (double)nextafterf(f, INFINITY)) * 0.5;
+        double tol = fabs(d) * 0x1p-40;
+        if (fabs(d - m1) <= tol || fabs(d - m2) <= tol) nr++;
+    }
+    *mismatch = mm;
+    *near = nr;
+}
+#endif

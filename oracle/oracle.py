@@ -247,6 +247,105 @@ class Oracle:
         return self.l1_loss(f["img"], gt, double_sum=True)[0]
 
 
+class Oracle3D:
+    """The eval_3D forward of the float oracle (gso_preprocess3d & co.)."""
+
+    def __init__(self):
+        path = os.path.join(BUILD, "libgs_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        self.lib = L
+        L.gso_frustum_contrib.restype = _f32
+        L.gso_frustum_contrib.argtypes = [_f32, _f32, _f32, _f32, _vp, _vp]
+        L.gso_ray_contrib.restype = _f32
+        L.gso_gauss2screen.restype = _f32
+        L.gso_gauss2screen.argtypes = [_vp, _vp, _vp, _f32, _vp, _vp, _vp, _f32, _f32, _f32, _i32, _i32, _vp, _vp]
+        L.gso_aabb_screen.argtypes = [_vp, _f32, _vp, _vp]
+        L.gso_aabb_view.argtypes = [_vp, _vp, _f32, _f32, _f32, _f32, _f32, _vp, _vp]
+        L.gso_count_pairs.restype = _i64
+        self._sort = L.gso_sort_pairs
+        self._sort.argtypes = [_i64, _vp, _vp, _vp, _vp, _i32, _vp]
+
+    @staticmethod
+    def _f(x, shape=None):
+        a = np.ascontiguousarray(x, dtype=np.float32)
+        return a if shape is None else a.reshape(shape)
+
+    def invert4(self, m):
+        inv = np.zeros(16, np.float32)
+        ok = self.lib.gso_invert4(_p(self._f(m)), _p(inv))
+        return bool(ok), inv
+
+    def ray_contrib(self, pa, pb):
+        mp = np.zeros(4, np.float32)
+        c = self.lib.gso_ray_contrib(_p(self._f(pa)), _p(self._f(pb)), _p(mp))
+        return float(c), mp
+
+    def preprocess(self, g, n, model_deg, max_coeffs, cam, cfg, filter3d=None):
+        f32 = np.float32
+        out = dict(means2D=np.zeros((n, 2), f32), depths=np.zeros(n, f32), op2d=np.zeros(n, f32),
+                   rgb=np.zeros((n, 3), f32), radii=np.zeros(n, np.int32), tiles=np.zeros(n, np.int32),
+                   g2s=np.zeros((n, 16), f32), cov3D_inv=np.zeros((n, 6), f32), mean_offset=np.zeros((n, 3), f32))
+        c, cf = cam_struct(cam), cfg_struct(cfg)
+        flt = None if filter3d is None else self._f(filter3d)
+        self.lib.gso_preprocess3d(_i32(n), _i32(model_deg), _i32(max_coeffs), _p(self._f(g["pos"])),
+                                  _p(self._f(g["sh"])), _p(self._f(g["scale"])), _p(self._f(g["rot"])),
+                                  _p(self._f(g["op"])), _p(flt), ctypes.byref(c), ctypes.byref(cf),
+                                  _p(out["means2D"]), _p(out["depths"]), _p(out["op2d"]), _p(out["rgb"]),
+                                  _p(out["radii"]), _p(out["tiles"]), _p(out["g2s"]), _p(out["cov3D_inv"]),
+                                  _p(out["mean_offset"]))
+        return out
+
+    def duplicate(self, pre, n, cam, tile=16):
+        tiles = np.ascontiguousarray(pre["tiles"], np.int32)
+        K = int(self.lib.gso_count_pairs(_i32(n), _p(tiles)))
+        keys = np.zeros(max(K, 1), np.uint64)
+        vals = np.zeros(max(K, 1), np.uint32)
+        c = cam_struct(cam)
+        self.lib.gso_duplicate3d(_i32(n), _p(self._f(pre["means2D"])), _p(self._f(pre["depths"])),
+                                 _p(np.ascontiguousarray(pre["radii"], np.int32)), _p(tiles),
+                                 _p(self._f(pre["cov3D_inv"])), _p(self._f(pre["mean_offset"])), ctypes.byref(c),
+                                 _i32(tile), _i32(tile), _p(keys), _p(vals))
+        return keys[:K], vals[:K]
+
+    def sort(self, keys, vals, num_tiles):
+        K = len(keys)
+        ko = np.zeros(max(K, 1), np.uint64)
+        vo = np.zeros(max(K, 1), np.uint32)
+        ranges = np.zeros((num_tiles, 2), np.uint32)
+        self._sort(_i64(K), _p(np.ascontiguousarray(keys, np.uint64)), _p(np.ascontiguousarray(vals, np.uint32)),
+                   _p(ko), _p(vo), _i32(num_tiles), _p(ranges))
+        return ko[:K], vo[:K], ranges
+
+    def rasterize(self, pre, values_sorted, ranges, cam, bg, tile=16):
+        H, W = cam.height, cam.width
+        img = np.zeros((H, W, 3), np.float32)
+        T = np.zeros((H, W), np.float32)
+        nc = np.zeros((H, W), np.int32)
+        c = cam_struct(cam)
+        vs = np.ascontiguousarray(values_sorted, np.uint32)
+        if vs.size == 0:
+            vs = np.zeros(1, np.uint32)
+        self.lib.gso_rasterize3d(_p(self._f(pre["depths"])), _p(self._f(pre["op2d"])), _p(self._f(pre["rgb"])),
+                                 _p(self._f(pre["g2s"])), _p(self._f(pre["cov3D_inv"])),
+                                 _p(self._f(pre["mean_offset"])), _p(vs),
+                                 _p(np.ascontiguousarray(ranges, np.uint32)), ctypes.byref(c), _i32(tile),
+                                 _i32(tile), _p(self._f(bg)), _p(img), _p(T), _p(nc))
+        return img, T, nc
+
+    def render(self, g, n, model_deg, max_coeffs, cam, cfg, filter3d=None):
+        """The whole eval_3D forward: preprocess, per-tile keys, stable sort, k-buffer blend."""
+        pre = self.preprocess(g, n, model_deg, max_coeffs, cam, cfg, filter3d)
+        keys, vals = self.duplicate(pre, n, cam)
+        gx, gy = (cam.width + 15) // 16, (cam.height + 15) // 16
+        ks, vs, ranges = self.sort(keys, vals, gx * gy)
+        img, T, nc = self.rasterize(pre, vs, ranges, cam, cfg.bg_color)
+        pre.update(keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges, img=img,
+                   T_final=T, n_contrib=nc)
+        return pre
+
+
 class RefLib:
     """The reference's own CPU classes compiled from source (oracle/_ref)."""
 
@@ -304,3 +403,34 @@ class RefLib:
             _i32(tc["max_steps"]), _f32(tc["beta1"]), _f32(tc["beta2"]), _f32(tc["eps"]), _i32(first_iteration),
             _i32(steps), ctypes.c_size_t(arena_bytes), _p(img))
         return float(loss), raw, img
+
+    def render_3d(self, g, n, model_deg, max_coeffs, cam, cfg, filter3d=None, max_pairs=None):
+        """The reference's eval_3D forward on activated parameters ``g``."""
+        f32 = np.float32
+        H, W = cam.height, cam.width
+        arr = [np.ascontiguousarray(g[k], f32).copy() for k in ("pos", "scale", "rot", "sh", "op")]
+        flt = None if filter3d is None else np.ascontiguousarray(filter3d, f32).copy()
+        if max_pairs is None:
+            max_pairs = 64 * n + 1024
+        gx, gy = (W + 15) // 16, (H + 15) // 16
+        o = dict(means2D=np.zeros((n, 2), f32), depths=np.zeros(n, f32), op2d=np.zeros(n, f32),
+                 rgb=np.zeros((n, 3), f32), radii=np.zeros(n, np.int32), tiles=np.zeros(n, np.int32),
+                 g2s=np.zeros((n, 16), f32), cov3D_inv=np.zeros((n, 6), f32), mean_offset=np.zeros((n, 3), f32),
+                 keys=np.zeros(max_pairs, np.uint64), values=np.zeros(max_pairs, np.uint32),
+                 values_sorted=np.zeros(max_pairs, np.uint32), ranges=np.zeros((gx * gy, 2), np.uint32),
+                 img=np.zeros((H, W, 3), f32), T_final=np.zeros((H, W), f32), n_contrib=np.zeros((H, W), np.int32))
+        c, cf = cam_struct(cam), cfg_struct(cfg)
+        fn = self.lib.ref_render_3d
+        fn.restype = _i64
+        arena = 256 * 1024 * 1024 + n * 4096 + H * W * 64 + max_pairs * 64
+        K = fn(_i32(n), _i32(model_deg), _i32(max_coeffs), *[_p(a) for a in arr], _p(flt), ctypes.byref(c),
+               ctypes.byref(cf), ctypes.c_size_t(arena), _i64(max_pairs), _p(o["means2D"]), _p(o["depths"]),
+               _p(o["op2d"]), _p(o["rgb"]), _p(o["radii"]), _p(o["tiles"]), _p(o["g2s"]), _p(o["cov3D_inv"]),
+               _p(o["mean_offset"]), _p(o["keys"]), _p(o["values"]), _p(o["values_sorted"]), _p(o["ranges"]),
+               _p(o["img"]), _p(o["T_final"]), _p(o["n_contrib"]))
+        if K < 0:
+            raise ValueError("max_pairs too small")
+        for k in ("keys", "values", "values_sorted"):
+            o[k] = o[k][:K]
+        o["K"] = int(K)
+        return o

@@ -18,6 +18,8 @@
 #include "cpu/rasterizer_cpu.h"
 #include "cpu/rasterizer_backward_cpu.h"
 #include "cpu/preprocessor_backward_cpu.h"
+#include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -142,6 +144,66 @@ float ref_train_steps(int N, int sh_degree, int max_coeffs, float* raw_pos, floa
     for (int s = 0; s < steps; s++) loss = tr->step(raw, cam, gt, cfg, tc, first_iteration + s).loss;
     if (img_out) std::memcpy(img_out, tr->last_rendered(), (size_t)cam.width * cam.height * 12);
     return loss;
+}
+
+// The eval_3D forward (PreprocessorCPU with RenderConfig::eval_3D, the
+// per-tile depth keys of TileBinnerCPU, the k-buffer RasterizerCPU) on
+// activated parameters.  The keys/values are returned in emission order; the
+// sorted values use a stable order of the keys (SorterCPU's std::sort can
+// only differ on key ties; its tile ranges are kept).  Returns the pair count,
+// or -1 if `max_pairs` is too small.
+long long ref_render_3d(int N, int sh_degree, int max_coeffs, float* pos, float* sc, float* rot, float* sh,
+                        float* op, float* filter3d, const gs_camera* camp, const gs_render_cfg* cfgp,
+                        size_t arena_bytes, long long max_pairs, float* means2D, float* depths, float* op2d,
+                        float* rgb, int* radii, int* tiles, float* g2s, float* cinv, float* moff, uint64_t* keys,
+                        uint32_t* values, uint32_t* values_sorted, uint32_t* ranges, float* img, float* T_final,
+                        int* n_contrib) {
+    GaussianData g;
+    g.count = N; g.sh_degree = sh_degree; g.max_coeffs = max_coeffs;
+    g.positions = pos; g.scales = sc; g.rotations = rot; g.sh_coeffs = sh; g.opacities = op;
+    g.filter_3D = filter3d;
+    Camera cam = to_cam(camp);
+    RenderConfig cfg = to_cfg(cfgp);
+    cfg.eval_3D = true;
+    FrameAllocator alloc(arena_bytes);
+    PreprocessorCPU pp; TileBinnerCPU bn; SorterCPU sr; RasterizerCPU rs;
+    PreprocessOutput pre = pp.process(g, cam, cfg, alloc, nullptr);
+    BinningOutput bin = bn.bin(pre, N, cam, cfg, alloc);
+    const long long K = bin.total_pairs;
+    if (K > max_pairs) return -1;
+    if (K > 0) {
+        std::memcpy(keys, bin.keys_unsorted, K * 8);
+        std::memcpy(values, bin.values_unsorted, K * 4);
+        sr.sort(bin, alloc);
+        std::vector<uint32_t> ord(K);
+        for (long long i = 0; i < K; i++) ord[i] = (uint32_t)i;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](uint32_t a, uint32_t b) { return bin.keys_unsorted[a] < bin.keys_unsorted[b]; });
+        for (long long i = 0; i < K; i++) {
+            bin.keys_sorted[i] = bin.keys_unsorted[ord[i]];
+            bin.values_sorted[i] = bin.values_unsorted[ord[i]];
+        }
+        std::memcpy(values_sorted, bin.values_sorted, K * 4);
+    }
+    std::memcpy(ranges, bin.tile_ranges, (size_t)bin.num_tiles * 8);
+    ForwardCache cache{};
+    rs.rasterize(pre, bin, cam, cfg, img, nullptr, &cache, &alloc);
+    const int npix = cam.width * cam.height;
+    std::memcpy(T_final, cache.T_final, npix * 4);
+    std::memcpy(n_contrib, cache.n_contrib, npix * 4);
+    for (int i = 0; i < N; i++) {
+        const bool vis = pre.radii[i] > 0;
+        radii[i] = pre.radii[i]; tiles[i] = pre.tiles_touched[i];
+        means2D[2 * i] = vis ? pre.means2D[2 * i] : 0; means2D[2 * i + 1] = vis ? pre.means2D[2 * i + 1] : 0;
+        depths[i] = vis ? pre.depths[i] : 0; op2d[i] = vis ? pre.opacities_2d[i] : 0;
+        for (int k = 0; k < 3; k++) {
+            rgb[3 * i + k] = vis ? pre.rgb[3 * i + k] : 0;
+            moff[3 * i + k] = vis ? pre.mean_offset[3 * i + k] : 0;
+        }
+        for (int k = 0; k < 6; k++) cinv[6 * i + k] = vis ? pre.cov3D_inv[6 * i + k] : 0;
+        for (int k = 0; k < 16; k++) g2s[16 * i + k] = vis ? pre.gauss2screen[16 * i + k] : 0;
+    }
+    return K;
 }
 
 }  // extern "C"

@@ -71,3 +71,54 @@ def u32(t):
     """torch int32/int64 tensor holding unsigned data -> numpy unsigned view."""
     a = t.cpu().numpy()
     return a.view(np.uint32) if a.dtype == np.int32 else a.view(np.uint64)
+
+
+# ---- eval_3D forward cases (activated parameters) ---------------------------
+def _activated(scene):
+    """RawGaussianParams::activate (train_types.cpp:6-36) via the oracle."""
+    from oracle.oracle import Oracle
+    return Oracle().activate(scene)
+
+
+def eval3d_reference_scene():
+    """PreprocessorEval3DTestScene of the reference's
+    tests/test_preprocessor_gpu.cpp:247-338: 3 Gaussians, 32x32, identity
+    view, w = z projection, SH degree 0."""
+    n, W, H = 3, 32, 32
+    pos = np.array([[i * 0.5 - 0.5, i * 0.3 - 0.3, 3.0 + i * 0.5] for i in range(n)], np.float32)
+    g = dict(pos=pos, scale=np.full((n, 3), 0.1, np.float32),
+             rot=np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1)), sh=np.full((n, 1, 3), 0.5, np.float32),
+             op=np.full(n, 0.8, np.float32))
+    view = np.zeros(16, np.float32)
+    view[[0, 5, 10, 15]] = 1.0
+    fx = np.float32(W) / (np.float32(2.0) * np.float32(0.5))
+    fy = np.float32(H) / (np.float32(2.0) * np.float32(0.5))
+    vp = np.zeros(16, np.float32)
+    vp[0], vp[5], vp[10], vp[11] = fx, fy, 1.0, 1.0
+    cam = Camera(view, vp, np.zeros(3, np.float32), 0.5, 0.5, W, H)
+    cfg = RenderCfg(bg_color=(0.0, 0.0, 0.0), training=True, sh_degree=0)
+    return dict(name="reference_eval3d_scene", g=g, n=n, deg=0, K=1, cam=cam, cfg=cfg,
+                filter3d=np.zeros(n, np.float32))
+
+
+def eval3d_cases(scale=1):
+    """Seeded eval_3D workloads: a cube seen from outside, a cloud with the
+    camera inside it (near-plane fallback bounds, camera-inside culls), and
+    large overlapping Gaussians (k-buffer overflow, > 16 blended per pixel)."""
+    from gs_scenes import gaussians_normal, gaussians_uniform_cube, make_camera
+    out = [eval3d_reference_scene()]
+    rng = np.random.Generator(np.random.Philox(77))
+    s = gaussians_uniform_cube(3000 * scale, 11, 1)
+    out.append(dict(name="cube_sh1", g=_activated(s), n=s.n, deg=1, K=s.max_coeffs,
+                    cam=make_camera((0.0, 0.0, -3.0), 160 * scale, 120 * scale, 60.0),
+                    cfg=RenderCfg(bg_color=(0.1, 0.2, 0.3), training=False, sh_degree=1),
+                    filter3d=rng.uniform(0.0, 0.02, s.n).astype(np.float32)))
+    s = gaussians_normal(4000 * scale, 12, 3, sigma=1.5, logscale=(-3.5, 0.6), op=(0.0, 1.5))
+    out.append(dict(name="inside_cloud_sh3", g=_activated(s), n=s.n, deg=3, K=s.max_coeffs,
+                    cam=make_camera((0.3, 0.2, -1.2), 128 * scale, 96 * scale, 70.0),
+                    cfg=RenderCfg(bg_color=(0.0, 0.0, 0.0), training=True, sh_degree=3), filter3d=None))
+    s = gaussians_uniform_cube(2000 * scale, 13, 0, logscale=(-2.2, 0.3), op=(1.5, 0.5))
+    out.append(dict(name="dense_kbuffer_sh0", g=_activated(s), n=s.n, deg=0, K=s.max_coeffs,
+                    cam=make_camera((0.0, 0.5, -2.8), 96 * scale, 64 * scale, 55.0),
+                    cfg=RenderCfg(bg_color=(1.0, 1.0, 1.0), training=False, sh_degree=0), filter3d=None))
+    return out
