@@ -468,8 +468,12 @@ __global__ __launch_bounds__(256) void tile_rays_kernel(Cam3 c, float4* __restri
     rays[t] = make_float4(d[0], d[1], d[2], 0.0f);
 }
 
-// One thread per Gaussian; its pairs in row-major tile order at the
-// exclusive offset (inclusive scan - own count), as tile_binner_cpu.cpp:64-98.
+// Pairs of each Gaussian in row-major tile order at its exclusive offset
+// (inclusive scan - own count), as tile_binner_cpu.cpp:64-98.  Each lane
+// reads one Gaussian; the wave then walks its 64 Gaussians one at a time,
+// all lanes emitting consecutive pairs of the current one, so the writes are
+// coalesced and a Gaussian covering thousands of tiles does not serialise
+// on one lane.
 __global__ __launch_bounds__(256) void duplicate3d_kernel(int n, const float2* __restrict__ means2D,
                                                           const float* __restrict__ depths,
                                                           const int* __restrict__ radii,
@@ -480,38 +484,55 @@ __global__ __launch_bounds__(256) void duplicate3d_kernel(int n, const float2* _
                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
                                                           uint32_t capacity, uint32_t* __restrict__ overflow) {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const int r = radii[i];
-    if (r <= 0) return;
-    const float2 m = means2D[i];
-    int mn[2], mx[2];
-    get_rect(m.x, m.y, r, gx, gy, mn, mx);
-    const uint32_t cnt = (uint32_t)((mx[1] - mn[1]) * (mx[0] - mn[0]));
-    uint32_t off = offsets_incl[i] - cnt;
-    const uint32_t db = __float_as_uint(depths[i]);
-    float ci[6], mo[3];
+    const int lane = threadIdx.x & 63;
+    int mn[2] = {0, 0}, mx[2] = {0, 0};
+    uint32_t off = 0, db = 0;
+    float ci[6] = {0, 0, 0, 0, 0, 0}, mo[3] = {0, 0, 0};
+    const int r = i < n ? radii[i] : 0;
+    if (r > 0) {
+        const float2 m = means2D[i];
+        get_rect(m.x, m.y, r, gx, gy, mn, mx);
+        off = offsets_incl[i] - (uint32_t)((mx[1] - mn[1]) * (mx[0] - mn[0]));
+        db = __float_as_uint(depths[i]);
 #pragma unroll
-    for (int k = 0; k < 6; k++) ci[k] = cinv[(size_t)i * 6 + k];
+        for (int k = 0; k < 6; k++) ci[k] = cinv[(size_t)i * 6 + k];
 #pragma unroll
-    for (int k = 0; k < 3; k++) mo[k] = moff[(size_t)i * 3 + k];
-    for (int y = mn[1]; y < mx[1]; y++)
-        for (int x = mn[0]; x < mx[0]; x++) {
+        for (int k = 0; k < 3; k++) mo[k] = moff[(size_t)i * 3 + k];
+    }
+    const int w = mx[0] - mn[0];
+    const int cnt = w * (mx[1] - mn[1]);
+    uint64_t busy = __ballot(cnt > 0);
+    while (busy) {
+        const int src = __ffsll((long long)busy) - 1;
+        busy &= busy - 1;
+        const int scnt = __shfl(cnt, src), sw = __shfl(w, src);
+        const int sx0 = __shfl(mn[0], src), sy0 = __shfl(mn[1], src);
+        const uint32_t soff = __shfl(off, src), sdb = __shfl(db, src);
+        float sci[6], smo[3];
+#pragma unroll
+        for (int k = 0; k < 6; k++) sci[k] = __shfl(ci[k], src);
+#pragma unroll
+        for (int k = 0; k < 3; k++) smo[k] = __shfl(mo[k], src);
+        const uint32_t gid = (uint32_t)(blockIdx.x * 256 + (threadIdx.x & ~63) + src);
+        for (int q = lane; q < scnt; q += 64) {
+            const int y = sy0 + q / sw, x = sx0 + q % sw;
             const uint32_t tile = (uint32_t)(y * gx + x);
-            uint32_t tb = db;
+            uint32_t tb = sdb;
             if (rays) {
                 const float4 rv = rays[tile];
                 const float vd[3] = {rv.x, rv.y, rv.z};
-                const float ptd = depth_along(ci, mo, vd);
+                const float ptd = depth_along(sci, smo, vd);
                 if (ptd > 0.0f) tb = __float_as_uint(ptd);
             }
-            if (off < capacity) {
-                keys[off] = ((uint64_t)tile << 32) | tb;
-                values[off] = (uint32_t)i;
+            const uint32_t o = soff + (uint32_t)q;
+            if (o < capacity) {
+                keys[o] = ((uint64_t)tile << 32) | tb;
+                values[o] = gid;
             } else if (overflow) {
                 *overflow = 1u;
             }
-            off++;
         }
+    }
 }
 
 // ---------------------------------------------------------------- raster ----
@@ -560,18 +581,19 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
         if (j < r1) {
             const uint32_t id = values[j];
             const float4* g4 = reinterpret_cast<const float4*>(g2s + (size_t)id * 16);
-            Entry3 e;
+            Entry3& e = sm[t];
             e.r0 = g4[0];
             e.r1 = g4[1];
             e.r3 = g4[3];
-#pragma unroll
-            for (int k = 0; k < 6; k++) e.ci[k] = cinv[(size_t)id * 6 + k];
-#pragma unroll
-            for (int k = 0; k < 3; k++) e.mo[k] = moff[(size_t)id * 3 + k];
+            const float2* c2 = reinterpret_cast<const float2*>(cinv + (size_t)id * 6);
+            const float2 a = c2[0], b = c2[1], d = c2[2];
+            e.ci[0] = a.x; e.ci[1] = a.y; e.ci[2] = b.x; e.ci[3] = b.y; e.ci[4] = d.x; e.ci[5] = d.y;
+            e.mo[0] = moff[(size_t)id * 3];
+            e.mo[1] = moff[(size_t)id * 3 + 1];
+            e.mo[2] = moff[(size_t)id * 3 + 2];
             e.op = op2d[id];
             e.depth = depths[id];
             e.id = id;
-            sm[t] = e;
         }
         __syncthreads();
         const int m = (int)((r1 - base) < (uint32_t)R3_BATCH ? (r1 - base) : (uint32_t)R3_BATCH);
@@ -819,6 +841,7 @@ extern "C" int gs_raster_fwd_3d(gs_ctx* c, const gs_camera* cam, const gs_prepro
                pre3d->mean_offset,
            "null array");
     G3_ARG(al(pre3d->gauss2screen, 16), "gauss2screen must be 16-byte aligned");
+    G3_ARG(al(pre3d->cov3D_inv, 8), "cov3D_inv must be 8-byte aligned");
     // rasterizer_cpu.cpp:16-21 reads the inverse uninitialised when the
     // view-projection matrix is singular; there the view depth orders the
     // k-buffer, as the binner's keys do (tile_binner_cpu.cpp:80).

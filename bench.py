@@ -227,6 +227,45 @@ def bench_large(hip, cfg, args):
     return out
 
 
+def bench_eval3d(hip, scene, cam, cfg, reps=5):
+    """The eval_3D forward (AAA-Gaussians 3D evaluation with the 16-entry
+    per-pixel k-buffer) on the headline scene and view: the whole stage chain
+    timed on the host around synchronized runs (the scan reads the pair count
+    back, as the reference's binner does), and each stage's device time alone."""
+    import torch
+    p = hip.upload_scene(scene)
+    out = hip.render_3d(p, cam, cfg, raw=True)  # warm-up: allocations, code load
+    hip.sync()
+    K = int(out["K"])
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o = hip.render_3d(p, cam, cfg, raw=True)
+        hip.sync()
+        ts.append(time.perf_counter() - t0)
+        del o
+    ms = 1e3 * sorted(ts)[len(ts) // 2]
+    pre, pre3d = out["pre"], out["pre3d"]
+    bg = tuple(cfg.bg_color)
+    st = {}
+    st["preprocess"] = median_ms(lambda: hip.preprocess_3d(p, cam, cfg, True, None, pre, pre3d))
+    offsets, _ = hip.scan_tiles(pre["tiles"])
+    st["scan"] = median_ms(lambda: hip.scan_tiles(pre["tiles"]))
+    st["duplicate"] = median_ms(lambda: hip.duplicate_3d(pre, pre3d, offsets, K, cam))
+    st["sort"] = median_ms(lambda: hip.sort_pairs(out["keys"], out["values"], cam.num_tiles))
+    img, T, nc = out["img"], out["T_final"], out["n_contrib"]
+    st["raster"] = median_ms(lambda: hip.raster_fwd_3d(pre, pre3d, out["values_sorted"], out["ranges"], cam, bg, img,
+                                                       T, nc), reps=3)
+    npix = cam.width * cam.height
+    res = {"workload": f"{scene.n} Gaussians SH{scene.sh_degree}, {cam.width}x{cam.height}, eval_3D",
+           "pairs": K, "ms": round(ms, 3), "fwd_mpix_s": round(npix / (ms * 1e-3) / 1e6, 1),
+           "stages_ms": {k: round(v, 4) for k, v in st.items()},
+           "mean_blended_per_pixel": round(float(nc.float().mean().item()), 2)}
+    del out, pre, pre3d, p
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(scene, cam, cfg, gt):
     """The reference's own Trainer::step (compiled from source into
     oracle/_ref) on the same workload, one full step on one core; falls back
@@ -308,6 +347,9 @@ def main():
                          "gs_tuning: radix_bits, raster_batch, raster_bwd_batch, bin_group, bin_tiles_per_lane, "
                          "persistent_per_cu, split_groups)")
     ap.add_argument("--large-workload", default="6m_8x1080p")
+    ap.add_argument("--no-eval3d", action="store_true", help="skip the eval_3D (k-buffer) forward leg")
+    ap.add_argument("--eval3d-only", action="store_true",
+                    help="only the eval_3D forward leg on the workload's first view (for profiling)")
     args = ap.parse_args()
 
     if args.traffic_probe:
@@ -332,6 +374,10 @@ def main():
     cfg = RenderCfg(sh_degree=3, training=True)
     tc = train_cfg(lr=1e-3)
     hip = GsHip(0)
+    if args.eval3d_only:
+        print(json.dumps({"eval3d": bench_eval3d(hip, scene, cam, RenderCfg(sh_degree=3, training=False))}),
+              flush=True)
+        return
     gt = torch.tensor(gt_np, device=hip.dev)
     tr = GsTrainer(hip, scene)
     tr.set_options(fused_adam=not args.no_fused_adam)  # the step at this size specifies a fused Adam
@@ -430,6 +476,11 @@ def main():
     del p
     log(f"hbm targets: {hbm_targets}")
 
+    eval3d = None
+    if not args.no_eval3d:
+        eval3d = bench_eval3d(hip, scene, cam, RenderCfg(sh_degree=3, training=False))
+        log(f"eval3d: {eval3d}")
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(hip, cfg, args)
@@ -464,6 +515,7 @@ def main():
         "fwd_mpix_s": round(mpix_s, 2),
         "batched": batched,
         "large": large,
+        "eval3d": eval3d,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "tuning": tuning,
         "roofline": roofline,

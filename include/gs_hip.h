@@ -91,7 +91,7 @@ typedef struct gs_preprocessed {
  * dilated 3D covariance, and position - camera position. */
 typedef struct gs_preprocessed_3d {
     float* gauss2screen; /* [N*16], 16-byte aligned */
-    float* cov3D_inv;    /* [N*6]  */
+    float* cov3D_inv;    /* [N*6], 8-byte aligned */
     float* mean_offset;  /* [N*3]  */
 } gs_preprocessed_3d;
 
