@@ -669,7 +669,12 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
         uint64_t* ko = to_out ? keys_out : pk;
         uint32_t* vo = vals ? (to_out ? vals_out : pv) : nullptr;
         hipError_t e;
+        // A last digit that would reach past end_bit is moved down to end at
+        // it: the bits above end_bit may be set (the depth-sort words carry
+        // the index there).  Re-sorting bits an earlier pass already sorted
+        // keeps a stable LSD sort correct.
         int shift = begin_bit + p * radix_bits;
+        if (shift + radix_bits > end_bit && end_bit - radix_bits >= begin_bit) shift = end_bit - radix_bits;
         switch (radix_bits) {
             case 4: e = sort_pass<4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
             case 5: e = sort_pass<5>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;

@@ -239,6 +239,28 @@ def test_radix_sort_stable_random_keys(hip, count, radix_bits):
     assert np.array_equal(u32(vo), vals[order])
 
 
+@pytest.mark.parametrize("radix_bits", [4, 5, 6, 7, 8])
+@pytest.mark.parametrize("key_bits", [32, 38])
+def test_radix_sort_ignores_bits_above_key_bits(hip, key_bits, radix_bits):
+    """Keys whose bits above key_bits are set (the depth-sort words carry the
+    Gaussian index there): the order is numpy's stable argsort of the low
+    key_bits, and whole keys travel.  A last digit that reads past key_bits
+    breaks this whenever radix_bits does not divide key_bits."""
+    _say("radix high bits", key_bits, radix_bits)
+    count = 50_000
+    rng = np.random.default_rng(key_bits * 10 + radix_bits)
+    low = rng.integers(0, 1 << 12, count, dtype=np.uint64) << np.uint64(key_bits - 12)  # ties
+    high = rng.integers(0, 1 << 20, count, dtype=np.uint64) << np.uint64(key_bits)
+    keys = high | low
+    vals = np.arange(count, dtype=np.uint32)
+    order = np.argsort(keys & np.uint64((1 << key_bits) - 1), kind="stable")
+    kt = torch.tensor(keys.view(np.int64), device=hip.dev)
+    vt = torch.tensor(vals.view(np.int32), device=hip.dev)
+    ko, vo = hip.radix_sort(kt, vt, key_bits=key_bits, radix_bits=radix_bits)
+    assert np.array_equal(u32(ko), keys[order])
+    assert np.array_equal(u32(vo), vals[order])
+
+
 def test_radix_sort_large_many_tiles(hip):
     """Above 4M keys (4096-key tiles, ~1300 of them): the one-sweep passes'
     look-back crosses many tiles; 40-bit keys with heavy ties, with and
