@@ -47,8 +47,19 @@ struct DeviceArray {
 // Device side of a PreprocessOutput (what device_data points at).
 struct PreprocessDevice {
     DeviceArray means2D, depths, conics, op2d, rgb, radii, tiles;
-    DeviceArray pos, scale, rot, sh, op;  // model upload
+    DeviceArray pos, scale, rot, sh, op, filter;  // model upload
+    DeviceArray g2s, cinv, moff;                   // eval_3D arrays
+    bool eval3d = false;
     int n = 0;
+    gs_preprocessed_3d view3d() const {
+        return gs_preprocessed_3d{g2s.as<float>(), cinv.as<float>(), moff.as<float>()};
+    }
+    void alloc3d(int N) {
+        size_t n = (size_t)N;
+        g2s.ensure(n * 64);
+        cinv.ensure(n * 24);
+        moff.ensure(n * 12);
+    }
     gs_preprocessed view() const {
         return gs_preprocessed{means2D.as<float>(), depths.as<float>(), conics.as<float>(), op2d.as<float>(),
                                rgb.as<float>(),     radii.as<int>(),    tiles.as<int>()};
@@ -98,7 +109,6 @@ static gs_camera to_gs(const Camera& c) {
 }
 
 static gs_render_cfg to_gs(const RenderConfig& c) {
-    if (c.eval_3D) throw std::runtime_error("eval_3D is not part of this path (2D rasterization only)");
     gs_render_cfg g;
     for (int i = 0; i < 3; i++) g.bg_color[i] = c.bg_color[i];
     g.scale_modifier = c.scale_modifier;
@@ -125,6 +135,13 @@ static const PreprocessDevice& pre_device(const PreprocessOutput& pre, int N, st
     upload(up->rgb.p, pre.rgb, n * 12, s);
     upload(up->radii.p, pre.radii, n * 4, s);
     upload(up->tiles.p, pre.tiles_touched, n * 4, s);
+    up->eval3d = pre.eval_3D && pre.cov3D_inv;
+    if (up->eval3d) {
+        up->alloc3d(N);
+        if (pre.gauss2screen) upload(up->g2s.p, pre.gauss2screen, n * 64, s);
+        upload(up->cinv.p, pre.cov3D_inv, n * 24, s);
+        upload(up->moff.p, pre.mean_offset, n * 12, s);
+    }
     return *up;
 }
 
@@ -160,7 +177,21 @@ PreprocessOutput PreprocessorHIP::process(const GaussianData& g, const Camera& c
     gs_camera c = to_gs(cam);
     gs_render_cfg f = to_gs(cfg);
     gs_preprocessed out = d.view();
-    gs_check(gs_preprocess_fwd(ctx_.get(), &gg, &c, &f, &out), "gs_preprocess_fwd");
+    d.eval3d = cfg.eval_3D;
+    if (cfg.eval_3D) {
+        // preprocessor_cpu.cpp:64-187; GaussianData::filter_3D is optional
+        d.alloc3d(N);
+        const float* flt = nullptr;
+        if (g.filter_3D) {
+            d.filter.ensure(n * 4);
+            upload(d.filter.p, g.filter_3D, n * 4, s);
+            flt = d.filter.as<float>();
+        }
+        gs_preprocessed_3d o3 = d.view3d();
+        gs_check(gs_preprocess_fwd_3d(ctx_.get(), &gg, flt, &c, &f, &out, &o3), "gs_preprocess_fwd_3d");
+    } else {
+        gs_check(gs_preprocess_fwd(ctx_.get(), &gg, &c, &f, &out), "gs_preprocess_fwd");
+    }
 
     PreprocessOutput o{};
     o.radii = alloc.allocate_array<int>(n);
@@ -179,9 +210,20 @@ PreprocessOutput PreprocessorHIP::process(const GaussianData& g, const Camera& c
         download(o.opacities_2d, d.op2d.p, n * 4, s);
         download(o.rgb, d.rgb.p, n * 12, s);
     }
-    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     o.gauss2screen = o.cov3D_inv = o.mean_offset = nullptr;
-    o.eval_3D = false;
+    o.eval_3D = cfg.eval_3D;
+    if (cfg.eval_3D) {
+        // types.h:77-79: allocated whenever eval_3D is on (preprocessor_cpu.cpp:25-29)
+        o.gauss2screen = alloc.allocate_array<float>(n * 16);
+        o.cov3D_inv = alloc.allocate_array<float>(n * 6);
+        o.mean_offset = alloc.allocate_array<float>(n * 3);
+        if (fill_host_outputs) {
+            download(o.gauss2screen, d.g2s.p, n * 64, s);
+            download(o.cov3D_inv, d.cinv.p, n * 24, s);
+            download(o.mean_offset, d.moff.p, n * 12, s);
+        }
+    }
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     o.device_data = dev_.get();
     return o;
 }
@@ -210,10 +252,20 @@ BinningOutput TileBinnerHIP::bin(const PreprocessOutput& pre, int N, const Camer
     d.K = (uint32_t)K;
     d.keys.ensure((size_t)K * 8 + 8);
     d.values.ensure((size_t)K * 4 + 4);
-    gs_check(gs_duplicate_keys(ctx_.get(), N, pd.means2D.as<float>(), pd.depths.as<float>(), pd.radii.as<int>(),
-                               d.offsets.as<uint32_t>(), cam.width, cam.height, 16, 16, d.keys.as<uint64_t>(),
-                               d.values.as<uint32_t>(), (uint32_t)K),
-             "gs_duplicate_keys");
+    if (pre.eval_3D && pd.eval3d) {
+        // tile_binner_cpu.cpp:78-89: per-tile depth keys
+        gs_camera c = to_gs(cam);
+        gs_preprocessed pv = pd.view();
+        gs_preprocessed_3d p3 = pd.view3d();
+        gs_check(gs_duplicate_keys_3d(ctx_.get(), N, &pv, &p3, d.offsets.as<uint32_t>(), &c, 16, 16,
+                                      d.keys.as<uint64_t>(), d.values.as<uint32_t>(), (uint32_t)K),
+                 "gs_duplicate_keys_3d");
+    } else {
+        gs_check(gs_duplicate_keys(ctx_.get(), N, pd.means2D.as<float>(), pd.depths.as<float>(), pd.radii.as<int>(),
+                                   d.offsets.as<uint32_t>(), cam.width, cam.height, 16, 16, d.keys.as<uint64_t>(),
+                                   d.values.as<uint32_t>(), (uint32_t)K),
+                 "gs_duplicate_keys");
+    }
     o.total_pairs = (int)K;
     o.keys_unsorted = o.keys_sorted = nullptr;
     o.values_unsorted = o.values_sorted = nullptr;
@@ -310,10 +362,19 @@ void RasterizerHIP::rasterize(const PreprocessOutput& pre, const BinningOutput& 
     nc_->ensure(npix * 4);
     last_->ensure(npix * 4);
     gs_preprocessed pv = pd.view();
-    gs_pixel_state px{T_->as<float>(), nc_->as<int>(), last_->as<uint32_t>()};
-    gs_check(gs_raster_fwd(ctx_.get(), cam.width, cam.height, &pv, bd.values_sorted.as<uint32_t>(),
-                           bd.ranges.as<uint32_t>(), cfg.bg_color, img_->as<float>(), &px),
-             "gs_raster_fwd");
+    if (pre.eval_3D && pd.eval3d) {
+        // rasterizer_cpu.cpp:61-135: the k-buffer blend
+        gs_camera c = to_gs(cam);
+        gs_preprocessed_3d p3 = pd.view3d();
+        gs_check(gs_raster_fwd_3d(ctx_.get(), &c, &pv, &p3, bd.values_sorted.as<uint32_t>(), bd.ranges.as<uint32_t>(),
+                                  cfg.bg_color, img_->as<float>(), T_->as<float>(), nc_->as<int>()),
+                 "gs_raster_fwd_3d");
+    } else {
+        gs_pixel_state px{T_->as<float>(), nc_->as<int>(), last_->as<uint32_t>()};
+        gs_check(gs_raster_fwd(ctx_.get(), cam.width, cam.height, &pv, bd.values_sorted.as<uint32_t>(),
+                               bd.ranges.as<uint32_t>(), cfg.bg_color, img_->as<float>(), &px),
+                 "gs_raster_fwd");
+    }
     download(out_img, img_->p, npix * 12, s);
     if (cache && allocator) {
         cache->T_final = allocator->allocate_array<float>(npix);
@@ -333,6 +394,7 @@ RasterizerBackwardHIP::~RasterizerBackwardHIP() = default;
 void RasterizerBackwardHIP::backward(const PreprocessOutput& pre, const BinningOutput& bin, const Camera& cam,
                                      const RenderConfig& cfg, const ForwardCache& cache, const float* d_image,
                                      int N, RasterGradOutput& rgrad) {
+    if (cfg.eval_3D) throw std::runtime_error("RasterizerBackwardHIP: eval_3D has no backward (the reference's 3D path is forward-only)");
     (void)cache;
     hipStream_t s = ctx_.get()->stream;
     const size_t npix = (size_t)cam.width * cam.height;
@@ -371,6 +433,7 @@ PreprocessorBackwardHIP::~PreprocessorBackwardHIP() = default;
 void PreprocessorBackwardHIP::backward(const GaussianData& g, const Camera& cam, const RenderConfig& cfg,
                                        const ForwardCache& cache, const RasterGradOutput& rgrad,
                                        const RawGaussianParams& raw, GradientOutput& grads) {
+    if (cfg.eval_3D) throw std::runtime_error("PreprocessorBackwardHIP: eval_3D has no backward (the reference's 3D path is forward-only)");
     if (!cache.pre || !cache.pre->radii) throw std::runtime_error("PreprocessorBackwardHIP: cache.pre->radii needed");
     hipStream_t s = ctx_.get()->stream;
     const size_t n = (size_t)g.count;
@@ -487,6 +550,7 @@ TrainerHIP::~TrainerHIP() {
 
 TrainerHIP::StepResult TrainerHIP::step(RawGaussianParams& params, const Camera& camera, const float* gt_image,
                                         const RenderConfig& cfg, const TrainConfig& tc, int iteration) {
+    if (cfg.eval_3D) throw std::runtime_error("TrainerHIP: eval_3D has no backward (the reference's 3D path is forward-only)");
     hipStream_t s = ctx_.get()->stream;
     if (!tr_ || params.count != n_ || params.max_coeffs != mc_ || params.sh_degree != deg_) {
         if (tr_) gs_trainer_destroy(tr_);

@@ -3,6 +3,7 @@
 // host arrays and compare them with the oracle.  Built into
 // tests/cpp/_build/libgs_stage_shim.so, linked against libgs_hip.so.
 #include <cstring>
+#include <stdexcept>
 #include <memory>
 #include <vector>
 
@@ -53,6 +54,37 @@ extern "C" int shim_render(int n, int deg, int mc, float* pos, float* sh, float*
                    std::unique_ptr<TileBinner>(new TileBinnerHIP(ctx)), std::unique_ptr<Sorter>(new SorterHIP(ctx)),
                    std::unique_ptr<Rasterizer>(new RasterizerHIP(ctx)), 512ull << 20);
         r.render(gd(n, deg, mc, pos, sh, sc, rot, op), cam_of(cam), cfg_of(cfg), img);
+    })
+}
+
+// The eval_3D forward through the stage objects (RenderConfig::eval_3D set),
+// with the ForwardCache's T_final / n_contrib; filter may be NULL.
+extern "C" int shim_render_3d(int n, int deg, int mc, float* pos, float* sh, float* sc, float* rot, float* op,
+                              float* filter, const gs_camera* cam, const gs_render_cfg* cfg, float* img,
+                              float* T_final, int* n_contrib, int* radii) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        PreprocessorHIP pp(ctx);
+        TileBinnerHIP bn(ctx);
+        SorterHIP sr(ctx);
+        RasterizerHIP rs(ctx);
+        FrameAllocator alloc(512ull << 20);
+        GaussianData g = gd(n, deg, mc, pos, sh, sc, rot, op);
+        g.filter_3D = filter;
+        Camera k = cam_of(cam);
+        RenderConfig f = cfg_of(cfg);
+        f.eval_3D = true;
+        PreprocessOutput pre = pp.process(g, k, f, alloc, nullptr);
+        if (!pre.eval_3D || !pre.gauss2screen || !pre.cov3D_inv || !pre.mean_offset)
+            throw std::runtime_error("eval_3D arrays missing from the preprocess output");
+        std::memcpy(radii, pre.radii, (size_t)n * 4);
+        BinningOutput bin = bn.bin(pre, n, k, f, alloc);
+        sr.sort(bin, alloc);
+        ForwardCache cache{};
+        rs.rasterize(pre, bin, k, f, img, nullptr, &cache, &alloc);
+        const size_t npix = (size_t)k.width * k.height;
+        std::memcpy(T_final, cache.T_final, npix * 4);
+        std::memcpy(n_contrib, cache.n_contrib, npix * 4);
     })
 }
 

@@ -166,3 +166,29 @@ def test_cpp_trainer_first_step_matches_oracle(shim, oracle, use_adam):
     ref = oracle.forward_backward(scene, cam, cfg, gt)
     assert abs(losses[0] - ref["loss"]) <= 1e-6
     assert losses[2] <= losses[0] + 1e-6  # the loss does not increase over steps
+
+
+def test_cpp_stages_eval3d_match_oracle(shim):
+    """RenderConfig::eval_3D through PreprocessorHIP, TileBinnerHIP, SorterHIP
+    and RasterizerHIP, against the oracle's 3D forward."""
+    from helpers import eval3d_cases
+    from oracle.oracle import Oracle3D
+    o3 = Oracle3D()
+    for c in eval3d_cases()[1:3]:
+        print("[case] cpp eval3d", c["name"], flush=True)
+        g = c["g"]
+        arr = [np.ascontiguousarray(g[k], np.float32) for k in ["pos", "sh", "scale", "rot", "op"]]
+        flt = None if c["filter3d"] is None else np.ascontiguousarray(c["filter3d"], np.float32)
+        H, W = c["cam"].height, c["cam"].width
+        img = np.zeros((H, W, 3), np.float32)
+        T = np.zeros((H, W), np.float32)
+        nc = np.zeros((H, W), np.int32)
+        radii = np.zeros(c["n"], np.int32)
+        cam, cfg = c["cam"].to_c(), c["cfg"].to_c()
+        _check(shim, shim.shim_render_3d(c["n"], c["deg"], c["K"], *[_p(a) for a in arr],
+                                         _p(flt) if flt is not None else None, ctypes.byref(cam), ctypes.byref(cfg),
+                                         _p(img), _p(T), _p(nc), _p(radii)), "shim_render_3d")
+        ref = o3.render(g, c["n"], c["deg"], c["K"], c["cam"], c["cfg"], c["filter3d"])
+        np.testing.assert_array_equal(radii, ref["radii"])
+        np.testing.assert_array_equal(nc, ref["n_contrib"])
+        assert_image_close(img, ref["img"])
