@@ -15,11 +15,10 @@
 // [N][16] row-major, cov3D_inv [N][6] (upper triangle), mean_offset [N][3].
 // The rasterizer stages rows 0, 1 and 3 of gauss2screen, the inverse
 // covariance, the mean offset, opacity and view depth of 256 Gaussians at a
-// time in LDS (96 B each); every lane of the tile's workgroup evaluates its
+// time in LDS (112 B each); every lane of the tile's workgroup evaluates its
 // pixel against each staged entry and keeps its 16 nearest entries (by depth
-// along its own ray) in registers, sorted, blending the front one whenever a
-// 17th arrives.  The k-buffer moves are unrolled selects over the 16 slots,
-// so no register array is indexed dynamically.
+// along its own ray) sorted in a ring in LDS, blending the front one whenever
+// a 17th arrives.
 #include <cmath>
 #include <cstring>
 
@@ -542,12 +541,19 @@ struct Entry3 {
     float mo[3];
     float op, depth;
     uint32_t id;
+    float cut;  // 1e-3 - ln(1 / (255 op)): half the squared distance beyond which alpha < 1/255
+    float pad[3];
 };
-static_assert(sizeof(Entry3) == 96, "one LDS record per Gaussian");
+static_assert(sizeof(Entry3) == 112, "one LDS record per Gaussian");
 
 constexpr int R3_BATCH = 256;
 
 // One workgroup per 16x16 tile, one pixel per lane (rasterizer_cpu.cpp:61-135).
+// Each lane's k-buffer is a 16-slot ring in LDS (slot-major, so the lanes of
+// a wave touch consecutive words): the front entry leaves by advancing the
+// head, and an entry is inserted from the back, moving only the deeper
+// entries.  The pairs arrive in order of depth at the tile's centre, so the
+// move count is usually zero or one.
 __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* __restrict__ ranges,
                                                        const uint32_t* __restrict__ values,
                                                        const float* __restrict__ g2s, const float* __restrict__ cinv,
@@ -556,6 +562,9 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
                                                        float bg0, float bg1, float bg2, float* __restrict__ img,
                                                        float* __restrict__ T_final, int* __restrict__ n_contrib) {
     __shared__ Entry3 sm[R3_BATCH];
+    __shared__ float s_kd[KBUF][256];
+    __shared__ float s_ka[KBUF][256];
+    __shared__ uint32_t s_ki[KBUF][256];
     const int tile = blockIdx.x;
     const int tx = tile % c.gx, ty = tile / c.gx;
     const int t = threadIdx.x;
@@ -567,12 +576,8 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
     const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
 
     float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
-    int cnt = 0, kc = 0;
+    int cnt = 0, kc = 0, head = 0;
     bool done = !inside;
-    float kd[KBUF], ka[KBUF];
-    uint32_t ki[KBUF];
-#pragma unroll
-    for (int k = 0; k < KBUF; k++) { kd[k] = 0.0f; ka[k] = 0.0f; ki[k] = 0u; }
 
     for (uint32_t base = r0; base < r1; base += R3_BATCH) {
         // the workgroup stops once every pixel is finished
@@ -591,9 +596,11 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
             e.mo[0] = moff[(size_t)id * 3];
             e.mo[1] = moff[(size_t)id * 3 + 1];
             e.mo[2] = moff[(size_t)id * 3 + 2];
-            e.op = op2d[id];
+            const float o = op2d[id];
+            e.op = o;
             e.depth = depths[id];
             e.id = id;
+            e.cut = 1e-3f + __logf(255.0f * o);
         }
         __syncthreads();
         const int m = (int)((r1 - base) < (uint32_t)R3_BATCH ? (r1 - base) : (uint32_t)R3_BATCH);
@@ -603,6 +610,22 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
                                   e.r0.w - e.r3.w * fx};
             const float ply[4] = {e.r1.x - e.r3.x * fy, e.r1.y - e.r3.y * fy, e.r1.z - e.r3.z * fy,
                                   e.r1.w - e.r3.w * fy};
+            {
+                // Far below the 1/255 cut the exact evaluation can be skipped:
+                // |m|^2 rcp(dd) is within 1e-6 (relative) of the exact squared
+                // distance, so 0.5 md (1 - 1e-5) > 1e-3 + ln(255 o) gives
+                // o e^power < e^-0.001 / 255 for the exact power, which the
+                // exact path also skips (NaN and inf fall through or agree).
+                const float dx = plx[1] * ply[2] - plx[2] * ply[1];
+                const float dy = plx[2] * ply[0] - plx[0] * ply[2];
+                const float dz = plx[0] * ply[1] - plx[1] * ply[0];
+                const float mx = plx[3] * ply[0] - plx[0] * ply[3];
+                const float my = plx[3] * ply[1] - plx[1] * ply[3];
+                const float mz = plx[3] * ply[2] - plx[2] * ply[3];
+                const float mm = mx * mx + my * my + mz * mz;  // must not overflow where the exact path does not
+                const float md = mm * __builtin_amdgcn_rcpf(dx * dx + dy * dy + dz * dz);
+                if (mm < 3.0e38f && 0.5f * md * (1.0f - 1e-5f) > e.cut) continue;
+            }
             const float power = -0.5f * ray_contrib(plx, ply, nullptr);
             if (power > 0.0f) continue;
             const float alpha = smin(0.99f, e.op * exp_ref(power));
@@ -614,68 +637,55 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
             }
             if (kc >= KBUF) {
                 // blend the front entry (the buffer is full)
-                const float a = ka[0];
+                const float a = s_ka[head][t];
                 const float tt = T * (1.0f - a);
                 if (tt < 0.0001f) {
                     done = true;
                     continue;
                 }
-                const uint32_t gid = ki[0];
+                const uint32_t gid = s_ki[head][t];
                 C0 += rgb[gid * 3] * a * T;
                 C1 += rgb[gid * 3 + 1] * a * T;
                 C2 += rgb[gid * 3 + 2] * a * T;
                 T = tt;
                 cnt++;
-#pragma unroll
-                for (int k = 0; k < KBUF - 1; k++) { kd[k] = kd[k + 1]; ka[k] = ka[k + 1]; ki[k] = ki[k + 1]; }
+                head = (head + 1) & (KBUF - 1);
                 kc = KBUF - 1;
             }
             // insertion after the last entry not deeper than pd
             int at = kc;
-            bool run = true;
-#pragma unroll
-            for (int k = KBUF - 1; k >= 0; k--) {
-                if (k < kc && run) {
-                    if (kd[k] > pd) at = k;
-                    else run = false;
-                }
+            while (at > 0) {
+                const int sl = (head + at - 1) & (KBUF - 1);
+                const float dprev = s_kd[sl][t];
+                if (!(dprev > pd)) break;
+                const int dst = (head + at) & (KBUF - 1);
+                s_kd[dst][t] = dprev;
+                s_ka[dst][t] = s_ka[sl][t];
+                s_ki[dst][t] = s_ki[sl][t];
+                at--;
             }
-#pragma unroll
-            for (int k = KBUF - 1; k >= 0; k--) {
-                const bool shift = k > at && k <= kc;
-                const bool put = k == at;
-                if (k > 0) {
-                    kd[k] = shift ? kd[k - 1] : (put ? pd : kd[k]);
-                    ka[k] = shift ? ka[k - 1] : (put ? alpha : ka[k]);
-                    ki[k] = shift ? ki[k - 1] : (put ? e.id : ki[k]);
-                } else {
-                    kd[0] = put ? pd : kd[0];
-                    ka[0] = put ? alpha : ka[0];
-                    ki[0] = put ? e.id : ki[0];
-                }
-            }
+            const int dst = (head + at) & (KBUF - 1);
+            s_kd[dst][t] = pd;
+            s_ka[dst][t] = alpha;
+            s_ki[dst][t] = e.id;
             kc++;
         }
         __syncthreads();
     }
     if (!inside) return;
     // flush, front to back
-    bool live = !done;
-#pragma unroll
-    for (int k = 0; k < KBUF; k++) {
-        if (live && k < kc) {
-            const float a = ka[k];
+    if (!done) {
+        for (int k = 0; k < kc; k++) {
+            const int sl = (head + k) & (KBUF - 1);
+            const float a = s_ka[sl][t];
             const float tt = T * (1.0f - a);
-            if (tt < 0.0001f) {
-                live = false;
-            } else {
-                const uint32_t gid = ki[k];
-                C0 += rgb[gid * 3] * a * T;
-                C1 += rgb[gid * 3 + 1] * a * T;
-                C2 += rgb[gid * 3 + 2] * a * T;
-                T = tt;
-                cnt++;
-            }
+            if (tt < 0.0001f) break;
+            const uint32_t gid = s_ki[sl][t];
+            C0 += rgb[gid * 3] * a * T;
+            C1 += rgb[gid * 3 + 1] * a * T;
+            C2 += rgb[gid * 3 + 2] * a * T;
+            T = tt;
+            cnt++;
         }
     }
     const int pix = py * c.W + px;
