@@ -337,13 +337,15 @@ static const BinningDevice& bin_device(const BinningOutput& bin, std::unique_ptr
 
 // ------------------------------------------------------------------ raster --
 RasterizerHIP::RasterizerHIP(HipContext& ctx)
-    : ctx_(ctx), img_(new DeviceArray()), T_(new DeviceArray()), nc_(new DeviceArray()), last_(new DeviceArray()) {}
+    : ctx_(ctx), img_(new DeviceArray()), T_(new DeviceArray()), nc_(new DeviceArray()), last_(new DeviceArray()),
+      depth_(new DeviceArray()) {}
 RasterizerHIP::~RasterizerHIP() = default;
 
 void RasterizerHIP::rasterize(const PreprocessOutput& pre, const BinningOutput& bin, const Camera& cam,
                               const RenderConfig& cfg, float* out_img, float* out_depth, ForwardCache* cache,
                               FrameAllocator* allocator) {
-    if (out_depth) throw std::runtime_error("RasterizerHIP: inverse-depth output is not part of this path");
+    if (out_depth && cfg.eval_3D)
+        throw std::runtime_error("RasterizerHIP: the eval_3D path has no inverse-depth output (nor does the reference's)");
     hipStream_t s = ctx_.get()->stream;
     const size_t npix = (size_t)cam.width * cam.height;
     int N = 0;
@@ -374,6 +376,14 @@ void RasterizerHIP::rasterize(const PreprocessOutput& pre, const BinningOutput& 
         gs_check(gs_raster_fwd(ctx_.get(), cam.width, cam.height, &pv, bd.values_sorted.as<uint32_t>(),
                                bd.ranges.as<uint32_t>(), cfg.bg_color, img_->as<float>(), &px),
                  "gs_raster_fwd");
+        if (out_depth) {
+            // rasterizer_cpu.cpp:170-172,197
+            depth_->ensure(npix * 4);
+            gs_check(gs_raster_inv_depth(ctx_.get(), cam.width, cam.height, &pv, bd.values_sorted.as<uint32_t>(),
+                                         bd.ranges.as<uint32_t>(), &px, depth_->as<float>()),
+                     "gs_raster_inv_depth");
+            download(out_depth, depth_->p, npix * 4, s);
+        }
     }
     download(out_img, img_->p, npix * 12, s);
     if (cache && allocator) {

@@ -105,7 +105,7 @@ public:
 
 private:
     HipContext& ctx_;
-    std::unique_ptr<gs3d_detail::DeviceArray> img_, T_, nc_, last_;
+    std::unique_ptr<gs3d_detail::DeviceArray> img_, T_, nc_, last_, depth_;
     std::unique_ptr<gs3d_detail::PreprocessDevice> pre_up_;
     std::unique_ptr<gs3d_detail::BinningDevice> bin_up_;
 };

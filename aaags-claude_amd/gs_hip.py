@@ -387,6 +387,18 @@ class GsHip:
         return dict(pre=pre, pre3d=pre3d, keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges,
                     img=img, T_final=T, n_contrib=nc, K=K)
 
+    def raster_inv_depth(self, pre, values_sorted, ranges, width, height, px):
+        """Inverse-depth output of the 2D forward, from its pixel state."""
+        d = self.dev
+        out = torch.empty((height, width), device=d)
+        ps = self._pre_struct(pre)
+        st = _PixelState(ptr(px["T_final"]), ptr(px["n_contrib"]), ptr(px["last_contrib"]))
+        vs = values_sorted if values_sorted.numel() else torch.zeros(1, dtype=torch.int32, device=d)
+        self._check(self.lib.gs_raster_inv_depth(self.ctx, width, height, ctypes.byref(ps), _vp(ptr(vs)),
+                                                 _vp(ptr(ranges)), ctypes.byref(st), _vp(ptr(out))),
+                    "gs_raster_inv_depth")
+        return out
+
     def l1_loss(self, img, gt):
         H, W = img.shape[0], img.shape[1]
         d_img = torch.empty_like(img)

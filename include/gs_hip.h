@@ -216,6 +216,14 @@ int gs_raster_fwd(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre
                   const uint32_t* values_sorted, const uint32_t* ranges, const float bg[3],
                   float* image, const gs_pixel_state* px);
 
+/* Rasterizer::rasterize's inverse-depth output, 2D path
+ * (cpu/rasterizer_cpu.cpp:170-172,197): per pixel, sum(alpha T / depth) over
+ * its contributors.  Run after gs_raster_fwd on the same inputs, with its
+ * pixel state (last_contrib is required); inv_depth is [H*W]. */
+int gs_raster_inv_depth(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre,
+                        const uint32_t* values_sorted, const uint32_t* ranges, const gs_pixel_state* px,
+                        float* inv_depth);
+
 /* gs_raster_fwd over the packed words of gs_bin_sorted_words. */
 int gs_raster_fwd_words(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre,
                         const uint64_t* words, const uint32_t* ranges, const float bg[3], float* image,

@@ -1434,3 +1434,40 @@ void gso_logf_check(float lo, float hi, long long* mismatch, long long* near) {
     *near = nr;
 }
 #endif
+
+#ifndef GS_ORACLE_DOUBLE
+/* ---- cpu/rasterizer_cpu.cpp:137-198 with out_depth (2D path): the
+ * inverse depth sum(alpha T / depth) over each pixel's contributors, in the
+ * forward's blend order (the `inv_depth` accumulator, lines 170-172, 197). ---- */
+void gso_rasterize_inv_depth(const float* means2D, const float* conics, const float* op2d, const float* depths,
+                             const uint32_t* values_sorted, const uint32_t* ranges, int W, int H, int tile_w,
+                             int tile_h, float* inv_depth) {
+    int gx = (W + tile_w - 1) / tile_w, gy = (H + tile_h - 1) / tile_h;
+    for (int ty = 0; ty < gy; ty++)
+        for (int tx = 0; tx < gx; tx++) {
+            int tid = ty * gx + tx;
+            uint32_t rs = ranges[tid * 2], re = ranges[tid * 2 + 1];
+            int x0 = tx * tile_w, y0 = ty * tile_h;
+            int x1 = SMIN(x0 + tile_w, W), y1 = SMIN(y0 + tile_h, H);
+            for (int py = y0; py < y1; py++)
+                for (int px = x0; px < x1; px++) {
+                    float T = 1.0f, inv = 0.0f;
+                    for (uint32_t j = rs; j < re; j++) {
+                        uint32_t id = values_sorted[j];
+                        float dx = means2D[id * 2] - (float)px;
+                        float dy = means2D[id * 2 + 1] - (float)py;
+                        float ca = conics[id * 3], cb = conics[id * 3 + 1], cc = conics[id * 3 + 2];
+                        float power = -0.5f * (ca * dx * dx + cc * dy * dy) - cb * dx * dy;
+                        if (power > 0.0f) continue;
+                        float alpha = SMIN(0.99f, op2d[id] * expf(power));
+                        if (alpha < 1.0f / 255.0f) continue;
+                        float test_T = T * (1.0f - alpha);
+                        if (test_T < 0.0001f) break;
+                        inv += (1.0f / depths[id]) * alpha * T;
+                        T = test_T;
+                    }
+                    inv_depth[py * W + px] = inv;
+                }
+        }
+}
+#endif

@@ -247,6 +247,20 @@ class Oracle:
         return self.l1_loss(f["img"], gt, double_sum=True)[0]
 
 
+def inv_depth_oracle(pre, values_sorted, ranges, width, height):
+    """gso_rasterize_inv_depth: the inverse-depth output of the 2D forward."""
+    L = ctypes.CDLL(os.path.join(BUILD, "libgs_oracle.so"))
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    out = np.zeros((height, width), np.float32)
+    vs = np.ascontiguousarray(values_sorted, np.uint32)
+    if vs.size == 0:
+        vs = np.zeros(1, np.uint32)
+    L.gso_rasterize_inv_depth(_p(f(pre["means2D"])), _p(f(pre["conics"])), _p(f(pre["op2d"])), _p(f(pre["depths"])),
+                              _p(vs), _p(np.ascontiguousarray(ranges, np.uint32)), _i32(width), _i32(height),
+                              _i32(16), _i32(16), _p(out))
+    return out
+
+
 class Oracle3D:
     """The eval_3D forward of the float oracle (gso_preprocess3d & co.)."""
 
@@ -434,3 +448,15 @@ class RefLib:
             o[k] = o[k][:K]
         o["K"] = int(K)
         return o
+
+    def inv_depth(self, g, n, model_deg, max_coeffs, cam, cfg):
+        """The reference's 2D forward with Rasterizer::rasterize's inverse-depth output."""
+        f32 = np.float32
+        arr = [np.ascontiguousarray(g[k], f32).copy() for k in ("pos", "scale", "rot", "sh", "op")]
+        img = np.zeros((cam.height, cam.width, 3), f32)
+        inv = np.zeros((cam.height, cam.width), f32)
+        c, cf = cam_struct(cam), cfg_struct(cfg)
+        arena = 256 * 1024 * 1024 + n * 4096 + cam.width * cam.height * 64
+        self.lib.ref_inv_depth(_i32(n), _i32(model_deg), _i32(max_coeffs), *[_p(a) for a in arr], ctypes.byref(c),
+                               ctypes.byref(cf), ctypes.c_size_t(arena), _p(img), _p(inv))
+        return img, inv

@@ -146,6 +146,38 @@ float ref_train_steps(int N, int sh_degree, int max_coeffs, float* raw_pos, floa
     return loss;
 }
 
+// The 2D forward with Rasterizer::rasterize's inverse-depth output
+// (rasterizer_cpu.cpp:170-172,197) on activated parameters; the pairs are
+// stably sorted as in ref_render_3d.
+int ref_inv_depth(int N, int sh_degree, int max_coeffs, float* pos, float* sc, float* rot, float* sh, float* op,
+                  const gs_camera* camp, const gs_render_cfg* cfgp, size_t arena_bytes, float* img,
+                  float* inv_depth) {
+    GaussianData g;
+    g.count = N; g.sh_degree = sh_degree; g.max_coeffs = max_coeffs;
+    g.positions = pos; g.scales = sc; g.rotations = rot; g.sh_coeffs = sh; g.opacities = op;
+    g.filter_3D = nullptr;
+    Camera cam = to_cam(camp);
+    RenderConfig cfg = to_cfg(cfgp);
+    FrameAllocator alloc(arena_bytes);
+    PreprocessorCPU pp; TileBinnerCPU bn; SorterCPU sr; RasterizerCPU rs;
+    PreprocessOutput pre = pp.process(g, cam, cfg, alloc, nullptr);
+    BinningOutput bin = bn.bin(pre, N, cam, cfg, alloc);
+    const long long K = bin.total_pairs;
+    if (K > 0) {
+        sr.sort(bin, alloc);
+        std::vector<uint32_t> ord(K);
+        for (long long i = 0; i < K; i++) ord[i] = (uint32_t)i;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](uint32_t a, uint32_t b) { return bin.keys_unsorted[a] < bin.keys_unsorted[b]; });
+        for (long long i = 0; i < K; i++) {
+            bin.keys_sorted[i] = bin.keys_unsorted[ord[i]];
+            bin.values_sorted[i] = bin.values_unsorted[ord[i]];
+        }
+    }
+    rs.rasterize(pre, bin, cam, cfg, img, inv_depth, nullptr, nullptr);
+    return 0;
+}
+
 // The eval_3D forward (PreprocessorCPU with RenderConfig::eval_3D, the
 // per-tile depth keys of TileBinnerCPU, the k-buffer RasterizerCPU) on
 // activated parameters.  The keys/values are returned in emission order; the

@@ -342,6 +342,25 @@ def test_raster_fwd_matches_oracle(hip, oracle, n, deg, seed, bg, batch):
     assert np.array_equal(u32(px["last_contrib"]), f["last_contrib"])
 
 
+@pytest.mark.parametrize("n,deg,seed", [(500, 3, 4), (2000, 3, 5)])
+def test_raster_inv_depth_matches_oracle(hip, oracle, n, deg, seed):
+    """gs_raster_inv_depth (Rasterizer::rasterize's out_depth) from the
+    forward's pixel state, against the oracle on the same stage inputs."""
+    from oracle.oracle import inv_depth_oracle
+    _say("inv depth", n, deg, seed)
+    scene, cam, gt = random_small_scene(n, deg, seed, width=100, height=75)
+    cfg = RenderCfg(sh_degree=3, training=True)
+    f = _oracle_stage_inputs(oracle, scene, cam, cfg)
+    pre = _to_dev(hip, {k: f["pre"][k] for k in ["means2D", "conics", "op2d", "rgb", "radii", "depths"]})
+    vs = torch.tensor(f["values_sorted"].view(np.int32), device=hip.dev)
+    rg = torch.tensor(f["ranges"].view(np.int32), device=hip.dev)
+    img, px = hip.raster_fwd(pre, vs, rg, cam.width, cam.height)
+    inv = _np(hip.raster_inv_depth(pre, vs, rg, cam.width, cam.height, px))
+    ref = inv_depth_oracle(f["pre"], f["values_sorted"], f["ranges"], cam.width, cam.height)
+    assert ref.max() > 0
+    assert np.abs(inv - ref).max() <= IMAGE_TOL * max(1.0, float(ref.max()))
+
+
 def test_l1_loss_fixtures(hip):
     """tests/test_data/l1_verify of the reference: bit-exact d_image, loss 1e-6 relative."""
     import json

@@ -106,3 +106,21 @@ def test_logf_vs_rounded_double_log(o3):
     total = int(np.float32(256.0).view(np.int32) - np.float32(0.99).view(np.int32)) + 1
     assert 0 < mm.value < 0.003 * total
     assert nr.value < 2000
+
+
+@pytest.mark.skipif(not RefLib.available(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("n,deg,seed", [(300, 1, 41), (3000, 3, 42)])
+def test_inv_depth_oracle_bitexact_vs_reference(oracle, n, deg, seed):
+    """gso_rasterize_inv_depth against RasterizerCPU::rasterize's out_depth
+    (rasterizer_cpu.cpp:170-172,197), both on the reference's own 2D forward."""
+    from gs_scenes import RenderCfg, random_small_scene
+    from oracle.oracle import inv_depth_oracle
+    scene, cam, _ = random_small_scene(n, deg, seed, width=90, height=70)
+    cfg = RenderCfg(sh_degree=3, training=False, bg_color=(0.1, 0.2, 0.3))
+    g = oracle.activate(scene)
+    rimg, rinv = RefLib().inv_depth(g, n, scene.sh_degree, scene.max_coeffs, cam, cfg)
+    f = oracle.forward(scene, cam, cfg)
+    np.testing.assert_array_equal(f["img"].view(np.uint32), rimg.view(np.uint32))
+    inv = inv_depth_oracle(f["pre"], f["values_sorted"], f["ranges"], cam.width, cam.height)
+    assert rinv.max() > 0
+    np.testing.assert_array_equal(inv.view(np.uint32), rinv.view(np.uint32))
