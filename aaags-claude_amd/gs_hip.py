@@ -399,6 +399,25 @@ class GsHip:
                     "gs_raster_inv_depth")
         return out
 
+    # ---------------------------------------------------------- PLY input --
+    def ply_info(self, path):
+        n, deg, mc, hf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.gs_ply_info(path.encode(), ctypes.byref(n), ctypes.byref(deg), ctypes.byref(mc),
+                                         ctypes.byref(hf)), "gs_ply_info")
+        return dict(n=n.value, sh_degree=deg.value, max_coeffs=mc.value, has_filter_3d=bool(hf.value))
+
+    def load_ply(self, path):
+        """loadPly into device tensors (activated, SH interleaved)."""
+        info = self.ply_info(path)
+        n, k, d = info["n"], info["max_coeffs"], self.dev
+        out = dict(pos=torch.empty((n, 3), device=d), scale=torch.empty((n, 3), device=d),
+                   rot=torch.empty((n, 4), device=d), sh=torch.empty((n, k, 3), device=d), op=torch.empty(n, device=d),
+                   filter3d=torch.empty(n, device=d) if info["has_filter_3d"] else None, **info)
+        self._check(self.lib.gs_ply_load(self.ctx, path.encode(), _vp(ptr(out["pos"])), _vp(ptr(out["scale"])),
+                                         _vp(ptr(out["rot"])), _vp(ptr(out["sh"])), _vp(ptr(out["op"])),
+                                         _vp(ptr(out["filter3d"]))), "gs_ply_load")
+        return out
+
     def l1_loss(self, img, gt):
         H, W = img.shape[0], img.shape[1]
         d_img = torch.empty_like(img)

@@ -229,6 +229,18 @@ int gs_raster_fwd_words(gs_ctx* ctx, int width, int height, const gs_preprocesse
                         const uint64_t* words, const uint32_t* ranges, const float bg[3], float* image,
                         const gs_pixel_state* px);
 
+/* ---- 3DGS PLY input: loadPly (include/ply_loader.h:12, src/ply_loader.cpp:32-189) ----
+ * gs_ply_info parses the header: vertex count, SH degree and max_coeffs
+ * (from the number of f_rest_* properties: 45/24/9 -> 3/2/1) and whether a
+ * filter_3D property is present.  gs_ply_load reads the vertices into
+ * device arrays ([N*3], [N*3], [N*4] 16-byte aligned, [N*max_coeffs*3], [N],
+ * filter_3d [N] or NULL), activated as loadPly does (exp scale, sigmoid
+ * opacity, normalised rotation) with the SH reordered to sh[k][ch].  Returns
+ * after the data is on the device. */
+int gs_ply_info(const char* path, int* count, int* sh_degree, int* max_coeffs, int* has_filter_3d);
+int gs_ply_load(gs_ctx* ctx, const char* path, float* positions, float* scales, float* rotations,
+                float* sh_coeffs, float* opacities, float* filter_3d);
+
 /* ---- eval_3D forward (AAA-Gaussians 3D evaluation, StopThePop k-buffer) ----
  * The stage-by-stage path of Renderer::render with RenderConfig::eval_3D
  * set (types.h:61): preprocess -> gs_scan_tiles -> gs_duplicate_keys_3d ->

@@ -460,3 +460,19 @@ class RefLib:
         self.lib.ref_inv_depth(_i32(n), _i32(model_deg), _i32(max_coeffs), *[_p(a) for a in arr], ctypes.byref(c),
                                ctypes.byref(cf), ctypes.c_size_t(arena), _p(img), _p(inv))
         return img, inv
+
+    def load_ply(self, path):
+        """The reference's loadPly (ply_loader.cpp:32-189)."""
+        n, mc, hf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        fn = self.lib.ref_load_ply
+        if fn(path.encode(), ctypes.byref(n), ctypes.byref(mc), ctypes.byref(hf), None, None, None, None, None,
+              None) != 0:
+            raise ValueError("loadPly failed: " + path)
+        N, K = n.value, mc.value
+        f32 = np.float32
+        o = dict(pos=np.zeros((N, 3), f32), scale=np.zeros((N, 3), f32), rot=np.zeros((N, 4), f32),
+                 sh=np.zeros((N, K, 3), f32), op=np.zeros(N, f32), filter3d=np.zeros(N, f32) if hf.value else None)
+        fn(path.encode(), ctypes.byref(n), ctypes.byref(mc), ctypes.byref(hf), _p(o["pos"]), _p(o["scale"]),
+           _p(o["rot"]), _p(o["sh"]), _p(o["op"]), _p(o["filter3d"]))
+        o.update(n=N, max_coeffs=K)
+        return o

@@ -18,6 +18,7 @@
 #include "cpu/rasterizer_cpu.h"
 #include "cpu/rasterizer_backward_cpu.h"
 #include "cpu/preprocessor_backward_cpu.h"
+#include "ply_loader.h"
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -144,6 +145,31 @@ float ref_train_steps(int N, int sh_degree, int max_coeffs, float* raw_pos, floa
     for (int s = 0; s < steps; s++) loss = tr->step(raw, cam, gt, cfg, tc, first_iteration + s).loss;
     if (img_out) std::memcpy(img_out, tr->last_rendered(), (size_t)cam.width * cam.height * 12);
     return loss;
+}
+
+// loadPly (ply_loader.cpp:32-189) into caller arrays sized from *count /
+// *max_coeffs (call with null arrays first to get them).  Returns 0, or -1 on
+// a loader exception.
+int ref_load_ply(const char* path, int* count, int* max_coeffs, int* has_filter, float* pos, float* sc,
+                 float* rot, float* sh, float* op, float* filter) {
+    try {
+        LoadedModel m = loadPly(path);
+        const GaussianData& g = m.data;
+        *count = g.count; *max_coeffs = g.max_coeffs; *has_filter = g.filter_3D != nullptr;
+        if (pos) {
+            const size_t n = (size_t)g.count;
+            std::memcpy(pos, g.positions, n * 12);
+            std::memcpy(sc, g.scales, n * 12);
+            std::memcpy(rot, g.rotations, n * 16);
+            std::memcpy(sh, g.sh_coeffs, n * g.max_coeffs * 12);
+            std::memcpy(op, g.opacities, n * 4);
+            if (filter && g.filter_3D) std::memcpy(filter, g.filter_3D, n * 4);
+        }
+        m.free();
+        return 0;
+    } catch (...) {
+        return -1;
+    }
 }
 
 // The 2D forward with Rasterizer::rasterize's inverse-depth output

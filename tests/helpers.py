@@ -122,3 +122,26 @@ def eval3d_cases(scale=1):
                     cam=make_camera((0.0, 0.5, -2.8), 96 * scale, 64 * scale, 55.0),
                     cfg=RenderCfg(bg_color=(1.0, 1.0, 1.0), training=False, sh_degree=0), filter3d=None))
     return out
+
+
+# ---- 3DGS PLY files ---------------------------------------------------------
+def write_ply(path, n, rest, seed, filter3d=False, normals=True, zero_rot_at=None):
+    """A binary little-endian 3DGS PLY with `rest` f_rest_* properties (raw,
+    pre-activation values), as the reference's loader reads (ply_loader.cpp:32-121).
+    Returns the raw property table and its names."""
+    rng = np.random.default_rng(seed)
+    names = ["x", "y", "z"] + (["nx", "ny", "nz"] if normals else []) + ["f_dc_0", "f_dc_1", "f_dc_2"]
+    names += [f"f_rest_{k}" for k in range(rest)] + ["opacity", "scale_0", "scale_1", "scale_2",
+                                                      "rot_0", "rot_1", "rot_2", "rot_3"]
+    if filter3d:
+        names.append("filter_3D")
+    raw = rng.normal(0.0, 1.0, (n, len(names))).astype(np.float32)
+    if zero_rot_at is not None:
+        for k in ("rot_0", "rot_1", "rot_2", "rot_3"):
+            raw[zero_rot_at, names.index(k)] = 0.0
+    hdr = ["ply", "format binary_little_endian 1.0", f"element vertex {n}"]
+    hdr += [f"property float {nm}" for nm in names] + ["end_header"]
+    with open(path, "wb") as f:
+        f.write(("\n".join(hdr) + "\n").encode())
+        f.write(raw.astype("<f4").tobytes())
+    return raw, names
