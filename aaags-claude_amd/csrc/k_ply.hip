@@ -60,6 +60,28 @@ __global__ __launch_bounds__(256) void ply_unpack_kernel(PlyLayout L, const floa
     if (filter && L.filter >= 0) filter[i] = v[L.filter];
 }
 
+// train_main.cpp:108-141, per vertex: x y z, f_dc, f_rest channel-major,
+// opacity, scale, rot -- the raw (pre-activation) values.
+__global__ __launch_bounds__(256) void ply_pack_kernel(int n, int max_coeffs, const float* __restrict__ pos,
+                                                       const float* __restrict__ scale, const float* __restrict__ rot,
+                                                       const float* __restrict__ sh, const float* __restrict__ op,
+                                                       float* __restrict__ rows) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int R = max_coeffs - 1;
+    const int props = 3 + 3 + 3 * R + 1 + 3 + 4;
+    float* v = rows + (size_t)i * props;
+    const float* s = sh + (size_t)i * max_coeffs * 3;
+    int k = 0;
+    v[k++] = pos[i * 3]; v[k++] = pos[i * 3 + 1]; v[k++] = pos[i * 3 + 2];
+    v[k++] = s[0]; v[k++] = s[1]; v[k++] = s[2];
+    for (int c = 0; c < 3; c++)
+        for (int b = 0; b < R; b++) v[k++] = s[3 + b * 3 + c];
+    v[k++] = op[i];
+    v[k++] = scale[i * 3]; v[k++] = scale[i * 3 + 1]; v[k++] = scale[i * 3 + 2];
+    v[k++] = rot[i * 4]; v[k++] = rot[i * 4 + 1]; v[k++] = rot[i * 4 + 2]; v[k++] = rot[i * 4 + 3];
+}
+
 // ply_loader.cpp:32-121: the header rules (first line "ply"; "element vertex
 // N"; "property float NAME" lines after it; required properties; the SH
 // degree from the count of f_rest_*).  Returns an error status or GS_OK; the
@@ -162,4 +184,50 @@ extern "C" int gs_ply_load(gs_ctx* c, const char* path, float* positions, float*
     (void)hipHostFree(host);
     if (e != hipSuccess) return gs_hip_status(e, "PLY upload / unpack kernel");
     return gs_hip_status(e2, "hipStreamSynchronize");
+}
+
+extern "C" int gs_ply_save(gs_ctx* c, const char* path, int n, int max_coeffs, const float* positions,
+                           const float* scales, const float* rotations, const float* sh_coeffs,
+                           const float* opacities) {
+    if (!c || !path) return gs_set_error(GS_ERR_INVALID, "null argument");
+    if (n <= 0) return gs_set_error(GS_ERR_INVALID, "gs_ply_save: no Gaussians");
+    if (!(max_coeffs == 1 || max_coeffs == 4 || max_coeffs == 9 || max_coeffs == 16))
+        return gs_set_error(GS_ERR_INVALID, "max_coeffs must be 1, 4, 9 or 16 (got %d)", max_coeffs);
+    if (!positions || !scales || !rotations || !sh_coeffs || !opacities)
+        return gs_set_error(GS_ERR_INVALID, "null parameter array");
+    const int R = max_coeffs - 1;
+    const int props = 3 + 3 + 3 * R + 1 + 3 + 4;
+    const size_t bytes = (size_t)n * props * 4;
+    void* dev;
+    int st = gs_ws_get(c, bytes, &dev);
+    if (st) return st;
+    void* host = nullptr;
+    hipError_t e = hipHostMalloc(&host, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return gs_hip_status(e, "hipHostMalloc(PLY vertices)");
+    gsp::ply_pack_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, max_coeffs, positions, scales, rotations,
+                                                                   sh_coeffs, opacities, static_cast<float*>(dev));
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipHostFree(host);
+        return gs_hip_status(e, "PLY pack / download");
+    }
+    // train_main.cpp:90-106: the header
+    std::string h = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(n) + "\n";
+    for (const char* nm : {"x", "y", "z", "f_dc_0", "f_dc_1", "f_dc_2"}) h += std::string("property float ") + nm + "\n";
+    for (int k = 0; k < 3 * R; k++) h += "property float f_rest_" + std::to_string(k) + "\n";
+    for (const char* nm : {"opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"})
+        h += std::string("property float ") + nm + "\n";
+    h += "end_header\n";
+    std::ofstream f(path, std::ios::binary);
+    bool ok = f.is_open();
+    if (ok) {
+        f.write(h.data(), (std::streamsize)h.size());
+        f.write(static_cast<const char*>(host), (std::streamsize)bytes);
+        ok = (bool)f;
+    }
+    (void)hipHostFree(host);
+    if (!ok) return gs_set_error(GS_ERR_INVALID, "failed to write PLY file: %s", path);
+    return GS_OK;
 }

@@ -418,6 +418,12 @@ class GsHip:
                                          _vp(ptr(out["filter3d"]))), "gs_ply_load")
         return out
 
+    def save_ply(self, path, p):
+        """savePly of raw device parameters (upload_scene's dict)."""
+        self._check(self.lib.gs_ply_save(self.ctx, path.encode(), p["n"], p["max_coeffs"], _vp(ptr(p["pos"])),
+                                         _vp(ptr(p["scale"])), _vp(ptr(p["rot"])), _vp(ptr(p["sh"])),
+                                         _vp(ptr(p["op"]))), "gs_ply_save")
+
     def l1_loss(self, img, gt):
         H, W = img.shape[0], img.shape[1]
         d_img = torch.empty_like(img)

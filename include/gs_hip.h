@@ -240,6 +240,10 @@ int gs_raster_fwd_words(gs_ctx* ctx, int width, int height, const gs_preprocesse
 int gs_ply_info(const char* path, int* count, int* sh_degree, int* max_coeffs, int* has_filter_3d);
 int gs_ply_load(gs_ctx* ctx, const char* path, float* positions, float* scales, float* rotations,
                 float* sh_coeffs, float* opacities, float* filter_3d);
+/* savePly (src/train_main.cpp:82-146): raw (pre-activation) device arrays
+ * written as a 3DGS PLY, SH reordered to the file's channel-major f_rest. */
+int gs_ply_save(gs_ctx* ctx, const char* path, int n, int max_coeffs, const float* positions,
+                const float* scales, const float* rotations, const float* sh_coeffs, const float* opacities);
 
 /* ---- eval_3D forward (AAA-Gaussians 3D evaluation, StopThePop k-buffer) ----
  * The stage-by-stage path of Renderer::render with RenderConfig::eval_3D

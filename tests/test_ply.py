@@ -81,3 +81,22 @@ def test_ply_header_errors(tmp_path):
     write_ply(str(p), 5, 24, seed=1)
     assert lib.gs_ply_info(str(p).encode(), *[ctypes.byref(x) for x in a]) == 0
     assert [x.value for x in a[:3]] == [5, 2, 9]
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not RefLib.available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("n,deg", [(500, 3), (300, 1), (200, 0)])
+def test_gpu_ply_save_roundtrip(hip, oracle, tmp_path, n, deg):
+    """gs_ply_save (savePly, train_main.cpp:82-146) read back by the
+    reference's loadPly equals the reference activation of the raw parameters
+    (the loader applies the same exp, sigmoid and normalisation)."""
+    from gs_scenes import gaussians_normal
+    print("[case] ply save", n, deg, flush=True)
+    scene = gaussians_normal(n, 60 + deg, deg)
+    path = str(tmp_path / "s.ply")
+    hip.save_ply(path, hip.upload_scene(scene))
+    got = RefLib().load_ply(path)
+    act = oracle.activate(scene)
+    assert got["n"] == n and got["max_coeffs"] == scene.max_coeffs
+    for k in ("pos", "scale", "rot", "sh", "op"):
+        np.testing.assert_array_equal(got[k].reshape(act[k].shape).view(np.uint32), act[k].view(np.uint32), err_msg=k)
