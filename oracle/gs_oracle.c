@@ -1471,3 +1471,281 @@ void gso_rasterize_inv_depth(const float* means2D, const float* conics, const fl
         }
 }
 #endif
+
+/* =====================================================================
+ * Tile-local central finite differences of the mean L1 loss (test harness
+ * for gradient checks on large scenes; no reference counterpart).
+ *
+ * L = (1/n) * sum over pixels and channels |img - gt|, n = 3 W H, with img the
+ * forward above (preprocess, stable (tile, depth, index) order, rasterize).
+ * One Gaussian reaches only the pixels of the tiles in its screen rectangle,
+ * so L(theta + h e_p) - L(theta - h e_p) is exactly the same difference summed
+ * over the union of its two perturbed rectangles, rendered with the Gaussians
+ * whose rectangles meet those tiles (everything else renders identically in
+ * both evaluations).  The caller passes that subset of the scene (raw
+ * parameters, ascending original index, so depth ties keep the full scene's
+ * index order), the index of the perturbed Gaussian inside it, and the tile
+ * region [rx0, rx1) x [ry0, ry1) the subset was gathered for.
+ *
+ * hs[p] is the step for parameter p (parameters with hs[p] <= 0 are skipped).
+ * grad[p] receives dL/dp for the perturbed Gaussian's raw parameters in the
+ * order pos[3], log-scale[3], quaternion[4], sh[max_coeffs*3], opacity logit.
+ * flags[p] counts the decisions that differ between the two evaluations, so a
+ * difference taken across a discontinuity can be recognised: the target's
+ * rectangle, cull or colour clamp, and per pixel the contributor count, the
+ * target's skip / clamp / blend state and the sign of img - gt.
+ * Returns 0, or 1 if a perturbed rectangle leaves the region (the caller
+ * widens it and calls again), or -1 on allocation failure.
+ * ===================================================================== */
+typedef struct { uint64_t key; int idx; } fdl_ent;
+
+static int fdl_cmp(const void* a, const void* b) {
+    uint64_t x = ((const fdl_ent*)a)->key, y = ((const fdl_ent*)b)->key;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+typedef struct {
+    real m2[2], depth, con[3], op, rgb[3], rgb_raw[3];
+    int r, mn[2], mx[2];
+} fdl_g;
+
+/* one Gaussian through activate + preprocess (N = 1 calls of the functions above) */
+static void fdl_one(int model_deg, int max_coeffs, const real* rp, const real* rs, const real* rr,
+                    const real* rsh, const real* ro, const gs_camera* cam, const gs_render_cfg* cfg, real* work,
+                    fdl_g* o) {
+    int K3 = max_coeffs * 3;
+    real *pos = work, *sc = work + 3, *rot = work + 6, *op = work + 10, *sh = work + 11;
+    PFX(activate)(1, max_coeffs, rp, rs, rr, rsh, ro, pos, sc, rot, sh, op);
+    real m2[2], dep, con[3], o2, rgb[3];
+    int rad, tl;
+    PFX(preprocess)(1, model_deg, max_coeffs, pos, sh, sc, rot, op, cam, cfg, m2, &dep, con, &o2, rgb, &rad, &tl,
+                    NULL, NULL, NULL, NULL, NULL);
+    o->m2[0] = m2[0]; o->m2[1] = m2[1]; o->depth = dep;
+    for (int k = 0; k < 3; k++) { o->con[k] = con[k]; o->rgb[k] = rgb[k]; }
+    o->op = o2; o->r = rad;
+    int gx = (cam->width + cfg->tile_w - 1) / cfg->tile_w, gy = (cam->height + cfg->tile_h - 1) / cfg->tile_h;
+    o->mn[0] = o->mn[1] = o->mx[0] = o->mx[1] = 0;
+    if (rad > 0) get_rect(m2, rad, gx, gy, cfg->tile_w, cfg->tile_h, o->mn, o->mx);
+    /* the unclamped colour, for the clamp flags */
+    real dir[3], len, cp[3] = {cam->cam_pos[0], cam->cam_pos[1], cam->cam_pos[2]};
+    int deg = SMIN(cfg->sh_degree, model_deg);
+    sh_raw(deg, sh, pos, cp, o->rgb_raw, dir, &len);
+    (void)K3;
+}
+
+/* Blend state of one pixel part-way through its list. */
+typedef struct { real T, C[3]; int cnt, stop; } fdl_state;
+
+/* Continues one pixel's blend from state st at merged position j0 (the list
+ * with the target, when tg is non-NULL, inserted at position ins); returns the
+ * sum over channels of |C + T bg - gt| and the decisions. */
+static double fdl_pixel(const fdl_ent* list, int n, int ins, const fdl_g* all, const fdl_g* tg, int px, int py,
+                        const float* bg, const float* gtp, fdl_state st, int j0, int* cnt_out, int* tstate,
+                        int* sgn) {
+    real T = st.T, C[3] = {st.C[0], st.C[1], st.C[2]};
+    int cnt = st.cnt, total = n + (tg ? 1 : 0);
+    *tstate = 0;
+    for (int j = st.stop ? total : j0; j < total; j++) {
+        const fdl_g* g;
+        int is_t = 0;
+        if (tg && j == ins) { g = tg; is_t = 1; }
+        else g = &all[list[(tg && j > ins) ? j - 1 : j].idx];
+        real dx = g->m2[0] - (real)px, dy = g->m2[1] - (real)py;
+        real power = (real)-0.5f * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
+        if (power > (real)0.0f) { if (is_t) *tstate = 1; continue; }
+        real a0 = g->op * EXPR(power);
+        real alpha = SMIN((real)0.99f, a0);
+        if (alpha < (real)(1.0f / 255.0f)) { if (is_t) *tstate = 2; continue; }
+        real test_T = T * ((real)1.0f - alpha);
+        if (test_T < (real)0.0001f) { if (is_t) *tstate = 3; break; }
+        for (int ch = 0; ch < 3; ch++) C[ch] += g->rgb[ch] * alpha * T;
+        T = test_T;
+        cnt++;
+        if (is_t) *tstate = a0 > (real)0.99f ? 5 : 4;
+    }
+    double s = 0.0;
+    int sg = 0;
+    for (int ch = 0; ch < 3; ch++) {
+        double d = (double)(C[ch] + T * (real)bg[ch]) - (double)gtp[ch];
+        s += fabs(d);
+        sg |= (d > 0 ? 1 : (d < 0 ? 2 : 0)) << (2 * ch);
+    }
+    *cnt_out = cnt;
+    *sgn = sg;
+    return s;
+}
+
+/* The blend state after the first k list entries (no target). */
+static fdl_state fdl_prefix(const fdl_ent* list, int k, const fdl_g* all, int px, int py) {
+    fdl_state st = {(real)1.0f, {0, 0, 0}, 0, 0};
+    for (int j = 0; j < k; j++) {
+        const fdl_g* g = &all[list[j].idx];
+        real dx = g->m2[0] - (real)px, dy = g->m2[1] - (real)py;
+        real power = (real)-0.5f * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
+        if (power > (real)0.0f) continue;
+        real alpha = SMIN((real)0.99f, g->op * EXPR(power));
+        if (alpha < (real)(1.0f / 255.0f)) continue;
+        real test_T = st.T * ((real)1.0f - alpha);
+        if (test_T < (real)0.0001f) { st.stop = 1; break; }
+        for (int ch = 0; ch < 3; ch++) st.C[ch] += g->rgb[ch] * alpha * st.T;
+        st.T = test_T;
+        st.cnt++;
+    }
+    return st;
+}
+
+/* whether a Gaussian passes the alpha cut at a pixel centre */
+static int fdl_passes(const fdl_g* g, int px, int py) {
+    if (g->r <= 0) return 0;
+    real dx = g->m2[0] - (real)px, dy = g->m2[1] - (real)py;
+    real power = (real)-0.5f * (g->con[0] * dx * dx + g->con[2] * dy * dy) - g->con[1] * dx * dy;
+    if (power > (real)0.0f) return 0;
+    return SMIN((real)0.99f, g->op * EXPR(power)) >= (real)(1.0f / 255.0f);
+}
+
+int PFX(fd_local)(int N, int model_deg, int max_coeffs, const real* rpos, const real* rscale, const real* rrot,
+                  const real* rsh, const real* rop, int target, const gs_camera* cam, const gs_render_cfg* cfg,
+                  const float* gt, int rx0, int ry0, int rx1, int ry1, const double* hs, double* grad, int* flags) {
+    int K3 = max_coeffs * 3, P = 3 + 3 + 4 + K3 + 1;
+    int W = cam->width, H = cam->height, tw = cfg->tile_w, th = cfg->tile_h;
+    int rw = rx1 - rx0, rh = ry1 - ry0, nt = rw * rh;
+    int ret = -1;
+    fdl_g* all = (fdl_g*)calloc((size_t)(N > 0 ? N : 1), sizeof(fdl_g));
+    real* work = (real*)malloc(sizeof(real) * (size_t)(11 + K3));
+    int* tstart = (int*)calloc((size_t)nt + 1, sizeof(int));
+    fdl_ent* lists = NULL;
+    if (!all || !work || !tstart) goto done;
+    /* every Gaussian of the subset at theta (the target's entry is not used) */
+    for (int i = 0; i < N; i++)
+        fdl_one(model_deg, max_coeffs, &rpos[i * 3], &rscale[i * 3], &rrot[i * 4], &rsh[(size_t)i * K3], &rop[i], cam,
+                cfg, work, &all[i]);
+    /* per region tile: the other Gaussians covering it, in (depth bits, index) order */
+    for (int pass = 0; pass < 2; pass++) {
+        int* fill = pass ? (int*)calloc((size_t)nt, sizeof(int)) : NULL;
+        if (pass && !fill) goto done;
+        for (int i = 0; i < N; i++) {
+            if (i == target || all[i].r <= 0) continue;
+            int x0 = SMAX(all[i].mn[0], rx0), x1 = SMIN(all[i].mx[0], rx1);
+            int y0 = SMAX(all[i].mn[1], ry0), y1 = SMIN(all[i].mx[1], ry1);
+            for (int y = y0; y < y1; y++)
+                for (int x = x0; x < x1; x++) {
+                    int t = (y - ry0) * rw + (x - rx0);
+                    if (!pass) tstart[t + 1]++;
+                    else {
+                        fdl_ent* e = &lists[tstart[t] + fill[t]++];
+                        e->key = ((uint64_t)depth_bits(all[i].depth) << 32) | (uint32_t)i;
+                        e->idx = i;
+                    }
+                }
+        }
+        if (!pass) {
+            for (int t = 0; t < nt; t++) tstart[t + 1] += tstart[t];
+            lists = (fdl_ent*)malloc(sizeof(fdl_ent) * (size_t)(tstart[nt] > 0 ? tstart[nt] : 1));
+            if (!lists) goto done;
+        } else free(fill);
+    }
+    for (int t = 0; t < nt; t++) qsort(&lists[tstart[t]], (size_t)(tstart[t + 1] - tstart[t]), sizeof(fdl_ent), fdl_cmp);
+
+    /* per region pixel: the blend state after the entries in front of the
+     * target's unperturbed depth key */
+    fdl_state* prefix = (fdl_state*)malloc(sizeof(fdl_state) * (size_t)nt * tw * th);
+    int pre_ok = prefix != NULL;
+    if (pre_ok) {
+        uint64_t key = ((uint64_t)depth_bits(all[target].depth) << 32) | (uint32_t)target;
+        for (int t = 0; t < nt; t++) {
+            const fdl_ent* L = &lists[tstart[t]];
+            int n = tstart[t + 1] - tstart[t], lo = 0, hi = n;
+            while (lo < hi) { int mid = (lo + hi) / 2; if (L[mid].key < key) lo = mid + 1; else hi = mid; }
+            int tx = rx0 + t % rw, ty = ry0 + t / rw;
+            int x0 = tx * tw, y0 = ty * th, x1 = SMIN(x0 + tw, W), y1 = SMIN(y0 + th, H);
+            for (int py = y0; py < y1; py++)
+                for (int px = x0; px < x1; px++)
+                    prefix[(size_t)t * tw * th + (py - y0) * tw + (px - x0)] = fdl_prefix(L, lo, all, px, py);
+        }
+    }
+    double inv_n = 1.0 / (3.0 * (double)W * (double)H);
+    real* raw = (real*)malloc(sizeof(real) * (size_t)P);
+    real* work2 = (real*)malloc(sizeof(real) * (size_t)(11 + K3));
+    if (!raw || !work2) { free(raw); free(work2); goto done; }
+    for (int p = 0; p < P; p++) {
+        fdl_g tg[2];
+        double h = hs[p];
+        if (!(h > 0.0)) continue;
+        for (int s = 0; s < 2; s++) {
+            for (int k = 0; k < 3; k++) raw[k] = rpos[target * 3 + k];
+            for (int k = 0; k < 3; k++) raw[3 + k] = rscale[target * 3 + k];
+            for (int k = 0; k < 4; k++) raw[6 + k] = rrot[target * 4 + k];
+            for (int k = 0; k < K3; k++) raw[10 + k] = rsh[(size_t)target * K3 + k];
+            raw[10 + K3] = rop[target];
+            raw[p] += s == 0 ? (real)h : (real)-h;
+            fdl_one(model_deg, max_coeffs, raw, raw + 3, raw + 6, raw + 10, raw + 10 + K3, cam, cfg, work2, &tg[s]);
+            if (tg[s].r > 0 && (tg[s].mn[0] < rx0 || tg[s].mn[1] < ry0 || tg[s].mx[0] > rx1 || tg[s].mx[1] > ry1)) {
+                ret = 1;
+                free(raw); free(work2); free(prefix);
+                goto done;
+            }
+        }
+        int fl = 0;
+        if ((tg[0].r > 0) != (tg[1].r > 0)) fl++;
+        for (int k = 0; k < 2; k++)
+            if (tg[0].mn[k] != tg[1].mn[k] || tg[0].mx[k] != tg[1].mx[k]) fl++;
+        for (int ch = 0; ch < 3; ch++)
+            if ((tg[0].rgb_raw[ch] + (real)0.5f < 0) != (tg[1].rgb_raw[ch] + (real)0.5f < 0)) fl++;
+        /* tiles of the union of the two rectangles */
+        int ux0 = rx1, uy0 = ry1, ux1 = rx0, uy1 = ry0;
+        for (int s = 0; s < 2; s++) {
+            if (tg[s].r <= 0) continue;
+            ux0 = SMIN(ux0, tg[s].mn[0]); uy0 = SMIN(uy0, tg[s].mn[1]);
+            ux1 = SMAX(ux1, tg[s].mx[0]); uy1 = SMAX(uy1, tg[s].mx[1]);
+        }
+        double diff = 0.0;
+        int same_depth = depth_bits(tg[0].depth) == depth_bits(all[target].depth) &&
+                         depth_bits(tg[1].depth) == depth_bits(all[target].depth);
+        for (int ty = uy0; ty < uy1; ty++)
+            for (int tx = ux0; tx < ux1; tx++) {
+                int t = (ty - ry0) * rw + (tx - rx0);
+                const fdl_ent* L = &lists[tstart[t]];
+                int n = tstart[t + 1] - tstart[t];
+                int ins[2];
+                const fdl_g* tgp[2];
+                for (int s = 0; s < 2; s++) {
+                    int in = tg[s].r > 0 && tx >= tg[s].mn[0] && tx < tg[s].mx[0] && ty >= tg[s].mn[1] &&
+                             ty < tg[s].mx[1];
+                    tgp[s] = in ? &tg[s] : NULL;
+                    uint64_t key = ((uint64_t)depth_bits(tg[s].depth) << 32) | (uint32_t)target;
+                    int lo = 0, hi = n;
+                    while (lo < hi) { int mid = (lo + hi) / 2; if (L[mid].key < key) lo = mid + 1; else hi = mid; }
+                    ins[s] = lo;
+                }
+                int x0 = tx * tw, y0 = ty * th, x1 = SMIN(x0 + tw, W), y1 = SMIN(y0 + th, H);
+                for (int py = y0; py < y1; py++)
+                    for (int px = x0; px < x1; px++) {
+                        const float* gp = &gt[((size_t)py * W + px) * 3];
+                        int c0, c1, s0, s1, g0, g1;
+                        fdl_state st = {(real)1.0f, {0, 0, 0}, 0, 0};
+                        int j0 = 0;
+                        if (same_depth && ins[0] == ins[1] && pre_ok) {
+                            /* the entries before the target render alike in both evaluations */
+                            int a0 = tgp[0] && fdl_passes(tgp[0], px, py), a1 = tgp[1] && fdl_passes(tgp[1], px, py);
+                            if (!a0 && !a1) continue;  /* the target is cut in both: identical pixels */
+                            st = prefix[(size_t)t * tw * th + (py - y0) * tw + (px - x0)];
+                            if (st.stop) continue;
+                            j0 = ins[0];
+                        }
+                        double lp = fdl_pixel(L, n, ins[0], all, tgp[0], px, py, cfg->bg_color, gp, st, j0, &c0, &s0,
+                                              &g0);
+                        double lm = fdl_pixel(L, n, ins[1], all, tgp[1], px, py, cfg->bg_color, gp, st, j0, &c1, &s1,
+                                              &g1);
+                        diff += lp - lm;
+                        if (c0 != c1 || s0 != s1 || g0 != g1) fl++;
+                    }
+            }
+        grad[p] = diff * inv_n / (2.0 * h);
+        flags[p] = fl;
+    }
+    free(raw); free(work2); free(prefix);
+    ret = 0;
+done:
+    free(all); free(work); free(tstart); free(lists);
+    return ret;
+}

@@ -247,6 +247,89 @@ class Oracle:
         return self.l1_loss(f["img"], gt, double_sum=True)[0]
 
 
+def _rects(means2D, radii, width, height, tile=16):
+    """getRect (math_utils.cpp:21-28) for every Gaussian: (int) truncation
+    toward zero, then the clamp to the tile grid.  Returns mn_x, mn_y, mx_x, mx_y."""
+    gx, gy = (width + tile - 1) // tile, (height + tile - 1) // tile
+    p = np.asarray(means2D, np.float64)
+    r = np.asarray(radii, np.float64)
+    out = []
+    for v, g, add in ((p[:, 0] - r, gx, 0), (p[:, 1] - r, gy, 0), (p[:, 0] + r, gx, tile - 1),
+                      (p[:, 1] + r, gy, tile - 1)):
+        out.append(np.clip(np.trunc((v + add) / tile), 0, g).astype(np.int64))
+    return out
+
+
+def fd_grads_local(scene, cam, cfg, gt, targets, h=1e-4, fallback=(1e-5, 1e-6), threads=None, margin=1):
+    """FP64 central finite differences (gsd_fd_local) of the mean L1 loss with
+    respect to the raw parameters of the Gaussians ``targets``, on a scene of
+    any size: each target is evaluated on the subset of Gaussians whose tile
+    rectangles meet its own rectangle grown by ``margin`` tiles (grown further
+    while a perturbed rectangle leaves it).
+
+    Every parameter is first differenced with step ``h``.  Where that
+    difference spans a discontinuity of the forward (a flipped alpha cut,
+    transmittance stop, clamp, rectangle or loss sign in any pixel), it is
+    taken again with the smaller steps of ``fallback`` in turn.  Returns
+    (grads, steps): both [len(targets), P] with P = 11 + 3 * max_coeffs in the
+    order pos, scale, rot, sh, opacity; steps holds the step each derivative
+    was taken with, or 0 where every step spans a discontinuity."""
+    from concurrent.futures import ThreadPoolExecutor
+    o = Oracle(double=True)
+    n, K = scene.n, scene.max_coeffs
+    g = o.activate(scene)
+    pre = o.preprocess(g, n, scene.sh_degree, K, cam, cfg, cache=False)
+    radii = pre["radii"]
+    mnx, mny, mxx, mxy = _rects(pre["means2D"], radii, cam.width, cam.height, cfg.tile_w)
+    gx, gy = (cam.width + cfg.tile_w - 1) // cfg.tile_w, (cam.height + cfg.tile_h - 1) // cfg.tile_h
+    raw = [np.ascontiguousarray(a, np.float64) for a in scene.arrays()]
+    gt = np.ascontiguousarray(gt, np.float32)
+    c, f = cam_struct(cam), cfg_struct(cfg)
+    fn = o.lib.gsd_fd_local
+    fn.restype = _i32
+    fn.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32, _i32, _i32,
+                   _vp, _vp, _vp]
+    P = 11 + 3 * K
+    live = radii > 0
+
+    def one(t):
+        m = margin
+        gr = np.zeros(P, np.float64)
+        used = np.zeros(P, np.float64)
+        todo = np.ones(P, bool)
+        for step in (h,) + tuple(fallback):
+            while True:
+                rx0, ry0 = max(0, int(mnx[t]) - m), max(0, int(mny[t]) - m)
+                rx1, ry1 = min(gx, int(mxx[t]) + m), min(gy, int(mxy[t]) + m)
+                sel = live & (mnx < rx1) & (mxx > rx0) & (mny < ry1) & (mxy > ry0)
+                sel[t] = True
+                idx = np.nonzero(sel)[0]
+                sub = [np.ascontiguousarray(a[idx]) for a in raw]
+                tl = int(np.searchsorted(idx, t))
+                hs = np.where(todo, step, 0.0)
+                g1 = np.zeros(P, np.float64)
+                fl = np.zeros(P, np.int32)
+                st = fn(len(idx), scene.sh_degree, K, *[_p(a) for a in sub], tl, ctypes.byref(c), ctypes.byref(f),
+                        _p(gt), rx0, ry0, rx1, ry1, _p(hs), _p(g1), _p(fl))
+                if st == 0:
+                    break
+                if st != 1 or (rx0 == 0 and ry0 == 0 and rx1 == gx and ry1 == gy):
+                    raise RuntimeError(f"gsd_fd_local failed ({st}) for Gaussian {t}")
+                m *= 2
+            clean = todo & (fl == 0)
+            gr[clean] = g1[clean]
+            used[clean] = step
+            todo &= ~clean
+            if not todo.any():
+                break
+        return gr, used
+
+    workers = threads or min(16, os.cpu_count() or 4)
+    with ThreadPoolExecutor(workers) as ex:
+        res = list(ex.map(one, [int(t) for t in targets]))
+    return np.stack([r[0] for r in res]), np.stack([r[1] for r in res])
+
+
 def inv_depth_oracle(pre, values_sorted, ranges, width, height):
     """gso_rasterize_inv_depth: the inverse-depth output of the 2D forward."""
     L = ctypes.CDLL(os.path.join(BUILD, "libgs_oracle.so"))
