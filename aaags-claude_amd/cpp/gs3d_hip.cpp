@@ -150,6 +150,45 @@ HipContext::HipContext(int device, void* stream) { gs_check(gs_ctx_create(device
 HipContext::~HipContext() { gs_ctx_destroy(ctx_); }
 void HipContext::synchronize() { gs_check(gs_ctx_synchronize(ctx_), "gs_ctx_synchronize"); }
 
+// ------------------------------------------------------------------- loss --
+double l1_loss_double(HipContext& ctx, const float* rendered, const float* gt, int H, int W, float* d_image) {
+    if (H < 0 || W < 0 || (H * W > 0 && (!rendered || !gt || !d_image)))
+        throw std::runtime_error("l1_loss: bad arguments");
+    hipStream_t s = ctx.get()->stream;
+    const size_t n = (size_t)H * W * 3;
+    DeviceArray r, g, d, loss;
+    r.ensure(n * 4);
+    g.ensure(n * 4);
+    d.ensure(n * 4);
+    loss.ensure(8);
+    upload(r.p, rendered, n * 4, s);
+    upload(g.p, gt, n * 4, s);
+    gs_check(gs_l1_loss(ctx.get(), W, H, r.as<float>(), g.as<float>(), d.as<float>(), loss.as<double>()),
+             "gs_l1_loss");
+    double out = 0.0;
+    download(d_image, d.p, n * 4, s);
+    download(&out, loss.p, 8, s);
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return out;
+}
+
+float l1_loss(HipContext& ctx, const float* rendered, const float* gt, int H, int W, float* d_image) {
+    return (float)l1_loss_double(ctx, rendered, gt, H, W, d_image);
+}
+
+static HipContext& default_context() {
+    static HipContext ctx(0);
+    return ctx;
+}
+
+double l1_loss_double(const float* rendered, const float* gt, int H, int W, float* d_image) {
+    return l1_loss_double(default_context(), rendered, gt, H, W, d_image);
+}
+
+float l1_loss(const float* rendered, const float* gt, int H, int W, float* d_image) {
+    return l1_loss(default_context(), rendered, gt, H, W, d_image);
+}
+
 // ------------------------------------------------------------ preprocess --
 PreprocessorHIP::PreprocessorHIP(HipContext& ctx) : ctx_(ctx), dev_(new PreprocessDevice()) {}
 PreprocessorHIP::~PreprocessorHIP() = default;
@@ -344,8 +383,6 @@ RasterizerHIP::~RasterizerHIP() = default;
 void RasterizerHIP::rasterize(const PreprocessOutput& pre, const BinningOutput& bin, const Camera& cam,
                               const RenderConfig& cfg, float* out_img, float* out_depth, ForwardCache* cache,
                               FrameAllocator* allocator) {
-    if (out_depth && cfg.eval_3D)
-        throw std::runtime_error("RasterizerHIP: the eval_3D path has no inverse-depth output (nor does the reference's)");
     hipStream_t s = ctx_.get()->stream;
     const size_t npix = (size_t)cam.width * cam.height;
     int N = 0;
@@ -371,6 +408,8 @@ void RasterizerHIP::rasterize(const PreprocessOutput& pre, const BinningOutput& 
         gs_check(gs_raster_fwd_3d(ctx_.get(), &c, &pv, &p3, bd.values_sorted.as<uint32_t>(), bd.ranges.as<uint32_t>(),
                                   cfg.bg_color, img_->as<float>(), T_->as<float>(), nc_->as<int>()),
                  "gs_raster_fwd_3d");
+        // rasterizer_cpu.cpp:53,197: the 3D branch accumulates no inverse depth
+        if (out_depth) std::fill(out_depth, out_depth + npix, 0.0f);
     } else {
         gs_pixel_state px{T_->as<float>(), nc_->as<int>(), last_->as<uint32_t>()};
         gs_check(gs_raster_fwd(ctx_.get(), cam.width, cam.height, &pv, bd.values_sorted.as<uint32_t>(),

@@ -39,6 +39,14 @@ private:
 
 void gs_check(int status, const char* what);  // throws std::runtime_error on failure
 
+// l1_loss / l1_loss_double (loss.h:6-10) on the device of `ctx`: d_image is
+// elementwise and bit-exact; the sum runs in double on the device, so the
+// float result is the mean rounded once (the reference's float version sums
+// serially in float).  The context-free declarations in gs3d_api.h use one
+// process-wide context on device 0.
+float l1_loss(HipContext& ctx, const float* rendered, const float* gt, int H, int W, float* d_image);
+double l1_loss_double(HipContext& ctx, const float* rendered, const float* gt, int H, int W, float* d_image);
+
 namespace gs3d_detail {
 struct DeviceArray;  // grow-only device allocation
 struct PreprocessDevice;
@@ -98,7 +106,9 @@ class RasterizerHIP : public Rasterizer {
 public:
     explicit RasterizerHIP(HipContext& ctx);
     ~RasterizerHIP() override;
-    // output_depth (inverse depth) is not produced by this path (nullptr only).
+    // output_depth receives the inverse depth sum(alpha T / depth) of each
+    // pixel's contributors (rasterizer_cpu.cpp:170-172,197); the eval_3D path
+    // writes 0 there, as the reference's 3D branch leaves it at 0.
     void rasterize(const PreprocessOutput& pre, const BinningOutput& bin, const Camera& cam, const RenderConfig& cfg,
                    float* output_image, float* output_depth = nullptr, ForwardCache* cache = nullptr,
                    FrameAllocator* allocator = nullptr) override;

@@ -149,9 +149,9 @@ extern "C" int gs_ply_info(const char* path, int* count, int* sh_degree, int* ma
     return GS_OK;
 }
 
-extern "C" int gs_ply_load(gs_ctx* c, const char* path, float* positions, float* scales, float* rotations,
-                           float* sh_coeffs, float* opacities, float* filter_3d) {
-    if (!c || !path) return gs_set_error(GS_ERR_INVALID, "null argument");
+extern "C" int gs_ply_load(gs_ctx* c, const char* path, int n, int max_coeffs, float* positions, float* scales,
+                           float* rotations, float* sh_coeffs, float* opacities, float* filter_3d) {
+    if (!path) return gs_set_error(GS_ERR_INVALID, "null path");
     if (!positions || !scales || !rotations || !sh_coeffs || !opacities)
         return gs_set_error(GS_ERR_INVALID, "null output array");
     if (reinterpret_cast<uintptr_t>(rotations) % 16) return gs_set_error(GS_ERR_INVALID, "rotations must be 16-byte aligned");
@@ -159,6 +159,14 @@ extern "C" int gs_ply_load(gs_ctx* c, const char* path, float* positions, float*
     gsp::PlyLayout L;
     int st = gsp::parse_header(f, path, L);
     if (st) return st;
+    // the caller's arrays were sized for (n, max_coeffs, filter): a file whose
+    // header says otherwise is refused before anything reaches the device
+    if (L.n != n || L.max_coeffs != max_coeffs)
+        return gs_set_error(GS_ERR_INVALID, "PLY %s holds %d Gaussians with %d SH coefficients; the arrays are for %d and %d",
+                            path, L.n, L.max_coeffs, n, max_coeffs);
+    if (filter_3d && L.filter < 0)
+        return gs_set_error(GS_ERR_INVALID, "PLY %s has no filter_3D property but a filter_3d array was passed", path);
+    if (!c) return gs_set_error(GS_ERR_INVALID, "null context");
     const size_t bytes = (size_t)L.n * L.props * 4;
     void* host = nullptr;
     hipError_t e = hipHostMalloc(&host, bytes, hipHostMallocDefault);

@@ -57,11 +57,32 @@ extern "C" int shim_render(int n, int deg, int mc, float* pos, float* sh, float*
     })
 }
 
+// Renderer::render with the inverse-depth output (RasterizerHIP out_depth).
+extern "C" int shim_render_depth(int n, int deg, int mc, float* pos, float* sh, float* sc, float* rot, float* op,
+                                 const gs_camera* cam, const gs_render_cfg* cfg, float* img, float* depth) {
+    SHIM_TRY({
+        HipContext ctx(0);
+        Renderer r(std::unique_ptr<Preprocessor>(new PreprocessorHIP(ctx)),
+                   std::unique_ptr<TileBinner>(new TileBinnerHIP(ctx)), std::unique_ptr<Sorter>(new SorterHIP(ctx)),
+                   std::unique_ptr<Rasterizer>(new RasterizerHIP(ctx)), 512ull << 20);
+        r.render(gd(n, deg, mc, pos, sh, sc, rot, op), cam_of(cam), cfg_of(cfg), img, depth);
+    })
+}
+
+// The free functions l1_loss / l1_loss_double (loss.h:6-10) on host arrays.
+extern "C" int shim_l1(const float* rendered, const float* gt, int H, int W, float* d_image, float* d_image2,
+                       float* loss_f, double* loss_d) {
+    SHIM_TRY({
+        *loss_f = l1_loss(rendered, gt, H, W, d_image);
+        *loss_d = l1_loss_double(rendered, gt, H, W, d_image2);
+    })
+}
+
 // The eval_3D forward through the stage objects (RenderConfig::eval_3D set),
 // with the ForwardCache's T_final / n_contrib; filter may be NULL.
 extern "C" int shim_render_3d(int n, int deg, int mc, float* pos, float* sh, float* sc, float* rot, float* op,
                               float* filter, const gs_camera* cam, const gs_render_cfg* cfg, float* img,
-                              float* T_final, int* n_contrib, int* radii) {
+                              float* T_final, int* n_contrib, int* radii, float* depth) {
     SHIM_TRY({
         HipContext ctx(0);
         PreprocessorHIP pp(ctx);
@@ -81,7 +102,7 @@ extern "C" int shim_render_3d(int n, int deg, int mc, float* pos, float* sh, flo
         BinningOutput bin = bn.bin(pre, n, k, f, alloc);
         sr.sort(bin, alloc);
         ForwardCache cache{};
-        rs.rasterize(pre, bin, k, f, img, nullptr, &cache, &alloc);
+        rs.rasterize(pre, bin, k, f, img, depth, &cache, &alloc);
         const size_t npix = (size_t)k.width * k.height;
         std::memcpy(T_final, cache.T_final, npix * 4);
         std::memcpy(n_contrib, cache.n_contrib, npix * 4);

@@ -54,6 +54,54 @@ def test_cpp_renderer_matches_oracle(shim, oracle, n, deg, seed):
     assert_image_close(img, ref["img"])
 
 
+def test_cpp_renderer_inverse_depth(shim, oracle):
+    """Renderer::render(..., output_depth) through RasterizerHIP: the image and
+    the inverse depth against the oracle (the depth replay restates the
+    reference's exact alpha, so it is bit-exact wherever the forward stopped at
+    the reference's last contributor)."""
+    from oracle.oracle import inv_depth_oracle
+    print("[case] cpp renderer inverse depth", flush=True)
+    scene, cam, gt = random_small_scene(1500, 3, 84, width=120, height=80)
+    cfg = RenderCfg(sh_degree=3, training=False)
+    act = _activated(oracle, scene)
+    img = np.zeros((cam.height, cam.width, 3), np.float32)
+    depth = np.zeros((cam.height, cam.width), np.float32)
+    c, f = cam.to_c(), cfg.to_c()
+    _check(shim, shim.shim_render_depth(scene.n, scene.sh_degree, scene.max_coeffs, *[_p(a) for a in act],
+                                        ctypes.byref(c), ctypes.byref(f), _p(img), _p(depth)), "shim_render_depth")
+    ref = oracle.forward(scene, cam, cfg)
+    assert_image_close(img, ref["img"])
+    inv = inv_depth_oracle(ref["pre"], ref["values_sorted"], ref["ranges"], cam.width, cam.height)
+    assert inv.max() > 0
+    assert (depth == inv).mean() > 0.99
+    assert np.abs(depth - inv).max() <= 1e-4 * max(1.0, float(inv.max()))
+
+
+def test_cpp_l1_loss_fixtures(shim):
+    """l1_loss / l1_loss_double (loss.h:6-10) on host arrays against the
+    reference's l1_verify fixtures: d_image bit-exact, the float loss within
+    1e-6 relative of the reference's serial float sum, the double loss within
+    1e-12 relative of an FP64 sum of the float differences (loss.cpp:18-30)."""
+    import json
+    from helpers import GOLDEN
+    base = os.path.join(GOLDEN, "l1_verify")
+    for case in json.load(open(os.path.join(base, "manifest.json"))):
+        print("[case] cpp l1", case["name"], flush=True)
+        H, W = case["H"], case["W"]
+        rd = lambda k: np.fromfile(os.path.join(base, f"{case['name']}_{k}.bin"), np.float32).reshape(H, W, 3)
+        r, g, want = rd("rendered"), rd("gt"), rd("d_image")
+        d1 = np.zeros_like(r)
+        d2 = np.zeros_like(r)
+        lf = ctypes.c_float()
+        ld = ctypes.c_double()
+        _check(shim, shim.shim_l1(_p(r), _p(g), H, W, _p(d1), _p(d2), ctypes.byref(lf), ctypes.byref(ld)), "shim_l1")
+        np.testing.assert_array_equal(d1, want)
+        np.testing.assert_array_equal(d2, want)
+        assert abs(lf.value - case["loss_serial"]) <= 1e-6 * abs(case["loss_serial"])
+        exact = np.abs((r - g).astype(np.float64)).sum() / r.size  # float differences, FP64 sum
+        assert abs(ld.value - exact) <= 1e-12 * exact
+
+
 def test_cpp_stages_outputs_match_oracle(shim, oracle):
     print("[case] cpp stages", flush=True)
     scene, cams, _ = make_workload("10k_256")
@@ -184,10 +232,13 @@ def test_cpp_stages_eval3d_match_oracle(shim):
         T = np.zeros((H, W), np.float32)
         nc = np.zeros((H, W), np.int32)
         radii = np.zeros(c["n"], np.int32)
+        depth = np.full((H, W), 7.0, np.float32)
         cam, cfg = c["cam"].to_c(), c["cfg"].to_c()
         _check(shim, shim.shim_render_3d(c["n"], c["deg"], c["K"], *[_p(a) for a in arr],
                                          _p(flt) if flt is not None else None, ctypes.byref(cam), ctypes.byref(cfg),
-                                         _p(img), _p(T), _p(nc), _p(radii)), "shim_render_3d")
+                                         _p(img), _p(T), _p(nc), _p(radii), _p(depth)), "shim_render_3d")
+        # rasterizer_cpu.cpp:53,197: the 3D branch writes an inverse depth of 0
+        assert not depth.any()
         ref = o3.render(g, c["n"], c["deg"], c["K"], c["cam"], c["cfg"], c["filter3d"])
         np.testing.assert_array_equal(radii, ref["radii"])
         np.testing.assert_array_equal(nc, ref["n_contrib"])

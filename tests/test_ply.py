@@ -100,3 +100,31 @@ def test_gpu_ply_save_roundtrip(hip, oracle, tmp_path, n, deg):
     assert got["n"] == n and got["max_coeffs"] == scene.max_coeffs
     for k in ("pos", "scale", "rot", "sh", "op"):
         np.testing.assert_array_equal(got[k].reshape(act[k].shape).view(np.uint32), act[k].view(np.uint32), err_msg=k)
+
+
+def test_ply_load_refuses_mismatched_header(tmp_path):
+    """gs_ply_load checks the file's header against the sizes the caller's
+    arrays were allocated for, before any device work (no GPU needed: with the
+    right sizes the call gets as far as the missing context)."""
+    import ctypes
+    import os
+    lib_path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "aaags-claude_amd", "lib",
+                            "libgs_hip.so")
+    if not os.path.exists(lib_path):
+        pytest.skip("libgs_hip.so not built")
+    lib = ctypes.CDLL(lib_path)
+    lib.gs_last_error.restype = ctypes.c_char_p
+    path = str(tmp_path / "m.ply")
+    write_ply(path, 50, 24, seed=5, filter3d=False)
+    n, deg, mc, hf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.gs_ply_info(path.encode(), ctypes.byref(n), ctypes.byref(deg), ctypes.byref(mc), ctypes.byref(hf)) == 0
+    assert (n.value, mc.value, hf.value) == (50, 9, 0)
+    fake = [ctypes.c_void_p(4096 * (k + 1)) for k in range(6)]  # never dereferenced: refused first
+
+    def load(nn, kk, filt):
+        return lib.gs_ply_load(None, path.encode(), nn, kk, *fake[:5], fake[5] if filt else None)
+
+    assert load(51, 9, False) == 1 and b"holds 50 Gaussians" in lib.gs_last_error()
+    assert load(50, 16, False) == 1 and b"9 SH coefficients" in lib.gs_last_error()
+    assert load(50, 9, True) == 1 and b"no filter_3D" in lib.gs_last_error()
+    assert load(50, 9, False) == 1 and b"null context" in lib.gs_last_error()
