@@ -77,7 +77,7 @@ struct gs_batch {
     DevBuf values, bviews_dev, bvblock_dev, bin_table, bin_scanned, bin_partial, bin_rects;
     DevBuf image, d_image, gt, T_final, n_contrib, last;
     DevBuf rg;  // screen-space gradients, one block of V*N: means2D (8 B), conics (12), rgb (12), opacities (4)
-    DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16
+    DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16, skipped @20
     int l1_nb = 0;
 
     gs_preprocessed pre() {
@@ -91,6 +91,8 @@ struct gs_batch {
     uint32_t* total32() { return reinterpret_cast<uint32_t*>(small.p); }
     unsigned long long* total64() { return reinterpret_cast<unsigned long long*>(static_cast<char*>(small.p) + 8); }
     uint32_t* overflow() { return reinterpret_cast<uint32_t*>(static_cast<char*>(small.p) + 16); }
+    // steps whose update the overflow flag suppressed since the last check
+    uint32_t* skipped() { return reinterpret_cast<uint32_t*>(static_cast<char*>(small.p) + 20); }
     ~gs_batch();
 };
 
@@ -435,9 +437,13 @@ static int enqueue_step(gs_batch* b, hipStream_t s) {
         Vv[k] = t->adam_v[k].f();
     }
     const gso::AdamDev* hp = static_cast<const gso::AdamDev*>(b->hp_dev.p);
+    // A step whose pair lists overflowed the buffers rendered truncated lists:
+    // its update is suppressed on the device (parameters and Adam moments are
+    // left as they were) and counted, so the caller can redo it after the
+    // regrow that batch_result performs.
     if (b->fused_adam) {
         B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), nullptr, b->V, b->radii.i(),
-                                          rg, pg, P, M, Vv, hp, nullptr, t->sh_degree, s),
+                                          rg, pg, P, M, Vv, hp, nullptr, t->sh_degree, s, b->overflow()),
               "multi-view preprocess backward + adam");
     } else {
         B_HIP(launch_preprocess_bwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), nullptr, b->V, b->radii.i(),
@@ -453,8 +459,10 @@ static int enqueue_step(gs_batch* b, hipStream_t s) {
         a.seg[3].g = pg.d_sh_coeffs;
         a.seg[4].g = pg.d_opacities;
         a.hp = hp;
+        a.skip = b->overflow();
         B_HIP(launch_adam(a, s), "adam");
     }
+    B_HIP(launch_count_if(b->overflow(), b->skipped(), s), "count suppressed updates");
     return GS_OK;
 }
 
@@ -498,11 +506,12 @@ static int batch_upload_hparams(gs_batch* b, const gs_train_cfg* tc, int iterati
 static int batch_result(gs_batch* b, gs_step_result* out, int iteration) {
     std::vector<double> L(b->V);
     unsigned long long K = 0;
-    uint32_t ovf = 0;
+    uint32_t flags[2] = {0, 0};  // overflow, suppressed updates
     B_HIP(hipMemcpyAsync(L.data(), b->losses.p, 8 * b->V, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
     B_HIP(hipMemcpyAsync(&K, b->total64(), 8, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
-    B_HIP(hipMemcpyAsync(&ovf, b->overflow(), 4, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
+    B_HIP(hipMemcpyAsync(flags, b->overflow(), 8, hipMemcpyDeviceToHost, b->stream), "hipMemcpyAsync");
     B_HIP(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    const uint32_t ovf = flags[0], skipped = flags[1];
     double s = 0.0;
     for (double x : L) s += x;
     if (out) {
@@ -512,13 +521,19 @@ static int batch_result(gs_batch* b, gs_step_result* out, int iteration) {
         out->pairs = (long long)K;
     }
     if (ovf) {
-        // some step since the last check dropped pairs: grow for the next one
-        B_HIP(launch_zero(b->overflow(), 4, b->stream), "zero overflow");
+        // some step since the last check dropped pairs: that step and every
+        // later one left the parameters and Adam state untouched; their Adam
+        // step numbers are handed back, and the buffers grow for the redo
+        B_HIP(launch_zero(b->overflow(), 8, b->stream), "zero overflow");
+        b->tr->adam_steps -= (int)skipped;
+        const uint32_t old_cap = b->capacity;
         unsigned long long cap = K + K / 4 + 65536;
         if (cap > 0xFFFFFFFFull) cap = 0xFFFFFFFFull;
         B_TRY(batch_size_pairs(b, (uint32_t)cap));
-        return gs_set_error(GS_ERR_CAPACITY, "a batched step had %llu pairs for a capacity of %u; buffers regrown", K,
-                            b->capacity);
+        return gs_set_error(GS_ERR_CAPACITY,
+                            "a batched step had %llu pairs for a capacity of %u: %u step(s) were not applied; "
+                            "buffers regrown to %u, repeat them",
+                            K, old_cap, skipped, b->capacity);
     }
     return GS_OK;
 }

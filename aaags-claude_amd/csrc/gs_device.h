@@ -418,5 +418,6 @@ struct AdamArgs {
     AdamSeg seg[5];
     float b1, b2, eps, bc1, bc2;
     const AdamDev* hp;  // when set, learning rates and the rest come from device memory
+    const uint32_t* skip = nullptr;  // when set and nonzero, the launch updates nothing
 };
 }  // namespace gso

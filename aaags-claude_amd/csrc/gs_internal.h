@@ -105,6 +105,9 @@ int l1_partials_needed(uint32_t n);
 hipError_t launch_l1(const float* r, const float* g, float* d, uint32_t n, double* partial, double* loss,
                      hipStream_t s);
 hipError_t launch_adam(const gso::AdamArgs& a, hipStream_t s);
+// One thread: *counter += 1 when *flag is nonzero (counts steps whose update a
+// capacity overflow suppressed).
+hipError_t launch_count_if(const uint32_t* flag, uint32_t* counter, hipStream_t s);
 hipError_t launch_sgd(const gso::AdamArgs& a, hipStream_t s);
 hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 
@@ -116,7 +119,7 @@ hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const gsd::CamArgs
                                        const gsd::CamArgs* cam_val, int V, const int* radii, const gs_raster_grads& rg,
                                        const gs_param_grads& out, float* const adam_p[5], float* const adam_m[5],
                                        float* const adam_v[5], const gso::AdamDev* hp, const gso::AdamDev* hp_val,
-                                       int sh_deg, hipStream_t s);
+                                       int sh_deg, hipStream_t s, const uint32_t* skip = nullptr);
 hipError_t launch_gather_tiles_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,
                                      const int* tiles, int* out, hipStream_t s);
 hipError_t launch_duplicate_multi(uint32_t total, int N, const uint64_t* skeys, const uint32_t* sidx,

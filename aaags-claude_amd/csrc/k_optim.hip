@@ -114,6 +114,7 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, flo
 }
 
 __global__ __launch_bounds__(256) void adam_kernel(AdamArgs args) {
+    if (args.skip && *args.skip) return;
     const AdamSeg& s = args.seg[blockIdx.y];
     AdamH a{args.b1, args.b2, args.eps, args.bc1, args.bc2};
     float lr = s.lr;
@@ -200,6 +201,15 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t s) {
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     adam_kernel<<<dim3(blocks, 5), 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+__global__ void count_if_kernel(const uint32_t* flag, uint32_t* counter) {
+    if (*flag) *counter += 1u;
+}
+
+hipError_t launch_count_if(const uint32_t* flag, uint32_t* counter, hipStream_t s) {
+    count_if_kernel<<<1, 1, 0, s>>>(flag, counter);
     return hipGetLastError();
 }
 

@@ -230,6 +230,8 @@ struct AdamState {
     float* v[5];
     const gso::AdamDev* hp;  // hyper-parameters in device memory (graph replays), or
     gso::AdamDev h0;         // by value when hp is null
+    const uint32_t* skip;    // when set and nonzero (a capacity overflow earlier in the
+                             // step), the launch returns without touching any state
 };
 
 // One Adam element (AdamOptimizer::adam_update, optimizer.cpp:60-101) on
@@ -289,6 +291,7 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
     constexpr int WROWS = 96;
     constexpr bool KEEP_P = FUSED_ADAM && (NB == MAXC);
     __shared__ float s_sh[FUSED_ADAM ? 4 * WROWS * ROW : 1];
+    if (FUSED_ADAM && ad.skip && *ad.skip) return;  // uniform over the grid
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = G.count;
     const bool valid = i < N;
@@ -511,7 +514,8 @@ hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const CamArgs* cam
 hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const CamArgs* cams_dev, const CamArgs* cam_val, int V,
                                        const int* radii, const gs_raster_grads& rg, const gs_param_grads& out,
                                        float* const adam_p[5], float* const adam_m[5], float* const adam_v[5],
-                                       const gso::AdamDev* hp, const gso::AdamDev* hp_val, int sh_deg, hipStream_t s) {
+                                       const gso::AdamDev* hp, const gso::AdamDev* hp_val, int sh_deg, hipStream_t s,
+                                       const uint32_t* skip) {
     dim3 grid((G.count + 255) / 256), block(256);
     CamArgs cam0{};
     if (!cams_dev) {
@@ -526,6 +530,7 @@ hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const CamArgs* cam
             ad.v[k] = adam_v[k];
         }
         ad.hp = hp;
+        ad.skip = skip;
         if (!hp) ad.h0 = *hp_val;
         GS_DISPATCH_SH(G.max_coeffs, sh_deg,
                        (preprocess_bwd_multi_kernel<MC, DG, true><<<grid, block, 0, s>>>(G, cams_dev, cam0, V, radii, rg,

@@ -127,3 +127,43 @@ def test_batched_loss_decreases(hip):
         b.step(tc, s)
     last, _ = b.result()
     assert last.loss < first * 0.97, (first, last.loss)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_overflowing_step_changes_nothing_and_redo_is_one_clean_step(hip, fused):
+    """A batched (graph-captured) step whose pairs outgrow the pre-sized
+    buffers: the step reports the overflow, leaves parameters and Adam
+    moments untouched, hands its Adam step number back, and regrows; the redo
+    then equals one clean step from the same parameters (a fresh trainer and
+    batch sized for them).  Exactly equal almost everywhere: the gradient
+    sums use float atomics, so a near-zero gradient can change sign, moving
+    that parameter by -+lr instead of +-lr."""
+    from gs_hip import GsBatch, GsError, GsTrainer, train_cfg
+    print("[case] batch overflow redo, fused", fused, flush=True)
+    scene, cams, gts = _setup()
+    cfg = RenderCfg(sh_degree=3, training=True)
+    big = scene.copy()
+    big.log_scale = big.log_scale + np.float32(2.0)  # ~50x the pairs
+    tc = train_cfg(lr=1e-3)
+    tr = GsTrainer(hip, scene)
+    b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=fused, use_graph=True)
+    b.render()  # sizes the pair buffers for the small Gaussians
+    _, _, cap = b.info()
+    tr.set_params(big)
+    with pytest.raises(GsError, match="not applied"):
+        b.step(tc, 0, sync=True)
+    for x, y in zip(tr.params(), big.arrays()):
+        assert np.array_equal(x, y.reshape(x.shape))
+    for k, shp in _shapes(scene).items():
+        assert not tr.buffer("adam_m_" + k, shp).any() and not tr.buffer("adam_v_" + k, shp).any()
+    assert b.info()[2] > cap
+    res = b.step(tc, 0, sync=True)  # the redo
+    redo = tr.params()
+    tr2 = GsTrainer(hip, big)
+    b2 = GsBatch(tr2, cams, cfg, gts, persistent=True, fused_adam=fused, use_graph=True)
+    res2 = b2.step(tc, 0, sync=True)
+    assert res.pairs == res2.pairs > cap
+    assert abs(res.loss - res2.loss) <= 1e-6
+    for x, y in zip(redo, tr2.params()):
+        assert np.abs(x - y).max() <= 2.5e-3
+        assert (x == y).mean() >= 0.99, (x == y).mean()
