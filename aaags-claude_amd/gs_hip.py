@@ -66,6 +66,12 @@ class _Tuning(ctypes.Structure):
                 for k, _ in self._fields_}
 
 
+class _Peaks(ctypes.Structure):
+    _fields_ = [("compute_units", _i32), ("clock_mhz", ctypes.c_double), ("hbm_copy_gbs", ctypes.c_double),
+                ("lds_read_gbs", ctypes.c_double), ("valu_wave_insts_per_s", ctypes.c_double),
+                ("exp_wave_insts_per_s", ctypes.c_double)]
+
+
 class GsError(RuntimeError):
     pass
 
@@ -399,6 +405,26 @@ class GsHip:
                     "gs_raster_inv_depth")
         return out
 
+    def raster_work(self, pre, values_sorted, ranges, width, height, px):
+        """gs_raster_work: the reference loops' pixel-Gaussian evaluations on
+        this frame: dict(fwd_evals, bwd_evals, contributors)."""
+        d = self.dev
+        out = torch.zeros(3, dtype=torch.int64, device=d)
+        ps = self._pre_struct(pre)
+        st = _PixelState(ptr(px["T_final"]), ptr(px["n_contrib"]), ptr(px["last_contrib"]))
+        vs = values_sorted if values_sorted.numel() else torch.zeros(1, dtype=torch.int32, device=d)
+        self._check(self.lib.gs_raster_work(self.ctx, width, height, ctypes.byref(ps), _vp(ptr(vs)),
+                                            _vp(ptr(ranges)), ctypes.byref(st), _vp(ptr(out))), "gs_raster_work")
+        c = [int(x) for x in out.cpu().tolist()]
+        return dict(fwd_evals=c[0], bwd_evals=c[1], contributors=c[2])
+
+    def device_peaks(self) -> dict:
+        """gs_device_peaks: measured HBM copy, LDS read and VALU issue ceilings."""
+        p = _Peaks()
+        self._check(self.lib.gs_device_peaks(self.ctx, ctypes.byref(p)), "gs_device_peaks")
+        return {k: (round(getattr(p, k), 1) if isinstance(getattr(p, k), float) else getattr(p, k))
+                for k, _ in p._fields_}
+
     # ---------------------------------------------------------- PLY input --
     def ply_info(self, path):
         n, deg, mc, hf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -413,7 +439,8 @@ class GsHip:
         out = dict(pos=torch.empty((n, 3), device=d), scale=torch.empty((n, 3), device=d),
                    rot=torch.empty((n, 4), device=d), sh=torch.empty((n, k, 3), device=d), op=torch.empty(n, device=d),
                    filter3d=torch.empty(n, device=d) if info["has_filter_3d"] else None, **info)
-        self._check(self.lib.gs_ply_load(self.ctx, path.encode(), _vp(ptr(out["pos"])), _vp(ptr(out["scale"])),
+        self._check(self.lib.gs_ply_load(self.ctx, path.encode(), _i32(n), _i32(k), _vp(ptr(out["pos"])),
+                                         _vp(ptr(out["scale"])),
                                          _vp(ptr(out["rot"])), _vp(ptr(out["sh"])), _vp(ptr(out["op"])),
                                          _vp(ptr(out["filter3d"]))), "gs_ply_load")
         return out
@@ -571,6 +598,14 @@ class GsTrainer:
                 self.lib.gs_trainer_destroy(self.tr)
         except Exception:
             pass
+
+    def set_params(self, scene):
+        """Replaces the raw parameters (same Gaussian count and SH size); the
+        optimizer state is kept."""
+        assert scene.n == self.n and scene.max_coeffs == self.k
+        arrs = [np.ascontiguousarray(a, np.float32) for a in scene.arrays()]
+        self.hip._check(self.lib.gs_trainer_set_params(self.tr, *[a.ctypes.data_as(_vp) for a in arrs], 1),
+                        "gs_trainer_set_params")
 
     def step(self, cam, cfg, gt, tc, iteration):
         res = _StepResult()

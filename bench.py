@@ -33,9 +33,23 @@ def log(*a):
 
 
 # a wave64 FP32 VALU instruction occupies a SIMD for 4 cycles (16 lanes a
-# cycle; aaags-claude_amd/lib/valu_probe measures it): 256 CUs x 4 SIMDs x
-# 2.4 GHz / 4
+# cycle; gs_device_peaks measures it): 256 CUs x 4 SIMDs x 2.4 GHz / 4
 VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
+
+# FP32 operations per pixel-Gaussian evaluation of the reference's raster
+# loops (cpu/rasterizer_cpu.cpp:141-176, cpu/rasterizer_backward_cpu.cpp:34-134),
+# counted operation by operation:
+# - every evaluated entry: dx and dy (2), the power -0.5 (a dx^2 + c dy^2) - b dx dy
+#   (6 multiplies, 2 adds, 1 scaling: 9), the power > 0 test, exp, opacity * exp,
+#   the 0.99 clamp and the 1/255 test (5): 16;
+# - forward, per contributor: 1 - alpha, T (1 - alpha), the 1e-4 test and three
+#   colour terms rgb * alpha * T accumulated (9): 12;
+# - backward, per contributor: 1 - alpha and T_i (2), d_rgb (7), d_alpha (12),
+#   accum_rec (9), d_power and d_opacity (3) and its sum (1), d_means2D (10),
+#   d_conics (12): 56.
+FLOPS_PER_EVAL, FLOPS_FWD_CONTRIB, FLOPS_BWD_CONTRIB = 16, 12, 56
+# LDS bytes an evaluation reads: mean 8, conic 12, opacity 4, colour 12
+LDS_BYTES_PER_EVAL = 36
 
 
 def load_traffic(workload):
@@ -266,6 +280,21 @@ def bench_eval3d(hip, scene, cam, cfg, reps=5):
     return res
 
 
+def raster_work(hip, tr, scene, cam, cfg):
+    """The reference loops' pixel-Gaussian evaluations on the trainer's current
+    parameters and the headline view (gs_raster_work on a forward with the
+    complete pair list)."""
+    from gs_scenes import RawScene
+    import torch
+    cur = RawScene(*tr.params(), scene.sh_degree)
+    p = hip.upload_scene(cur)
+    f = hip.forward_fast(p, cam, cfg)
+    w = hip.raster_work(f["pre"], f["values_sorted"], f["ranges"], cam.width, cam.height, f["px"])
+    del f, p
+    torch.cuda.empty_cache()
+    return w
+
+
 def cpu_baseline(scene, cam, cfg, gt):
     """The reference's own Trainer::step (compiled from source into
     oracle/_ref) on the same workload, one full step on one core; falls back
@@ -440,16 +469,46 @@ def main():
     batch = (tuning or {}).get("raster_batch" if dom == "raster_fwd" else "raster_bwd_batch")
     by_batch = st_meas.get("valu_wave_insts_by_batch", {})
     insts = by_batch.get(str(batch)) or (sorted(by_batch.values())[len(by_batch) // 2] if by_batch else None)
-    if dom in ("raster_fwd", "raster_bwd") and insts:
-        # the rasterizers are VALU-bound: wave64 FP32 VALU issue, 4 cycles per
-        # instruction per SIMD, 4 SIMDs x 256 CUs at 2.4 GHz
-        achieved = insts / (dom_ms * 1e-3) / 1e9
-        peak = VALU_WAVE_INSTS_PER_S / 1e9
-        roofline = {"kernel": kernel_names[dom], "bound": "valu", "achieved": round(achieved, 1),
-                    "peak": round(peak, 1), "unit": "G VALU wave-instructions/s", "frac": round(achieved / peak, 4),
-                    "traffic": st_meas.get("traffic_bytes"), "valu_wave_insts": insts,
-                    "hbm_achieved_gbs": round(hbm_gbs, 1), "hbm_frac": round(hbm_gbs / HBM_PEAK_GBS, 4),
-                    "algorithmic_bytes": int(byts), "ms": round(dom_ms, 4)}
+    peaks = hip.device_peaks()
+    log(f"measured peaks: {peaks}")
+    if dom in ("raster_fwd", "raster_bwd"):
+        # algorithmic work of this frame, counted on the device (outside the timed region)
+        work = raster_work(hip, tr, scene, cam, cfg)
+        log(f"raster work: {work}")
+        if dom == "raster_fwd":
+            evals = work["fwd_evals"]
+            flops = FLOPS_PER_EVAL * evals + FLOPS_FWD_CONTRIB * work["contributors"]
+        else:
+            evals = work["bwd_evals"]
+            flops = FLOPS_PER_EVAL * evals + FLOPS_BWD_CONTRIB * work["contributors"]
+        sec = dom_ms * 1e-3
+        tflops = flops / sec / 1e12
+        lds_gbs = evals * LDS_BYTES_PER_EVAL / sec / 1e9
+        roofline = {"kernel": kernel_names[dom], "bound": "valu", "achieved": round(tflops, 2),
+                    "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                    "traffic": st_meas.get("traffic_bytes"), "ms": round(dom_ms, 4),
+                    "algorithmic_flops": int(flops), "evaluations": int(evals),
+                    "contributors": int(work["contributors"]),
+                    "flops_per_evaluation": FLOPS_PER_EVAL,
+                    "flops_per_contributor": FLOPS_FWD_CONTRIB if dom == "raster_fwd" else FLOPS_BWD_CONTRIB,
+                    # the same work against the FP32 rate these kernels can reach: one
+                    # unfused, unpacked FP32 operation per lane and instruction at the
+                    # measured issue rate (no FMA contraction, no packed FP32)
+                    "measured_fp32_unfused_tflops": round(peaks["valu_wave_insts_per_s"] * 64 / 1e12, 2),
+                    "lds": {"achieved_gbs": round(lds_gbs, 1), "peak_gbs_measured": peaks["lds_read_gbs"],
+                            "frac": round(lds_gbs / peaks["lds_read_gbs"], 4) if peaks["lds_read_gbs"] else None,
+                            "bytes_per_evaluation": LDS_BYTES_PER_EVAL},
+                    "hbm": {"achieved_gbs": round(hbm_gbs, 1), "peak_gbs": HBM_PEAK_GBS,
+                            "frac": round(hbm_gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": int(byts)}}
+        if insts:
+            # secondary: VALU issue occupancy (instructions executed, from the committed
+            # rocprofv3 SQ_INSTS_VALU summary, per second against 4 cycles per
+            # wave64 instruction); a measure of how busy the SIMDs are, not of work done
+            ach = insts / sec / 1e9
+            roofline["valu_issue"] = {"achieved": round(ach, 1), "peak": round(VALU_WAVE_INSTS_PER_S / 1e9, 1),
+                                      "unit": "G VALU wave-instructions/s",
+                                      "frac": round(ach / (VALU_WAVE_INSTS_PER_S / 1e9), 4),
+                                      "valu_wave_insts": insts}
     else:
         roofline = {"kernel": kernel_names[dom], "bound": "hbm", "achieved": round(hbm_gbs, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(hbm_gbs / HBM_PEAK_GBS, 4), "traffic": st_meas.get("traffic_bytes"),
@@ -519,6 +578,7 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "tuning": tuning,
         "roofline": roofline,
+        "device_peaks": peaks,
         "hbm_targets": hbm_targets,
         "cpu_baseline": cpu,
     }

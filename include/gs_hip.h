@@ -235,11 +235,14 @@ int gs_raster_fwd_words(gs_ctx* ctx, int width, int height, const gs_preprocesse
  * filter_3D property is present.  gs_ply_load reads the vertices into
  * device arrays ([N*3], [N*3], [N*4] 16-byte aligned, [N*max_coeffs*3], [N],
  * filter_3d [N] or NULL), activated as loadPly does (exp scale, sigmoid
- * opacity, normalised rotation) with the SH reordered to sh[k][ch].  Returns
+ * opacity, normalised rotation) with the SH reordered to sh[k][ch].  n and
+ * max_coeffs are the sizes the arrays were allocated for (from gs_ply_info);
+ * a file whose header differs, or that has no filter_3D when filter_3d is
+ * non-NULL, is refused with GS_ERR_INVALID before any device work.  Returns
  * after the data is on the device. */
 int gs_ply_info(const char* path, int* count, int* sh_degree, int* max_coeffs, int* has_filter_3d);
-int gs_ply_load(gs_ctx* ctx, const char* path, float* positions, float* scales, float* rotations,
-                float* sh_coeffs, float* opacities, float* filter_3d);
+int gs_ply_load(gs_ctx* ctx, const char* path, int n, int max_coeffs, float* positions, float* scales,
+                float* rotations, float* sh_coeffs, float* opacities, float* filter_3d);
 /* savePly (src/train_main.cpp:82-146): raw (pre-activation) device arrays
  * written as a 3DGS PLY, SH reordered to the file's channel-major f_rest. */
 int gs_ply_save(gs_ctx* ctx, const char* path, int n, int max_coeffs, const float* positions,
@@ -477,6 +480,36 @@ int gs_autotune(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, 
  * the multi-view rasterizers, on the batch's views and targets.  The batch's
  * captured graph, if any, is rebuilt on the next step. */
 int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out);
+
+/* ------------------------------------------------------------------------
+ * Measurement (no counterpart in the reference).
+ * ---------------------------------------------------------------------- */
+
+/* The algorithmic work of the 2D rasterizers on one frame, counted on the
+ * device by a replay with the reference's exact arithmetic
+ * (cpu/rasterizer_cpu.cpp:141-176, cpu/rasterizer_backward_cpu.cpp:34-134):
+ * counts[0] = pixel-Gaussian evaluations of the forward loop (up to and
+ * including each pixel's stopping entry), counts[1] = entries from each tile's
+ * start to each pixel's last contributor (what the backward evaluates),
+ * counts[2] = the sum of n_contrib.  counts: device uint64[3], overwritten.
+ * px comes from the forward of the same inputs. */
+int gs_raster_work(gs_ctx* ctx, int width, int height, const gs_preprocessed* pre,
+                   const uint32_t* values_sorted, const uint32_t* ranges, const gs_pixel_state* px,
+                   unsigned long long* counts);
+
+/* Measured ceilings of the context's device (synchronous; allocates and frees
+ * 2 GiB for the copy): HBM copy bandwidth (read + write bytes), LDS read
+ * bandwidth (conflict-free 16-byte reads), wave64 FP32 VALU issue rate
+ * (multiply chains) and v_exp_f32 issue rate. */
+typedef struct gs_peaks {
+    int compute_units;
+    double clock_mhz;
+    double hbm_copy_gbs;
+    double lds_read_gbs;
+    double valu_wave_insts_per_s;
+    double exp_wave_insts_per_s;
+} gs_peaks;
+int gs_device_peaks(gs_ctx* ctx, gs_peaks* out);
 
 #ifdef __cplusplus
 }
