@@ -4,7 +4,8 @@
 // CPU path (harmonyos_3dgs/src/math_utils.cpp, sh_eval.cpp,
 // cpu/preprocessor_cpu.cpp) so that, with -ffp-contract=off and IEEE division
 // and square root, the GPU reproduces the CPU results bit for bit.  The
-// exponential the CPU path calls (glibc's expf) is restated as exp_ref.
+// exponential and logarithm the CPU path calls (glibc's expf and logf) are
+// restated as exp_ref and log_ref.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -22,17 +23,29 @@ namespace gsd {
 // from a cubic; the double result is rounded to float once.  The table is
 // tab[i] = bits(2^(i/32) rounded to double) - (i << 47).  Equal to glibc's
 // expf on all 2^32 float inputs (checked exhaustively on the CPU).
-static __constant__ uint64_t kExp2Tab[32] = {
-    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
-    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
-    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
-    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
-    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
-    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
-    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
-    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull};
+//
+// The tables below are defined once (GS_EXP2_TAB, GS_LOGF_TAB) and
+// instantiated for the device (__constant__) and for the host, so the same
+// functions compile for both; a host build of them is checked against the
+// system's glibc on every float input.
+#define GS_EXP2_TAB                                                                    \
+    {0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull, \
+     0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull, \
+     0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull, \
+     0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull, \
+     0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull, \
+     0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull, \
+     0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull, \
+     0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull}
+static __constant__ uint64_t kExp2TabDev[32] = GS_EXP2_TAB;
+static const uint64_t kExp2TabHost[32] = GS_EXP2_TAB;
 
-__device__ __forceinline__ float exp_ref(float x) {
+__host__ __device__ __forceinline__ float exp_ref(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t* tab = kExp2TabDev;
+#else
+    const uint64_t* tab = kExp2TabHost;
+#endif
     if (!(x <= 0x1.62e42ep6f)) return x != x ? x + x : __builtin_inff();  // NaN, overflow
     if (x < -0x1.9fe368p6f) return 0.0f;                                   // below 2^-150
     constexpr double kInvLn2N = 0x1.71547652b82fep+0 * 32.0, kShift = 0x1.8p+52;
@@ -40,15 +53,78 @@ __device__ __forceinline__ float exp_ref(float x) {
                      kC2 = 0x1.62e42ff0c52d6p-1 / 32;
     const double xd = (double)x;
     double kd = __builtin_fma(kInvLn2N, xd, kShift);
-    const uint64_t ki = (uint64_t)__double_as_longlong(kd);
+    uint64_t ki;
+    __builtin_memcpy(&ki, &kd, 8);
     kd -= kShift;
     const double r = __builtin_fma(kInvLn2N, xd, -kd);
-    const double sc = __longlong_as_double((long long)(kExp2Tab[ki & 31u] + (ki << 47)));
+    const uint64_t sb = tab[ki & 31u] + (ki << 47);
+    double sc;
+    __builtin_memcpy(&sc, &sb, 8);
     const double z = __builtin_fma(kC0, r, kC1);
     const double r2 = r * r;
     double y = __builtin_fma(kC2, r, 1.0);
     y = __builtin_fma(z, r2, y);
     return (float)(y * sc);
+}
+
+// logf exactly as the reference's CPU path evaluates it: glibc's logf (2.35,
+// sysdeps/ieee754/flt-32/e_logf.c, the published optimized-routines
+// algorithm, fused multiply-adds as in the FMA build).  With x = 2^k z,
+// z in [0x1.66p-1, 0x1.66p0) split into 16 intervals by its top mantissa
+// bits: log(x) = k ln2 + log(c) + log1p(z/c - 1), where invc = 1/c and
+// logc = log(c) (double) are tabulated per interval and log1p(r) is a
+// quartic in r evaluated in double; one rounding to float.  Equal to glibc's
+// logf on all 2^32 float inputs (NaN payloads aside).
+#define GS_LOGF_TAB                                                               \
+    {{0x1.661ec79f8f3bep+0, -0x1.57bf7808caadep-2}, {0x1.571ed4aaf883dp+0, -0x1.2bef0a7c06ddbp-2}, \
+     {0x1.49539f0f010bp+0, -0x1.01eae7f513a67p-2},  {0x1.3c995b0b80385p+0, -0x1.b31d8a68224e9p-3}, \
+     {0x1.30d190c8864a5p+0, -0x1.6574f0ac07758p-3}, {0x1.25e227b0b8eap+0, -0x1.1aa2bc79c81p-3},   \
+     {0x1.1bb4a4a1a343fp+0, -0x1.a4e76ce8c0e5ep-4}, {0x1.12358f08ae5bap+0, -0x1.1973c5a611cccp-4}, \
+     {0x1.0953f419900a7p+0, -0x1.252f438e10c1ep-5}, {0x1p+0, 0x0p+0},                            \
+     {0x1.e608cfd9a47acp-1, 0x1.aa5aa5df25984p-5},  {0x1.ca4b31f026aap-1, 0x1.c5e53aa362eb4p-4},  \
+     {0x1.b2036576afce6p-1, 0x1.526e57720db08p-3},  {0x1.9c2d163a1aa2dp-1, 0x1.bc2860d22477p-3},  \
+     {0x1.886e6037841edp-1, 0x1.1058bc8a07ee1p-2},  {0x1.767dcf5534862p-1, 0x1.4043057b6ee09p-2}}
+struct LogfEntry {
+    double invc, logc;
+};
+static __constant__ LogfEntry kLogfTabDev[16] = GS_LOGF_TAB;
+static const LogfEntry kLogfTabHost[16] = GS_LOGF_TAB;
+
+__host__ __device__ __forceinline__ float log_ref(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const LogfEntry* tab = kLogfTabDev;
+#else
+    const LogfEntry* tab = kLogfTabHost;
+#endif
+    constexpr double kLn2 = 0x1.62e42fefa39efp-1;
+    constexpr double kA0 = -0x1.00ea348b88334p-2, kA1 = 0x1.5575b0be00b6ap-2, kA2 = -0x1.ffffef20a4123p-2;
+    constexpr uint32_t kOff = 0x3f330000u;
+    uint32_t ix;
+    __builtin_memcpy(&ix, &x, 4);
+    if (ix == 0x3f800000u) return 0.0f;
+    if (ix - 0x00800000u >= 0x7f800000u - 0x00800000u) {
+        // zero, subnormal, negative, infinity or NaN
+        if (ix * 2u == 0u) return -__builtin_inff();
+        if (ix == 0x7f800000u) return x;
+        if ((ix & 0x80000000u) || ix * 2u >= 0xff000000u) return (x - x) / (x - x);
+        const float xs = x * 0x1p23f;  // subnormal: normalise
+        __builtin_memcpy(&ix, &xs, 4);
+        ix -= 23u << 23;
+    }
+    const uint32_t tmp = ix - kOff;
+    const int i = (int)((tmp >> (23 - 4)) % 16u);
+    const int k = (int32_t)tmp >> 23;
+    const uint32_t iz = ix - (tmp & 0xff800000u);
+    float zf;
+    __builtin_memcpy(&zf, &iz, 4);
+    const double z = (double)zf;
+    const double r = __builtin_fma(z, tab[i].invc, -1.0);
+    const double y0 = __builtin_fma((double)k, kLn2, tab[i].logc);
+    const double r2 = r * r;
+    double y = __builtin_fma(kA1, r, kA2);
+    y = __builtin_fma(kA0, r2, y);
+    y = __builtin_fma(y, r2, y0 + r);
+    return (float)y;
 }
 
 // std::min / std::max semantics: the first argument wins unless strictly beaten.

@@ -294,15 +294,7 @@ __device__ __forceinline__ int to_int_x86(float v) {
     return (v < 2147483648.0f && v >= -2147483648.0f) ? (int)v : (int)0x80000000u;
 }
 
-// logf of opacity / (1/255), an argument in [1, 255]: the double logarithm
-// rounded to float (correctly rounded there but for ~1400 of the 68M floats,
-// whose double result lies within 2^-40 of a rounding midpoint).  glibc's
-// logf, which the reference's host build calls, is within 0.82 ulp and
-// differs from it by one ulp on 0.22% of those floats; the value feeds the
-// screen-bound cutoff and the frustum cull, where one ulp decides the
-// outcome only for a Gaussian whose bound lies within ~1e-7 (relative) of an
-// integer pixel radius or whose contribution ties the threshold.
-__device__ __forceinline__ float log_rn(float x) { return (float)log((double)x); }
+
 
 // ------------------------------------------------------------ preprocess ----
 // One thread per Gaussian: cpu/preprocessor_cpu.cpp:64-187.
@@ -368,7 +360,7 @@ __global__ __launch_bounds__(256) void preprocess3d_kernel(gs_gaussians G, const
         opacity *= dil;
         vis = !(opacity < kAlpha);
         if (vis) {
-            opt = log_rn(opacity / kAlpha);
+            opt = log_ref(opacity / kAlpha);  // std::log(float), preprocessor_cpu.cpp:86
             cutoff = smin(11.11f, 2.0f * opt);
         }
     }

@@ -1413,26 +1413,6 @@ void gso_rasterize3d(const float* depths, const float* op2d, const float* rgb, c
 #endif /* !GS_ORACLE_DOUBLE */
 
 #ifndef GS_ORACLE_DOUBLE
-/* Over every float x in [lo, hi]: *mismatch counts the x where logf(x) differs
- * from the double logarithm rounded to float, *near those whose double
- * logarithm lies within 2^-40 (relative) of a midpoint between adjacent
- * floats (where a double log a few ulps off could round the other way).  The
- * eval_3D preprocess (preprocessor_cpu.cpp:89) takes logf of values in
- * [1, 255]; the device evaluates it as the rounded double logarithm. */
-void gso_logf_check(float lo, float hi, long long* mismatch, long long* near) {
-    long long mm = 0, nr = 0;
-    for (float x = lo; x <= hi; x = nextafterf(x, INFINITY)) {
-        double d = log((double)x);
-        float f = (float)d;
-        if (f != logf(x)) mm++;
-        double m1 = ((double)f + (double)nextafterf(f, -INFINITY)) * 0.5;
-        double m2 = ((double)f + (double)nextafterf(f, INFINITY)) * 0.5;
-        double tol = fabs(d) * 0x1p-40;
-        if (fabs(d - m1) <= tol || fabs(d - m2) <= tol) nr++;
-    }
-    *mismatch = mm;
-    *near = nr;
-}
 #endif
 
 #ifndef GS_ORACLE_DOUBLE

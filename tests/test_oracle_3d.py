@@ -94,20 +94,6 @@ def test_invert4_matches_numpy(o3):
     assert not ok
 
 
-def test_logf_vs_rounded_double_log(o3):
-    """The facts the device's logarithm relies on, over every float in
-    [0.99, 256]: glibc's logf differs from the rounded double logarithm on
-    fewer than 0.3% of them, and fewer than 2000 double results lie within
-    2^-40 of a float rounding midpoint."""
-    L = o3.lib
-    L.gso_logf_check.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
-    mm, nr = ctypes.c_longlong(), ctypes.c_longlong()
-    L.gso_logf_check(0.99, 256.0, ctypes.byref(mm), ctypes.byref(nr))
-    total = int(np.float32(256.0).view(np.int32) - np.float32(0.99).view(np.int32)) + 1
-    assert 0 < mm.value < 0.003 * total
-    assert nr.value < 2000
-
-
 @pytest.mark.skipif(not RefLib.available(), reason="oracle/_ref not built (no reference tree)")
 @pytest.mark.parametrize("n,deg,seed", [(300, 1, 41), (3000, 3, 42)])
 def test_inv_depth_oracle_bitexact_vs_reference(oracle, n, deg, seed):
