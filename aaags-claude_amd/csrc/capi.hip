@@ -2,9 +2,11 @@
 // workspace, and kernel launches on the context stream.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "gs_device.h"
@@ -41,6 +43,34 @@ extern "C" const char* gs_last_error(void) { return g_err.c_str(); }
 
 static bool aligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
+// ------------------------------------------------ allocation-free scan ----
+// The fixed device scratch of gs_scan_tiles_noalloc: the 64-bit pair total at
+// offset 0 and the per-block partial sums (one per 4096 Gaussians, plus one)
+// from offset 256, sized for GS_NOALLOC_SCAN_MAX Gaussians (33 KB).  Each
+// device holds its own copy until the process ends.  In host code the name
+// `scratch` is only a stand-in: its device address is taken once per device
+// with hipGetSymbolAddress (gs_ctx_create, with that device current) and only
+// that address is used.
+constexpr size_t kNoallocPartials = (GS_NOALLOC_SCAN_MAX + 4095u) / 4096u + 1u;
+constexpr size_t kNoallocBytes = 256 + ((kNoallocPartials * 8 + 255) & ~size_t(255));
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static std::atomic<void*> g_scratch_ptr[kMaxDevices];
+
+// Resolves this device's copy of `scratch` (the device must be current).
+static int scratch_init(int device) {
+    GS_CHECK_ARG(device >= 0 && device < kMaxDevices, "device %d beyond the %d supported", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device].load()) return GS_OK;
+    void* p = nullptr;
+    GS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    GS_CHECK_ARG(p != nullptr, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[device].store(p);
+    return GS_OK;
+}
+
 // ------------------------------------------------------------- context ----
 extern "C" int gs_ctx_create(int device, void* stream, gs_ctx** out) {
     GS_CHECK_ARG(out != nullptr, "gs_ctx_create: out is null");
@@ -48,6 +78,8 @@ extern "C" int gs_ctx_create(int device, void* stream, gs_ctx** out) {
     GS_HIP(hipGetDeviceCount(&n), "hipGetDeviceCount");
     GS_CHECK_ARG(device >= 0 && device < n, "gs_ctx_create: device %d out of range (%d devices)", device, n);
     GS_HIP(hipSetDevice(device), "hipSetDevice");
+    int st = scratch_init(device);
+    if (st) return st;
     gs_ctx* c = new gs_ctx();
     c->device = device;
     c->stream = static_cast<hipStream_t>(stream);
@@ -262,6 +294,28 @@ extern "C" int gs_scan_tiles(gs_ctx* c, int n, const int* tiles, uint32_t* offse
         *total_host = (long long)t;
         GS_CHECK_ARG(t <= 0xFFFFFFFFull, "pair count %llu exceeds 2^32-1", t);
     }
+    return GS_OK;
+}
+
+extern "C" int gs_scan_tiles_noalloc(gs_ctx* c, int n, const int* tiles, uint32_t* offsets, uint32_t* total_dev) {
+    GS_CHECK_ARG(c != nullptr, "null context");
+    GS_CHECK_ARG(n >= 0, "negative count");
+    GS_CHECK_ARG((uint32_t)n <= GS_NOALLOC_SCAN_MAX,
+                 "gs_scan_tiles_noalloc: %d Gaussians exceed its fixed scratch (at most %u; use gs_scan_tiles)", n,
+                 (unsigned)GS_NOALLOC_SCAN_MAX);
+    void* base = g_scratch_ptr[c->device].load();
+    GS_CHECK_ARG(base != nullptr, "no scratch address for device %d", c->device);
+    auto* total64 = static_cast<unsigned long long*>(base);
+    auto* partial = reinterpret_cast<unsigned long long*>(static_cast<char*>(base) + 256);
+    if (n == 0) {
+        // fill kernels, not hipMemsetAsync: this call may be graph-captured
+        GS_HIP(launch_zero(total64, 8, c->stream), "zero kernel");
+        if (total_dev) GS_HIP(launch_zero(total_dev, 4, c->stream), "zero kernel");
+        return GS_OK;
+    }
+    GS_CHECK_ARG(tiles && offsets, "null array");
+    // every partial and the total are written before they are read: nothing to clear
+    GS_HIP(launch_scan_tiles(tiles, (uint32_t)n, offsets, partial, total_dev, total64, c->stream), "scan kernels");
     return GS_OK;
 }
 

@@ -481,6 +481,18 @@ int gs_autotune(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, 
  * captured graph, if any, is rebuilt on the next step. */
 int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out);
 
+/* gs_scan_tiles without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same inclusive scan of
+ * tiles_touched into offsets, and the pair count into total_dev (device
+ * uint32, may be NULL), through a fixed per-device scratch of the library
+ * (33 KB, held until the process ends) instead of the context's workspace;
+ * no host read-back.  At most GS_NOALLOC_SCAN_MAX Gaussians (GS_ERR_INVALID
+ * beyond).  Calls that overlap on one device share that scratch and race:
+ * serialise them (one stream, or events). */
+#define GS_NOALLOC_SCAN_MAX 16777216u
+int gs_scan_tiles_noalloc(gs_ctx* ctx, int n, const int* tiles, uint32_t* offsets, uint32_t* total_dev);
+
 /* ------------------------------------------------------------------------
  * Measurement (no counterpart in the reference).
  * ---------------------------------------------------------------------- */
