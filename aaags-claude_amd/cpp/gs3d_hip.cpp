@@ -592,6 +592,95 @@ void AdamOptimizerHIP::step(RawGaussianParams& params, const GradientOutput& gra
 }
 
 // ----------------------------------------------------------------- trainer --
+// ------------------------------------------------------- density control --
+DensityControllerHIP::DensityControllerHIP(HipContext& ctx) : ctx_(ctx), up_(new DeviceArray()) {}
+DensityControllerHIP::~DensityControllerHIP() {
+    if (tr_) gs_trainer_destroy(tr_);
+}
+
+// A device state for N Gaussians of the given SH layout; statistics already
+// gathered for the same N survive a change of layout.
+void DensityControllerHIP::ensure_state(int N, int sh_degree, int max_coeffs) {
+    if (tr_ && tr_->n == N && mc_ == max_coeffs && deg_ == sh_degree) return;
+    std::vector<float> keep[2];
+    std::vector<int> keep_r;
+    const bool carry = tr_ && tr_->n == N && tr_->dc_n == N && N > 0;
+    hipStream_t s = ctx_.get()->stream;
+    if (carry) {
+        keep[0].resize(N);
+        keep[1].resize(N);
+        keep_r.resize(N);
+        download(keep[0].data(), tr_->dc_accum.p, (size_t)N * 4, s);
+        download(keep[1].data(), tr_->dc_denom.p, (size_t)N * 4, s);
+        download(keep_r.data(), tr_->dc_maxr.p, (size_t)N * 4, s);
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    if (tr_) gs_trainer_destroy(tr_);
+    tr_ = nullptr;
+    gs_check(gs_trainer_create(ctx_.get(), N, sh_degree, max_coeffs, &tr_), "gs_trainer_create");
+    deg_ = sh_degree;
+    mc_ = max_coeffs;
+    gs_check(gs_trainer_density_reset(tr_), "gs_trainer_density_reset");
+    if (carry) {
+        upload(tr_->dc_accum.p, keep[0].data(), (size_t)N * 4, s);
+        upload(tr_->dc_denom.p, keep[1].data(), (size_t)N * 4, s);
+        upload(tr_->dc_maxr.p, keep_r.data(), (size_t)N * 4, s);
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+}
+
+void DensityControllerHIP::reset(int N) {
+    if (N < 0) throw std::runtime_error("DensityControllerHIP::reset: negative count");
+    current_N_ = N;
+    if (tr_ && tr_->n == N) gs_check(gs_trainer_density_reset(tr_), "gs_trainer_density_reset");
+    else ensure_state(N, deg_, mc_);
+}
+
+void DensityControllerHIP::accumulate_stats(const float* d_means2D, const int* radii, int N) {
+    if (N != current_N_ || !tr_ || tr_->n != N) reset(N);  // density_controller.cpp:30
+    if (N == 0) return;
+    hipStream_t s = ctx_.get()->stream;
+    up_->ensure((size_t)N * 12);
+    upload(up_->p, d_means2D, (size_t)N * 8, s);
+    const int* rd = nullptr;
+    if (radii) {
+        upload(static_cast<char*>(up_->p) + (size_t)N * 8, radii, (size_t)N * 4, s);
+        rd = reinterpret_cast<const int*>(static_cast<char*>(up_->p) + (size_t)N * 8);
+    }
+    gs_check(gs_trainer_density_accumulate(tr_, up_->as<float>(), rd), "gs_trainer_density_accumulate");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+int DensityControllerHIP::densify_and_prune(OwnedRawParams& p, const DensifyConfig& cfg, float scene_extent) {
+    const int N = p.count();
+    if (N == 0) return 0;
+    if (N != current_N_) reset(N);  // statistics for another count do not apply
+    ensure_state(N, p.sh_degree, p.max_coeffs);
+    gs_check(gs_trainer_set_params(tr_, p.positions.data(), p.scales.data(), p.rotations.data(), p.sh_coeffs.data(),
+                                   p.opacities.data(), 1),
+             "gs_trainer_set_params");
+    gs_densify_cfg c{cfg.densify_from_iter, cfg.densify_until_iter, cfg.densify_interval, cfg.grad_threshold,
+                     cfg.min_opacity, cfg.percent_dense, cfg.opacity_reset_interval, cfg.max_screen_size,
+                     cfg.enabled ? 1 : 0};
+    tr_->dc_rng = rng_;
+    int M = 0;
+    gs_check(gs_trainer_densify_and_prune(tr_, &c, scene_extent, &M), "gs_trainer_densify_and_prune");
+    rng_ = tr_->dc_rng;
+    p.resize(M);
+    if (M > 0)
+        gs_check(gs_trainer_get_params(tr_, p.positions.data(), p.scales.data(), p.rotations.data(),
+                                       p.sh_coeffs.data(), p.opacities.data(), 1),
+                 "gs_trainer_get_params");
+    current_N_ = M;
+    return M;
+}
+
+// density_controller.cpp:199-204 (host arrays, as the reference's)
+void DensityControllerHIP::reset_opacity(OwnedRawParams& p) {
+    const float v = std::log(0.01f / 0.99f);
+    std::fill(p.opacities.begin(), p.opacities.end(), v);
+}
+
 TrainerHIP::TrainerHIP(HipContext& ctx) : ctx_(ctx), gt_(new DeviceArray()) {}
 TrainerHIP::~TrainerHIP() {
     if (tr_) gs_trainer_destroy(tr_);
@@ -601,15 +690,7 @@ TrainerHIP::StepResult TrainerHIP::step(RawGaussianParams& params, const Camera&
                                         const RenderConfig& cfg, const TrainConfig& tc, int iteration) {
     if (cfg.eval_3D) throw std::runtime_error("TrainerHIP: eval_3D has no backward (the reference's 3D path is forward-only)");
     hipStream_t s = ctx_.get()->stream;
-    if (!tr_ || params.count != n_ || params.max_coeffs != mc_ || params.sh_degree != deg_) {
-        if (tr_) gs_trainer_destroy(tr_);
-        tr_ = nullptr;
-        gs_check(gs_trainer_create(ctx_.get(), params.count, params.sh_degree, params.max_coeffs, &tr_),
-                 "gs_trainer_create");
-        n_ = params.count;
-        mc_ = params.max_coeffs;
-        deg_ = params.sh_degree;
-    }
+    ensure_trainer(params);
     gs_check(gs_trainer_set_params(tr_, params.raw_positions, params.raw_scales, params.raw_rotations,
                                    params.raw_sh_coeffs, params.raw_opacities, 1),
              "gs_trainer_set_params");
@@ -629,4 +710,57 @@ TrainerHIP::StepResult TrainerHIP::step(RawGaussianParams& params, const Camera&
     download(last_rendered_.data(), gs_trainer_buffer(tr_, "image"), npix * 12, s);
     hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     return StepResult{(float)r.loss, iteration, params.count};
+}
+
+// A device trainer for this count and SH layout (optimizer state restarts
+// when they change, as AdamOptimizer::step does); the density generator's
+// state carries over.
+void TrainerHIP::ensure_trainer(const RawGaussianParams& params) {
+    if (tr_ && params.count == n_ && params.max_coeffs == mc_ && params.sh_degree == deg_) return;
+    if (tr_) {
+        rng_ = tr_->dc_rng;
+        gs_trainer_destroy(tr_);
+    }
+    tr_ = nullptr;
+    gs_check(gs_trainer_create(ctx_.get(), params.count, params.sh_degree, params.max_coeffs, &tr_),
+             "gs_trainer_create");
+    tr_->dc_rng = rng_;
+    n_ = params.count;
+    mc_ = params.max_coeffs;
+    deg_ = params.sh_degree;
+}
+
+TrainerHIP::StepResult TrainerHIP::step_with_densify(OwnedRawParams& owned, const Camera& camera,
+                                                     const float* gt_image, const RenderConfig& cfg,
+                                                     const TrainConfig& tc, const DensifyConfig& dcfg,
+                                                     float scene_extent, int iteration) {
+    if (cfg.eval_3D) throw std::runtime_error("TrainerHIP: eval_3D has no backward (the reference's 3D path is forward-only)");
+    hipStream_t s = ctx_.get()->stream;
+    RawGaussianParams params = owned.as_raw();
+    ensure_trainer(params);
+    gs_check(gs_trainer_set_params(tr_, params.raw_positions, params.raw_scales, params.raw_rotations,
+                                   params.raw_sh_coeffs, params.raw_opacities, 1),
+             "gs_trainer_set_params");
+    const size_t npix = (size_t)camera.width * camera.height;
+    gt_->ensure(npix * 12);
+    upload(gt_->p, gt_image, npix * 12, s);
+    gs_camera c = to_gs(camera);
+    gs_render_cfg f = to_gs(cfg);
+    gs_train_cfg t{tc.lr_position_init, tc.lr_position_final, tc.lr_feature, tc.lr_opacity, tc.lr_scaling,
+                   tc.lr_rotation, tc.max_steps, tc.use_adam ? 1 : 0, tc.adam_beta1, tc.adam_beta2, tc.adam_eps};
+    gs_densify_cfg d{dcfg.densify_from_iter, dcfg.densify_until_iter, dcfg.densify_interval, dcfg.grad_threshold,
+                     dcfg.min_opacity, dcfg.percent_dense, dcfg.opacity_reset_interval, dcfg.max_screen_size,
+                     dcfg.enabled ? 1 : 0};
+    gs_step_result r;
+    gs_check(gs_trainer_step_densify(tr_, &c, &f, gt_->as<float>(), &t, &d, scene_extent, iteration, &r),
+             "gs_trainer_step_densify");
+    // the rendering of the step's forward is not kept: the statistics pass
+    // re-rendered the updated parameters into the same buffer
+    owned.resize(r.n_gaussians);
+    if (r.n_gaussians > 0)
+        gs_check(gs_trainer_get_params(tr_, owned.positions.data(), owned.scales.data(), owned.rotations.data(),
+                                       owned.sh_coeffs.data(), owned.opacities.data(), 1),
+                 "gs_trainer_get_params");
+    n_ = r.n_gaussians;
+    return StepResult{(float)r.loss, iteration, r.n_gaussians};
 }

@@ -8,7 +8,9 @@
 //   RasterizerBackwardHIP ... RasterizerBackwardGPU::backward (gpu/rasterizer_backward_gpu.h:16-19)
 //   PreprocessorBackwardHIP . PreprocessorBackwardGPU::backward (gpu/preprocessor_backward_gpu.h:18-21)
 //   AdamOptimizerHIP ........ AdamOptimizer::step (optimizer.h:13-14)
-//   TrainerHIP .............. Trainer::step / TrainerGPU::step (trainer.h:21-23, trainer_gpu.h:27-29)
+//   DensityControllerHIP .... DensityController (density_controller.h:7-40)
+//   TrainerHIP .............. Trainer::step / step_with_densify / TrainerGPU::step (trainer.h:21-30,
+//                             trainer_gpu.h:27-29)
 //
 // Conventions follow the GPU stages of the reference: each stage owns its
 // device buffers and exposes them to the next stage through device_data
@@ -165,6 +167,31 @@ private:
     int step_count_ = 0, last_N_ = 0, last_mc_ = 0;
 };
 
+// DensityController (include/density_controller.h:7-40) on the device: the
+// statistics live in HBM; parameters cross PCIe per densification, as the
+// reference's host arrays require.  The split noise continues the reference's
+// generator, so results equal the reference's bit for bit.
+class DensityControllerHIP {
+public:
+    explicit DensityControllerHIP(HipContext& ctx);
+    ~DensityControllerHIP();
+    DensityControllerHIP(const DensityControllerHIP&) = delete;
+    DensityControllerHIP& operator=(const DensityControllerHIP&) = delete;
+    void accumulate_stats(const float* d_means2D, const int* radii, int N);  // host arrays
+    int densify_and_prune(OwnedRawParams& params, const DensifyConfig& cfg, float scene_extent);
+    static void reset_opacity(OwnedRawParams& params);
+    void reset(int N);
+    int current_N() const { return current_N_; }
+
+private:
+    void ensure_state(int N, int sh_degree, int max_coeffs);
+    HipContext& ctx_;
+    gs_trainer* tr_ = nullptr;  // holds the device statistics and the parameters while densifying
+    int deg_ = 0, mc_ = 1, current_N_ = 0;
+    uint64_t rng_ = 12345;  // density_controller.h:39
+    std::unique_ptr<gs3d_detail::DeviceArray> up_;
+};
+
 class TrainerHIP {
 public:
     struct StepResult {
@@ -181,12 +208,20 @@ public:
     // when the Gaussian count changes.
     StepResult step(RawGaussianParams& params, const Camera& camera, const float* gt_image, const RenderConfig& cfg,
                     const TrainConfig& train_cfg, int iteration);
+    // Trainer::step_with_densify (trainer.h:26-30): the step, the statistics at
+    // the updated parameters, densification and opacity reset on their
+    // schedules; params is resized to the new count.
+    StepResult step_with_densify(OwnedRawParams& params, const Camera& camera, const float* gt_image,
+                                 const RenderConfig& cfg, const TrainConfig& train_cfg, const DensifyConfig& densify_cfg,
+                                 float scene_extent, int iteration);
     const float* last_rendered() const { return last_rendered_.empty() ? nullptr : last_rendered_.data(); }
 
 private:
+    void ensure_trainer(const RawGaussianParams& params);
     HipContext& ctx_;
     gs_trainer* tr_ = nullptr;
     int n_ = -1, mc_ = 0, deg_ = -1;
+    uint64_t rng_ = 12345;  // the density controller's generator, kept across trainers
     std::unique_ptr<gs3d_detail::DeviceArray> gt_;
     std::vector<float> last_rendered_;
 };

@@ -178,7 +178,10 @@ gs_trainer::~gs_trainer() {
         grad[k].release();
         adam_m[k].release();
         adam_v[k].release();
+        param_alt[k].release();
     }
+    gs::DevBuf* dc[] = {&dc_accum, &dc_denom, &dc_maxr, &dc_scratch, &dc_map, &dc_noise};
+    for (gs::DevBuf* b : dc) b->release();
     for (auto& v : views) v.release_all();
     keys.release();
     vals.release();
@@ -465,17 +468,16 @@ extern "C" int gs_trainer_set_options(gs_trainer* t, int fused_adam) {
     return GS_OK;
 }
 
-extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
-                               const gs_train_cfg* tc, int iteration, gs_step_result* out) {
-    if (!t || !cam || !cfg || !gt || !tc) return gs_set_error(GS_ERR_INVALID, "gs_trainer_step: null argument");
-    TR_TRY(check_cam_cfg(cam, cfg));
+// Forward, L1 loss and raster backward of one view into the trainer's view
+// buffers.  With a pending pair count the host checks it here, while the
+// rasterizers run, and redoes the three stages in the rare case that the pairs
+// outgrew the buffer.  *K: the pair count when it was read back.
+int tr_forward_raster_bwd(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                          long long* Kout) {
     gs_ctx* c = t->ctx;
     if (t->views.empty()) t->views.resize(1);
     gs::ViewBufs& v = t->views[0];
     long long K = 0;
-    // forward, loss and raster backward; with a pending pair count the host
-    // checks it here, while the rasterizers run, and redoes the three stages
-    // in the rare case that the pairs outgrew the buffer
     auto fwd_bwd = [&](bool* pending) -> int {
         TR_TRY(tr_forward(t, v, cam, cfg, &K, pending));
         TR_TRY(tr_mark(t, GS_STAGE_LOSS, true));
@@ -510,6 +512,18 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
             TR_TRY(st);
         }
     }
+    *Kout = K;
+    return GS_OK;
+}
+
+extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                               const gs_train_cfg* tc, int iteration, gs_step_result* out) {
+    if (!t || !cam || !cfg || !gt || !tc) return gs_set_error(GS_ERR_INVALID, "gs_trainer_step: null argument");
+    TR_TRY(check_cam_cfg(cam, cfg));
+    gs_ctx* c = t->ctx;
+    long long K = 0;
+    TR_TRY(tr_forward_raster_bwd(t, cam, cfg, gt, &K));
+    gs::ViewBufs& v = t->views[0];
     gs_preprocessed pre = v.pre();
     gs_raster_grads rg = v.rgrads();
     gs_gaussians g = t->gaussians();

@@ -76,10 +76,21 @@ struct gs_trainer {
     bool allow_async = true;
     gs::StageTimer timer;
     std::vector<gs::GraphSlot> graphs;
+    // adaptive density control (DensityController): per-Gaussian statistics,
+    // the split noise generator's state, and densification scratch
+    gs::DevBuf dc_accum, dc_denom, dc_maxr;
+    int dc_n = -1;                   // Gaussian count the statistics are for (-1: none yet)
+    uint64_t dc_rng = 12345;         // DensityController::rng_state_ (density_controller.h:39)
+    gs::DevBuf param_alt[5];         // the arrays a densification writes, swapped in afterwards
+    gs::DevBuf dc_scratch, dc_map, dc_noise;
     gs_gaussians gaussians() const;
     size_t tensor_elems(int k) const;
     ~gs_trainer();
 };
+
+// Forward, L1 and raster backward of one view into views[0] (gs_trainer.cpp).
+int tr_forward_raster_bwd(gs_trainer* t, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                          long long* pairs);
 
 extern "C" int gs_sgd_step(gs_ctx* ctx, int n, int max_coeffs, float* const params[5], const float* const grads[5],
                            const float lr[5]);

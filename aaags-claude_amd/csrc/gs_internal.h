@@ -142,5 +142,20 @@ hipError_t launch_raster_fwd_multi(const gsr::MultiRaster& m, const gsr::MultiPi
                                    int persistent_groups, hipStream_t s);
 hipError_t launch_raster_bwd_multi(const gsr::MultiRaster& m, const gsr::MultiPix& o, const gsr::MultiGrads& g,
                                    int batch, int persistent_groups, hipStream_t s);
+// adaptive density control (k_density.hip)
+hipError_t launch_density_accumulate(int n, const float* d_means2D, const int* radii, float* accum, float* denom,
+                                     int* max_radii, hipStream_t s);
+hipError_t launch_density_classify(int n, const float* accum, const float* denom, const float* log_scales,
+                                   float threshold, float size_threshold, uint32_t* clone, uint32_t* split,
+                                   hipStream_t s);
+hipError_t launch_density_map(int n, const uint32_t* clone, const uint32_t* split, const uint32_t* rank_c,
+                              const uint32_t* rank_s, uint32_t n_clone, uint32_t n_split, int2* map, hipStream_t s);
+hipError_t launch_density_prune(int m1, int n, const int2* map, const float* log_scales, const float* opacity,
+                                const int* max_radii, float scale_factor, float min_opacity, int max_screen_size,
+                                float world_threshold, uint32_t* keep, hipStream_t s);
+hipError_t launch_density_scatter(int m1, int max_coeffs, const int2* map, const uint32_t* keep, const uint32_t* slot,
+                                  const float* const src[5], const float* noise, float scale_factor,
+                                  float* const dst[5], hipStream_t s);
+hipError_t launch_fill(int n, float v, float* out, hipStream_t s);
 hipError_t launch_l1_multi(const float* r, const float* g, float* d, const int* pix_off_dev, int V, int nb,
                            double* partial, double* loss, hipStream_t s);

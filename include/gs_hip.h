@@ -481,6 +481,54 @@ int gs_autotune(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, 
  * captured graph, if any, is rebuilt on the next step. */
 int gs_batch_autotune(gs_batch* b, int reps, gs_tuning* out);
 
+/* ------------------------------------------------------------------------
+ * Adaptive density control on the trainer's device parameters
+ * (DensityController, include/density_controller.h:7-40;
+ * src/density_controller.cpp:9-204; Trainer::step_with_densify,
+ * src/trainer.cpp:72-182).  The split noise continues the reference's own
+ * generator (an LCG seeded 12345 with Box-Muller normals, :9-16) from the
+ * trainer's state, so densifications reproduce the reference's parameters
+ * bit for bit.  A densification changes the trainer's Gaussian count: buffers
+ * returned by gs_trainer_buffer and batches created on the trainer before it
+ * are invalid afterwards.
+ * ---------------------------------------------------------------------- */
+typedef struct gs_densify_cfg { /* DensifyConfig, include/train_types.h:71-81 (same defaults) */
+    int densify_from_iter;      /* 500 */
+    int densify_until_iter;     /* 15000 */
+    int densify_interval;       /* 100 */
+    float grad_threshold;       /* 0.0002 */
+    float min_opacity;          /* 0.005 */
+    float percent_dense;        /* 0.01; as in the reference, the clone/split size cut is 0.01 * extent */
+    int opacity_reset_interval; /* 3000 */
+    int max_screen_size;        /* 20 */
+    int enabled;                /* 0 */
+} gs_densify_cfg;
+
+/* Gaussians currently held by the trainer. */
+int gs_trainer_count(gs_trainer* tr, int* n);
+/* DensityController::reset (:21-26): zero the statistics for the current count. */
+int gs_trainer_density_reset(gs_trainer* tr);
+/* DensityController::accumulate_stats (:29-41): adds |d_means2D| (device
+ * float[N*2]) to each Gaussian's sum, 1 to its count, and raises its largest
+ * screen radius to radii (device int[N], may be NULL).  Resets first when the
+ * statistics are for another count. */
+int gs_trainer_density_accumulate(gs_trainer* tr, const float* d_means2D, const int* radii);
+/* DensityController::densify_and_prune (:148-197): clone, split (noise from
+ * the generator), prune, compact, reset the statistics; the Adam moments are
+ * zeroed and its step count reset when the count changes (trainer.cpp:159-163).
+ * *new_n receives the new count (may be NULL). */
+int gs_trainer_densify_and_prune(gs_trainer* tr, const gs_densify_cfg* cfg, float scene_extent, int* new_n);
+/* DensityController::reset_opacity (:199-204): every opacity logit to
+ * log(0.01 / 0.99). */
+int gs_trainer_reset_opacity(gs_trainer* tr);
+/* Trainer::step_with_densify (trainer.cpp:72-182): gs_trainer_step, then the
+ * statistics of a forward and raster backward at the updated parameters, then
+ * densification and opacity reset on their schedules.  out->n_gaussians is the
+ * count after the step. */
+int gs_trainer_step_densify(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
+                            const gs_train_cfg* tc, const gs_densify_cfg* dcfg, float scene_extent, int iteration,
+                            gs_step_result* out);
+
 /* gs_scan_tiles without any allocation (no counterpart in the reference; for
  * callers that cannot allocate: code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The same inclusive scan of

@@ -1729,3 +1729,134 @@ done:
     free(all); free(work); free(tstart); free(lists);
     return ret;
 }
+
+#ifndef GS_ORACLE_DOUBLE
+/* =====================================================================
+ * Adaptive density control: DensityController (density_controller.cpp:9-204)
+ * with OwnedRawParams' append / compact (train_types.h:107-136), on raw
+ * parameter arrays.  Follows the reference operation by operation, with one
+ * difference in what is read: the reference's append copies its source through
+ * pointers into the vectors it grows (invalid once push_back reallocates), and
+ * its compact after the split reads the remove mask past its end for the
+ * appended children; here the copies are exact and the children are kept,
+ * which is what the reference computes whenever those reads are defined
+ * (enough capacity; the children within the mask's last 64-bit word).
+ * ===================================================================== */
+float gso_dc_randn(uint64_t* st) {  /* :9-16 */
+    *st = *st * 6364136223846793005ULL + 1442695040888963407ULL;
+    float u1 = (float)((*st >> 33) & 0xFFFFFF) / (float)0xFFFFFF;
+    if (u1 < 1e-7f) u1 = 1e-7f;
+    *st = *st * 6364136223846793005ULL + 1442695040888963407ULL;
+    float u2 = (float)((*st >> 33) & 0xFFFFFF) / (float)0xFFFFFF;
+    return sqrtf(-2.0f * logf(u1)) * cosf(6.28318530718f * u2);
+}
+
+void gso_dc_accumulate(int N, const float* d_means2D, const int* radii, float* accum, float* denom, int* max_radii) {
+    for (int i = 0; i < N; i++) {  /* :29-41 */
+        float dx = d_means2D[i * 2], dy = d_means2D[i * 2 + 1];
+        float norm = sqrtf(dx * dx + dy * dy);
+        accum[i] += norm;
+        denom[i] += 1.0f;
+        if (radii && radii[i] > max_radii[i]) max_radii[i] = radii[i];
+    }
+}
+
+typedef struct { float *pos, *sc, *rot, *sh, *op; int n, k3; } dc_arr;
+
+static void dc_copy(dc_arr* d, int di, const dc_arr* s, int si) {
+    memcpy(&d->pos[di * 3], &s->pos[si * 3], 12);
+    memcpy(&d->sc[di * 3], &s->sc[si * 3], 12);
+    memcpy(&d->rot[di * 4], &s->rot[si * 4], 16);
+    memcpy(&d->sh[(size_t)di * d->k3], &s->sh[(size_t)si * s->k3], (size_t)d->k3 * 4);
+    d->op[di] = s->op[si];
+}
+
+static float dc_max_scale(const float* ls) {
+    float m = 0;
+    for (int j = 0; j < 3; j++) m = SMAX(m, expf(ls[j]));
+    return m;
+}
+
+/* compact (train_types.h:119-136): removes the masked entries, keeping order */
+static void dc_compact(dc_arr* a, const unsigned char* remove) {
+    int dst = 0;
+    for (int src = 0; src < a->n; src++) {
+        if (remove[src]) continue;
+        if (dst != src) dc_copy(a, dst, a, src);
+        dst++;
+    }
+    a->n = dst;
+}
+
+/* densify_and_prune (:148-197) on N Gaussians: statistics accum / denom /
+ * max_radii, generator state *rng.  The arrays hold room for 3N Gaussians
+ * (clones and split children) and are rewritten in place; returns the new count. */
+int gso_densify_and_prune(int N, int max_coeffs, float* pos, float* sc, float* rot, float* sh, float* op,
+                          const float* accum, const float* denom, const int* max_radii, float grad_threshold,
+                          float min_opacity, int max_screen_size, float extent, uint64_t* rng) {
+    if (N == 0) return 0;
+    dc_arr a = {pos, sc, rot, sh, op, N, max_coeffs * 3};
+    float* avg = (float*)calloc((size_t)N, sizeof(float));
+    unsigned char* mask = (unsigned char*)calloc((size_t)N * 3 + 1, 1);
+    for (int i = 0; i < N; i++)
+        if (denom[i] > 0) avg[i] = accum[i] / denom[i];
+    float size_threshold = 0.01f * extent;
+    /* densify_and_clone (:47-73) */
+    for (int i = 0; i < N; i++) {
+        if (avg[i] < grad_threshold) continue;
+        if (dc_max_scale(&sc[i * 3]) > size_threshold) continue;
+        dc_copy(&a, a.n, &a, i);
+        a.n++;
+    }
+    /* densify_and_split (:75-138); the clones (index >= N) are small and never split */
+    int n2 = a.n;
+    for (int i = 0; i < N; i++) {
+        if (avg[i] < grad_threshold) continue;
+        if (dc_max_scale(&sc[i * 3]) <= size_threshold) continue;
+        float scale[3], q[4];
+        for (int j = 0; j < 3; j++) scale[j] = expf(sc[i * 3 + j]);
+        for (int j = 0; j < 4; j++) q[j] = rot[i * 4 + j];
+        float qlen = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        if (qlen > 1e-8f) for (int j = 0; j < 4; j++) q[j] /= qlen;
+        float r = q[0], x = q[1], y = q[2], z = q[3];
+        float R[9] = {1 - 2 * (y * y + z * z), 2 * (x * y + r * z), 2 * (x * z - r * y),
+                      2 * (x * y - r * z), 1 - 2 * (x * x + z * z), 2 * (y * z + r * x),
+                      2 * (x * z + r * y), 2 * (y * z - r * x), 1 - 2 * (x * x + y * y)};
+        float scale_factor = logf(0.8f * 2.0f);
+        for (int k = 0; k < 2; k++) {
+            float s0 = gso_dc_randn(rng) * scale[0];
+            float s1 = gso_dc_randn(rng) * scale[1];
+            float s2 = gso_dc_randn(rng) * scale[2];
+            float sample[3] = {s0, s1, s2};
+            int c = a.n++;
+            for (int row = 0; row < 3; row++) {
+                float o = 0;
+                for (int col = 0; col < 3; col++) o += R[row * 3 + col] * sample[col];
+                pos[c * 3 + row] = pos[i * 3 + row] + o;
+                sc[c * 3 + row] = sc[i * 3 + row] - scale_factor;
+            }
+            memcpy(&rot[c * 4], &rot[i * 4], 16);
+            memcpy(&sh[(size_t)c * a.k3], &sh[(size_t)i * a.k3], (size_t)a.k3 * 4);
+            op[c] = op[i];
+        }
+        mask[i] = 1;
+    }
+    (void)n2;
+    dc_compact(&a, mask);
+    /* prune (:157-186) */
+    int new_n = a.n;
+    memset(mask, 0, (size_t)N * 3 + 1);
+    for (int i = 0; i < new_n; i++) {
+        float o = 1.0f / (1.0f + expf(-op[i]));
+        if (o < min_opacity) mask[i] = 1;
+    }
+    for (int i = 0; i < SMIN(N, new_n); i++)
+        if (max_radii[i] > max_screen_size) mask[i] = 1;
+    for (int i = 0; i < new_n; i++)
+        if (dc_max_scale(&sc[i * 3]) > 0.1f * extent) mask[i] = 1;
+    dc_compact(&a, mask);
+    free(avg);
+    free(mask);
+    return a.n;
+}
+#endif

@@ -330,6 +330,52 @@ def fd_grads_local(scene, cam, cfg, gt, targets, h=1e-4, fallback=(1e-5, 1e-6), 
     return np.stack([r[0] for r in res]), np.stack([r[1] for r in res])
 
 
+class OracleDensity:
+    """The oracle's density controller (gso_dc_*, gso_densify_and_prune): the
+    statistics, the generator state and the parameter arrays on the host."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL(os.path.join(BUILD, "libgs_oracle.so"))
+        self.L.gso_densify_and_prune.restype = _i32
+        self.L.gso_densify_and_prune.argtypes = [_i32, _i32] + [_vp] * 8 + [_f32, _f32, _i32, _f32, _vp]
+        self.L.gso_dc_accumulate.argtypes = [_i32, _vp, _vp, _vp, _vp, _vp]
+        self.L.gso_dc_randn.restype = _f32
+        self.L.gso_dc_randn.argtypes = [_vp]
+        self.rng = np.array([12345], np.uint64)
+        self.n = -1
+
+    def reset(self, n):
+        self.n = n
+        self.accum = np.zeros(n, np.float32)
+        self.denom = np.zeros(n, np.float32)
+        self.maxr = np.zeros(n, np.int32)
+
+    def accumulate(self, d_means2D, radii=None):
+        d = np.ascontiguousarray(d_means2D, np.float32)
+        n = d.size // 2
+        if n != self.n:
+            self.reset(n)
+        r = None if radii is None else np.ascontiguousarray(radii, np.int32)
+        self.L.gso_dc_accumulate(n, _p(d), _p(r), _p(self.accum), _p(self.denom), _p(self.maxr))
+
+    def densify(self, params, extent, grad_threshold=0.0002, min_opacity=0.005, max_screen_size=20, **_):
+        """params: [pos, scale, rot, sh, op] (raw); returns the new list."""
+        n, k = params[0].shape[0], params[3].shape[1]
+        if n != self.n:
+            self.reset(n)
+        cap = 3 * n + 1
+        bufs = [np.zeros((cap,) + a.shape[1:], np.float32) for a in params]
+        for b, a in zip(bufs, params):
+            b[:n] = a
+        m = self.L.gso_densify_and_prune(n, k, *[_p(b) for b in bufs], _p(self.accum), _p(self.denom), _p(self.maxr),
+                                          grad_threshold, min_opacity, max_screen_size, extent, _p(self.rng))
+        self.reset(m)
+        return [b[:m].copy() for b in bufs]
+
+    def randn(self, count):
+        return np.array([self.L.gso_dc_randn(_p(self.rng)) for _ in range(count)], np.float32)
+
+
 def inv_depth_oracle(pre, values_sorted, ranges, width, height):
     """gso_rasterize_inv_depth: the inverse-depth output of the 2D forward."""
     L = ctypes.CDLL(os.path.join(BUILD, "libgs_oracle.so"))

@@ -12,6 +12,7 @@
 #include "loss.h"
 #include "optimizer.h"
 #include "trainer.h"
+#include "density_controller.h"
 #include "cpu/preprocessor_cpu.h"
 #include "cpu/tile_binner_cpu.h"
 #include "cpu/sorter_cpu.h"
@@ -262,6 +263,83 @@ long long ref_render_3d(int N, int sh_degree, int max_coeffs, float* pos, float*
         for (int k = 0; k < 16; k++) g2s[16 * i + k] = vis ? pre.gauss2screen[16 * i + k] : 0;
     }
     return K;
+}
+
+// ---- DensityController and OwnedRawParams (density_controller.cpp:9-204,
+// train_types.h:85-157) behind opaque handles, and Trainer::step_with_densify
+// (trainer.cpp:72-182). ----
+void* ref_params_new(int N, int sh_degree, int max_coeffs, const float* pos, const float* sc, const float* rot,
+                     const float* sh, const float* op) {
+    OwnedRawParams* p = new OwnedRawParams();
+    p->sh_degree = sh_degree;
+    p->max_coeffs = max_coeffs;
+    // OwnedRawParams::append copies from pointers into the vectors it grows
+    // (train_types.h:108-116): with this much capacity no push_back reallocates
+    // during a densification, so those reads stay valid
+    const size_t cap = (size_t)N * 16 + 64;
+    p->positions.reserve(cap * 3);
+    p->scales.reserve(cap * 3);
+    p->rotations.reserve(cap * 4);
+    p->sh_coeffs.reserve(cap * max_coeffs * 3);
+    p->opacities.reserve(cap);
+    p->resize(N);
+    std::memcpy(p->positions.data(), pos, (size_t)N * 12);
+    std::memcpy(p->scales.data(), sc, (size_t)N * 12);
+    std::memcpy(p->rotations.data(), rot, (size_t)N * 16);
+    std::memcpy(p->sh_coeffs.data(), sh, (size_t)N * max_coeffs * 12);
+    std::memcpy(p->opacities.data(), op, (size_t)N * 4);
+    return p;
+}
+void ref_params_free(void* h) { delete static_cast<OwnedRawParams*>(h); }
+int ref_params_count(void* h) { return static_cast<OwnedRawParams*>(h)->count(); }
+void ref_params_get(void* h, float* pos, float* sc, float* rot, float* sh, float* op) {
+    OwnedRawParams* p = static_cast<OwnedRawParams*>(h);
+    const size_t n = (size_t)p->count();
+    std::memcpy(pos, p->positions.data(), n * 12);
+    std::memcpy(sc, p->scales.data(), n * 12);
+    std::memcpy(rot, p->rotations.data(), n * 16);
+    std::memcpy(sh, p->sh_coeffs.data(), n * p->max_coeffs * 12);
+    std::memcpy(op, p->opacities.data(), n * 4);
+}
+void* ref_dc_new() { return new DensityController(); }
+void ref_dc_free(void* h) { delete static_cast<DensityController*>(h); }
+void ref_dc_reset(void* h, int N) { static_cast<DensityController*>(h)->reset(N); }
+void ref_dc_accumulate(void* h, const float* d_means2D, const int* radii, int N) {
+    static_cast<DensityController*>(h)->accumulate_stats(d_means2D, radii, N);
+}
+// icfg: from, until, interval, opacity_reset_interval, max_screen_size, enabled;
+// fcfg: grad_threshold, min_opacity, percent_dense
+static DensifyConfig dcfg_of(const int* icfg, const float* fcfg) {
+    DensifyConfig c;
+    c.densify_from_iter = icfg[0]; c.densify_until_iter = icfg[1]; c.densify_interval = icfg[2];
+    c.opacity_reset_interval = icfg[3]; c.max_screen_size = icfg[4]; c.enabled = icfg[5] != 0;
+    c.grad_threshold = fcfg[0]; c.min_opacity = fcfg[1]; c.percent_dense = fcfg[2];
+    return c;
+}
+int ref_dc_densify(void* h, void* params, const int* icfg, const float* fcfg, float extent) {
+    return static_cast<DensityController*>(h)->densify_and_prune(*static_cast<OwnedRawParams*>(params),
+                                                                  dcfg_of(icfg, fcfg), extent);
+}
+void ref_dc_reset_opacity(void* params) { DensityController::reset_opacity(*static_cast<OwnedRawParams*>(params)); }
+
+void* ref_trainer_new(size_t arena_bytes) { return new Trainer(arena_bytes); }
+void ref_trainer_free(void* h) { delete static_cast<Trainer*>(h); }
+// tcf: lr_pos_init, lr_pos_final, lr_feature, lr_opacity, lr_scaling, lr_rotation, beta1, beta2, eps;
+// tci: max_steps, use_adam.  Returns the loss; *n_out the count after the step.
+float ref_trainer_step_densify(void* h, void* params, const gs_camera* camp, const gs_render_cfg* cfgp,
+                               const float* gt, const float* tcf, const int* tci, const int* icfg, const float* fcfg,
+                               float extent, int iteration, int* n_out) {
+    TrainConfig tc;
+    tc.lr_position_init = tcf[0]; tc.lr_position_final = tcf[1]; tc.lr_feature = tcf[2];
+    tc.lr_opacity = tcf[3]; tc.lr_scaling = tcf[4]; tc.lr_rotation = tcf[5];
+    tc.adam_beta1 = tcf[6]; tc.adam_beta2 = tcf[7]; tc.adam_eps = tcf[8];
+    tc.max_steps = tci[0]; tc.use_adam = tci[1] != 0;
+    Camera cam = to_cam(camp);
+    RenderConfig cfg = to_cfg(cfgp);
+    Trainer::StepResult r = static_cast<Trainer*>(h)->step_with_densify(
+        *static_cast<OwnedRawParams*>(params), cam, gt, cfg, tc, dcfg_of(icfg, fcfg), extent, iteration);
+    *n_out = r.n_gaussians;
+    return r.loss;
 }
 
 }  // extern "C"

@@ -236,3 +236,90 @@ extern "C" int shim_adam(int n, int deg, int mc, float* p_pos, float* p_sc, floa
         for (int s = 0; s < steps; s++) adam.step(p, g, tc, s);
     })
 }
+
+// ---- DensityControllerHIP, OwnedRawParams and TrainerHIP::step_with_densify
+// behind opaque handles (the same calls as the reference's, for the tests). ----
+static HipContext& shim_ctx() {
+    static HipContext ctx(0);
+    return ctx;
+}
+
+extern "C" void* shim_params_new(int N, int sh_degree, int max_coeffs, const float* pos, const float* sc,
+                                 const float* rot, const float* sh, const float* op) {
+    OwnedRawParams* p = new OwnedRawParams();
+    p->sh_degree = sh_degree;
+    p->max_coeffs = max_coeffs;
+    p->resize(N);
+    std::memcpy(p->positions.data(), pos, (size_t)N * 12);
+    std::memcpy(p->scales.data(), sc, (size_t)N * 12);
+    std::memcpy(p->rotations.data(), rot, (size_t)N * 16);
+    std::memcpy(p->sh_coeffs.data(), sh, (size_t)N * max_coeffs * 12);
+    std::memcpy(p->opacities.data(), op, (size_t)N * 4);
+    return p;
+}
+extern "C" void shim_params_free(void* h) { delete static_cast<OwnedRawParams*>(h); }
+extern "C" int shim_params_count(void* h) { return static_cast<OwnedRawParams*>(h)->count(); }
+extern "C" void shim_params_get(void* h, float* pos, float* sc, float* rot, float* sh, float* op) {
+    OwnedRawParams* p = static_cast<OwnedRawParams*>(h);
+    const size_t n = (size_t)p->count();
+    std::memcpy(pos, p->positions.data(), n * 12);
+    std::memcpy(sc, p->scales.data(), n * 12);
+    std::memcpy(rot, p->rotations.data(), n * 16);
+    std::memcpy(sh, p->sh_coeffs.data(), n * p->max_coeffs * 12);
+    std::memcpy(op, p->opacities.data(), n * 4);
+}
+
+static DensifyConfig shim_dcfg(const int* icfg, const float* fcfg) {
+    DensifyConfig c;
+    c.densify_from_iter = icfg[0]; c.densify_until_iter = icfg[1]; c.densify_interval = icfg[2];
+    c.opacity_reset_interval = icfg[3]; c.max_screen_size = icfg[4]; c.enabled = icfg[5] != 0;
+    c.grad_threshold = fcfg[0]; c.min_opacity = fcfg[1]; c.percent_dense = fcfg[2];
+    return c;
+}
+
+extern "C" void* shim_dc_new() {
+    try {
+        return new DensityControllerHIP(shim_ctx());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+extern "C" void shim_dc_free(void* h) { delete static_cast<DensityControllerHIP*>(h); }
+extern "C" int shim_dc_reset(void* h, int N) { SHIM_TRY(static_cast<DensityControllerHIP*>(h)->reset(N)) }
+extern "C" int shim_dc_accumulate(void* h, const float* d_means2D, const int* radii, int N) {
+    SHIM_TRY(static_cast<DensityControllerHIP*>(h)->accumulate_stats(d_means2D, radii, N))
+}
+extern "C" int shim_dc_densify(void* h, void* params, const int* icfg, const float* fcfg, float extent, int* n_out) {
+    SHIM_TRY(*n_out = static_cast<DensityControllerHIP*>(h)->densify_and_prune(*static_cast<OwnedRawParams*>(params),
+                                                                                shim_dcfg(icfg, fcfg), extent))
+}
+extern "C" void shim_dc_reset_opacity(void* params) {
+    DensityControllerHIP::reset_opacity(*static_cast<OwnedRawParams*>(params));
+}
+
+extern "C" void* shim_trainer_new() {
+    try {
+        return new TrainerHIP(shim_ctx());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+extern "C" void shim_trainer_free(void* h) { delete static_cast<TrainerHIP*>(h); }
+extern "C" int shim_trainer_step_densify(void* h, void* params, const gs_camera* cam, const gs_render_cfg* cfg,
+                                         const float* gt, const float* tcf, const int* tci, const int* icfg,
+                                         const float* fcfg, float extent, int iteration, float* loss, int* n_out) {
+    SHIM_TRY({
+        TrainConfig tc;
+        tc.lr_position_init = tcf[0]; tc.lr_position_final = tcf[1]; tc.lr_feature = tcf[2];
+        tc.lr_opacity = tcf[3]; tc.lr_scaling = tcf[4]; tc.lr_rotation = tcf[5];
+        tc.adam_beta1 = tcf[6]; tc.adam_beta2 = tcf[7]; tc.adam_eps = tcf[8];
+        tc.max_steps = tci[0]; tc.use_adam = tci[1] != 0;
+        TrainerHIP::StepResult r = static_cast<TrainerHIP*>(h)->step_with_densify(
+            *static_cast<OwnedRawParams*>(params), cam_of(cam), gt, cfg_of(cfg), tc, shim_dcfg(icfg, fcfg), extent,
+            iteration);
+        *loss = r.loss;
+        *n_out = r.n_gaussians;
+    })
+}
