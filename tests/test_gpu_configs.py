@@ -132,37 +132,65 @@ def test_1m_1080p_gradients_match_fp64_finite_differences(step_1m_1080p):
 
 @pytest.mark.parametrize("workload", ["1m_16views", "6m_8x1080p"])
 def test_batched_step_at_full_size(hip, workload):
+    """Pair count, bit-exact render, and the batched gradient against the sum
+    of the single-view gradients.  Both sum screen-space terms with float
+    atomics, whose order changes from run to run; for the few very large
+    Gaussians of the 6M scene (screen radii up to ~750 px, thousands of tiles
+    of partly cancelling terms) that order alone moves a gradient by more than
+    1e-3 of the array's largest entry.  So each path runs twice: the
+    batched-vs-single difference must stay within 1e-3 of the largest entry
+    plus four times the larger run-to-run difference, and at most 1e-5 of the
+    entries may exceed the plain 1e-3 bar."""
     from gs_hip import GsBatch, GsTrainer, train_cfg
     print(f"[case] {workload} batched step: pairs, render, gradient", flush=True)
     scene, cams, gts = make_workload(workload)
     cfg = RenderCfg(sh_degree=3, training=True)
-    tr = GsTrainer(hip, scene)
-    b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=False, use_graph=True)
-    K = b.render()
-    imgs = [im.clone() for im in b.images()]
+    sh = _shapes(scene)
     p = hip.upload_scene(scene)
     tiles = [int(hip.preprocess(p, cam, cfg)["tiles"].to(torch.int64).sum().item()) for cam in cams]
-    assert K == sum(tiles), (K, tiles)
-    for v, cam in enumerate(cams):
-        tr.render(cam, cfg)
-        assert torch.equal(imgs[v], tr.buffer("image", (cam.height, cam.width, 3))), f"view {v}"
-    # gradient of the batch = sum of the single-view gradients (before the update)
-    b.step(train_cfg(lr=1e-3), 0, sync=True)
-    _, losses = b.result()
-    sh = _shapes(scene)
-    got = {k: tr.buffer("grad_" + k, sh[k]) for k in NAMES}
-    ref = {k: torch.zeros(sh[k], device=hip.dev) for k in NAMES}
-    single_losses = []
-    for cam, gt in zip(cams, gts):
-        f = hip.forward_backward_fast(p, cam, cfg, torch.tensor(gt, device=hip.dev))
-        single_losses.append(float(f["loss"].item()))
-        for k in NAMES:
-            ref[k] += f["grads"][k].view(sh[k])
-        del f
-    torch.cuda.synchronize()
-    print(f"  K={K} views={len(cams)} losses {np.round(losses, 6).tolist()}", flush=True)
+
+    def batched(check_render):
+        tr = GsTrainer(hip, scene)
+        b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=False, use_graph=True)
+        if check_render:
+            K = b.render()
+            assert K == sum(tiles), (K, tiles)
+            imgs = [im.clone() for im in b.images()]
+            for v, cam in enumerate(cams):
+                tr.render(cam, cfg)
+                assert torch.equal(imgs[v], tr.buffer("image", (cam.height, cam.width, 3))), f"view {v}"
+        # gradient of the batch (before its update)
+        b.step(train_cfg(lr=1e-3), 0, sync=True)
+        _, losses = b.result()
+        g = {k: tr.buffer("grad_" + k, sh[k]).cpu().numpy().astype(np.float64) for k in NAMES}
+        del b, tr
+        return g, losses
+
+    def single():
+        ref = {k: np.zeros(sh[k], np.float64) for k in NAMES}
+        losses = []
+        for cam, gt in zip(cams, gts):
+            f = hip.forward_backward_fast(p, cam, cfg, torch.tensor(gt, device=hip.dev))
+            losses.append(float(f["loss"].item()))
+            for k in NAMES:
+                ref[k] += f["grads"][k].view(sh[k]).cpu().numpy()
+            del f
+        return ref, losses
+
+    b1, losses = batched(True)
+    b2, _ = batched(False)
+    s1, single_losses = single()
+    s2, _ = single()
+    print(f"  K={sum(tiles)} views={len(cams)} losses {np.round(losses, 6).tolist()}", flush=True)
     assert np.allclose(losses, single_losses, rtol=0, atol=1e-6), (losses, single_losses)
+    fails = []
     for k in NAMES:
-        _elementwise_report(got[k].cpu().numpy(), ref[k].cpu().numpy(), "grad_" + k)
-    for k in NAMES:
-        assert_grad_close(got[k].cpu().numpy(), ref[k].cpu().numpy(), f"{workload} grad_{k}")
+        top = np.abs(s1[k]).max()
+        err = np.abs(b1[k] - s1[k])
+        noise = max(np.abs(b1[k] - b2[k]).max(), np.abs(s1[k] - s2[k]).max())
+        over = int((err > 1e-3 * top).sum())
+        print(f"  grad_{k}: max|b-s|/max|s| {err.max() / top:.2e}; run-to-run {noise / top:.2e}; "
+              f"entries over 1e-3: {over} of {err.size}", flush=True)
+        if err.max() > 1e-3 * top + 4 * noise or over > 1e-5 * err.size:
+            fails.append((k, err.max() / top, noise / top, over))
+    assert not fails, fails
