@@ -17,6 +17,7 @@ struct Geom {
     float det;
     float conic[3];
     float hconv;
+    float det_cov, c2a, c2c;  // determinant and diagonal before the filter (antialiasing backward)
     int radius;
     int tiles;
 };
@@ -39,6 +40,9 @@ __device__ __forceinline__ bool preprocess_geom(const float p[3], const float s[
     proj_jacobian(g.pv, c.view, c.tanx, c.tany, c.fx, c.fy, pj);
     cov2d_from(pj, g.c3, g.c2);
     float det_cov = g.c2[0] * g.c2[2] - g.c2[1] * g.c2[1];
+    g.det_cov = det_cov;
+    g.c2a = g.c2[0];
+    g.c2c = g.c2[2];
     g.c2[0] += 0.3f;
     g.c2[2] += 0.3f;
     float det_h = g.c2[0] * g.c2[2] - g.c2[1] * g.c2[1];
@@ -216,14 +220,29 @@ struct RGrad {
 __device__ __forceinline__ void geom_backward(const Params& P, const Geom& g, const CamArgs& c, const RGrad& r,
                                               float gpos[3], float gsc[3], float grot[4], float& gop) {
     const float sig = P.op;
-    gop = r.dop * sig * (1.0f - sig);
+    // antialiasing (preprocessor_cpu.cpp:213-216,250): op2d = op * h with
+    // h = sqrt(max(2.5e-5, det_cov / det)); the reference's backward has no
+    // term for h, this one carries it into the opacity and, through det_cov
+    // and det, into the 2D covariance (h = 1 without antialiasing)
+    gop = r.dop * g.hconv * sig * (1.0f - sig);
 
     const float a = g.c2[0], b = g.c2[1], cc = g.c2[2];
     const float inv_det = 1.0f / g.det;
     const float id2 = inv_det * inv_det;
-    const float d_a = r.dc0 * (-cc * cc * id2) + r.dc1 * (b * cc * id2) + r.dc2 * (-b * b * id2);
-    const float d_b = r.dc0 * (2.0f * b * cc * id2) + r.dc1 * (-(a * cc + b * b) * id2) + r.dc2 * (2.0f * a * b * id2);
-    const float d_c = r.dc0 * (-b * b * id2) + r.dc1 * (a * b * id2) + r.dc2 * (-a * a * id2);
+    float d_a = r.dc0 * (-cc * cc * id2) + r.dc1 * (b * cc * id2) + r.dc2 * (-b * b * id2);
+    float d_b = r.dc0 * (2.0f * b * cc * id2) + r.dc1 * (-(a * cc + b * b) * id2) + r.dc2 * (2.0f * a * b * id2);
+    float d_c = r.dc0 * (-b * b * id2) + r.dc1 * (a * b * id2) + r.dc2 * (-a * a * id2);
+    if (c.antialiasing) {
+        const float ratio = g.det_cov / g.det;
+        if (ratio > 0.000025f) {
+            const float dL_dratio = r.dop * sig * 0.5f / g.hconv;
+            const float dL_dcovdet = dL_dratio / g.det;
+            const float dL_ddet = -dL_dratio * ratio / g.det;
+            d_a += dL_dcovdet * g.c2c + dL_ddet * cc;
+            d_c += dL_dcovdet * g.c2a + dL_ddet * a;
+            d_b += (dL_dcovdet + dL_ddet) * (-2.0f * b);
+        }
+    }
 
     ProjJac pj;
     proj_jacobian(g.pv, c.view, c.tanx, c.tany, c.fx, c.fy, pj);

@@ -690,10 +690,30 @@ void PFX(preprocess_backward)(int N, int model_deg, int max_coeffs, const real* 
         sh_backward(model_deg, max_coeffs, &sh[i * (long)max_coeffs * 3], &pos[i * 3], cam.campos,
                     &d_rgb[i * 3], &g_sh[i * (long)max_coeffs * 3], &g_pos[i * 3]);
         real sig = op[i];
-        g_op[i] = d_op2d[i] * sig * ((real)1.0f - sig);
-
         real a = cache_cov2D[i * 3], b = cache_cov2D[i * 3 + 1], c = cache_cov2D[i * 3 + 2];
         real det = cache_det[i];
+        /* Antialiasing (preprocessor_cpu.cpp:213-216,250): op2d = op * h with
+         * h = sqrt(max(2.5e-5, det_cov / det)), det_cov the determinant before
+         * the 0.3 dilation.  The reference's backward has no term for h; this
+         * one carries it into the opacity and, through det_cov and det, into
+         * the 2D covariance (checked against FP64 central differences). */
+        real hconv = (real)1.0f, dL_dcovdet = 0, dL_ddet = 0, a0 = 0, c0 = 0;
+        if (cfg->antialiasing) {
+            real pre2[3];
+            cov2d_of(&pos[i * 3], &cache_cov3D[i * 6], &cam, fx, fy, pre2);
+            a0 = pre2[0];
+            c0 = pre2[2];
+            real det_cov = pre2[0] * pre2[2] - pre2[1] * pre2[1];
+            real ratio = det_cov / det;
+            hconv = SQRTR(SMAX((real)0.000025f, ratio));
+            if (ratio > (real)0.000025f) {
+                real dL_dratio = d_op2d[i] * sig * (real)0.5f / hconv;
+                dL_dcovdet = dL_dratio / det;
+                dL_ddet = -dL_dratio * ratio / det;
+            }
+        }
+        g_op[i] = d_op2d[i] * hconv * sig * ((real)1.0f - sig);
+
         if (det == (real)0.0f) continue;
         real inv_det = (real)1.0f / det;
         real dc0 = d_conics[i * 3], dc1 = d_conics[i * 3 + 1], dc2 = d_conics[i * 3 + 2];
@@ -702,6 +722,11 @@ void PFX(preprocess_backward)(int N, int model_deg, int max_coeffs, const real* 
         real d_b = dc0 * ((real)2.0f * b * c * id2) + dc1 * (-(a * c + b * b) * id2) +
                    dc2 * ((real)2.0f * a * b * id2);
         real d_c = dc0 * (-b * b * id2) + dc1 * (a * b * id2) + dc2 * (-a * a * id2);
+        if (cfg->antialiasing) {
+            d_a += dL_dcovdet * c0 + dL_ddet * c;
+            d_c += dL_dcovdet * a0 + dL_ddet * a;
+            d_b += (dL_dcovdet + dL_ddet) * ((real)-2.0f * b);
+        }
 
         proj_jac pj;
         projection_jacobian(&cache_pview[i * 3], &cam, fx, fy, &pj);
