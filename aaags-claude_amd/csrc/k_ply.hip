@@ -134,6 +134,18 @@ static int parse_header(std::ifstream& f, const char* path, PlyLayout& L) {
     return GS_OK;
 }
 
+// initRawFromGaussianData (train_main.cpp:31-63) after the activation of
+// loadPly: the log of the activated scale (floored at 1e-12) and the inverse
+// sigmoid of the clamped opacity, with the reference's logf (log_ref).
+__global__ __launch_bounds__(256) void raw_from_activated_kernel(int n, float* __restrict__ scale,
+                                                                 float* __restrict__ op) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < 3; k++) scale[i * 3 + k] = log_ref(smax(1e-12f, scale[i * 3 + k]));
+    const float y = smax(1e-6f, smin(1.0f - 1e-6f, op[i]));
+    op[i] = log_ref(y / (1.0f - y));
+}
+
 }  // namespace gsp
 
 extern "C" int gs_ply_info(const char* path, int* count, int* sh_degree, int* max_coeffs, int* has_filter_3d) {
@@ -192,6 +204,16 @@ extern "C" int gs_ply_load(gs_ctx* c, const char* path, int n, int max_coeffs, f
     (void)hipHostFree(host);
     if (e != hipSuccess) return gs_hip_status(e, "PLY upload / unpack kernel");
     return gs_hip_status(e2, "hipStreamSynchronize");
+}
+
+extern "C" int gs_ply_load_raw(gs_ctx* c, const char* path, int n, int max_coeffs, float* positions,
+                               float* log_scales, float* rotations, float* sh_coeffs, float* opacity_logits) {
+    int st = gs_ply_load(c, path, n, max_coeffs, positions, log_scales, rotations, sh_coeffs, opacity_logits, nullptr);
+    if (st) return st;
+    gsp::raw_from_activated_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, log_scales, opacity_logits);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return gs_hip_status(e, "raw parameter kernel");
+    return gs_hip_status(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
 }
 
 extern "C" int gs_ply_save(gs_ctx* c, const char* path, int n, int max_coeffs, const float* positions,

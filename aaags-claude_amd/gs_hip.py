@@ -445,6 +445,19 @@ class GsHip:
                                          _vp(ptr(out["filter3d"]))), "gs_ply_load")
         return out
 
+    def load_ply_raw(self, path):
+        """loadPly + initRawFromGaussianData into device tensors: the raw
+        parameters a training run starts from (upload_scene's layout)."""
+        info = self.ply_info(path)
+        n, k, d = info["n"], info["max_coeffs"], self.dev
+        out = dict(pos=torch.empty((n, 3), device=d), scale=torch.empty((n, 3), device=d),
+                   rot=torch.empty((n, 4), device=d), sh=torch.empty((n, k, 3), device=d), op=torch.empty(n, device=d),
+                   n=n, max_coeffs=k, sh_degree=info["sh_degree"])
+        self._check(self.lib.gs_ply_load_raw(self.ctx, path.encode(), _i32(n), _i32(k), _vp(ptr(out["pos"])),
+                                             _vp(ptr(out["scale"])), _vp(ptr(out["rot"])), _vp(ptr(out["sh"])),
+                                             _vp(ptr(out["op"]))), "gs_ply_load_raw")
+        return out
+
     def save_ply(self, path, p):
         """savePly of raw device parameters (upload_scene's dict)."""
         self._check(self.lib.gs_ply_save(self.ctx, path.encode(), p["n"], p["max_coeffs"], _vp(ptr(p["pos"])),
@@ -666,6 +679,32 @@ class GsTrainer:
         if st != 0:
             raise GsError(f"hipMemcpy failed ({st})")
         return out.view(*shape)
+
+    def train_views(self, cams, cfg, targets, tc, iterations, seed=42, save_every=0, output=None,
+                    first_iteration=0, log_every=0):
+        """gs_trainer_train_views: the multi-view training loop (one random view
+        per iteration, checkpoints); returns (losses, views)."""
+        V = len(cams)
+        carr = (GsCamera * V)(*[c.to_c() for c in cams])
+        tg = [t if isinstance(t, torch.Tensor) else torch.tensor(np.asarray(t, np.float32), device=self.hip.dev)
+              for t in targets]
+        ptrs = (_vp * V)(*[_vp(ptr(t)) for t in tg])
+        losses = np.zeros(max(iterations, 1), np.float64)
+        views = np.zeros(max(iterations, 1), np.int32)
+        f = cfg.to_c()
+        self.lib.gs_trainer_train_views.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _i32, _i32, ctypes.c_uint, _i32,
+                                                    ctypes.c_char_p, _i32, _vp, _vp]
+        self.hip._check(self.lib.gs_trainer_train_views(self.tr, V, carr, ptrs, ctypes.byref(f), ctypes.byref(tc),
+                                                        first_iteration, iterations, seed, save_every,
+                                                        output.encode() if output else None, log_every,
+                                                        losses.ctypes.data_as(_vp), views.ctypes.data_as(_vp)),
+                        "gs_trainer_train_views")
+        return losses[:iterations], views[:iterations]
+
+    def count(self):
+        n = _i32()
+        self.hip._check(self.lib.gs_trainer_count(self.tr, ctypes.byref(n)), "gs_trainer_count")
+        return n.value
 
     def params(self):
         out = [np.empty((self.n, 3), np.float32), np.empty((self.n, 3), np.float32), np.empty((self.n, 4), np.float32),

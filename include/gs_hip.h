@@ -243,6 +243,12 @@ int gs_raster_fwd_words(gs_ctx* ctx, int width, int height, const gs_preprocesse
 int gs_ply_info(const char* path, int* count, int* sh_degree, int* max_coeffs, int* has_filter_3d);
 int gs_ply_load(gs_ctx* ctx, const char* path, int n, int max_coeffs, float* positions, float* scales,
                 float* rotations, float* sh_coeffs, float* opacities, float* filter_3d);
+/* loadPly followed by initRawFromGaussianData (src/train_main.cpp:31-63):
+ * the raw parameters a training run starts from (log of the activated scale,
+ * inverse sigmoid of the clamped opacity, normalised rotations, SH as read).
+ * Same arrays and checks as gs_ply_load; returns after the data is ready. */
+int gs_ply_load_raw(gs_ctx* ctx, const char* path, int n, int max_coeffs, float* positions, float* log_scales,
+                    float* rotations, float* sh_coeffs, float* opacity_logits);
 /* savePly (src/train_main.cpp:82-146): raw (pre-activation) device arrays
  * written as a 3DGS PLY, SH reordered to the file's channel-major f_rest. */
 int gs_ply_save(gs_ctx* ctx, const char* path, int n, int max_coeffs, const float* positions,
@@ -528,6 +534,35 @@ int gs_trainer_reset_opacity(gs_trainer* tr);
 int gs_trainer_step_densify(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
                             const gs_train_cfg* tc, const gs_densify_cfg* dcfg, float scene_extent, int iteration,
                             gs_step_result* out);
+
+/* ------------------------------------------------------------------------
+ * Camera files, images and the multi-view training loop (host side).
+ * ---------------------------------------------------------------------- */
+/* loadCamerasJson (src/train_main.cpp:262-377; src/main.cpp:146-264 reads one
+ * camera of the same file): the 3DGS cameras.json -- objects with id, width,
+ * height, fx, fy, position (camera centre) and rotation (camera-to-world
+ * 3x3) -- as Cameras: view = [R^T | -R^T c], viewproj = perspective
+ * (znear 0.01, zfar 100) * view, tan_fov = size / (2 f).  With cams NULL only
+ * *count is set; img_names (optional) receives each camera's img_name in slots
+ * of name_stride bytes. */
+int gs_cameras_json(const char* path, gs_camera* cams, int max_cams, int* count, char* img_names, int name_stride);
+/* readPPM / writePPM (src/image_io.cpp): binary 8-bit PPM (P6) to and from
+ * host HWC float images in [0, 1].  gs_read_ppm with image NULL only reads the
+ * size. */
+int gs_read_ppm(const char* path, float* image, size_t max_floats, int* width, int* height);
+int gs_write_ppm(const char* path, const float* image, int width, int height);
+/* The training loop of src/train_main.cpp:633-673 on the trainer: each
+ * iteration trains on one view, drawn by std::mt19937(seed) through
+ * std::uniform_int_distribution over the views (view 0 when there is one;
+ * the reference seeds 42), with iteration numbers from first_iteration;
+ * every save_every iterations the raw parameters go to
+ * "<output_path>.iter<N>.ply", and at the end to output_path (if not NULL).
+ * targets: per-view device HWC images.  losses / views (optional, host,
+ * `iterations` entries) receive each step's loss and view. */
+int gs_trainer_train_views(gs_trainer* tr, int n_views, const gs_camera* cams, const float* const* targets,
+                           const gs_render_cfg* cfg, const gs_train_cfg* tc, int first_iteration, int iterations,
+                           unsigned seed, int save_every, const char* output_path, int log_every, double* losses,
+                           int* views);
 
 /* gs_scan_tiles without any allocation (no counterpart in the reference; for
  * callers that cannot allocate: code under HIP graph capture, tight serving
