@@ -10,8 +10,14 @@
 //             stably with wave ballots (match-any on the digit bits), stages
 //             the block's keys in LDS in digit order and writes them out in
 //             runs, so global writes are contiguous within each digit bucket.
-//             1024 or 4096 keys per 256-thread block (by sort size).
+//             1024 or 4096 keys per 256-thread block (by sort size).  Sorts
+//             of 4M keys and more instead count every pass's digits in one
+//             read up front and run each pass as one kernel (decoupled
+//             look-back in place of the upsweep and the scan).
+#include <algorithm>
+
 #include "gs_device.h"
+#include "gs_internal.h"
 
 namespace gsk {
 
@@ -531,6 +537,229 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
     }
 }
 
+// ------------------------------------------------- one-sweep passes ----
+// Sorts of at least RS_SMALL_LIMIT keys: one kernel counts the digits of every
+// pass in a single read of the keys (rs_hist_all_kernel); then each pass is
+// one kernel (rs_onesweep_kernel).  Its blocks take consecutive tickets, rank
+// their keys as the downsweep does, publish their per-digit counts and obtain
+// the counts of all earlier blocks by decoupled look-back over per-(block,
+// digit) status words: bits 31..30 the state (0 none yet, 1 this block's count,
+// 2 the inclusive count of all blocks up to it), bits 29..0 the count.  A
+// block reaches back until it meets a state-2 word.  Tickets are handed out in
+// the order blocks start, so every block looked at has started and publishes
+// its own count before it waits on anything: the waits end.  A wait is
+// nevertheless bounded (OS_SPIN_LIMIT sleeps, about a tenth of a second);
+// past it the kernel sets *err and goes on, so a fault elsewhere cannot hang
+// the device.  Each pass clears the status words of the next (its block b
+// clears row b), so a sort needs one clearing launch.
+constexpr int OS_MAX_PASSES = 8;
+constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_CNT = (1u << 30) - 1u;
+constexpr uint32_t OS_SPIN_LIMIT = 1u << 22;
+constexpr uint32_t OS_HIST_BLOCKS = 1024;
+
+struct PassShifts {
+    int s[OS_MAX_PASSES];
+};
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Digit counts of every pass, summed into hist[pass * RADIX + digit] (zeroed
+// beforehand).  Four LDS copies, chosen by lane % 4, cut same-address
+// conflicts on the near-constant high digits by four.
+template <int BITS>
+__global__ __launch_bounds__(256) void rs_hist_all_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
+                                                          const uint32_t* __restrict__ n_dev, PassShifts sh,
+                                                          int passes, uint32_t* __restrict__ hist) {
+    constexpr int RADIX = 1 << BITS, COPIES = 4;
+    __shared__ uint32_t s_h[COPIES][OS_MAX_PASSES * RADIX];
+    const uint32_t n = live_count(n_cap, n_dev);
+    for (int i = threadIdx.x; i < COPIES * OS_MAX_PASSES * RADIX; i += 256) (&s_h[0][0])[i] = 0u;
+    __syncthreads();
+    uint32_t* mine = s_h[threadIdx.x & (COPIES - 1)];
+    const uint32_t stride = gridDim.x * 256u;
+    for (uint32_t i0 = blockIdx.x * 256u + threadIdx.x; i0 < n; i0 += 4u * stride) {
+        uint64_t k[4];
+        bool ok[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t i = (uint64_t)i0 + (uint64_t)q * stride;
+            ok[q] = i < n;
+            k[q] = ok[q] ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (!ok[q]) continue;
+#pragma unroll
+            for (int p = 0; p < OS_MAX_PASSES; p++)
+                if (p < passes) atomicAdd(&mine[p * RADIX + digit_of<BITS>(k[q], sh.s[p])], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * RADIX; i += 256) {
+        uint32_t c = 0u;
+#pragma unroll
+        for (int k = 0; k < COPIES; k++) c += s_h[k][i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// One pass: keys (and values) in, scattered stably by the digit at `shift`.
+// hist: this pass's digit totals; status / status_next: this pass's and the
+// next pass's status words ([block][digit]); ticket: zeroed counter.
+template <int BITS, bool VALS, int ROUNDS>
+__global__ __launch_bounds__(RS_THREADS) void rs_onesweep_kernel(
+    const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n_cap,
+    const uint32_t* __restrict__ n_dev, int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ status_next, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err,
+    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int RS_TILE = RS_THREADS * ROUNDS, RS_WAVE_SPAN = ROUNDS * 64;
+    static_assert(RADIX <= RS_THREADS, "one digit per thread");
+    const uint32_t n = live_count(n_cap, n_dev);
+    // keys in digit order, then (digit << 32 | value) in the same slots
+    __shared__ uint64_t s_k[RS_TILE];
+    __shared__ uint32_t s_wh[RS_WAVES][RADIX];
+    __shared__ uint32_t s_dstart[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+    __shared__ uint32_t s_scan[RS_WAVES];
+    __shared__ uint32_t s_b;
+
+    if (threadIdx.x == 0) s_b = atomicAdd(ticket, 1u);
+    for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_wh[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t b = s_b;
+    if (status_next && threadIdx.x < RADIX) status_next[(size_t)b * RADIX + threadIdx.x] = 0u;
+    const uint32_t bbase = b * (uint32_t)RS_TILE;
+    if (bbase >= n) return;  // past the live count: no block before it reads its words
+    {
+        // the digits' starts in the output (the look-back adds the earlier blocks' counts)
+        const uint32_t t = threadIdx.x < RADIX ? hist[threadIdx.x] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(t, s_scan, &total);
+        if (threadIdx.x < RADIX) s_goff[threadIdx.x] = ex;
+    }
+
+    const int w = threadIdx.x >> 6, l = lane_id();
+    const uint32_t wbase = bbase + w * RS_WAVE_SPAN;
+    uint64_t k_r[ROUNDS];
+    uint32_t v_r[VALS ? ROUNDS : 1];
+    uint32_t rank_r[ROUNDS];  // digits are recomputed from the keys (fewer registers)
+    // 1. per-wave stable ranks: rounds in key order, lanes in key order
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        const bool valid = i < n;
+        k_r[r] = valid ? keys_in[i] : 0ull;
+        if constexpr (VALS) v_r[r] = valid ? vals_in[i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        const bool valid = i < n;
+        const uint64_t active = __ballot(valid);
+        const uint32_t d = digit_of<BITS>(k_r[r], shift);
+        rank_r[r] = 0u;
+        if (active != 0) {
+            const uint64_t peers = match_digit<BITS>(d, active);
+            uint32_t before = 0u;
+            if (valid) before = s_wh[w][d];
+            __builtin_amdgcn_wave_barrier();
+            if (valid && (peers & lanemask_lt()) == 0) s_wh[w][d] = before + (uint32_t)__popcll(peers);
+            __builtin_amdgcn_wave_barrier();
+            rank_r[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
+        }
+    }
+    __syncthreads();
+    // 2. the block's count of each digit (published at once) and per-wave offsets
+    uint32_t cnt = 0u;
+    if (threadIdx.x < RADIX) {
+        const int d = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < RS_WAVES; k++) {
+            const uint32_t c = s_wh[k][d];
+            s_wh[k][d] = cnt;
+            cnt += c;
+        }
+        st_agent(&status[(size_t)b * RADIX + d], (b == 0 ? OS_INC : OS_AGG) | cnt);
+    }
+    {
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(cnt, s_scan, &total);
+        if (threadIdx.x < RADIX) s_dstart[threadIdx.x] = ex;
+    }
+    // 3. look-back: the counts of this digit in all earlier blocks, four words at a time
+    if (threadIdx.x < RADIX && b > 0) {
+        const int d = threadIdx.x;
+        const uint32_t* col = status + d;
+        uint32_t acc = 0u;
+        int j = (int)b - 1;
+        bool done = false;
+        while (!done) {
+            uint32_t s[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) s[q] = (j - q >= 0) ? ld_agent(&col[(size_t)(j - q) * RADIX]) : OS_INC;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (done) break;
+                uint32_t v = s[q];
+                uint32_t spins = 0u;
+                while ((v & ~OS_CNT) == 0u) {
+                    if (++spins > OS_SPIN_LIMIT) {
+                        *err = 1u;
+                        v = OS_INC;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    v = ld_agent(&col[(size_t)(j - q) * RADIX]);
+                }
+                acc += v & OS_CNT;
+                if (v & OS_INC) done = true;
+            }
+            j -= 4;
+        }
+        st_agent(&status[(size_t)b * RADIX + d], OS_INC | (acc + cnt));
+        s_goff[d] += acc;
+    }
+    __syncthreads();
+    // 4. stage the keys in LDS in (digit, input order) order and write them out in runs
+    uint32_t p_r[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        const uint32_t d = digit_of<BITS>(k_r[r], shift);
+        p_r[r] = s_dstart[d] + s_wh[w][d] + rank_r[r];
+        if (i < n && p_r[r] < (uint32_t)RS_TILE) s_k[p_r[r]] = k_r[r];
+    }
+    __syncthreads();
+    const uint32_t count = (n - bbase < (uint32_t)RS_TILE) ? (n - bbase) : (uint32_t)RS_TILE;
+    for (uint32_t p = threadIdx.x; p < count; p += RS_THREADS) {
+        const uint64_t k = s_k[p];
+        const uint32_t d = digit_of<BITS>(k, shift);
+        const uint32_t g = s_goff[d] + (p - s_dstart[d]);
+        if (g < n) keys_out[g] = k;
+    }
+    if constexpr (VALS) {
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            const uint32_t i = wbase + r * 64 + l;
+            if (i < n && p_r[r] < (uint32_t)RS_TILE) s_k[p_r[r]] = ((uint64_t)digit_of<BITS>(k_r[r], shift) << 32) | v_r[r];
+        }
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < count; p += RS_THREADS) {
+            const uint64_t e = s_k[p];
+            const uint32_t d = (uint32_t)(e >> 32);
+            const uint32_t g = s_goff[d] + (p - s_dstart[d]);
+            if (g < n) vals_out[g] = (uint32_t)e;
+        }
+    }
+}
+
 // identify_tile_ranges (radix_sort.cl:14-46), ranges pre-zeroed.
 // Tile ids at or beyond num_tiles are ignored (they cannot come from binning).
 __global__ __launch_bounds__(256) void tile_ranges_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
@@ -596,17 +825,65 @@ uint32_t radix_blocks(uint32_t n) {
 }
 
 // Workspace layout for one sort of n items with the given radix:
-//   [ping keys n*8][ping vals n*4][hist R*nb*4][goff R*nb*4][digit totals R*4]
-// Row stride of the digit-major count table: the block count rounded up to a
-// multiple of 4 (16-byte rows for the digit scan).
+//   [ping keys n*8][ping vals n*4][small: OS_MAX_PASSES*R + 64 words][table A R*nb*4][table B R*nb*4]
+// Three-launch passes: table A = per-block digit counts, table B = their
+// scans, small = digit totals.  One-sweep passes: small = every pass's digit
+// totals, the per-pass tickets and the error word; tables A and B = the status
+// words of alternate passes.  Row stride of the digit-major count table: the
+// block count rounded up to a multiple of 4 (16-byte rows for the digit scan).
 static uint32_t radix_row_stride(uint32_t nb) { return (nb + 3u) & ~3u; }
+
+static size_t small_words(int radix_bits) { return ((size_t)OS_MAX_PASSES << radix_bits) + 64; }
 
 size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
     size_t nb = radix_row_stride(radix_blocks(n));
     size_t R = (size_t)1 << radix_bits;
     size_t tab = R * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return al((size_t)n * 8) + al((size_t)n * 4) + 2 * al(tab * 4) + al(R * 4);
+    return al((size_t)n * 8) + al((size_t)n * 4) + al(small_words(radix_bits) * 4) + 2 * al(tab * 4);
+}
+
+// the one-sweep path: large sorts whose counts fit the status words' 30 bits
+static bool use_onesweep(uint32_t n, int passes) {
+    return n >= RS_SMALL_LIMIT && n <= OS_CNT && passes <= OS_MAX_PASSES;
+}
+
+template <int BITS>
+static hipError_t onesweep_sort(const uint64_t* ki, const uint32_t* vi, uint64_t* keys_out, uint32_t* vals_out,
+                                uint64_t* pk, uint32_t* pv, uint32_t n, const uint32_t* n_dev, const int* shifts,
+                                int passes, uint32_t* small, uint32_t* tab_a, uint32_t* tab_b, hipStream_t s) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int ROUNDS = 16;
+    const uint32_t nb = (n + RS_THREADS * ROUNDS - 1) / (RS_THREADS * ROUNDS);
+    uint32_t* hist = small;                                       // [passes][RADIX]
+    uint32_t* tickets = small + ((size_t)OS_MAX_PASSES << BITS);  // [OS_MAX_PASSES]
+    uint32_t* err = tickets + OS_MAX_PASSES;
+    // the small words and the first pass's status words (table A follows them)
+    hipError_t e = launch_zero(small, small_words(BITS) * 4, s);
+    if (e == hipSuccess) e = launch_zero(tab_a, (size_t)nb * RADIX * 4, s);
+    if (e != hipSuccess) return e;
+    PassShifts sh{};
+    for (int p = 0; p < passes; p++) sh.s[p] = shifts[p];
+    const uint32_t hb = std::min<uint32_t>(OS_HIST_BLOCKS, (n + 4095) / 4096);
+    rs_hist_all_kernel<BITS><<<hb, 256, 0, s>>>(ki, n, n_dev, sh, passes, hist);
+    const bool vals = vi != nullptr;
+    for (int p = 0; p < passes; p++) {
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t* ko = to_out ? keys_out : pk;
+        uint32_t* vo = vals ? (to_out ? vals_out : pv) : nullptr;
+        uint32_t* st = (p % 2 == 0) ? tab_a : tab_b;
+        uint32_t* st_next = (p + 1 < passes) ? ((p % 2 == 0) ? tab_b : tab_a) : nullptr;
+        if (vals)
+            rs_onesweep_kernel<BITS, true, ROUNDS><<<nb, RS_THREADS, 0, s>>>(
+                ki, vi, n, n_dev, shifts[p], hist + (size_t)p * RADIX, st, st_next, tickets + p, err, ko, vo);
+        else
+            rs_onesweep_kernel<BITS, false, ROUNDS><<<nb, RS_THREADS, 0, s>>>(
+                ki, nullptr, n, n_dev, shifts[p], hist + (size_t)p * RADIX, st, st_next, tickets + p, err, ko,
+                nullptr);
+        ki = ko;
+        vi = vo;
+    }
+    return hipGetLastError();
 }
 
 template <int BITS, int ROUNDS>
@@ -647,11 +924,11 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
     w += al((size_t)n * 8);
     uint32_t* pv = reinterpret_cast<uint32_t*>(w);
     w += al((size_t)n * 4);
-    uint32_t* hist = reinterpret_cast<uint32_t*>(w);
+    uint32_t* small = reinterpret_cast<uint32_t*>(w);
+    w += al(small_words(radix_bits) * 4);
+    uint32_t* tab_a = reinterpret_cast<uint32_t*>(w);
     w += al(tab * 4);
-    uint32_t* goff = reinterpret_cast<uint32_t*>(w);
-    w += al(tab * 4);
-    uint32_t* dtot = reinterpret_cast<uint32_t*>(w);
+    uint32_t* tab_b = reinterpret_cast<uint32_t*>(w);
 
     const bool vals = vals_in != nullptr;
     int bits = end_bit - begin_bit;
@@ -661,6 +938,34 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
         if (e != hipSuccess || !vals) return e;
         return hipMemcpyAsync(vals_out, vals_in, (size_t)n * 4, hipMemcpyDeviceToDevice, s);
     }
+    // A last digit that would reach past end_bit is moved down to end at it:
+    // the bits above end_bit may be set (the depth-sort words carry the index
+    // there).  Re-sorting bits an earlier pass already sorted keeps a stable
+    // LSD sort correct.
+    int shifts[64];
+    for (int p = 0; p < passes; p++) {
+        int shift = begin_bit + p * radix_bits;
+        if (shift + radix_bits > end_bit && end_bit - radix_bits >= begin_bit) shift = end_bit - radix_bits;
+        shifts[p] = shift;
+    }
+    if (use_onesweep(n, passes)) {
+        switch (radix_bits) {
+            case 4: return onesweep_sort<4>(keys_in, vals_in, keys_out, vals_out, pk, pv, n, n_dev, shifts, passes,
+                                            small, tab_a, tab_b, s);
+            case 5: return onesweep_sort<5>(keys_in, vals_in, keys_out, vals_out, pk, pv, n, n_dev, shifts, passes,
+                                            small, tab_a, tab_b, s);
+            case 6: return onesweep_sort<6>(keys_in, vals_in, keys_out, vals_out, pk, pv, n, n_dev, shifts, passes,
+                                            small, tab_a, tab_b, s);
+            case 7: return onesweep_sort<7>(keys_in, vals_in, keys_out, vals_out, pk, pv, n, n_dev, shifts, passes,
+                                            small, tab_a, tab_b, s);
+            case 8: return onesweep_sort<8>(keys_in, vals_in, keys_out, vals_out, pk, pv, n, n_dev, shifts, passes,
+                                            small, tab_a, tab_b, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    uint32_t* hist = tab_a;
+    uint32_t* goff = tab_b;
+    uint32_t* dtot = small;
     const uint64_t* ki = keys_in;
     const uint32_t* vi = vals_in;
     for (int p = 0; p < passes; p++) {
@@ -669,12 +974,7 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
         uint64_t* ko = to_out ? keys_out : pk;
         uint32_t* vo = vals ? (to_out ? vals_out : pv) : nullptr;
         hipError_t e;
-        // A last digit that would reach past end_bit is moved down to end at
-        // it: the bits above end_bit may be set (the depth-sort words carry
-        // the index there).  Re-sorting bits an earlier pass already sorted
-        // keeps a stable LSD sort correct.
-        int shift = begin_bit + p * radix_bits;
-        if (shift + radix_bits > end_bit && end_bit - radix_bits >= begin_bit) shift = end_bit - radix_bits;
+        const int shift = shifts[p];
         switch (radix_bits) {
             case 4: e = sort_pass<4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;
             case 5: e = sort_pass<5>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s); break;

@@ -261,21 +261,45 @@ def test_radix_sort_ignores_bits_above_key_bits(hip, key_bits, radix_bits):
     assert np.array_equal(u32(vo), vals[order])
 
 
-def test_radix_sort_large_many_tiles(hip):
-    """Above 4M keys (4096-key tiles, ~1300 of them): the one-sweep passes'
-    look-back crosses many tiles; 40-bit keys with heavy ties, with and
-    without values."""
-    _say("radix large")
-    rng = np.random.default_rng(11)
+@pytest.mark.parametrize("radix_bits", [4, 5, 6, 7, 8])
+def test_radix_sort_large_many_tiles(hip, radix_bits):
+    """Above 4M keys (4096-key blocks, ~1300 of them) every digit width with at
+    most eight passes runs the one-sweep passes, whose look-back crosses many
+    blocks (4-bit digits need ten passes over 40 bits and keep the three-launch
+    passes); 40-bit keys with heavy ties and a near-constant top digit, a count
+    that is no multiple of the block size."""
+    _say("radix large", radix_bits)
+    rng = np.random.default_rng(11 + radix_bits)
     n = 5_300_001
     keys = (rng.integers(0, 1 << 20, n, dtype=np.uint64) << np.uint64(20)) | rng.integers(0, 3, n, dtype=np.uint64)
+    keys[rng.integers(0, n, 1000)] |= np.uint64(1 << 39)
     vals = np.arange(n, dtype=np.uint32)
     order = np.argsort(keys, kind="stable")
     kt = torch.tensor(keys.view(np.int64), device=hip.dev)
     vt = torch.tensor(vals.view(np.int32), device=hip.dev)
-    ko, vo = hip.radix_sort(kt, vt, key_bits=40, radix_bits=8)
+    ko, vo = hip.radix_sort(kt, vt, key_bits=40, radix_bits=radix_bits)
     assert np.array_equal(u32(ko), keys[order])
     assert np.array_equal(u32(vo), vals[order])
+
+
+def test_radix_sort_48m_pair_keys(hip):
+    """A sort the size of the 1080p pair list (48M keys, 45-bit tile | depth
+    keys as the duplicate step makes them, sorted twice in a row): equal to
+    torch's stable sort on the device."""
+    _say("radix 48m")
+    n = 48_000_017
+    g = torch.Generator(device=hip.dev).manual_seed(5)
+    tile = torch.randint(0, 8160, (n,), device=hip.dev, generator=g, dtype=torch.int64)
+    depth = torch.rand(n, device=hip.dev, generator=g) * 8.0 + 0.5
+    keys = (tile << 32) | (depth.view(torch.int32).to(torch.int64) & 0xFFFFFFFF)
+    keys[::97] = keys[5]  # ties
+    vals = torch.arange(n, device=hip.dev, dtype=torch.int32)
+    ref_k, ref_i = torch.sort(keys, stable=True)
+    for _ in range(2):
+        ko, vo = hip.radix_sort(keys, vals, key_bits=45, radix_bits=8)
+        torch.cuda.synchronize()
+        assert torch.equal(ko, ref_k)
+        assert torch.equal(vo, ref_i.to(torch.int32))
 
 
 def test_radix_sort_full_64bit(hip):
