@@ -357,16 +357,28 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) {
     return (uint32_t)(k >> shift) & ((1u << BITS) - 1u);
 }
 
-// Lanes (among `active`) whose digit equals this lane's.
+// Lanes (among `active`) whose digit equals this lane's: one ballot per digit
+// bit; a lane differs from this one where its bit differs, i.e. where the
+// ballot XOR this lane's bit (spread to all ones) is set.  Two bits per step,
+// so the two XORs fold into one three-input OR.
 template <int BITS>
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
-    uint64_t peers = active;
+    uint32_t dlo = 0u, dhi = 0u;
 #pragma unroll
-    for (int b = 0; b < BITS; b++) {
-        uint64_t bal = __ballot((d >> b) & 1u);
-        peers &= ((d >> b) & 1u) ? bal : ~bal;
+    for (int b = 0; b < BITS; b += 2) {
+        const uint32_t m0 = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // all ones where bit b is set
+        const uint64_t bal0 = __builtin_amdgcn_ballot_w64(m0 != 0u);
+        if (b + 1 < BITS) {
+            const uint32_t m1 = (uint32_t)((int32_t)(d << (30 - b)) >> 31);
+            const uint64_t bal1 = __builtin_amdgcn_ballot_w64(m1 != 0u);
+            dlo |= ((uint32_t)bal0 ^ m0) | ((uint32_t)bal1 ^ m1);
+            dhi |= ((uint32_t)(bal0 >> 32) ^ m0) | ((uint32_t)(bal1 >> 32) ^ m1);
+        } else {
+            dlo |= (uint32_t)bal0 ^ m0;
+            dhi |= (uint32_t)(bal0 >> 32) ^ m0;
+        }
     }
-    return peers;
+    return active & ~(((uint64_t)dhi << 32) | dlo);
 }
 
 template <int BITS, int ROUNDS>
@@ -555,7 +567,18 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
 constexpr int OS_MAX_PASSES = 8;
 constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_CNT = (1u << 30) - 1u;
 constexpr uint32_t OS_SPIN_LIMIT = 1u << 22;
-constexpr uint32_t OS_HIST_BLOCKS = 1024;
+constexpr uint32_t OS_HIST_BLOCKS = 512;
+// keys per one-sweep block = OS_THREADS x OS_ROUNDS
+constexpr int OS_THREADS = 256, OS_ROUNDS = 16;
+// status words of earlier blocks read at once per look-back step (8, 16 and
+// 32 measured slower at 47M keys)
+constexpr int OS_WINDOW = 4;
+// large sorts: decoupled look-back (true), or a counting kernel and a scan per
+// pass in front of the same scatter kernel (false; 16% slower at 47M keys)
+constexpr bool OS_LOOKBACK = true;
+// the smallest sort that takes the one-sweep passes (below it, e.g. the ~1M-key
+// depth sort, the three-launch passes on 1024-key blocks measured faster)
+constexpr uint32_t OS_MIN_KEYS = RS_SMALL_LIMIT;
 
 struct PassShifts {
     int s[OS_MAX_PASSES];
@@ -569,20 +592,39 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
 }
 
 // Digit counts of every pass, summed into hist[pass * RADIX + digit] (zeroed
-// beforehand).  Four LDS copies, chosen by lane % 4, cut same-address
-// conflicts on the near-constant high digits by four.
+// beforehand).  LDS atomics into OS_HIST_COPIES copies of the counters, the
+// copy chosen by lane % OS_HIST_COPIES, rows padded by one word so that the
+// copies of one counter sit in different banks: a digit that most keys share
+// (the exponent byte of a depth, the high tile bits of one Gaussian's run of
+// pairs) costs a few conflicting lanes per instruction instead of a wave.
+constexpr int OS_HIST_COPIES = 8;
+constexpr int OS_HIST_THREADS = 1024;
+
+// copies that fit 64 KB of LDS (eight, or four for eight 8-bit passes)
+static int hist_copies(int radix_bits, int passes) {
+    int c = OS_HIST_COPIES;
+    while (c > 1 && (size_t)c * (((size_t)passes << radix_bits) + 1) * 4 > 65536) c /= 2;
+    return c;
+}
+static size_t hist_lds_bytes(int radix_bits, int passes) {
+    return (size_t)hist_copies(radix_bits, passes) * (((size_t)passes << radix_bits) + 1) * 4;
+}
+
 template <int BITS>
-__global__ __launch_bounds__(256) void rs_hist_all_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
-                                                          const uint32_t* __restrict__ n_dev, PassShifts sh,
-                                                          int passes, uint32_t* __restrict__ hist) {
-    constexpr int RADIX = 1 << BITS, COPIES = 4;
-    __shared__ uint32_t s_h[COPIES][OS_MAX_PASSES * RADIX];
+__global__ __launch_bounds__(OS_HIST_THREADS) void rs_hist_all_kernel(const uint64_t* __restrict__ keys,
+                                                                      uint32_t n_cap,
+                                                                      const uint32_t* __restrict__ n_dev,
+                                                                      PassShifts sh, int passes, int copies,
+                                                                      uint32_t* __restrict__ hist) {
+    constexpr int RADIX = 1 << BITS;
+    extern __shared__ uint32_t s_hc[];  // `copies` rows of passes * RADIX + 1 words
+    const int row = passes * RADIX + 1;
     const uint32_t n = live_count(n_cap, n_dev);
-    for (int i = threadIdx.x; i < COPIES * OS_MAX_PASSES * RADIX; i += 256) (&s_h[0][0])[i] = 0u;
+    for (int i = threadIdx.x; i < copies * row; i += OS_HIST_THREADS) s_hc[i] = 0u;
     __syncthreads();
-    uint32_t* mine = s_h[threadIdx.x & (COPIES - 1)];
-    const uint32_t stride = gridDim.x * 256u;
-    for (uint32_t i0 = blockIdx.x * 256u + threadIdx.x; i0 < n; i0 += 4u * stride) {
+    uint32_t* mine = s_hc + (threadIdx.x & (copies - 1)) * row;
+    const uint32_t stride = gridDim.x * (uint32_t)OS_HIST_THREADS;
+    for (uint32_t i0 = blockIdx.x * (uint32_t)OS_HIST_THREADS + threadIdx.x; i0 < n; i0 += 4u * stride) {
         uint64_t k[4];
         bool ok[4];
 #pragma unroll
@@ -600,48 +642,53 @@ __global__ __launch_bounds__(256) void rs_hist_all_kernel(const uint64_t* __rest
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * RADIX; i += 256) {
+    for (int i = threadIdx.x; i < passes * RADIX; i += OS_HIST_THREADS) {
         uint32_t c = 0u;
-#pragma unroll
-        for (int k = 0; k < COPIES; k++) c += s_h[k][i];
+        for (int k = 0; k < copies; k++) c += s_hc[k * row + i];
         if (c) atomicAdd(&hist[i], c);
     }
 }
 
 // One pass: keys (and values) in, scattered stably by the digit at `shift`.
-// hist: this pass's digit totals; status / status_next: this pass's and the
-// next pass's status words ([block][digit]); ticket: zeroed counter.
-template <int BITS, bool VALS, int ROUNDS>
-__global__ __launch_bounds__(RS_THREADS) void rs_onesweep_kernel(
+// LOOKBACK: the block's place among the blocks comes from its ticket, its
+// digits' starts from the decoupled look-back (hist: this pass's digit totals;
+// status / status_next: this pass's and the next pass's status words
+// ([block][digit]); ticket: zeroed counter).  Otherwise from the scanned
+// per-block counts of a counting kernel (goff: scanned digit-major table with
+// row stride hs; dtot: digit totals), block = blockIdx.x.
+template <int BITS, bool VALS, int ROUNDS, int THREADS, bool LOOKBACK>
+__global__ __launch_bounds__(THREADS) void rs_scatter_kernel(
     const uint64_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n_cap,
     const uint32_t* __restrict__ n_dev, int shift, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ status_next, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err,
-    uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
+    uint32_t* __restrict__ status_next, uint32_t* __restrict__ ticket, uint32_t* __restrict__ err, uint32_t hs,
+    const uint32_t* __restrict__ goff, uint64_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out) {
     constexpr int RADIX = 1 << BITS;
-    constexpr int RS_TILE = RS_THREADS * ROUNDS, RS_WAVE_SPAN = ROUNDS * 64;
-    static_assert(RADIX <= RS_THREADS, "one digit per thread");
+    constexpr int WAVES = THREADS / 64;
+    constexpr int RS_TILE = THREADS * ROUNDS, RS_WAVE_SPAN = ROUNDS * 64;
+    static_assert(RADIX <= THREADS, "one digit per thread");
     const uint32_t n = live_count(n_cap, n_dev);
     // keys in digit order, then (digit << 32 | value) in the same slots
     __shared__ uint64_t s_k[RS_TILE];
-    __shared__ uint32_t s_wh[RS_WAVES][RADIX];
+    __shared__ uint32_t s_wh[WAVES][RADIX];
     __shared__ uint32_t s_dstart[RADIX];
     __shared__ uint32_t s_goff[RADIX];
-    __shared__ uint32_t s_scan[RS_WAVES];
+    __shared__ uint32_t s_scan[WAVES];
     __shared__ uint32_t s_b;
 
-    if (threadIdx.x == 0) s_b = atomicAdd(ticket, 1u);
-    for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_wh[0][0])[i] = 0u;
+    if (LOOKBACK && threadIdx.x == 0) s_b = atomicAdd(ticket, 1u);
+    for (int i = threadIdx.x; i < WAVES * RADIX; i += THREADS) (&s_wh[0][0])[i] = 0u;
     __syncthreads();
-    const uint32_t b = s_b;
-    if (status_next && threadIdx.x < RADIX) status_next[(size_t)b * RADIX + threadIdx.x] = 0u;
+    const uint32_t b = LOOKBACK ? s_b : blockIdx.x;
+    if (LOOKBACK && status_next && threadIdx.x < RADIX) status_next[(size_t)b * RADIX + threadIdx.x] = 0u;
     const uint32_t bbase = b * (uint32_t)RS_TILE;
     if (bbase >= n) return;  // past the live count: no block before it reads its words
     {
-        // the digits' starts in the output (the look-back adds the earlier blocks' counts)
+        // the digits' starts in the output (the look-back adds the earlier
+        // blocks' counts; without it the scanned table holds them)
         const uint32_t t = threadIdx.x < RADIX ? hist[threadIdx.x] : 0u;
         uint32_t total;
         const uint32_t ex = block_exclusive_scan<uint32_t>(t, s_scan, &total);
-        if (threadIdx.x < RADIX) s_goff[threadIdx.x] = ex;
+        if (threadIdx.x < RADIX) s_goff[threadIdx.x] = ex + (LOOKBACK ? 0u : goff[(size_t)threadIdx.x * hs + b]);
     }
 
     const int w = threadIdx.x >> 6, l = lane_id();
@@ -680,31 +727,32 @@ __global__ __launch_bounds__(RS_THREADS) void rs_onesweep_kernel(
     if (threadIdx.x < RADIX) {
         const int d = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < RS_WAVES; k++) {
+        for (int k = 0; k < WAVES; k++) {
             const uint32_t c = s_wh[k][d];
             s_wh[k][d] = cnt;
             cnt += c;
         }
-        st_agent(&status[(size_t)b * RADIX + d], (b == 0 ? OS_INC : OS_AGG) | cnt);
+        if (LOOKBACK) st_agent(&status[(size_t)b * RADIX + d], (b == 0 ? OS_INC : OS_AGG) | cnt);
     }
     {
         uint32_t total;
         const uint32_t ex = block_exclusive_scan<uint32_t>(cnt, s_scan, &total);
         if (threadIdx.x < RADIX) s_dstart[threadIdx.x] = ex;
     }
-    // 3. look-back: the counts of this digit in all earlier blocks, four words at a time
-    if (threadIdx.x < RADIX && b > 0) {
+    // 3. look-back: the counts of this digit in all earlier blocks, OS_WINDOW words at a time
+    if (LOOKBACK && threadIdx.x < RADIX && b > 0) {
         const int d = threadIdx.x;
         const uint32_t* col = status + d;
         uint32_t acc = 0u;
         int j = (int)b - 1;
         bool done = false;
         while (!done) {
-            uint32_t s[4];
+            uint32_t s[OS_WINDOW];
 #pragma unroll
-            for (int q = 0; q < 4; q++) s[q] = (j - q >= 0) ? ld_agent(&col[(size_t)(j - q) * RADIX]) : OS_INC;
+            for (int q = 0; q < OS_WINDOW; q++)
+                s[q] = (j - q >= 0) ? ld_agent(&col[(size_t)(j - q) * RADIX]) : OS_INC;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < OS_WINDOW; q++) {
                 if (done) break;
                 uint32_t v = s[q];
                 uint32_t spins = 0u;
@@ -720,12 +768,15 @@ __global__ __launch_bounds__(RS_THREADS) void rs_onesweep_kernel(
                 acc += v & OS_CNT;
                 if (v & OS_INC) done = true;
             }
-            j -= 4;
+            j -= OS_WINDOW;
         }
         st_agent(&status[(size_t)b * RADIX + d], OS_INC | (acc + cnt));
         s_goff[d] += acc;
     }
     __syncthreads();
+    // from here on s_goff[d] is the output index of the block's slot 0 of digit d's run
+    // minus that slot: output index = s_goff[d] + slot
+    if (threadIdx.x < RADIX) s_goff[threadIdx.x] -= s_dstart[threadIdx.x];
     // 4. stage the keys in LDS in (digit, input order) order and write them out in runs
     uint32_t p_r[ROUNDS];
 #pragma unroll
@@ -737,10 +788,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_onesweep_kernel(
     }
     __syncthreads();
     const uint32_t count = (n - bbase < (uint32_t)RS_TILE) ? (n - bbase) : (uint32_t)RS_TILE;
-    for (uint32_t p = threadIdx.x; p < count; p += RS_THREADS) {
+    for (uint32_t p = threadIdx.x; p < count; p += THREADS) {
         const uint64_t k = s_k[p];
-        const uint32_t d = digit_of<BITS>(k, shift);
-        const uint32_t g = s_goff[d] + (p - s_dstart[d]);
+        const uint32_t g = s_goff[digit_of<BITS>(k, shift)] + p;
         if (g < n) keys_out[g] = k;
     }
     if constexpr (VALS) {
@@ -751,12 +801,44 @@ __global__ __launch_bounds__(RS_THREADS) void rs_onesweep_kernel(
             if (i < n && p_r[r] < (uint32_t)RS_TILE) s_k[p_r[r]] = ((uint64_t)digit_of<BITS>(k_r[r], shift) << 32) | v_r[r];
         }
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < count; p += RS_THREADS) {
+        for (uint32_t p = threadIdx.x; p < count; p += THREADS) {
             const uint64_t e = s_k[p];
-            const uint32_t d = (uint32_t)(e >> 32);
-            const uint32_t g = s_goff[d] + (p - s_dstart[d]);
+            const uint32_t g = s_goff[(uint32_t)(e >> 32)] + p;
             if (g < n) vals_out[g] = (uint32_t)e;
         }
+    }
+}
+
+// Per-block digit counts for the scatter kernel without look-back:
+// hist[digit * hs + block] (digit-major, as rs_digit_scan_kernel reads it).
+// LDS atomics into eight padded copies of the counters (see rs_hist_all_kernel).
+template <int BITS, int ROUNDS, int THREADS>
+__global__ __launch_bounds__(THREADS) void rs_count_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
+                                                           const uint32_t* __restrict__ n_dev, int shift, uint32_t hs,
+                                                           uint32_t* __restrict__ hist) {
+    constexpr int RADIX = 1 << BITS, COPIES = 8, ROW = RADIX + 1;
+    constexpr int TILE = THREADS * ROUNDS;
+    __shared__ uint32_t s_h[COPIES * ROW];
+    const uint32_t n = live_count(n_cap, n_dev);
+    for (int i = threadIdx.x; i < COPIES * ROW; i += THREADS) s_h[i] = 0u;
+    __syncthreads();
+    uint32_t* mine = s_h + (threadIdx.x & (COPIES - 1)) * ROW;
+    const uint32_t base = blockIdx.x * (uint32_t)TILE + threadIdx.x;
+    uint64_t k[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = base + r * THREADS;
+        k[r] = i < n ? keys[i] : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++)
+        if (base + r * THREADS < n) atomicAdd(&mine[digit_of<BITS>(k[r], shift)], 1u);
+    __syncthreads();
+    for (int d = threadIdx.x; d < RADIX; d += THREADS) {
+        uint32_t c = 0u;
+#pragma unroll
+        for (int q = 0; q < COPIES; q++) c += s_h[q * ROW + d];
+        hist[(size_t)d * hs + blockIdx.x] = c;
     }
 }
 
@@ -835,17 +917,24 @@ static uint32_t radix_row_stride(uint32_t nb) { return (nb + 3u) & ~3u; }
 
 static size_t small_words(int radix_bits) { return ((size_t)OS_MAX_PASSES << radix_bits) + 64; }
 
+static uint32_t os_blocks(uint32_t n) { return (n + OS_THREADS * OS_ROUNDS - 1) / (OS_THREADS * OS_ROUNDS); }
+
+// rows of the two tables: the three-launch passes' blocks or the one-sweep blocks
+static size_t table_rows(uint32_t n) {
+    return std::max<size_t>(radix_row_stride(radix_blocks(n)), radix_row_stride(os_blocks(n)));
+}
+
 size_t radix_workspace_bytes(uint32_t n, int radix_bits) {
-    size_t nb = radix_row_stride(radix_blocks(n));
+    size_t nb = table_rows(n);
     size_t R = (size_t)1 << radix_bits;
     size_t tab = R * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     return al((size_t)n * 8) + al((size_t)n * 4) + al(small_words(radix_bits) * 4) + 2 * al(tab * 4);
 }
 
-// the one-sweep path: large sorts whose counts fit the status words' 30 bits
+// the one-sweep path: sorts of at least OS_MIN_KEYS whose counts fit the status words' 30 bits
 static bool use_onesweep(uint32_t n, int passes) {
-    return n >= RS_SMALL_LIMIT && n <= OS_CNT && passes <= OS_MAX_PASSES;
+    return n >= OS_MIN_KEYS && n <= OS_CNT && passes <= OS_MAX_PASSES;
 }
 
 template <int BITS>
@@ -853,33 +942,50 @@ static hipError_t onesweep_sort(const uint64_t* ki, const uint32_t* vi, uint64_t
                                 uint64_t* pk, uint32_t* pv, uint32_t n, const uint32_t* n_dev, const int* shifts,
                                 int passes, uint32_t* small, uint32_t* tab_a, uint32_t* tab_b, hipStream_t s) {
     constexpr int RADIX = 1 << BITS;
-    constexpr int ROUNDS = 16;
-    const uint32_t nb = (n + RS_THREADS * ROUNDS - 1) / (RS_THREADS * ROUNDS);
+    constexpr int ROUNDS = OS_ROUNDS, THREADS = OS_THREADS;
+    const uint32_t nb = os_blocks(n), hs = radix_row_stride(nb);
     uint32_t* hist = small;                                       // [passes][RADIX]
     uint32_t* tickets = small + ((size_t)OS_MAX_PASSES << BITS);  // [OS_MAX_PASSES]
     uint32_t* err = tickets + OS_MAX_PASSES;
-    // the small words and the first pass's status words (table A follows them)
-    hipError_t e = launch_zero(small, small_words(BITS) * 4, s);
-    if (e == hipSuccess) e = launch_zero(tab_a, (size_t)nb * RADIX * 4, s);
-    if (e != hipSuccess) return e;
-    PassShifts sh{};
-    for (int p = 0; p < passes; p++) sh.s[p] = shifts[p];
-    const uint32_t hb = std::min<uint32_t>(OS_HIST_BLOCKS, (n + 4095) / 4096);
-    rs_hist_all_kernel<BITS><<<hb, 256, 0, s>>>(ki, n, n_dev, sh, passes, hist);
     const bool vals = vi != nullptr;
+    if (OS_LOOKBACK) {
+        // the small words and the first pass's status words
+        hipError_t e = launch_zero(small, small_words(BITS) * 4, s);
+        if (e == hipSuccess) e = launch_zero(tab_a, (size_t)nb * RADIX * 4, s);
+        if (e != hipSuccess) return e;
+        PassShifts sh{};
+        for (int p = 0; p < passes; p++) sh.s[p] = shifts[p];
+        const uint32_t hb = std::min<uint32_t>(OS_HIST_BLOCKS, (n + 16383) / 16384);
+        rs_hist_all_kernel<BITS><<<hb, OS_HIST_THREADS, hist_lds_bytes(BITS, passes), s>>>(
+            ki, n, n_dev, sh, passes, hist_copies(BITS, passes), hist);
+    }
     for (int p = 0; p < passes; p++) {
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint64_t* ko = to_out ? keys_out : pk;
         uint32_t* vo = vals ? (to_out ? vals_out : pv) : nullptr;
-        uint32_t* st = (p % 2 == 0) ? tab_a : tab_b;
-        uint32_t* st_next = (p + 1 < passes) ? ((p % 2 == 0) ? tab_b : tab_a) : nullptr;
-        if (vals)
-            rs_onesweep_kernel<BITS, true, ROUNDS><<<nb, RS_THREADS, 0, s>>>(
-                ki, vi, n, n_dev, shifts[p], hist + (size_t)p * RADIX, st, st_next, tickets + p, err, ko, vo);
-        else
-            rs_onesweep_kernel<BITS, false, ROUNDS><<<nb, RS_THREADS, 0, s>>>(
-                ki, nullptr, n, n_dev, shifts[p], hist + (size_t)p * RADIX, st, st_next, tickets + p, err, ko,
-                nullptr);
+        if (OS_LOOKBACK) {
+            uint32_t* st = (p % 2 == 0) ? tab_a : tab_b;
+            uint32_t* st_next = (p + 1 < passes) ? ((p % 2 == 0) ? tab_b : tab_a) : nullptr;
+            if (vals)
+                rs_scatter_kernel<BITS, true, ROUNDS, THREADS, true><<<nb, THREADS, 0, s>>>(
+                    ki, vi, n, n_dev, shifts[p], hist + (size_t)p * RADIX, st, st_next, tickets + p, err, 0u, nullptr,
+                    ko, vo);
+            else
+                rs_scatter_kernel<BITS, false, ROUNDS, THREADS, true><<<nb, THREADS, 0, s>>>(
+                    ki, nullptr, n, n_dev, shifts[p], hist + (size_t)p * RADIX, st, st_next, tickets + p, err, 0u,
+                    nullptr, ko, nullptr);
+        } else {
+            // per-block counts (table A), their scan per digit (table B) and the digit totals
+            rs_count_kernel<BITS, ROUNDS, THREADS><<<nb, THREADS, 0, s>>>(ki, n, n_dev, shifts[p], hs, tab_a);
+            rs_digit_scan_kernel<<<RADIX, 64, 0, s>>>(tab_a, nb, hs, tab_b, hist);
+            if (vals)
+                rs_scatter_kernel<BITS, true, ROUNDS, THREADS, false><<<nb, THREADS, 0, s>>>(
+                    ki, vi, n, n_dev, shifts[p], hist, nullptr, nullptr, nullptr, nullptr, hs, tab_b, ko, vo);
+            else
+                rs_scatter_kernel<BITS, false, ROUNDS, THREADS, false><<<nb, THREADS, 0, s>>>(
+                    ki, nullptr, n, n_dev, shifts[p], hist, nullptr, nullptr, nullptr, nullptr, hs, tab_b, ko,
+                    nullptr);
+        }
         ki = ko;
         vi = vo;
     }
@@ -916,7 +1022,7 @@ hipError_t launch_radix_sort_bits(const uint64_t* keys_in, const uint32_t* vals_
                                   uint32_t* vals_out, uint32_t n, int begin_bit, int end_bit, int radix_bits,
                                   void* workspace, hipStream_t s, const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
-    size_t nb = radix_row_stride(radix_blocks(n));
+    size_t nb = table_rows(n);
     size_t tab = ((size_t)1 << radix_bits) * nb;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     char* w = static_cast<char*>(workspace);

@@ -238,6 +238,28 @@ def bench_large(hip, cfg, args):
            "views_per_s": round(steps * len(cams) / dt, 2), "ms_per_batch_step": round(1e3 * dt / steps, 3),
            "device_gb_held": round((free0 - free1) / 1e9, 2), "device_gb_total": round(total / 1e9, 1)}
     del b, tr
+    torch.cuda.empty_cache()
+    # the HBM-bound stages of this workload, each timed alone: preprocess of one
+    # view over all Gaussians, and the batch's depth sort (the V x N
+    # (view << 32 | depth bits) keys with their indices, view bits + 32 key bits)
+    p = hip.upload_scene(scene)
+    ms_pre = median_ms(lambda: hip.preprocess(p, cams[0], cfg))
+    b_pre = algorithmic_bytes("preprocess", scene.n, 0, 0, scene.max_coeffs, 1)
+    V = len(cams)
+    view_bits = max(1, (V - 1).bit_length())
+    keys = torch.cat([(hip.preprocess(p, c, cfg)["depths"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF) | (v << 32)
+                      for v, c in enumerate(cams)])
+    vals = torch.arange(keys.numel(), device=hip.dev, dtype=torch.int32)
+    ms_sort = median_ms(lambda: hip.radix_sort(keys, vals, key_bits=32 + view_bits, radix_bits=8))
+    b_sort = sort_bytes(keys.numel(), 32 + view_bits, 8)
+    out["hbm_targets"] = {
+        "preprocess": {"ms": round(ms_pre, 4), "algorithmic_bytes": int(b_pre),
+                       "gb_s": round(b_pre / ms_pre / 1e6, 1), "frac": round(b_pre / ms_pre / 1e6 / HBM_PEAK_GBS, 4)},
+        "depth_sort": {"count": int(keys.numel()), "key_bits": 32 + view_bits, "radix_bits": 8, "ms": round(ms_sort, 4),
+                       "algorithmic_bytes": int(b_sort), "gb_s": round(b_sort / ms_sort / 1e6, 1),
+                       "frac": round(b_sort / ms_sort / 1e6 / HBM_PEAK_GBS, 4)}}
+    del p, keys, vals
+    torch.cuda.empty_cache()
     return out
 
 
@@ -295,25 +317,55 @@ def raster_work(hip, tr, scene, cam, cfg):
     return w
 
 
-def cpu_baseline(scene, cam, cfg, gt):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(scene, cam, cfg, gt, threads=1):
     """The reference's own Trainer::step (compiled from source into
-    oracle/_ref) on the same workload, one full step on one core; falls back
-    to the oracle (C restatement) if that library is absent."""
+    oracle/_ref) on the same workload: one full step on one core; with
+    threads > 1, also `threads` independent steps at once, one per core (ctypes
+    releases the interpreter lock during the call; the reference keeps no
+    global state), as the host's all-core training throughput.  Falls back to
+    the oracle (C restatement, forward + backward only) if that library is
+    absent."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle.oracle import Oracle, RefLib
     tc = dict(lr_pos_init=1e-3, lr_pos_final=1e-3, lr_feature=1e-3, lr_opacity=1e-3, lr_scaling=1e-3,
               lr_rotation=1e-3, max_steps=30000, beta1=0.9, beta2=0.999, eps=1e-8)
-    t0 = time.perf_counter()
     if RefLib.available():
-        RefLib().train_steps(scene, cam, cfg, gt, tc, 0, 1)
+        ref = RefLib()
+        one = lambda _=None: ref.train_steps(scene, cam, cfg, gt, tc, 0, 1)
         kind = "reference"
-        sample = "one full Trainer::step (reference CPU classes, Adam) at 1M Gaussians, 1920x1080"
+        what = "full Trainer::step (reference CPU classes, Adam) at 1M Gaussians, 1920x1080"
     else:
-        Oracle().forward_backward(scene, cam, cfg, gt)
+        orc = Oracle()
+        one = lambda _=None: orc.forward_backward(scene, cam, cfg, gt)
         kind = "port"
-        sample = "one forward+backward of the C oracle at 1M Gaussians, 1920x1080 (no optimizer)"
+        what = "forward+backward of the C oracle at 1M Gaussians, 1920x1080 (no optimizer)"
+    t0 = time.perf_counter()
+    one()
     dt = time.perf_counter() - t0
-    return {"value": round(1.0 / dt, 5), "unit": "train steps/s", "cores": 1, "kind": kind, "sample": sample,
-            "seconds": round(dt, 2)}
+    out = {"value": round(1.0 / dt, 5), "unit": "train steps/s", "cores": 1, "kind": kind, "sample": "one " + what,
+           "seconds": round(dt, 2), "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+    if threads > 1:
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, range(threads)))
+        dtn = time.perf_counter() - t0
+        out["all_cores"] = {"value": round(threads / dtn, 5), "unit": "train steps/s", "cores": threads,
+                            "sample": f"{threads} independent {what}, one per core, at once",
+                            "seconds": round(dtn, 2)}
+    return out
 
 
 def traffic_probe(args):
@@ -379,10 +431,18 @@ def main():
     ap.add_argument("--no-eval3d", action="store_true", help="skip the eval_3D (k-buffer) forward leg")
     ap.add_argument("--eval3d-only", action="store_true",
                     help="only the eval_3D forward leg on the workload's first view (for profiling)")
+    ap.add_argument("--sorts-only", action="store_true",
+                    help="only the two sorts of hbm_targets on the workload's first view (for profiling)")
+    ap.add_argument("--peaks-only", action="store_true",
+                    help="only the measured device ceilings (gs_device_peaks), as one JSON line")
     args = ap.parse_args()
 
     if args.traffic_probe:
         traffic_probe(args)
+        return
+    if args.peaks_only:
+        from gs_hip import GsHip
+        print(json.dumps({"device_peaks": GsHip(0).device_peaks()}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is design-only",
@@ -406,6 +466,9 @@ def main():
     if args.eval3d_only:
         print(json.dumps({"eval3d": bench_eval3d(hip, scene, cam, RenderCfg(sh_degree=3, training=False))}),
               flush=True)
+        return
+    if args.sorts_only:
+        print(json.dumps({"hbm_targets": bench_sorts(hip, hip.upload_scene(scene), cam, cfg, 8)}), flush=True)
         return
     gt = torch.tensor(gt_np, device=hip.dev)
     tr = GsTrainer(hip, scene)
@@ -543,6 +606,17 @@ def main():
     batched = None
     if not args.no_batched:
         batched = bench_batched(hip, cfg, args)
+        # The batch takes one Adam update per batch; the plain path one per view.
+        # Like for like: charge the batch the per-Gaussian backward + Adam sweeps
+        # it skips, at the single-view step's measured time for that stage
+        # (same 1M Gaussians).
+        batched["adam_updates_per_batch"] = 1
+        batched["plain_adam_updates"] = batched["views"]
+        bwd_ms = stages.get("backward")
+        if bwd_ms:
+            ms_eq = batched["ms_per_batch_step"] + (batched["views"] - 1) * bwd_ms
+            batched["batched_vs_single_same_updates"] = round(
+                batched["views"] / (ms_eq * 1e-3) / batched["plain_views_per_s"], 3)
         log(f"batched: {batched}")
 
     large = None
@@ -552,7 +626,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, cam, cfg, gt_np)
+        cpu = cpu_baseline(scene, cam, cfg, gt_np, threads=min(16, os.cpu_count() or 1))
         log(f"cpu baseline: {cpu}")
 
     out = {
