@@ -439,8 +439,10 @@ int gs_batch_set_options(gs_batch* b, int persistent, int fused_adam, int use_gr
 int gs_batch_set_targets(gs_batch* b, const float* const* targets, int from_host);
 /* One batched step with Adam.  out == NULL: no host synchronisation (results
  * and the capacity check are read by a later gs_batch_result / step with out).
- * Returns GS_ERR_CAPACITY (and regrows) if any step since the last check had
- * more pairs than the pre-sized buffers. */
+ * A step whose pairs exceed the pre-sized buffers updates nothing (its
+ * backward and Adam kernels skip on the device's overflow word, and its Adam
+ * step count is rolled back); the next check returns GS_ERR_CAPACITY, regrows
+ * the buffers, and the caller repeats the steps that were not applied. */
 int gs_batch_step(gs_batch* b, const gs_train_cfg* tc, int iteration, gs_step_result* out);
 int gs_batch_result(gs_batch* b, gs_step_result* out, double* view_losses);
 /* Forward only for all views into the batch's concatenated image buffer. */
