@@ -40,12 +40,10 @@ namespace gsd {
 static __constant__ uint64_t kExp2TabDev[32] = GS_EXP2_TAB;
 static const uint64_t kExp2TabHost[32] = GS_EXP2_TAB;
 
-__host__ __device__ __forceinline__ float exp_ref(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const uint64_t* tab = kExp2TabDev;
-#else
-    const uint64_t* tab = kExp2TabHost;
-#endif
+// exp_ref with the 2^(i/32) table at `tab` (a copy of GS_EXP2_TAB, e.g. in LDS:
+// the table index differs from lane to lane, so a copy in LDS saves a global
+// load's latency per call in loops that evaluate many exponentials).
+__host__ __device__ __forceinline__ float exp_ref_tab(float x, const uint64_t* tab) {
     if (!(x <= 0x1.62e42ep6f)) return x != x ? x + x : __builtin_inff();  // NaN, overflow
     if (x < -0x1.9fe368p6f) return 0.0f;                                   // below 2^-150
     constexpr double kInvLn2N = 0x1.71547652b82fep+0 * 32.0, kShift = 0x1.8p+52;
@@ -65,6 +63,14 @@ __host__ __device__ __forceinline__ float exp_ref(float x) {
     double y = __builtin_fma(kC2, r, 1.0);
     y = __builtin_fma(z, r2, y);
     return (float)(y * sc);
+}
+
+__host__ __device__ __forceinline__ float exp_ref(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return exp_ref_tab(x, kExp2TabDev);
+#else
+    return exp_ref_tab(x, kExp2TabHost);
+#endif
 }
 
 // logf exactly as the reference's CPU path evaluates it: glibc's logf (2.35,
@@ -373,6 +379,18 @@ struct CamArgs {
 }  // namespace gsd
 
 namespace gsd {
+// The depth of a Gaussian along a unit ray (math_utils.cpp:616-630): ci its
+// inverse 3D covariance (upper triangle), mo its mean relative to the camera.
+__device__ __forceinline__ float depth_along(const float* ci, const float* mo, const float* vd) {
+    const float s0 = ci[0] * vd[0] + ci[1] * vd[1] + ci[2] * vd[2];
+    const float s1 = ci[1] * vd[0] + ci[3] * vd[1] + ci[4] * vd[2];
+    const float s2 = ci[2] * vd[0] + ci[4] * vd[1] + ci[5] * vd[2];
+    const float num = mo[0] * s0 + mo[1] * s1 + mo[2] * s2;
+    const float den = vd[0] * s0 + vd[1] * s1 + vd[2] * s2;
+    if (fabsf(den) < 1e-10f) return sqrtf(mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2]);
+    return num / den;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8) and each XCD has its own L2.  This bijection on [0, nb) gives
 // every XCD one contiguous run of work items, so items that share data

@@ -25,6 +25,7 @@
 #include "gs_device.h"
 #include "gs_internal.h"
 #include "gs_pre_device.h"
+#include "gs_wave.h"
 
 namespace gs3 {
 using namespace gsd;
@@ -256,16 +257,7 @@ __device__ void aabb_view(const float* g2v, const float* mv, float fx, float fy,
     ext2d[1] = (yh - yl) / 2.0f;
 }
 
-// ---- math_utils.cpp:616-630 ----
-__device__ __forceinline__ float depth_along(const float* ci, const float* mo, const float* vd) {
-    const float s0 = ci[0] * vd[0] + ci[1] * vd[1] + ci[2] * vd[2];
-    const float s1 = ci[1] * vd[0] + ci[3] * vd[1] + ci[4] * vd[2];
-    const float s2 = ci[2] * vd[0] + ci[4] * vd[1] + ci[5] * vd[2];
-    const float num = mo[0] * s0 + mo[1] * s1 + mo[2] * s2;
-    const float den = vd[0] * s0 + vd[1] * s1 + vd[2] * s2;
-    if (fabsf(den) < 1e-10f) return sqrtf(mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2]);
-    return num / den;
-}
+using gsd::depth_along;
 
 // ---- math_utils.cpp:632-651: unit world-space ray through pixel (px, py) ----
 __device__ __forceinline__ void pixel_dir(float px, float py, const Cam3& c, float* d) {
@@ -557,9 +549,11 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
     __shared__ float s_kd[KBUF][256];
     __shared__ float s_ka[KBUF][256];
     __shared__ uint32_t s_ki[KBUF][256];
-    const int tile = blockIdx.x;
+    __shared__ uint64_t s_e2[32];  // exp_ref's table (read before the first barrier below)
+    const int tile = gsd::xcd_remap(blockIdx.x, gridDim.x);  // neighbouring tiles share an XCD's L2
     const int tx = tile % c.gx, ty = tile / c.gx;
     const int t = threadIdx.x;
+    if (t < 32) s_e2[t] = kExp2TabDev[t];
     const int px = tx * 16 + (t & 15), py = ty * 16 + (t >> 4);
     const bool inside = px < c.W && py < c.H;
     const uint32_t r0 = ranges[2 * tile], r1 = ranges[2 * tile + 1];
@@ -620,7 +614,7 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
             }
             const float power = -0.5f * ray_contrib(plx, ply, nullptr);
             if (power > 0.0f) continue;
-            const float alpha = smin(0.99f, e.op * exp_ref(power));
+            const float alpha = smin(0.99f, e.op * exp_ref_tab(power, s_e2));
             if (alpha < kAlpha) continue;
             float pd = e.depth;
             if (c.have_ivp) {
@@ -754,6 +748,200 @@ static Cam3 make_cam3(const gs_camera* cam, const gs_render_cfg* cfg, int model_
     return c;
 }
 
+// ------------------------------------------------- direct 3D binning ----
+// The pair order the reference's duplicate + stable sort produce is, per tile
+// (in tile order), the Gaussians that touch it ordered by their per-tile depth
+// key, equal keys in Gaussian order.  The direct-placement binning of the 2D
+// path, fed the Gaussians in index order, writes every tile's list in Gaussian
+// order; each tile's list is then sorted stably by its per-tile key in LDS
+// (TS_CAP pairs at most; longer lists go through the general radix sort).
+
+constexpr int TS_THREADS = 1024, TS_ITEMS = 16, TS_WAVES = TS_THREADS / 64;
+constexpr int TS_CAP = TS_THREADS * TS_ITEMS;  // 16384 pairs per tile
+
+__global__ __launch_bounds__(256) void iota_kernel(uint32_t n, uint32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = i;
+}
+
+// A pair key's inputs per Gaussian in one 64-byte line: cov3D_inv (6),
+// mean_offset (3), depth, so that a pair's key costs three 16-byte loads of
+// one cache line instead of ten loads from three arrays.
+__global__ __launch_bounds__(256) void pack_keys3d_kernel(int n, const float* __restrict__ depths,
+                                                          const float* __restrict__ cinv,
+                                                          const float* __restrict__ moff, float4* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float* c = cinv + (size_t)i * 6;
+    const float* m = moff + (size_t)i * 3;
+    out[(size_t)i * 4] = make_float4(c[0], c[1], c[2], c[3]);
+    out[(size_t)i * 4 + 1] = make_float4(c[4], c[5], m[0], m[1]);
+    out[(size_t)i * 4 + 2] = make_float4(m[2], depths[i], 0.0f, 0.0f);
+}
+
+// The low word of a pair's key, as duplicate3d_kernel computes it (ray: the
+// tile's ray, used when have_ray).
+__device__ __forceinline__ uint32_t tile_depth_key(uint32_t g, const float4* __restrict__ packed, bool have_ray,
+                                                   float4 ray) {
+    const float4 c = packed[(size_t)g * 4 + 2];
+    uint32_t tb = __float_as_uint(c.y);
+    if (have_ray) {
+        const float4 a = packed[(size_t)g * 4], b = packed[(size_t)g * 4 + 1];
+        const float ci[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+        const float mo[3] = {b.z, b.w, c.x};
+        const float vd[3] = {ray.x, ray.y, ray.z};
+        const float ptd = depth_along(ci, mo, vd);
+        if (ptd > 0.0f) tb = __float_as_uint(ptd);
+    }
+    return tb;
+}
+
+// One workgroup per tile: its list (Gaussian order) sorted stably by the
+// per-tile key, four 8-bit LSD passes through LDS; values rewritten in place,
+// keys (tile << 32 | key) written when keys_out is set.  A list longer than
+// TS_CAP is left as it is and its length recorded in bigcnt (0 otherwise).
+__global__ __launch_bounds__(TS_THREADS) void tile_sort3d_kernel(const uint32_t* __restrict__ ranges,
+                                                                 uint32_t* __restrict__ values,
+                                                                 uint64_t* __restrict__ keys_out,
+                                                                 const float4* __restrict__ packed,
+                                                                 const float4* __restrict__ rays,
+                                                                 uint32_t* __restrict__ bigcnt) {
+    using namespace gsw;
+    __shared__ uint32_t s_k[TS_CAP];
+    __shared__ uint32_t s_v[TS_CAP];
+    __shared__ uint32_t s_wh[TS_WAVES][256];
+    __shared__ uint32_t s_dstart[256];
+    __shared__ uint32_t s_scan[TS_WAVES];
+    const uint32_t tile = (uint32_t)gsd::xcd_remap(blockIdx.x, gridDim.x);  // neighbouring tiles share an L2
+    const uint32_t s = ranges[2 * tile], e = ranges[2 * tile + 1];
+    const uint32_t cnt = e > s ? e - s : 0u;
+    if (threadIdx.x == 0) bigcnt[tile] = cnt > (uint32_t)TS_CAP ? cnt : 0u;
+    if (cnt == 0u || cnt > (uint32_t)TS_CAP) return;
+    const int w = threadIdx.x >> 6, l = lane_id();
+    const uint32_t wbase = (uint32_t)w * (TS_ITEMS * 64);
+    const float4 ray = rays ? rays[tile] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t k_r[TS_ITEMS], v_r[TS_ITEMS], rank_r[TS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < TS_ITEMS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        v_r[r] = i < cnt ? values[s + i] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < TS_ITEMS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        k_r[r] = i < cnt ? tile_depth_key(v_r[r], packed, rays != nullptr, ray) : 0u;
+    }
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 8 * pass;
+        for (int i = threadIdx.x; i < TS_WAVES * 256; i += TS_THREADS) (&s_wh[0][0])[i] = 0u;
+        __syncthreads();
+        // per-wave stable ranks (rounds in list order, lanes in list order)
+#pragma unroll
+        for (int r = 0; r < TS_ITEMS; r++) {
+            const uint32_t i = wbase + r * 64 + l;
+            const bool valid = i < cnt;
+            const uint64_t active = __ballot(valid);
+            rank_r[r] = 0u;
+            if (active != 0) {
+                const uint32_t d = (k_r[r] >> shift) & 255u;
+                const uint64_t peers = match_digit<8>(d, active);
+                uint32_t before = 0u;
+                if (valid) before = s_wh[w][d];
+                __builtin_amdgcn_wave_barrier();
+                if (valid && (peers & lanemask_lt()) == 0) s_wh[w][d] = before + (uint32_t)__popcll(peers);
+                __builtin_amdgcn_wave_barrier();
+                rank_r[r] = before + (uint32_t)__popcll(peers & lanemask_lt());
+            }
+        }
+        __syncthreads();
+        // digit totals, per-wave offsets inside each digit, digit starts
+        uint32_t tot = 0u;
+        if (threadIdx.x < 256) {
+            const int d = threadIdx.x;
+#pragma unroll
+            for (int k = 0; k < TS_WAVES; k++) {
+                const uint32_t c = s_wh[k][d];
+                s_wh[k][d] = tot;
+                tot += c;
+            }
+        }
+        {
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan<uint32_t>(tot, s_scan, &total);
+            if (threadIdx.x < 256) s_dstart[threadIdx.x] = ex;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < TS_ITEMS; r++) {
+            const uint32_t i = wbase + r * 64 + l;
+            if (i < cnt) {
+                const uint32_t d = (k_r[r] >> shift) & 255u;
+                const uint32_t p = s_dstart[d] + s_wh[w][d] + rank_r[r];
+                s_k[p] = k_r[r];
+                s_v[p] = v_r[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < TS_ITEMS; r++) {
+            const uint32_t i = wbase + r * 64 + l;
+            if (i < cnt) {
+                k_r[r] = s_k[i];
+                v_r[r] = s_v[i];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < TS_ITEMS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        if (i < cnt) {
+            values[s + i] = v_r[r];
+            if (keys_out) keys_out[s + i] = ((uint64_t)tile << 32) | k_r[r];
+        }
+    }
+}
+
+// Lists longer than TS_CAP: their (tile << 32 | key, Gaussian) pairs copied to
+// [boff[tile], boff[tile] + bigcnt[tile]) of bk / bv (tiles in order), one
+// workgroup per tile; tiles with bigcnt 0 return at once.
+__global__ __launch_bounds__(256) void big_gather3d_kernel(const uint32_t* __restrict__ ranges,
+                                                           const uint32_t* __restrict__ bigcnt,
+                                                           const uint32_t* __restrict__ boff,
+                                                           const uint32_t* __restrict__ values,
+                                                           const float4* __restrict__ packed,
+                                                           const float4* __restrict__ rays, uint64_t* __restrict__ bk,
+                                                           uint32_t* __restrict__ bv) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t cnt = bigcnt[tile];
+    if (cnt == 0u) return;
+    const uint32_t s = ranges[2 * tile], o = boff[tile];
+    const float4 ray = rays ? rays[tile] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256u) {
+        const uint32_t g = values[s + i];
+        bk[o + i] = ((uint64_t)tile << 32) | tile_depth_key(g, packed, rays != nullptr, ray);
+        bv[o + i] = g;
+    }
+}
+
+// The sorted long lists back into their tiles' ranges.
+__global__ __launch_bounds__(256) void big_scatter3d_kernel(const uint32_t* __restrict__ ranges,
+                                                            const uint32_t* __restrict__ bigcnt,
+                                                            const uint32_t* __restrict__ boff,
+                                                            const uint64_t* __restrict__ bk,
+                                                            const uint32_t* __restrict__ bv,
+                                                            uint32_t* __restrict__ values,
+                                                            uint64_t* __restrict__ keys_out) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t cnt = bigcnt[tile];
+    if (cnt == 0u) return;
+    const uint32_t s = ranges[2 * tile], o = boff[tile];
+    for (uint32_t i = threadIdx.x; i < cnt; i += 256u) {
+        values[s + i] = bv[o + i];
+        if (keys_out) keys_out[s + i] = bk[o + i];
+    }
+}
+
 }  // namespace gs3
 
 // ------------------------------------------------------------------- ABI ----
@@ -829,6 +1017,134 @@ extern "C" int gs_duplicate_keys_3d(gs_ctx* c, int n, const gs_preprocessed* pre
         n, reinterpret_cast<const float2*>(pre->means2D), pre->depths, pre->radii, pre3d->cov3D_inv,
         pre3d->mean_offset, offsets, rays, k.gx, k.gy, keys, values, capacity, ovf);
     G3_HIP(hipGetLastError(), "duplicate3d kernel");
+    return GS_OK;
+}
+
+extern "C" int gs_bin_tiles_3d(gs_ctx* c, int n, const gs_preprocessed* pre, const gs_preprocessed_3d* pre3d,
+                               const gs_camera* cam, uint32_t* values, uint64_t* keys, uint32_t capacity,
+                               uint32_t* ranges, long long* total) {
+    G3_ARG(c != nullptr, "null context");
+    G3_ARG(cam != nullptr && cam->width > 0 && cam->height > 0, "bad camera");
+    G3_ARG(n >= 0, "negative count");
+    G3_ARG(ranges != nullptr, "null tile ranges");
+    const gs3::Cam3 k = gs3::make_cam3(cam, nullptr, 0);
+    const int num_tiles = k.gx * k.gy;
+    if (k.gx > GS_BIN_MAX_TILES_XY || k.gy > GS_BIN_MAX_TILES_XY)
+        return gs_set_error(GS_ERR_UNSUPPORTED,
+                            "gs_bin_tiles_3d: %d x %d tiles exceed %d per side (use gs_duplicate_keys_3d + gs_sort_pairs)",
+                            k.gx, k.gy, GS_BIN_MAX_TILES_XY);
+    if (total) *total = 0;
+    if (n == 0) {
+        G3_HIP(launch_zero(ranges, (size_t)num_tiles * 8, c->stream), "zero kernel");
+        return GS_OK;
+    }
+    G3_ARG(pre && pre3d, "null argument");
+    G3_ARG(pre->means2D && pre->depths && pre->radii && pre3d->cov3D_inv && pre3d->mean_offset, "null array");
+    G3_ARG(al(pre->means2D, 8), "means2D must be 8-byte aligned");
+    G3_ARG(values == nullptr || al(values, 16), "values must be 16-byte aligned");
+    void* sm;
+    int st = gs_small_get(c, &sm);
+    if (st) return st;
+    uint32_t* ovf = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_OVERFLOW);
+    unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
+    auto alg = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    // Gaussian-sized scratch: index order, rectangles, the (tile, group) table
+    // and its scan, the tiles' rays, the long lists' counts and offsets
+    const size_t un = (size_t)n;
+    const int G = c->bin_chunk;
+    const int C = (n + G - 1) / G;
+    const size_t L = (size_t)C * num_tiles + 1;
+    G3_ARG(L < 0xFFFFFFFFull, "binning table too large");
+    const size_t np = scan_partials_needed((uint32_t)std::max<size_t>(L, (size_t)num_tiles + 1)) + 1;
+    void* ws2;
+    if ((st = gs_ws2_get(c, alg(un * 4) + alg(un * 8) + 2 * alg(L * 4) + alg(np * 8) + alg((size_t)num_tiles * 16) +
+                                2 * alg(((size_t)num_tiles + 1) * 4) + alg(un * 64),
+                         &ws2)))
+        return st;
+    char* w = static_cast<char*>(ws2);
+    uint32_t* order = reinterpret_cast<uint32_t*>(w); w += alg(un * 4);
+    uint2* rects = reinterpret_cast<uint2*>(w); w += alg(un * 8);
+    uint32_t* table = reinterpret_cast<uint32_t*>(w); w += alg(L * 4);
+    uint32_t* scanned = reinterpret_cast<uint32_t*>(w); w += alg(L * 4);
+    unsigned long long* partial = reinterpret_cast<unsigned long long*>(w); w += alg(np * 8);
+    float4* rays = reinterpret_cast<float4*>(w); w += alg((size_t)num_tiles * 16);
+    uint32_t* bigcnt = reinterpret_cast<uint32_t*>(w); w += alg(((size_t)num_tiles + 1) * 4);
+    uint32_t* boff = reinterpret_cast<uint32_t*>(w); w += alg(((size_t)num_tiles + 1) * 4);
+    float4* packed = reinterpret_cast<float4*>(w);
+    if (k.have_ivp) {
+        gs3::tile_rays_kernel<<<(num_tiles + 255) / 256, 256, 0, c->stream>>>(k, rays);
+        G3_HIP(hipGetLastError(), "tile rays kernel");
+    }
+    gs3::iota_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>((uint32_t)n, order);
+    G3_HIP(hipGetLastError(), "iota kernel");
+    gsb::BinArgs a{};
+    a.V = 1;
+    a.N = n;
+    a.C = C;
+    a.G = G;
+    a.S = 0;
+    a.G1 = G;
+    a.sidx = order;
+    a.sidx_stride = 1;
+    a.means2D = reinterpret_cast<const float2*>(pre->means2D);
+    a.radii = pre->radii;
+    a.views = nullptr;
+    a.view0 = make_int4(k.gx, k.gy, 0, 0);
+    a.vblock0 = nullptr;
+    a.table = table;
+    a.table_len = L;
+    a.rects = rects;
+    a.out = values;
+    a.capacity = capacity;
+    a.overflow = ovf;
+    a.diff_len = (k.gx + 1) * (k.gy + 1);
+    const int blocks = bin_blocks(C, k.gx, k.gy, c->bin_tpl);
+    G3_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");
+    G3_HIP(launch_scan_u32_exclusive(table, (uint32_t)L, scanned, partial, c->stream, nullptr, total64),
+           "binning scan");
+    a.table = scanned;
+    unsigned long long K = 0;
+    G3_HIP(hipMemcpyAsync(&K, total64, 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(total)");
+    G3_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (total) *total = (long long)K;
+    if (K > capacity)
+        return gs_set_error(GS_ERR_CAPACITY, "gs_bin_tiles_3d: %llu pairs exceed capacity %u", K, capacity);
+    G3_HIP(launch_bin_ranges(a, scanned, (uint32_t)num_tiles, ranges, c->stream), "binning ranges kernel");
+    if (K == 0) return GS_OK;
+    G3_ARG(values != nullptr, "null values buffer");
+    G3_HIP(launch_bin_place(a, blocks, c->bin_tpl, c->stream), "binning place kernel");
+    const float4* rays_arg = k.have_ivp ? rays : nullptr;
+    gs3::pack_keys3d_kernel<<<(n + 255) / 256, 256, 0, c->stream>>>(n, pre->depths, pre3d->cov3D_inv,
+                                                                     pre3d->mean_offset, packed);
+    G3_HIP(hipGetLastError(), "key pack kernel");
+    gs3::tile_sort3d_kernel<<<num_tiles, gs3::TS_THREADS, 0, c->stream>>>(ranges, values, keys, packed, rays_arg,
+                                                                          bigcnt);
+    G3_HIP(hipGetLastError(), "tile sort kernel");
+    // lists longer than one workgroup's LDS: the general radix sort over all of them
+    uint32_t B = 0;
+    uint32_t* B_dev = reinterpret_cast<uint32_t*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
+    G3_HIP(launch_scan_u32_exclusive(bigcnt, (uint32_t)num_tiles, boff, partial, c->stream, B_dev, nullptr),
+           "long-list scan");
+    G3_HIP(hipMemcpyAsync(&B, B_dev, 4, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(long lists)");
+    G3_HIP(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (B == 0) return GS_OK;
+    const size_t ub = (size_t)B;
+    const int key_bits = gs_tile_key_bits(num_tiles);
+    void* ws;
+    if ((st = gs_ws_get(c, 2 * alg(ub * 8) + 2 * alg(ub * 4) + radix_workspace_bytes(B, c->radix_bits), &ws)))
+        return st;
+    char* q = static_cast<char*>(ws);
+    uint64_t* bk = reinterpret_cast<uint64_t*>(q); q += alg(ub * 8);
+    uint64_t* bk2 = reinterpret_cast<uint64_t*>(q); q += alg(ub * 8);
+    uint32_t* bv = reinterpret_cast<uint32_t*>(q); q += alg(ub * 4);
+    uint32_t* bv2 = reinterpret_cast<uint32_t*>(q); q += alg(ub * 4);
+    gs3::big_gather3d_kernel<<<num_tiles, 256, 0, c->stream>>>(ranges, bigcnt, boff, values, packed, rays_arg, bk,
+                                                               bv);
+    G3_HIP(hipGetLastError(), "long-list gather kernel");
+    G3_HIP(launch_radix_sort_bits(bk, bv, bk2, bv2, B, 0, key_bits, c->radix_bits, q, c->stream, nullptr),
+           "long-list sort");
+    gs3::big_scatter3d_kernel<<<num_tiles, 256, 0, c->stream>>>(ranges, bigcnt, boff, bk2, bv2, values, keys);
+    G3_HIP(hipGetLastError(), "long-list scatter kernel");
     return GS_OK;
 }
 

@@ -14,13 +14,21 @@
 
 namespace gsp {
 
+// four independent 16-byte loads in flight per lane before their stores
 __global__ __launch_bounds__(256) void copy_kernel(const float4* __restrict__ in, float4* __restrict__ out, size_t n) {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * 256;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
+        out[i] = a;
+        out[i + stride] = b;
+        out[i + 2 * stride] = c;
+        out[i + 3 * stride] = d;
+    }
     for (; i < n; i += stride) out[i] = in[i];
 }
 
-constexpr int kLdsF4 = 4096;  // 64 KiB of float4
+constexpr int kLdsF4 = 2048;  // 32 KiB of float4 (five workgroups fit in a CU's LDS)
 
 __global__ __launch_bounds__(256) void lds_read_kernel(int iters, float* __restrict__ out) {
     __shared__ float4 buf[kLdsF4];
@@ -122,15 +130,20 @@ extern "C" int gs_device_peaks(gs_ctx* c, gs_peaks* out) {
     if ((e = hipMalloc(&dst.p, bytes)) != hipSuccess) return gs_hip_status(e, "hipMalloc");
     if ((e = launch_zero(src.p, bytes, s)) != hipSuccess) return gs_hip_status(e, "zero");
     float ms = 0.f;
-    const int copy_blocks = cus * 16;
-    int st = best_ms(s, tm, [&] {
-        gsp::copy_kernel<<<copy_blocks, 256, 0, s>>>(static_cast<const float4*>(src.p), static_cast<float4*>(dst.p), n4);
-        return hipGetLastError();
-    }, &ms);
-    if (st != GS_OK) return st;
-    out->hbm_copy_gbs = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+    int st = GS_OK;
+    for (int per_cu : {4, 8, 16, 32}) {  // the best of a few grid sizes
+        const int copy_blocks = cus * per_cu;
+        st = best_ms(s, tm, [&] {
+            gsp::copy_kernel<<<copy_blocks, 256, 0, s>>>(static_cast<const float4*>(src.p), static_cast<float4*>(dst.p),
+                                                         n4);
+            return hipGetLastError();
+        }, &ms);
+        if (st != GS_OK) return st;
+        const double gbs = 2.0 * (double)bytes / (ms * 1e-3) / 1e9;
+        if (gbs > out->hbm_copy_gbs) out->hbm_copy_gbs = gbs;
+    }
 
-    // LDS reads: eight 256-thread workgroups per CU, 64 KiB of LDS each
+    // LDS reads: eight 256-thread workgroups per CU (five resident), 32 KiB of LDS each
     const int blocks = cus * 8, lds_iters = 4096;
     DevBuf sink;
     if ((e = hipMalloc(&sink.p, (size_t)blocks * 256 * sizeof(float))) != hipSuccess) return gs_hip_status(e, "hipMalloc");

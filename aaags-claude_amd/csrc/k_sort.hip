@@ -18,8 +18,10 @@
 
 #include "gs_device.h"
 #include "gs_internal.h"
+#include "gs_wave.h"
 
 namespace gsk {
+using namespace gsw;
 
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
@@ -31,50 +33,6 @@ constexpr int RS_WAVES = RS_THREADS / 64;
 // runs per digit), 1024 below 4M keys so that a sort of ~1M keys still spreads
 // over ~1000 blocks instead of ~250.
 constexpr uint32_t RS_SMALL_LIMIT = 1u << 22;
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    int l = lane_id();
-    return (l == 0) ? 0ull : ((~0ull) >> (64 - l));
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_inclusive_scan(T v) {
-    int l = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        T u = __shfl_up(v, o, 64);
-        if (l >= o) v += u;
-    }
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Exclusive block scan of one value per thread (256 threads); returns the
-// exclusive prefix and writes the block total to *total.
-template <typename T>
-__device__ __forceinline__ T block_exclusive_scan(T v, T* s_wave, T* total) {
-    int w = threadIdx.x >> 6, l = lane_id();
-    T inc = wave_inclusive_scan(v);
-    if (l == 63) s_wave[w] = inc;
-    __syncthreads();
-    T wbase = 0, tot = 0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-        T s = s_wave[k];
-        if (k < w) wbase += s;
-        tot += s;
-    }
-    *total = tot;
-    __syncthreads();
-    return wbase + inc - v;
-}
 
 // n_dev (optional): the item count lives on the device (graph capture); the
 // launch is sized for n (the capacity) and the kernels use min(*n_dev, n).
@@ -352,35 +310,6 @@ __global__ __launch_bounds__(256) void tile_ranges_multi_kernel(const uint64_t* 
 }
 
 // ---------------------------------------------------------- radix sort ----
-template <int BITS>
-__device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift) {
-    return (uint32_t)(k >> shift) & ((1u << BITS) - 1u);
-}
-
-// Lanes (among `active`) whose digit equals this lane's: one ballot per digit
-// bit; a lane differs from this one where its bit differs, i.e. where the
-// ballot XOR this lane's bit (spread to all ones) is set.  Two bits per step,
-// so the two XORs fold into one three-input OR.
-template <int BITS>
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
-    uint32_t dlo = 0u, dhi = 0u;
-#pragma unroll
-    for (int b = 0; b < BITS; b += 2) {
-        const uint32_t m0 = (uint32_t)((int32_t)(d << (31 - b)) >> 31);  // all ones where bit b is set
-        const uint64_t bal0 = __builtin_amdgcn_ballot_w64(m0 != 0u);
-        if (b + 1 < BITS) {
-            const uint32_t m1 = (uint32_t)((int32_t)(d << (30 - b)) >> 31);
-            const uint64_t bal1 = __builtin_amdgcn_ballot_w64(m1 != 0u);
-            dlo |= ((uint32_t)bal0 ^ m0) | ((uint32_t)bal1 ^ m1);
-            dhi |= ((uint32_t)(bal0 >> 32) ^ m0) | ((uint32_t)(bal1 >> 32) ^ m1);
-        } else {
-            dlo |= (uint32_t)bal0 ^ m0;
-            dhi |= (uint32_t)(bal0 >> 32) ^ m0;
-        }
-    }
-    return active & ~(((uint64_t)dhi << 32) | dlo);
-}
-
 template <int BITS, int ROUNDS>
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* __restrict__ keys, uint32_t n_cap,
                                                                 const uint32_t* __restrict__ n_dev, int shift,

@@ -381,14 +381,41 @@ class GsHip:
                                               _vp(ptr(n_contrib))), "gs_raster_fwd_3d")
         return img, T_final, n_contrib
 
-    def render_3d(self, p, cam, cfg, raw=True, filter3d=None):
-        """The eval_3D forward stage by stage: preprocess, scan (K read back),
-        per-tile depth keys, stable pair sort, k-buffer raster."""
-        pre, pre3d = self.preprocess_3d(p, cam, cfg, raw, filter3d)
-        offsets, K = self.scan_tiles(pre["tiles"])
-        keys, vals = self.duplicate_3d(pre, pre3d, offsets, K, cam)
+    def bin_tiles_3d(self, pre, pre3d, cam, capacity=None, keys=True, out=None):
+        """gs_bin_tiles_3d: sorted pair values, keys (optional) and tile ranges
+        of the eval_3D binning in one call; returns (values, keys, ranges, K).
+        out: (values, keys, ranges) buffers to reuse (capacity from values)."""
+        n = pre["radii"].numel()
         gx, gy = (cam.width + 15) // 16, (cam.height + 15) // 16
-        ks, vs, ranges = self.sort_pairs(keys, vals, gx * gy)
+        if out is None:
+            cap = capacity if capacity is not None else int(pre["tiles"].to(torch.int64).sum().item())
+            vo = torch.empty(max(cap, 4), dtype=torch.int32, device=self.dev)
+            ko = torch.empty(max(cap, 1), dtype=torch.int64, device=self.dev) if keys else None
+            ranges = torch.empty(gx * gy * 2, dtype=torch.int32, device=self.dev)
+        else:
+            vo, ko, ranges = out
+            cap = vo.numel()
+        ps, p3, c = self._pre_struct(pre), self._pre3d_struct(pre3d), cam.to_c()
+        K = ctypes.c_longlong()
+        self._check(self.lib.gs_bin_tiles_3d(self.ctx, n, ctypes.byref(ps), ctypes.byref(p3), ctypes.byref(c),
+                                             _vp(ptr(vo)), _vp(ptr(ko)), ctypes.c_uint32(cap), _vp(ptr(ranges)),
+                                             ctypes.byref(K)), "gs_bin_tiles_3d")
+        k = K.value
+        return vo[:k], (ko[:k] if ko is not None else None), ranges, k
+
+    def render_3d(self, p, cam, cfg, raw=True, filter3d=None, fast=False):
+        """The eval_3D forward stage by stage: preprocess, scan (K read back),
+        per-tile depth keys, stable pair sort, k-buffer raster.  fast: the
+        binning (keys, sort, ranges) by gs_bin_tiles_3d instead."""
+        pre, pre3d = self.preprocess_3d(p, cam, cfg, raw, filter3d)
+        if fast:
+            vs, ks, ranges, K = self.bin_tiles_3d(pre, pre3d, cam)
+            keys = vals = None
+        else:
+            offsets, K = self.scan_tiles(pre["tiles"])
+            keys, vals = self.duplicate_3d(pre, pre3d, offsets, K, cam)
+            gx, gy = (cam.width + 15) // 16, (cam.height + 15) // 16
+            ks, vs, ranges = self.sort_pairs(keys, vals, gx * gy)
         img, T, nc = self.raster_fwd_3d(pre, pre3d, vs, ranges, cam, tuple(cfg.bg_color))
         return dict(pre=pre, pre3d=pre3d, keys=keys, values=vals, keys_sorted=ks, values_sorted=vs, ranges=ranges,
                     img=img, T_final=T, n_contrib=nc, K=K)

@@ -265,39 +265,57 @@ def bench_large(hip, cfg, args):
 
 def bench_eval3d(hip, scene, cam, cfg, reps=5):
     """The eval_3D forward (AAA-Gaussians 3D evaluation with the 16-entry
-    per-pixel k-buffer) on the headline scene and view: the whole stage chain
-    timed on the host around synchronized runs (the scan reads the pair count
-    back, as the reference's binner does), and each stage's device time alone."""
+    per-pixel k-buffer) on the headline scene and view, timed on the host
+    around synchronized runs (the pair count is read back, as the reference's
+    binner does): `ms` with the fast binning (gs_bin_tiles_3d: direct placement
+    + per-tile sort), `ms_stage_path` with the reference's stage chain (scan,
+    duplicate, 64-bit pair sort); and each stage's device time alone."""
     import torch
     p = hip.upload_scene(scene)
     out = hip.render_3d(p, cam, cfg, raw=True)  # warm-up: allocations, code load
     hip.sync()
     K = int(out["K"])
-    ts = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        o = hip.render_3d(p, cam, cfg, raw=True)
-        hip.sync()
-        ts.append(time.perf_counter() - t0)
-        del o
-    ms = 1e3 * sorted(ts)[len(ts) // 2]
     pre, pre3d = out["pre"], out["pre3d"]
     bg = tuple(cfg.bg_color)
+    img, T, nc = out["img"], out["T_final"], out["n_contrib"]
+
+    def host_ms(f):
+        f()
+        hip.sync()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            hip.sync()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * sorted(ts)[len(ts) // 2]
+
+    ms_stage = host_ms(lambda: hip.render_3d(p, cam, cfg, raw=True))
+    vo = torch.empty(K + 4, dtype=torch.int32, device=hip.dev)
+    ranges = torch.empty(cam.num_tiles * 2, dtype=torch.int32, device=hip.dev)
+
+    def fast():
+        hip.preprocess_3d(p, cam, cfg, True, None, pre, pre3d)
+        vs, _, rg, _ = hip.bin_tiles_3d(pre, pre3d, cam, keys=False, out=(vo, None, ranges))
+        hip.raster_fwd_3d(pre, pre3d, vs, rg, cam, bg, img, T, nc)
+
+    ms = host_ms(fast)
     st = {}
     st["preprocess"] = median_ms(lambda: hip.preprocess_3d(p, cam, cfg, True, None, pre, pre3d))
+    st["bin_fast"] = median_ms(lambda: hip.bin_tiles_3d(pre, pre3d, cam, keys=False, out=(vo, None, ranges)))
     offsets, _ = hip.scan_tiles(pre["tiles"])
     st["scan"] = median_ms(lambda: hip.scan_tiles(pre["tiles"]))
     st["duplicate"] = median_ms(lambda: hip.duplicate_3d(pre, pre3d, offsets, K, cam))
     st["sort"] = median_ms(lambda: hip.sort_pairs(out["keys"], out["values"], cam.num_tiles))
-    img, T, nc = out["img"], out["T_final"], out["n_contrib"]
     st["raster"] = median_ms(lambda: hip.raster_fwd_3d(pre, pre3d, out["values_sorted"], out["ranges"], cam, bg, img,
                                                        T, nc), reps=3)
     npix = cam.width * cam.height
     res = {"workload": f"{scene.n} Gaussians SH{scene.sh_degree}, {cam.width}x{cam.height}, eval_3D",
            "pairs": K, "ms": round(ms, 3), "fwd_mpix_s": round(npix / (ms * 1e-3) / 1e6, 1),
+           "ms_stage_path": round(ms_stage, 3),
            "stages_ms": {k: round(v, 4) for k, v in st.items()},
            "mean_blended_per_pixel": round(float(nc.float().mean().item()), 2)}
-    del out, pre, pre3d, p
+    del out, pre, pre3d, p, vo
     torch.cuda.empty_cache()
     return res
 

@@ -288,6 +288,17 @@ int gs_duplicate_keys_3d(gs_ctx* ctx, int n, const gs_preprocessed* pre, const g
 int gs_raster_fwd_3d(gs_ctx* ctx, const gs_camera* cam, const gs_preprocessed* pre,
                      const gs_preprocessed_3d* pre3d, const uint32_t* values_sorted, const uint32_t* ranges,
                      const float bg[3], float* image, float* T_final, int* n_contrib);
+/* The fast path for gs_duplicate_keys_3d + gs_sort_pairs (tile_binner_cpu.cpp:
+ * 59-99, sorter_cpu.cpp:6-43 on eval_3D keys): values and ranges equal theirs
+ * bit for bit, and keys (optional, K entries) their sorted keys.  Direct
+ * placement of every tile's pairs in Gaussian order, then a stable sort of
+ * each tile's list by its per-tile depth key (in LDS for lists of up to 16384
+ * pairs, by the radix sort beyond).  *total receives the pair count K; with K
+ * above capacity nothing is placed and GS_ERR_CAPACITY is returned.  Views up
+ * to 255 x 255 tiles.  Synchronises the stream (K is read back). */
+int gs_bin_tiles_3d(gs_ctx* ctx, int n, const gs_preprocessed* pre, const gs_preprocessed_3d* pre3d,
+                    const gs_camera* cam, uint32_t* values, uint64_t* keys, uint32_t capacity, uint32_t* ranges,
+                    long long* total);
 
 /* L1 loss and its image gradient: l1_loss / l1_loss_double (loss.cpp:4-30).
  * loss_dev: device double, the mean absolute difference (sum in FP64). */

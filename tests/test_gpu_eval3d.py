@@ -1,6 +1,7 @@
 """GPU parity of the eval_3D forward (gs_preprocess_fwd_3d, gs_duplicate_keys_3d,
-gs_sort_pairs on 3D keys, gs_raster_fwd_3d) against the oracle's restatement
-of the reference's CPU 3D path, stage by stage on identical inputs.
+gs_sort_pairs on 3D keys, gs_raster_fwd_3d, and the fast binning
+gs_bin_tiles_3d) against the oracle's restatement of the reference's CPU 3D
+path, stage by stage on identical inputs.
 
 Bars: radii, tiles, keys, values, sorted values, tile ranges and n_contrib
 bit-exact; the per-Gaussian floats bit-exact (the same IEEE operations in the
@@ -71,6 +72,12 @@ def test_eval3d_stages_match_oracle(hip, o3, case):
     sk, sv, sr = o3.sort(rk, rv, gx * gy)
     np.testing.assert_array_equal(_np(vs).view(np.uint32), sv)
     np.testing.assert_array_equal(_np(ranges).view(np.uint32).reshape(-1, 2), sr)
+    # the fast path (direct placement + per-tile sort) gives the same pairs, keys and ranges
+    fv, fk, fr, fK = hip.bin_tiles_3d(pre, pre3d, c["cam"])
+    assert fK == K
+    np.testing.assert_array_equal(_np(fv).view(np.uint32), sv)
+    np.testing.assert_array_equal(_np(fk).view(np.uint64), sk)
+    np.testing.assert_array_equal(_np(fr).view(np.uint32).reshape(-1, 2), sr)
 
     img, T, nc = hip.raster_fwd_3d(pre, pre3d, vs, ranges, c["cam"], tuple(c["cfg"].bg_color))
     hip.sync()
@@ -111,3 +118,58 @@ def test_eval3d_nothing_visible(hip):
     assert int(_np(out["pre"]["radii"]).max()) == 0
     np.testing.assert_array_equal(_np(out["img"]), np.broadcast_to(np.float32([0.25, 0.5, 0.75]), (40, 48, 3)))
     assert int(_np(out["n_contrib"]).max()) == 0
+
+
+@pytest.mark.parametrize("n,w,h", [(40_000, 64, 64), (60_000, 160, 96)])
+def test_eval3d_fast_binning_long_tile_lists(hip, o3, n, w, h):
+    """Tiles holding more pairs than one workgroup sorts in LDS (16384): a
+    dense cluster in front of the camera.  The fast binning's long lists go
+    through the radix sort; values, keys and ranges equal the oracle's
+    duplicate + stable sort bit for bit, and the image the stage path's."""
+    _say("eval3d long lists", n, w, h)
+    scene = gaussians_normal(n, 31, 1, sigma=0.25, logscale=(-1.8, 0.3), op=(0.0, 1.0))
+    cam = make_camera((0.0, 0.0, -3.0), w, h, 60.0)
+    cfg = RenderCfg(training=False, sh_degree=1)
+    p = hip.upload_scene(scene)
+    pre, pre3d = hip.preprocess_3d(p, cam, cfg, raw=True)
+    fv, fk, fr, K = hip.bin_tiles_3d(pre, pre3d, cam)
+    hip.sync()
+    per_tile = np.diff(_np(fr).view(np.uint32).reshape(-1, 2), axis=1).ravel()
+    assert per_tile.max() > 16384, per_tile.max()
+    got = dict(means2D=_np(pre["means2D"]), depths=_np(pre["depths"]), radii=_np(pre["radii"]),
+               tiles=_np(pre["tiles"]), cov3D_inv=_np(pre3d["cov3D_inv"]),
+               mean_offset=_np(pre3d["mean_offset"]))
+    rk, rv = o3.duplicate(got, n, cam)
+    gx, gy = (w + 15) // 16, (h + 15) // 16
+    sk, sv, sr = o3.sort(rk, rv, gx * gy)
+    assert K == len(sv)
+    np.testing.assert_array_equal(_np(fv).view(np.uint32), sv)
+    np.testing.assert_array_equal(_np(fk).view(np.uint64), sk)
+    np.testing.assert_array_equal(_np(fr).view(np.uint32).reshape(-1, 2), sr)
+    a = hip.render_3d(p, cam, cfg, raw=True)
+    b = hip.render_3d(p, cam, cfg, raw=True, fast=True)
+    hip.sync()
+    assert torch.equal(a["img"], b["img"]) and torch.equal(a["n_contrib"], b["n_contrib"])
+
+
+def test_eval3d_fast_binning_full_size(hip):
+    """1M Gaussians (SH3), 1920x1080: the fast binning equals the stage path
+    (gs_duplicate_keys_3d + gs_sort_pairs, bit-exact against the oracle in the
+    tests above) bit for bit: values, keys and ranges."""
+    from gs_scenes import make_workload
+    _say("eval3d fast binning 1m 1080p")
+    scene, cams, _ = make_workload("1m_1080p")
+    cam = cams[0]
+    cfg = RenderCfg(training=False, sh_degree=3)
+    p = hip.upload_scene(scene)
+    pre, pre3d = hip.preprocess_3d(p, cam, cfg, raw=True)
+    offsets, K = hip.scan_tiles(pre["tiles"])
+    keys, vals = hip.duplicate_3d(pre, pre3d, offsets, K, cam)
+    ks, vs, ranges = hip.sort_pairs(keys, vals, ((cam.width + 15) // 16) * ((cam.height + 15) // 16))
+    del keys, vals
+    fv, fk, fr, fK = hip.bin_tiles_3d(pre, pre3d, cam)
+    hip.sync()
+    assert fK == K and K > 40_000_000
+    assert torch.equal(fv, vs)
+    assert torch.equal(fk, ks)
+    assert torch.equal(fr, ranges)
