@@ -145,3 +145,36 @@ def write_ply(path, n, rest, seed, filter3d=False, normals=True, zero_rot_at=Non
         f.write(("\n".join(hdr) + "\n").encode())
         f.write(raw.astype("<f4").tobytes())
     return raw, names
+
+
+def fd_mismatches(grads, idx, fd, steps, max_coeffs, group_bar, rel_bar, abs_frac, min_clean=0.9):
+    """Analytic gradients (dict pos/scale/rot/sh/op of full arrays) of the
+    Gaussians `idx` against FP64 finite differences `fd` / `steps` from
+    oracle.fd_grads_local (rows in idx order; a step of 0 marks a derivative
+    whose every step spans a discontinuity of the forward, left out).  Within
+    each parameter group: max |g - fd| <= group_bar * max |fd|; each element:
+    |g - fd| <= rel_bar |fd| + abs_frac max |fd|; at least min_clean of the
+    derivatives clean.  Returns (failures, report lines)."""
+    n = len(idx)
+    g = np.concatenate([np.asarray(grads["pos"])[idx], np.asarray(grads["scale"])[idx],
+                        np.asarray(grads["rot"])[idx], np.asarray(grads["sh"])[idx].reshape(n, -1),
+                        np.asarray(grads["op"])[idx].reshape(n, 1)], 1).astype(np.float64)
+    ok = steps > 0
+    fails, report = [], [f"clean derivatives: {ok.mean():.3f} of {ok.size}"]
+    if ok.mean() < min_clean:
+        fails.append(("clean", float(ok.mean())))
+    K = max_coeffs
+    groups = {"pos": (0, 3), "scale": (3, 3), "rot": (6, 4), "sh": (10, 3 * K), "op": (10 + 3 * K, 1)}
+    for nm, (a, l) in groups.items():
+        m = ok[:, a:a + l]
+        gg, ff = g[:, a:a + l][m], fd[:, a:a + l][m]
+        if ff.size == 0:
+            continue
+        top = max(np.abs(ff).max(), 1e-30)
+        e_group = np.abs(gg - ff).max() / top
+        excess = (np.abs(gg - ff) - (rel_bar * np.abs(ff) + abs_frac * top)).max()
+        report.append(f"{nm}: {gg.size} values, max|g-fd|/max|fd| {e_group:.2e}, elementwise excess {excess:.2e}")
+        if e_group > group_bar or excess > 0:
+            fails.append((nm, e_group, excess))
+    return fails, report
+

@@ -1,16 +1,15 @@
 """Test-harness autograd wrapper over the C ABI, checked against FP64 central
-finite differences (h = 1e-4) of the double-precision oracle forward.
+finite differences (h = 1e-4, smaller where 1e-4 spans a discontinuity) of
+the double-precision oracle forward, every parameter of a small scene.
 
 The wrapper lives here, in tests/, on purpose: it is a checking device for the
 hand-written adjoint kernels, not part of the product.
 """
-import sys
-
 import numpy as np
 import pytest
 import torch
 
-from gs_scenes import RawScene, RenderCfg, random_small_scene
+from gs_scenes import RenderCfg, random_small_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -36,21 +35,15 @@ class RenderFn(torch.autograd.Function):
         return g["pos"], g["scale"], g["rot"], g["sh"], g["op"], None, None, None, None
 
 
-def _fd(oracle64, scene, cam, cfg, gt, param, idx, h=1e-4):
-    attr = {"pos": "pos", "scale": "log_scale", "rot": "rot", "sh": "sh", "op": "opacity_logit"}[param]
-
-    def loss(delta):
-        s64 = RawScene(*(a.astype(np.float64) for a in scene.arrays()), scene.sh_degree)
-        a = getattr(s64, attr).reshape(-1).copy()
-        a[idx] += delta
-        setattr(s64, attr, a.reshape(getattr(s64, attr).shape))
-        return oracle64.loss(s64, cam, cfg, gt)
-
-    return (loss(h) - loss(-h)) / (2 * h)
-
-
 @pytest.mark.parametrize("deg,seed", [(0, 71), (1, 72), (3, 73)])
-def test_autograd_gradients_match_fp64_finite_differences(hip, oracle64, deg, seed):
+def test_autograd_gradients_match_fp64_finite_differences(hip, deg, seed):
+    """Every raw parameter of every Gaussian of a small scene: the gradients
+    through the wrapper against FP64 central differences of the oracle's
+    double forward (oracle.fd_grads_local: h = 1e-4, smaller steps where 1e-4
+    spans a discontinuity).  Bars: within each parameter group 1e-3 of its
+    largest derivative; each element 1e-2 of itself plus 1e-4 of that largest."""
+    from helpers import fd_mismatches
+    from oracle.oracle import fd_grads_local
     print("[case] autograd fd", deg, seed, flush=True)
     scene, cam, gt = random_small_scene(12, deg, seed, width=48, height=40, logscale=(-1.6, 0.2), op=(0.0, 0.5),
                                         spread=0.6)
@@ -61,14 +54,9 @@ def test_autograd_gradients_match_fp64_finite_differences(hip, oracle64, deg, se
     img = RenderFn.apply(*leaves, hip, meta, cam, cfg)
     loss = (img - torch.tensor(gt, device=d)).abs().mean()
     loss.backward()
-    rng = np.random.default_rng(seed)
-    worst = 0.0
-    for li, param in enumerate(["pos", "scale", "rot", "sh", "op"]):
-        g = leaves[li].grad.detach().cpu().numpy().reshape(-1)
-        scale = max(np.abs(g).max(), 1e-12)
-        for idx in rng.choice(g.size, size=min(6, g.size), replace=False):
-            fd = _fd(oracle64, scene, cam, cfg, gt, param, int(idx))
-            err = abs(fd - g[idx]) / scale
-            worst = max(worst, err)
-            assert abs(fd - g[idx]) <= 2e-2 * scale + 1e-7, (param, idx, fd, g[idx])
-    print("worst FD discrepancy (fraction of max |grad|):", worst, flush=True)
+    grads = {k: leaves[li].grad.detach().cpu().numpy() for li, k in enumerate(["pos", "scale", "rot", "sh", "op"])}
+    idx = np.arange(scene.n)
+    fd, steps = fd_grads_local(scene, cam, cfg, gt, idx, h=1e-4)
+    fails, report = fd_mismatches(grads, idx, fd, steps, scene.max_coeffs, 1e-3, 1e-2, 1e-4)
+    print("\n".join(report), flush=True)
+    assert not fails, fails

@@ -17,7 +17,7 @@ import pytest
 import torch
 
 from gs_scenes import RenderCfg, make_workload
-from helpers import GRAD_RTOL, assert_grad_close, assert_image_close, rel_err
+from helpers import GRAD_RTOL, assert_grad_close, assert_image_close, fd_mismatches, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -106,27 +106,10 @@ def test_1m_1080p_gradients_match_fp64_finite_differences(step_1m_1080p):
     assert len(live) > 1000
     idx = np.sort(np.random.default_rng(3).choice(live, 64, replace=False))
     fd, steps = fd_grads_local(scene, cam, RenderCfg(sh_degree=3, training=True), gt, idx, h=1e-4)
-    n = len(idx)
-    g = np.concatenate([gpu["grads"]["pos"][idx], gpu["grads"]["scale"][idx], gpu["grads"]["rot"][idx],
-                        gpu["grads"]["sh"][idx].reshape(n, -1), gpu["grads"]["op"][idx][:, None]], 1)
-    g = g.astype(np.float64)
-    ok = steps > 0
     print(f"  steps: h=1e-4 {int((steps == 1e-4).sum())}, 1e-5 {int((steps == 1e-5).sum())}, "
-          f"1e-6 {int((steps == 1e-6).sum())}, none {int((~ok).sum())} of {steps.size}", flush=True)
-    assert ok.mean() >= 0.9, ok.mean()
-    K = scene.max_coeffs
-    groups = {"pos": (0, 3), "scale": (3, 3), "rot": (6, 4), "sh": (10, 3 * K), "op": (10 + 3 * K, 1)}
-    fails = []
-    for nm, (a, l) in groups.items():
-        m = ok[:, a:a + l]
-        gg, ff = g[:, a:a + l][m], fd[:, a:a + l][m]
-        top = np.abs(ff).max()
-        e_group = np.abs(gg - ff).max() / top
-        worst_elem = (np.abs(gg - ff) - (1e-2 * np.abs(ff) + 1e-3 * top)).max()
-        print(f"  {nm}: {gg.size} values, max|g-fd|/max|fd| {e_group:.2e}, "
-              f"elementwise excess over bar {worst_elem:.2e} (max|fd| {top:.3e})", flush=True)
-        if e_group > GRAD_RTOL or worst_elem > 0:
-            fails.append((nm, e_group, worst_elem))
+          f"1e-6 {int((steps == 1e-6).sum())}, none {int((steps == 0).sum())} of {steps.size}", flush=True)
+    fails, report = fd_mismatches(gpu["grads"], idx, fd, steps, scene.max_coeffs, GRAD_RTOL, 1e-2, 1e-3)
+    print("  " + "\n  ".join(report), flush=True)
     assert not fails, fails
 
 
