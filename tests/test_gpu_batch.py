@@ -167,3 +167,40 @@ def test_overflowing_step_changes_nothing_and_redo_is_one_clean_step(hip, fused)
     for x, y in zip(redo, tr2.params()):
         assert np.abs(x - y).max() <= 2.5e-3
         assert (x == y).mean() >= 0.99, (x == y).mean()
+
+
+def test_batched_word_sort_path_with_views_wider_than_255_tiles(hip):
+    """A view 4160 px wide (260 tiles) takes the whole batch off the direct
+    placement onto the word sort: (tile | index) words sorted by the radix
+    sort over the pre-sized capacity with the live pair count on the device.
+    With ~28M pairs that sort runs the one-sweep passes.  Graph-captured and
+    eager batches render every view as the single-view path does, bit for bit,
+    and one step's per-view losses equal the single-view losses."""
+    from gs_hip import GsBatch, GsTrainer, train_cfg
+    from gs_scenes import gaussians_normal
+    print("[case] batch word-sort path 4160x2400 + 1280x720", flush=True)
+    scene = gaussians_normal(300_000, 91, 1, sigma=1.5, logscale=(-4.0, 0.6), op=(-1.0, 1.5))
+    rng = np.random.Generator(np.random.Philox(92))
+    cams = [make_camera(sphere_eye(rng, 4.0), 4160, 2400, 60.0), make_camera(sphere_eye(rng, 4.0), 1280, 720, 50.0)]
+    gts = [target_image(c.width, c.height, 93 + i) for i, c in enumerate(cams)]
+    cfg = RenderCfg(sh_degree=1, training=True)
+    tr = GsTrainer(hip, scene)
+    singles, single_losses = [], []
+    for cam, gt in zip(cams, gts):
+        tr.render(cam, cfg)
+        singles.append(tr.buffer("image", (cam.height, cam.width, 3)))
+        # a fresh trainer per view: its step's loss is the loss at the scene's parameters
+        # (a zero learning rate is no no-op: the position schedule's log-interpolation of
+        # 0 and 0 is NaN at step 0, in the reference as here)
+        single_losses.append(GsTrainer(hip, scene).step(cam, cfg, torch.tensor(gt, device=hip.dev),
+                                                        train_cfg(lr=1e-3), 0).loss)
+    for graph in (True, False):
+        b = GsBatch(GsTrainer(hip, scene), cams, cfg, gts, persistent=True, fused_adam=False, use_graph=graph)
+        K = b.render()
+        assert K > 4_000_000, K
+        for v, im in enumerate(b.images()):
+            assert torch.equal(im, singles[v]), (graph, v)
+        b.step(train_cfg(lr=1e-3), 0, sync=True)
+        _, losses = b.result()
+        assert np.allclose(losses, single_losses, rtol=0, atol=1e-6), (losses, single_losses)
+        del b
