@@ -75,6 +75,7 @@ struct gs_batch {
     DevBuf dkeys, didx, dkeys_s, didx_s, rtiles, roff, scan_partial, dsort_ws;
     DevBuf words, words_tmp, tsort_ws, ranges;
     DevBuf values, bviews_dev, bvblock_dev, bin_table, bin_scanned, bin_partial, bin_rects;
+    DevBuf rect_idx;  // direct binning: packed tile rectangles by (view, Gaussian), from the preprocess
     DevBuf image, d_image, gt, T_final, n_contrib, last;
     DevBuf rg;  // screen-space gradients, one block of V*N: means2D (8 B), conics (12), rgb (12), opacities (4)
     DevBuf losses, l1_partial, small;  // small: total32 @0, total64 @8, overflow @16, skipped @20
@@ -240,6 +241,7 @@ extern "C" int gs_batch_create(gs_trainer* tr, int n_views, const gs_camera* cam
         (st = b->conics.ensure(VN * 12)) || (st = b->op2d.ensure(VN * 4)) || (st = b->rgb.ensure(VN * 12)) ||
         (st = b->radii.ensure(VN * 4)) || (st = b->tiles.ensure(VN * 4)) || (st = b->dkeys.ensure(VN * 8)) ||
         (st = b->didx.ensure(VN * 4)) || (st = b->dkeys_s.ensure(VN * 8)) || (st = b->didx_s.ensure(VN * 4)) ||
+        (b->direct && (st = b->rect_idx.ensure(VN * 4))) ||
         (st = b->scan_partial.ensure(((VN + SCAN_BLOCK - 1) / SCAN_BLOCK + 1) * 8)) ||
         (st = b->dsort_ws.ensure(radix_workspace_bytes((uint32_t)VN, b->radix_bits))) ||
         (st = b->ranges.ensure((size_t)b->total_tiles * 8)) || (st = b->image.ensure(P * 12)) ||
@@ -305,7 +307,8 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
     gs_preprocessed pre = b->pre();
     const int deg = std::min(b->cfg.sh_degree, t->sh_degree);
     B_HIP(launch_preprocess_fwd_multi(g, static_cast<const gsd::CamArgs*>(b->cams_dev.p), b->V, pre,
-                                      static_cast<uint64_t*>(b->dkeys.p), b->didx.u(), deg, s),
+                                      static_cast<uint64_t*>(b->dkeys.p), b->didx.u(), deg, s,
+                                      b->direct ? b->rect_idx.u() : nullptr),
           "multi-view preprocess");
     B_HIP(launch_radix_sort_bits(static_cast<const uint64_t*>(b->dkeys.p), b->didx.u(),
                                  static_cast<uint64_t*>(b->dkeys_s.p), b->didx_s.u(), VN, 0, 32 + b->view_bits,
@@ -332,6 +335,7 @@ static int enqueue_binning(gs_batch* b, hipStream_t s) {
         a.capacity = b->capacity;
         a.overflow = b->overflow();
         a.diff_len = b->bin_diff_len;
+        a.rect_idx = b->rect_idx.u();
         B_HIP(launch_bin_count(a, b->bin_blocks, b->bin_tpl, s), "binning count");
         B_HIP(launch_scan_u32_exclusive(b->bin_table.u(), (uint32_t)b->bin_L, b->bin_scanned.u(),
                                         static_cast<unsigned long long*>(b->bin_partial.p), s, b->total32(),

@@ -547,19 +547,21 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     GS_CHECK_ARG(L < 0xFFFFFFFFull, "binning table too large");
     const size_t np = scan_partials_needed((uint32_t)L) + 1;
     void* ws2;
-    if ((st = gs_ws2_get(c, 2 * al(un * 8) + al(un * 8) + radix_workspace_bytes((uint32_t)n, 8) + 2 * al(L * 4) +
-                                al(np * 8),
+    if ((st = gs_ws2_get(c, 2 * al(un * 8) + al(un * 8) + al(un * 4) + radix_workspace_bytes((uint32_t)n, 8) +
+                                2 * al(L * 4) + al(np * 8),
                          &ws2)))
         return st;
     char* w = static_cast<char*>(ws2);
     uint64_t* dkeys = reinterpret_cast<uint64_t*>(w); w += al(un * 8);    // index << 32 | depth key
     uint64_t* dkeys_s = reinterpret_cast<uint64_t*>(w); w += al(un * 8);  // the same, in depth order
     uint2* rects = reinterpret_cast<uint2*>(w); w += al(un * 8);
+    uint32_t* rect_idx = reinterpret_cast<uint32_t*>(w); w += al(un * 4);  // packed rectangles, index order
     uint32_t* table = reinterpret_cast<uint32_t*>(w); w += al(L * 4);
     uint32_t* scanned = reinterpret_cast<uint32_t*>(w); w += al(L * 4);
     unsigned long long* partial = reinterpret_cast<unsigned long long*>(w); w += al(np * 8);
     void* dsort_ws = w;
-    GS_HIP(launch_depth_words(n, pre->depths, pre->radii, dkeys, c->stream), "depth keys kernel");
+    GS_HIP(launch_depth_words(n, pre->depths, pre->radii, dkeys, c->stream, pre->means2D, gx, gy, rect_idx),
+           "depth keys kernel");
     GS_HIP(launch_radix_sort_bits(dkeys, nullptr, dkeys_s, nullptr, (uint32_t)n, 0, 32, c->radix_bits, dsort_ws,
                                   c->stream, nullptr),
            "depth sort");
@@ -584,6 +586,7 @@ int gs_bin_tiles_impl(gs_ctx* c, int n, const gs_preprocessed* pre, int width, i
     a.capacity = capacity;
     a.overflow = ovf;
     a.diff_len = (gx + 1) * (gy + 1);
+    a.rect_idx = rect_idx;
     unsigned long long* total64 = reinterpret_cast<unsigned long long*>(static_cast<char*>(sm) + GS_SMALL_TOTAL64);
     const int blocks = bin_blocks(C, gx, gy, c->bin_tpl);
     GS_HIP(launch_bin_count(a, blocks, c->bin_tpl, c->stream), "binning count kernel");

@@ -490,6 +490,7 @@ struct BinArgs {
     uint32_t capacity;
     uint32_t* overflow;
     int diff_len;           // max over views of (tiles per row + 1) * (tile rows + 1); 0: no LDS count
+    const uint32_t* rect_idx;  // optional [V][N]: packed rectangles by Gaussian index (else from means2D, radii)
 };
 }  // namespace gsb
 

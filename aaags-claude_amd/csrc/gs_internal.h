@@ -90,7 +90,8 @@ hipError_t launch_tile_ranges(const uint64_t* keys_sorted, uint32_t n, uint32_t 
                               hipStream_t s, const uint32_t* n_dev = nullptr);
 hipError_t launch_depth_keys(int n, const float* depths, const int* radii, uint64_t* keys, uint32_t* idx,
                              hipStream_t s);
-hipError_t launch_depth_words(int n, const float* depths, const int* radii, uint64_t* words, hipStream_t s);
+hipError_t launch_depth_words(int n, const float* depths, const int* radii, uint64_t* words, hipStream_t s,
+                              const float* means2D = nullptr, int gx = 0, int gy = 0, uint32_t* rects = nullptr);
 hipError_t launch_gather_tiles(int n, const uint32_t* order, const int* tiles, int* out, hipStream_t s);
 hipError_t launch_duplicate_ranked(int n, const uint32_t* order, const float* means2D, const int* radii,
                                    const uint32_t* offsets, int gx, int gy, uint64_t* words, uint32_t capacity,
@@ -114,7 +115,7 @@ hipError_t launch_zero(void* ptr, size_t bytes, hipStream_t s);
 // multi-view
 hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev, int V,
                                        const gs_preprocessed& out, uint64_t* dkeys, uint32_t* didx, int deg,
-                                       hipStream_t s);
+                                       hipStream_t s, uint32_t* rects = nullptr);
 hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const gsd::CamArgs* cams_dev,
                                        const gsd::CamArgs* cam_val, int V, const int* radii, const gs_raster_grads& rg,
                                        const gs_param_grads& out, float* const adam_p[5], float* const adam_m[5],

@@ -51,6 +51,10 @@ __global__ __launch_bounds__(256) void bin_rects_kernel(BinArgs a) {
     const int4 vi = a.views ? a.views[v] : a.view0;
     const size_t seg = (size_t)v * a.N;
     const uint32_t id = a.sidx[r * (size_t)a.sidx_stride];
+    if (a.rect_idx) {  // packed beforehand in index order: one word gathered
+        a.rects[r] = make_uint2(a.rect_idx[seg + id], id);
+        return;
+    }
     const int rad = a.radii[seg + id];
     uint2 rec = make_uint2(0u, id);
     if (rad > 0) {

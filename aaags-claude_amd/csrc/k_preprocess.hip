@@ -179,12 +179,15 @@ __global__ __launch_bounds__(256) void preprocess_bwd_kernel(gs_gaussians G, Cam
 // ------------------------------------------------------------ multi-view ----
 // Forward over V views: parameters and SH read once, per-view outputs written
 // view-major ([v][N]), plus the depth-sort keys view << 32 | depth bits
-// (culled: view << 32 | 0xFFFFFFFF) and Gaussian indices.
+// (culled: view << 32 | 0xFFFFFFFF) and Gaussian indices; rects (optional,
+// views of at most 255 x 255 tiles): each Gaussian's packed tile rectangle
+// x0 | x1 << 8 | y0 << 16 | y1 << 24 (0 when culled), for the binning.
 template <int MAXC, int DEG>
 __global__ __launch_bounds__(256) void preprocess_fwd_multi_kernel(gs_gaussians G, const CamArgs* __restrict__ cams,
                                                                    int V, gs_preprocessed out,
                                                                    uint64_t* __restrict__ dkeys,
-                                                                   uint32_t* __restrict__ didx) {
+                                                                   uint32_t* __restrict__ didx,
+                                                                   uint32_t* __restrict__ rects) {
     constexpr int NB = (DEG + 1) * (DEG + 1);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = G.count;
@@ -220,6 +223,15 @@ __global__ __launch_bounds__(256) void preprocess_fwd_multi_kernel(gs_gaussians 
         out.tiles_touched[o] = tiles;
         dkeys[o] = ((uint64_t)v << 32) | (vis ? (uint64_t)__float_as_uint(depth) : 0xFFFFFFFFull);
         didx[o] = (uint32_t)i;
+        if (rects) {
+            uint32_t rec = 0u;
+            if (radius > 0) {
+                int mn[2], mx[2];
+                get_rect(m2.x, m2.y, radius, c.gx, c.gy, mn, mx);
+                rec = (uint32_t)mn[0] | ((uint32_t)mx[0] << 8) | ((uint32_t)mn[1] << 16) | ((uint32_t)mx[1] << 24);
+            }
+            rects[o] = rec;
+        }
     }
 }
 
@@ -501,10 +513,11 @@ hipError_t launch_preprocess_bwd(const gs_gaussians& G, const CamArgs& c, const 
 
 hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const CamArgs* cams_dev, int V,
                                        const gs_preprocessed& out, uint64_t* dkeys, uint32_t* didx, int deg,
-                                       hipStream_t s) {
+                                       hipStream_t s, uint32_t* rects) {
     dim3 grid((G.count + 255) / 256), block(256);
     GS_DISPATCH_SH(G.max_coeffs, deg,
-                   (preprocess_fwd_multi_kernel<MC, DG><<<grid, block, 0, s>>>(G, cams_dev, V, out, dkeys, didx)));
+                   (preprocess_fwd_multi_kernel<MC, DG>
+                    <<<grid, block, 0, s>>>(G, cams_dev, V, out, dkeys, didx, rects)));
     return hipGetLastError();
 }
 

@@ -202,12 +202,29 @@ __global__ __launch_bounds__(256) void depth_keys_kernel(int n, const float* __r
 // The same keys with the index in the high word (index << 32 | key): sorted
 // on bits [0, 32) as self-contained words, the index travels with its key
 // (no separate value array through the passes).
+// rects (optional, views of at most 255 x 255 tiles): each Gaussian's tile
+// rectangle packed as x0 | x1 << 8 | y0 << 16 | y1 << 24 (0 when culled), in
+// index order, so that the binning gathers one word per Gaussian in depth
+// order instead of a mean and a radius.
 __global__ __launch_bounds__(256) void depth_words_kernel(int n, const float* __restrict__ depths,
-                                                          const int* __restrict__ radii, uint64_t* __restrict__ words) {
+                                                          const int* __restrict__ radii, uint64_t* __restrict__ words,
+                                                          const float2* __restrict__ means2D, int gx, int gy,
+                                                          uint32_t* __restrict__ rects) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t key = radii[i] > 0 ? __float_as_uint(depths[i]) : 0xFFFFFFFFu;
+    const int rad = radii[i];
+    const uint32_t key = rad > 0 ? __float_as_uint(depths[i]) : 0xFFFFFFFFu;
     words[i] = ((uint64_t)(uint32_t)i << 32) | key;
+    if (rects) {
+        uint32_t rec = 0u;
+        if (rad > 0) {
+            const float2 m = means2D[i];
+            int mn[2], mx[2];
+            gsd::get_rect(m.x, m.y, rad, gx, gy, mn, mx);
+            rec = (uint32_t)mn[0] | ((uint32_t)mx[0] << 8) | ((uint32_t)mn[1] << 16) | ((uint32_t)mx[1] << 24);
+        }
+        rects[i] = rec;
+    }
 }
 
 __global__ __launch_bounds__(256) void gather_tiles_kernel(int n, const uint32_t* __restrict__ order,
@@ -1038,9 +1055,11 @@ hipError_t launch_depth_keys(int n, const float* depths, const int* radii, uint6
     return hipGetLastError();
 }
 
-hipError_t launch_depth_words(int n, const float* depths, const int* radii, uint64_t* words, hipStream_t s) {
+hipError_t launch_depth_words(int n, const float* depths, const int* radii, uint64_t* words, hipStream_t s,
+                              const float* means2D, int gx, int gy, uint32_t* rects) {
     if (n <= 0) return hipSuccess;
-    depth_words_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, depths, radii, words);
+    depth_words_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, depths, radii, words,
+                                                       reinterpret_cast<const float2*>(means2D), gx, gy, rects);
     return hipGetLastError();
 }
 
