@@ -173,3 +173,25 @@ def test_eval3d_fast_binning_full_size(hip):
     assert torch.equal(fv, vs)
     assert torch.equal(fk, ks)
     assert torch.equal(fr, ranges)
+
+
+def test_eval3d_fast_binning_errors(hip):
+    """gs_bin_tiles_3d refuses a pair buffer smaller than the pair count
+    (GS_ERR_CAPACITY, the count still reported) and views wider than 255 tiles
+    (GS_ERR_UNSUPPORTED: the stage path serves those)."""
+    from gs_hip import GsError
+    scene = gaussians_normal(3000, 41, 0, sigma=0.8, logscale=(-3.0, 0.4))
+    cam = make_camera((0.0, 0.0, -3.0), 96, 64, 60.0)
+    cfg = RenderCfg(training=False, sh_degree=0)
+    p = hip.upload_scene(scene)
+    pre, pre3d = hip.preprocess_3d(p, cam, cfg, raw=True)
+    K = int(pre["tiles"].to(torch.int64).sum().item())
+    assert K > 8
+    with pytest.raises(GsError, match="capacity"):
+        hip.bin_tiles_3d(pre, pre3d, cam, capacity=K - 1)
+    vs, ks, rg, K2 = hip.bin_tiles_3d(pre, pre3d, cam, capacity=K)
+    assert K2 == K
+    wide = make_camera((0.0, 0.0, -3.0), 4100, 64, 60.0)
+    pre_w, pre3d_w = hip.preprocess_3d(p, wide, cfg, raw=True)
+    with pytest.raises(GsError, match="255"):
+        hip.bin_tiles_3d(pre_w, pre3d_w, wide, capacity=1 << 20)
