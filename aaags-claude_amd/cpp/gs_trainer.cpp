@@ -536,6 +536,10 @@ extern "C" int gs_trainer_step(gs_trainer* t, const gs_camera* cam, const gs_ren
         TR_TRY(tr_optimizer(t, tc, iteration));
     }
     TR_TRY(tr_mark(t, GS_STAGE_BACKWARD, false));
+    // out == NULL and no stage timing: the step stays queued on the stream (no
+    // read-back, no wait), so the next step's launches follow it back to back;
+    // the loss stays in the "loss" buffer until the next step
+    if (!out && !t->timer.mask) return GS_OK;
     double loss = 0.0;
     hipError_t e = hipMemcpyAsync(&loss, v.loss.p, 8, hipMemcpyDeviceToHost, c->stream);
     if (e != hipSuccess) return gs_hip_status(e, "hipMemcpyAsync(loss)");

@@ -647,12 +647,15 @@ class GsTrainer:
         self.hip._check(self.lib.gs_trainer_set_params(self.tr, *[a.ctypes.data_as(_vp) for a in arrs], 1),
                         "gs_trainer_set_params")
 
-    def step(self, cam, cfg, gt, tc, iteration):
+    def step(self, cam, cfg, gt, tc, iteration, sync=True):
+        """gs_trainer_step; sync=False passes no result (the step stays queued,
+        the loss is not read back) and returns None."""
         res = _StepResult()
         c, f = cam.to_c(), cfg.to_c()
         self.hip._check(self.lib.gs_trainer_step(self.tr, ctypes.byref(c), ctypes.byref(f), _vp(ptr(gt)),
-                                                 ctypes.byref(tc), iteration, ctypes.byref(res)), "gs_trainer_step")
-        return res
+                                                 ctypes.byref(tc), iteration, ctypes.byref(res) if sync else None),
+                        "gs_trainer_step")
+        return res if sync else None
 
     def set_options(self, fused_adam=False):
         """fused_adam: Adam inside the preprocess backward (gradient buffers not written)."""

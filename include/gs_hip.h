@@ -393,7 +393,10 @@ void* gs_trainer_buffer(gs_trainer* tr, const char* which);
 
 /* Trainer::step (trainer.h:21-23) / TrainerGPU::step (trainer_gpu.h:27-29):
  * one single-view step.  The pair count is read back to size the pair
- * buffers, as the reference GPU stages do; gt is a device HWC image. */
+ * buffers, as the reference GPU stages do; gt is a device HWC image.  With
+ * out == NULL (and no stage timing) the step returns without reading the
+ * loss back or waiting for the stream, so consecutive steps run back to back;
+ * the loss stays in gs_trainer_buffer(tr, "loss") (double) until the next. */
 int gs_trainer_step(gs_trainer* tr, const gs_camera* cam, const gs_render_cfg* cfg, const float* gt,
                     const gs_train_cfg* tc, int iteration, gs_step_result* out);
 
