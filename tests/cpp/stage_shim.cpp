@@ -2,6 +2,7 @@
 // (gs3d_hip.h), so the tests can run the reference-shaped stage objects on
 // host arrays and compare them with the oracle.  Built into
 // tests/cpp/_build/libgs_stage_shim.so, linked against libgs_hip.so.
+#include <cstdint>
 #include <cstring>
 #include <stdexcept>
 #include <memory>
@@ -322,4 +323,41 @@ extern "C" int shim_trainer_step_densify(void* h, void* params, const gs_camera*
         *loss = r.loss;
         *n_out = r.n_gaussians;
     })
+}
+
+// FrameAllocator's contract as the reference's tests/test_types.cpp:10-41
+// states it (host only, no device call): 64-byte aligned blocks, a
+// std::runtime_error on exhaustion, the whole arena reusable after reset().
+// Returns 0 when every check holds, else the number of the failed check.
+extern "C" int shim_frame_allocator_contract() {
+    FrameAllocator a(1024);
+    if (a.capacity() < 1024 || a.used() != 0) return 1;
+    void* p = a.allocate(10);
+    void* q = a.allocate(7);
+    if (reinterpret_cast<uintptr_t>(p) % 64 || reinterpret_cast<uintptr_t>(q) % 64) return 2;
+    if (static_cast<char*>(q) - static_cast<char*>(p) != 64 || a.used() != 64 + 7) return 3;
+    float* f = a.allocate_array<float>(16);
+    if (reinterpret_cast<uintptr_t>(f) % 64) return 4;
+    a.reset();
+    if (a.used() != 0 || a.allocate(1) != p) return 5;
+    a.reset();
+    if (a.allocate(a.capacity()) != p || a.used() != a.capacity()) return 6;
+    try {
+        a.allocate(1);
+        return 7;
+    } catch (const std::runtime_error&) {
+    }
+    FrameAllocator small(64);
+    try {
+        small.allocate(128);
+        return 8;
+    } catch (const std::runtime_error&) {
+    }
+    if (small.used() != 0) return 9;
+    try {
+        small.allocate(SIZE_MAX - 8);  // no wrap-around past the end
+        return 10;
+    } catch (const std::runtime_error&) {
+    }
+    return 0;
 }
