@@ -430,7 +430,13 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 // (rasterizer_backward_cpu.cpp:85), with the hardware
                 // reciprocal (1 ulp): gradients need 1e-3, not bits
                 const float Ti = (oma > 1e-6f) ? T * __builtin_amdgcn_rcpf(oma) : 0.0f;
-                const float dal = Ti * ((cl.x - acc0) * dC0 + (cl.y - acc1) * dC1 + (cl.z - acc2) * dC2);
+                float dal;
+                {
+                    // gradient-only arithmetic: contracted to FMAs (the
+                    // contributor decisions above stay uncontracted)
+#pragma clang fp contract(fast)
+                    dal = Ti * ((cl.x - acc0) * dC0 + (cl.y - acc1) * dC1 + (cl.z - acc2) * dC2);
+                }
                 // dL/d(opacity * G) = dal * G; dL/dpower = opacity times that,
                 // and the opacity is applied once per Gaussian when the sums
                 // leave LDS (alpha >= 0.99 is the clamp: no gradient)
@@ -449,9 +455,12 @@ __device__ __forceinline__ void bwd_tile(const RasterArgs& a, int tile, BwdSmem<
                 gv[8] = dop;
                 // colour behind the next entry: alpha c + (1 - alpha) acc
                 const float ac = contrib ? alpha : 0.0f;
-                acc0 += ac * (cl.x - acc0);
-                acc1 += ac * (cl.y - acc1);
-                acc2 += ac * (cl.z - acc2);
+                {
+#pragma clang fp contract(fast)
+                    acc0 += ac * (cl.x - acc0);
+                    acc1 += ac * (cl.y - acc1);
+                    acc2 += ac * (cl.z - acc2);
+                }
                 T = contrib ? Ti : T;
                 if (__ballot(contrib) != 0ull) {
                     const float v = wave_reduce9(gv, role);

@@ -120,10 +120,12 @@ def test_batched_step_at_full_size(hip, workload):
     atomics, whose order changes from run to run; for the few very large
     Gaussians of the 6M scene (screen radii up to ~750 px, thousands of tiles
     of partly cancelling terms) that order alone moves a gradient by more than
-    1e-3 of the array's largest entry.  So each path runs twice: the
+    1e-3 of the array's largest entry, by an amount that itself varies by 10x
+    between runs.  So each path runs three times, and the run-to-run spread
+    is the largest difference among the six same-path pairs: the
     batched-vs-single difference must stay within 1e-3 of the largest entry
-    plus four times the larger run-to-run difference, and at most 1e-5 of the
-    entries may exceed the plain 1e-3 bar."""
+    plus four times that spread, and at most 1e-5 of the entries may exceed
+    the plain 1e-3 bar."""
     from gs_hip import GsBatch, GsTrainer, train_cfg
     print(f"[case] {workload} batched step: pairs, render, gradient", flush=True)
     scene, cams, gts = make_workload(workload)
@@ -161,16 +163,17 @@ def test_batched_step_at_full_size(hip, workload):
         return ref, losses
 
     b1, losses = batched(True)
-    b2, _ = batched(False)
+    bs = [b1] + [batched(False)[0] for _ in range(2)]
     s1, single_losses = single()
-    s2, _ = single()
+    ss = [s1] + [single()[0] for _ in range(2)]
+    pairs = [(0, 1), (0, 2), (1, 2)]
     print(f"  K={sum(tiles)} views={len(cams)} losses {np.round(losses, 6).tolist()}", flush=True)
     assert np.allclose(losses, single_losses, rtol=0, atol=1e-6), (losses, single_losses)
     fails = []
     for k in NAMES:
         top = np.abs(s1[k]).max()
         err = np.abs(b1[k] - s1[k])
-        noise = max(np.abs(b1[k] - b2[k]).max(), np.abs(s1[k] - s2[k]).max())
+        noise = max(max(np.abs(bs[i][k] - bs[j][k]).max(), np.abs(ss[i][k] - ss[j][k]).max()) for i, j in pairs)
         over = int((err > 1e-3 * top).sum())
         print(f"  grad_{k}: max|b-s|/max|s| {err.max() / top:.2e}; run-to-run {noise / top:.2e}; "
               f"entries over 1e-3: {over} of {err.size}", flush=True)
