@@ -22,8 +22,11 @@ using gs::DevBuf;
 
 namespace gs {
 
+// Grows the buffer to at least `need` bytes; always leaves an allocation
+// (256 bytes at least), so that buffers sized by an empty scene or view are
+// valid, non-null pointers for the argument checks.
 int DevBuf::ensure(size_t need) {
-    if (need <= bytes) return GS_OK;
+    if (p && need <= bytes) return GS_OK;
     if (p) {
         hipError_t e = hipFree(p);
         if (e != hipSuccess) return gs_hip_status(e, "hipFree");

@@ -456,6 +456,7 @@ using namespace gsd;
 // ---- launchers (called from capi.hip / the drivers) ----
 hipError_t launch_preprocess_fwd(const gs_gaussians& G, const CamArgs& c, const gs_preprocessed& out, int deg,
                                  hipStream_t s) {
+    if (G.count <= 0) return hipSuccess;  // no empty grids
     dim3 grid((G.count + 255) / 256), block(256);
     GS_DISPATCH_SH(G.max_coeffs, deg, (preprocess_fwd_kernel<MC, DG><<<grid, block, 0, s>>>(G, c, out)));
     return hipGetLastError();
@@ -463,12 +464,14 @@ hipError_t launch_preprocess_fwd(const gs_gaussians& G, const CamArgs& c, const 
 
 hipError_t launch_activate(const gs_gaussians& G, float* pos, float* sc, float* rot, float* sh, float* op,
                            hipStream_t s) {
+    if (G.count <= 0) return hipSuccess;
     activate_kernel<<<(G.count + 255) / 256, 256, 0, s>>>(G, pos, sc, rot, sh, op);
     return hipGetLastError();
 }
 
 hipError_t launch_preprocess_bwd(const gs_gaussians& G, const CamArgs& c, const int* radii, const gs_raster_grads& rg,
                                  const gs_param_grads& out, int accumulate, int sh_deg, hipStream_t s) {
+    if (G.count <= 0) return hipSuccess;
     dim3 grid((G.count + 255) / 256), block(256);
     GS_DISPATCH_SH(G.max_coeffs, sh_deg,
                    (preprocess_bwd_kernel<MC, DG><<<grid, block, 0, s>>>(G, c, radii, rg, out, accumulate)));
@@ -478,6 +481,7 @@ hipError_t launch_preprocess_bwd(const gs_gaussians& G, const CamArgs& c, const 
 hipError_t launch_preprocess_fwd_multi(const gs_gaussians& G, const CamArgs* cams_dev, int V,
                                        const gs_preprocessed& out, uint64_t* dkeys, uint32_t* didx, int deg,
                                        hipStream_t s, uint32_t* rects) {
+    if (G.count <= 0) return hipSuccess;
     dim3 grid((G.count + 255) / 256), block(256);
     GS_DISPATCH_SH(G.max_coeffs, deg,
                    (preprocess_fwd_multi_kernel<MC, DG>
@@ -493,6 +497,7 @@ hipError_t launch_preprocess_bwd_multi(const gs_gaussians& G, const CamArgs* cam
                                        float* const adam_p[5], float* const adam_m[5], float* const adam_v[5],
                                        const gso::AdamDev* hp, const gso::AdamDev* hp_val, int sh_deg, hipStream_t s,
                                        const uint32_t* skip) {
+    if (G.count <= 0) return hipSuccess;
     dim3 grid((G.count + 255) / 256), block(256);
     CamArgs cam0{};
     if (!cams_dev) {
