@@ -126,3 +126,26 @@ def test_batch_with_a_view_that_sees_nothing(hip, oracle):
             want = sum(r["grads"][k] for r in refs)
             assert_grad_close(tr.buffer("grad_" + k, sh[k]).cpu().numpy(), want, "grad_" + k)
         del b, tr
+
+
+def test_batch_on_an_empty_scene(hip, oracle):
+    """No Gaussians at all in a batched, graph-captured step: every view is the
+    background and its loss the oracle's."""
+    from gs_hip import GsBatch, GsTrainer, train_cfg
+    print("[case] batch, empty scene", flush=True)
+    s, _, _ = random_small_scene(1, 3, 908)
+    scene = RawScene(s.pos[:0], s.log_scale[:0], s.rot[:0], s.sh[:0], s.opacity_logit[:0], 3)
+    rng = np.random.Generator(np.random.Philox(909))
+    cams = [make_camera(sphere_eye(rng, 4.0), w, h, 60.0) for (w, h) in [(40, 30), (17, 9)]]
+    gts = [target_image(c.width, c.height, 910 + i) for i, c in enumerate(cams)]
+    cfg = RenderCfg(sh_degree=3, training=True, bg_color=BG)
+    tr = GsTrainer(hip, scene)
+    b = GsBatch(tr, cams, cfg, gts, persistent=True, fused_adam=True, use_graph=True)
+    for it in range(2):
+        b.step(train_cfg(lr=1e-3), it, sync=True)
+    _, losses = b.result()
+    for v, (c, g) in enumerate(zip(cams, gts)):
+        ref = oracle.forward_backward(scene, c, cfg, g)
+        assert abs(losses[v] - ref["loss"]) <= 1e-6, (v, losses[v], ref["loss"])
+        img = b.images()[v].cpu().numpy()
+        assert np.array_equal(img, np.broadcast_to(np.array(BG, np.float32), img.shape)), v
