@@ -28,8 +28,10 @@ struct DeviceArray {
     ~DeviceArray() {
         if (p) (void)hipFree(p);
     }
+    // always leaves an allocation (256 bytes at least), so that arrays sized
+    // by an empty scene are valid, non-null device pointers
     void* ensure(size_t need) {
-        if (need > bytes) {
+        if (!p || need > bytes) {
             if (p) hip_check(hipFree(p), "hipFree");
             p = nullptr;
             bytes = 0;

@@ -103,9 +103,20 @@ def load_library(path: str = LIB_PATH):
     return _LIB
 
 
+_PLACEHOLDER = {}
+
+
 def ptr(t) -> int | None:
+    """Device address of a tensor for the C ABI.  The ABI takes no null array
+    pointers, even for an empty scene or view, but torch reports address 0
+    for every tensor without elements; those get a 256-byte allocation of
+    their device (never read or written through a count of zero)."""
     if t is None:
         return None
+    if t.numel() == 0:
+        if t.device not in _PLACEHOLDER:
+            _PLACEHOLDER[t.device] = torch.empty(64, dtype=torch.float32, device=t.device)
+        return _PLACEHOLDER[t.device].data_ptr()
     return t.data_ptr()
 
 

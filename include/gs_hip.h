@@ -19,6 +19,13 @@
  *   values     uint32[K] = Gaussian index
  *   ranges     uint32[num_tiles*2] = [start, end) into the sorted pair list
  *
+ * Array pointers are never null, also for an empty scene or view (N == 0,
+ * K == 0): pass any valid device allocation there; it is not read or written
+ * through a count of zero.  The raster entry points take no Gaussian count,
+ * so a null per-Gaussian array is rejected (GS_ERR_INVALID) rather than
+ * taken to mean an empty scene.  Entry points that take the count return at
+ * once for N == 0, before their array checks.
+ *
  * Each declaration cites the reference interface it replaces.
  */
 #ifndef GS_HIP_H

@@ -79,6 +79,10 @@ def test_density_unit_cases_match_reference(shim_loaded):
         a[4][:] = inv_sigmoid(0.001)
     (mr, _), (mh, _) = _case([R, H], 3, np.zeros(6), all_low)
     assert mr == mh == 0
+    # no Gaussians at all
+    (mr, pr), (mh, ph) = _case([R, H], 0, np.zeros(0), None, min_opacity=0.001)
+    assert mr == mh == 0
+    _same(pr, ph, "empty")
     # the screen-size rule reads the statistics' largest radius
     (mr, pr), (mh, ph) = _case([R, H], 5, np.zeros(10), None, radii=np.array([0, 25, 3, 40, 20], np.int32))
     assert mr == mh == 3

@@ -149,3 +149,17 @@ def test_batch_on_an_empty_scene(hip, oracle):
         assert abs(losses[v] - ref["loss"]) <= 1e-6, (v, losses[v], ref["loss"])
         img = b.images()[v].cpu().numpy()
         assert np.array_equal(img, np.broadcast_to(np.array(BG, np.float32), img.shape)), v
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_eval3d_on_an_empty_scene(hip, fast):
+    """The eval_3D forward with no Gaussians: no pairs, the background."""
+    print("[case] eval3d empty scene", fast, flush=True)
+    s, cam, _ = random_small_scene(1, 3, 911, width=40, height=24)
+    scene = RawScene(s.pos[:0], s.log_scale[:0], s.rot[:0], s.sh[:0], s.opacity_logit[:0], 3)
+    cfg = RenderCfg(sh_degree=3, training=False, bg_color=BG)
+    out = hip.render_3d(hip.upload_scene(scene), cam, cfg, raw=True, fast=fast)
+    torch.cuda.synchronize()
+    assert out["K"] == 0
+    img = out["img"].cpu().numpy()
+    assert np.array_equal(img, np.broadcast_to(np.array(BG, np.float32), img.shape))

@@ -40,7 +40,7 @@ def _activated(oracle, scene):
     return [np.ascontiguousarray(g[k], np.float32) for k in ["pos", "sh", "scale", "rot", "op"]]
 
 
-@pytest.mark.parametrize("n,deg,seed", [(400, 3, 81), (2500, 1, 82)])
+@pytest.mark.parametrize("n,deg,seed", [(400, 3, 81), (2500, 1, 82), (0, 3, 83)])
 def test_cpp_renderer_matches_oracle(shim, oracle, n, deg, seed):
     print("[case] cpp renderer", n, deg, seed, flush=True)
     scene, cam, gt = random_small_scene(n, deg, seed, width=120, height=80)
