@@ -339,12 +339,21 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep_kernel(const uint64_t* 
     for (int i = threadIdx.x; i < RS_WAVES * RADIX; i += RS_THREADS) (&s_h[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t wbase = blockIdx.x * RS_TILE + w * RS_WAVE_SPAN;
+    // every round's key is loaded before any is counted: one memory latency
+    // per wave instead of one per round
+    uint64_t k_r[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        k_r[r] = i < n ? keys[i] : 0ull;
+    }
+#pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         uint32_t i = wbase + r * 64 + l;
         bool valid = i < n;
         uint64_t active = __ballot(valid);
         if (active == 0) break;
-        uint32_t d = valid ? digit_of<BITS>(keys[i], shift) : 0u;
+        uint32_t d = valid ? digit_of<BITS>(k_r[r], shift) : 0u;
         uint64_t peers = match_digit<BITS>(d, active);
         if (valid && (peers & lanemask_lt()) == 0) s_h[w][d] += (uint32_t)__popcll(peers);
     }
@@ -371,19 +380,31 @@ __global__ __launch_bounds__(64) void rs_digit_scan_kernel(const uint32_t* __res
     const uint32_t nq = hs / 4u;
     const int l = lane_id();
     uint32_t carry = 0u;
-    for (uint32_t q0 = 0; q0 < nq; q0 += 64u) {
-        const uint32_t q = q0 + (uint32_t)l;
-        uint4 x = q < nq ? h[q] : make_uint4(0u, 0u, 0u, 0u);
-        const uint32_t i0 = 4u * q;  // counts past the last block are padding
-        if (i0 >= nb) x.x = 0u;
-        if (i0 + 1u >= nb) x.y = 0u;
-        if (i0 + 2u >= nb) x.z = 0u;
-        if (i0 + 3u >= nb) x.w = 0u;
-        const uint32_t tot = x.x + x.y + x.z + x.w;
-        const uint32_t inc = wave_inclusive_scan(tot);
-        const uint32_t ex = carry + inc - tot;
-        if (q < nq) o[q] = make_uint4(ex, ex + x.x, ex + x.x + x.y, ex + x.x + x.y + x.z);
-        carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+    // DS_LOADS steps of 64 quads are loaded before they are scanned, so the
+    // loads of a row (1024 blocks per group of steps) overlap
+    constexpr int DS_LOADS = 4;
+    for (uint32_t q00 = 0; q00 < nq; q00 += 64u * DS_LOADS) {
+        uint4 xs[DS_LOADS];
+#pragma unroll
+        for (int j = 0; j < DS_LOADS; j++) {
+            const uint32_t q = q00 + 64u * j + (uint32_t)l;
+            xs[j] = q < nq ? h[q] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int j = 0; j < DS_LOADS; j++) {
+            const uint32_t q = q00 + 64u * j + (uint32_t)l;
+            uint4 x = xs[j];
+            const uint32_t i0 = 4u * q;  // counts past the last block are padding
+            if (i0 >= nb) x.x = 0u;
+            if (i0 + 1u >= nb) x.y = 0u;
+            if (i0 + 2u >= nb) x.z = 0u;
+            if (i0 + 3u >= nb) x.w = 0u;
+            const uint32_t tot = x.x + x.y + x.z + x.w;
+            const uint32_t inc = wave_inclusive_scan(tot);
+            const uint32_t ex = carry + inc - tot;
+            if (q < nq) o[q] = make_uint4(ex, ex + x.x, ex + x.x + x.y, ex + x.x + x.y + x.z);
+            carry += (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+        }
     }
     if (l == 0) dtot[blockIdx.x] = carry;
 }
@@ -427,14 +448,19 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep_kernel(const uint64_t
     uint32_t v_r[VALS ? ROUNDS : 1];
     uint32_t rank_r[ROUNDS];
     uint32_t d_r[ROUNDS];
+    // every round's key (and value) is loaded before the ranking starts
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = wbase + r * 64 + l;
+        k_r[r] = i < n ? keys_in[i] : 0ull;
+        if constexpr (VALS) v_r[r] = i < n ? vals_in[i] : 0u;
+    }
     // 1. per-wave stable ranks: rounds in key order, lanes in key order.
 #pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         uint32_t i = wbase + r * 64 + l;
         bool valid = i < n;
         uint64_t active = __ballot(valid);
-        k_r[r] = valid ? keys_in[i] : 0ull;
-        if constexpr (VALS) v_r[r] = valid ? vals_in[i] : 0u;
         uint32_t d = digit_of<BITS>(k_r[r], shift);
         d_r[r] = d;
         rank_r[r] = 0u;
