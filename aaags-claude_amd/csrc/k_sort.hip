@@ -872,7 +872,7 @@ hipError_t launch_duplicate(int n, const float* means2D, const float* depths, co
     return hipGetLastError();
 }
 
-static int radix_rounds(uint32_t n) { return n >= RS_SMALL_LIMIT ? 16 : 4; }
+static int radix_rounds(uint32_t n) { return n >= RS_SMALL_LIMIT ? 16 : 8; }
 uint32_t radix_blocks(uint32_t n) {
     uint32_t tile = (uint32_t)(RS_THREADS * radix_rounds(n));
     return (n + tile - 1) / tile;
@@ -985,6 +985,7 @@ static hipError_t sort_pass(const uint64_t* ki, const uint32_t* vi, uint64_t* ko
                             const uint32_t* n_dev, int shift, uint32_t* hist, uint32_t* goff, uint32_t* dtot,
                             hipStream_t s) {
     if (radix_rounds(n) == 16) return sort_pass_r<BITS, 16>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s);
+    if (radix_rounds(n) == 8) return sort_pass_r<BITS, 8>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s);
     return sort_pass_r<BITS, 4>(ki, vi, ko, vo, n, n_dev, shift, hist, goff, dtot, s);
 }
 
