@@ -549,7 +549,7 @@ constexpr int OS_WINDOW = 4;
 // pass in front of the same scatter kernel (false; 16% slower at 47M keys)
 constexpr bool OS_LOOKBACK = true;
 // the smallest sort that takes the one-sweep passes (below it, e.g. the ~1M-key
-// depth sort, the three-launch passes on 1024-key blocks measured faster)
+// depth sort, the three-launch passes on 2048-key blocks measured faster)
 constexpr uint32_t OS_MIN_KEYS = RS_SMALL_LIMIT;
 
 struct PassShifts {
