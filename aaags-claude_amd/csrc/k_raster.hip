@@ -232,11 +232,16 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 const bool room = test_T >= 0.0001f;  // T and alpha are finite here
                 const bool take = use && room;
                 done = done || (use && !room);
-                // colour += c alpha T, with the weight alpha T selected in or out
+                // colour += c alpha T, with the weight alpha T selected in or out;
+                // the colour feeds no decision (T and alpha do), so its sums are
+                // contracted to FMAs
                 const float wt = take ? alpha * T : 0.0f;
-                C0 += cl.x * wt;
-                C1 += cl.y * wt;
-                C2 += cl.z * wt;
+                {
+#pragma clang fp contract(fast)
+                    C0 += cl.x * wt;
+                    C1 += cl.y * wt;
+                    C2 += cl.z * wt;
+                }
                 T = take ? test_T : T;
                 cnt += take ? 1 : 0;
                 last = take ? base + (uint32_t)k + 1u : last;
