@@ -410,7 +410,28 @@ def cpu_baseline(scene, cam, cfg, gt, threads=1):
         out["all_cores"] = {"value": round(threads / dtn, 5), "unit": "train steps/s", "cores": threads,
                             "sample": f"{threads} independent {what}, one per core, at once",
                             "seconds": round(dtn, 2)}
+    out["reference_opencl"] = reference_opencl_status()
     return out
+
+
+def reference_opencl_status():
+    """Whether the reference's OpenCL kernels could be timed on a CPU OpenCL
+    device here: that needs the reference tree (its .cl files, named by
+    REFERENCE_PATH, never copied into this repository) and a CPU OpenCL
+    platform such as PoCL.  Reports which is missing; nothing is loaded."""
+    import glob
+    ref = os.environ.get("REFERENCE_PATH", "")
+    has_ref = bool(ref) and os.path.isdir(os.path.join(ref, "harmonyos_3dgs", "src", "gpu", "kernels"))
+    icds = [os.path.basename(f) for f in glob.glob("/etc/OpenCL/vendors/*.icd")]
+    has_cpu_cl = any("pocl" in f.lower() for f in icds)
+    if has_ref and has_cpu_cl:
+        return "available (not run by this benchmark)"
+    missing = []
+    if not has_ref:
+        missing.append("no reference tree (REFERENCE_PATH unset or without the .cl kernels)")
+    if not has_cpu_cl:
+        missing.append("no CPU OpenCL platform (ICDs: " + (", ".join(icds) or "none") + ")")
+    return "reference OpenCL on PoCL: unavailable: " + "; ".join(missing)
 
 
 def traffic_probe(args):

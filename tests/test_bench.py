@@ -56,3 +56,10 @@ def test_committed_traffic_summary_is_readable():
     for batch in ("64", "128", "256"):
         assert st["raster_bwd"]["valu_wave_insts_by_batch"][batch] > 0
     assert b.load_traffic("no_such_workload") == {}
+
+
+def test_reference_opencl_status_names_what_is_missing(monkeypatch):
+    b = _bench()
+    monkeypatch.delenv("REFERENCE_PATH", raising=False)
+    s = b.reference_opencl_status()
+    assert s.startswith("reference OpenCL on PoCL: unavailable") and "reference tree" in s
