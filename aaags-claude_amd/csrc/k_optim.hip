@@ -113,6 +113,17 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, flo
     p -= lr * mh / (sqrtf(vh) + a.eps);
 }
 
+// Non-temporal 16-byte load / store (the builtins take native vector types).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 nt_load4(const float4* p) {
+    const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void nt_store4(float4 x, float4* p) {
+    const nt_f4 v = {x.x, x.y, x.z, x.w};
+    __builtin_nontemporal_store(v, reinterpret_cast<nt_f4*>(p));
+}
+
 __global__ __launch_bounds__(256) void adam_kernel(AdamArgs args) {
     if (args.skip && *args.skip) return;
     const AdamSeg& s = args.seg[blockIdx.y];
@@ -129,15 +140,17 @@ __global__ __launch_bounds__(256) void adam_kernel(AdamArgs args) {
     const float4* g4 = reinterpret_cast<const float4*>(s.g);
     float4* m4 = reinterpret_cast<float4*>(s.m);
     float4* v4 = reinterpret_cast<float4*>(s.v);
+    // the moments and gradients are touched once per step: non-temporal, so
+    // that the parameters the next forward reads stay in the caches
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
-        float4 p = p4[i], g = g4[i], m = m4[i], v = v4[i];
+        float4 p = p4[i], g = nt_load4(g4 + i), m = nt_load4(m4 + i), v = nt_load4(v4 + i);
         adam1(p.x, g.x, m.x, v.x, lr, a);
         adam1(p.y, g.y, m.y, v.y, lr, a);
         adam1(p.z, g.z, m.z, v.z, lr, a);
         adam1(p.w, g.w, m.w, v.w, lr, a);
         p4[i] = p;
-        m4[i] = m;
-        v4[i] = v;
+        nt_store4(m, m4 + i);
+        nt_store4(v, v4 + i);
     }
     if (blockIdx.x == 0) {
         for (uint32_t i = n4 * 4 + threadIdx.x; i < n; i += blockDim.x) {

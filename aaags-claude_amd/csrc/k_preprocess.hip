@@ -231,15 +231,15 @@ __device__ __forceinline__ void adam_run(float* __restrict__ p, float* __restric
 #pragma unroll
     for (int k = 0; k < K; k++) {
         pp[k] = p[k0 + k];
-        mm[k] = m[k0 + k];
-        vv[k] = v[k0 + k];
+        mm[k] = __builtin_nontemporal_load(m + k0 + k);
+        vv[k] = __builtin_nontemporal_load(v + k0 + k);
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
         pp[k] = adam_math(pp[k], &mm[k], &vv[k], g[k], lr, h);
         p[k0 + k] = pp[k];
-        m[k0 + k] = mm[k];
-        v[k0 + k] = vv[k];
+        __builtin_nontemporal_store(mm[k], m + k0 + k);
+        __builtin_nontemporal_store(vv[k], v + k0 + k);
     }
 }
 
@@ -292,10 +292,16 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
             preprocess_geom(P.p, P.s, P.q, c, g);
             float gpos[3] = {0.0f, 0.0f, 0.0f};
             float gsh[E];
-            const float drgb[3] = {rg.d_rgb[o * 3], rg.d_rgb[o * 3 + 1], rg.d_rgb[o * 3 + 2]};
+            // the screen-space sums are read once: non-temporal, so that they
+            // do not push the parameters out of the caches
+            const float drgb[3] = {__builtin_nontemporal_load(rg.d_rgb + o * 3),
+                                   __builtin_nontemporal_load(rg.d_rgb + o * 3 + 1),
+                                   __builtin_nontemporal_load(rg.d_rgb + o * 3 + 2)};
             sh_backward_from<MAXC, DEG>(sh, sh_dir(P.p, c.campos), drgb, gsh, gpos);
-            RGrad r{rg.d_means2D[o * 2], rg.d_means2D[o * 2 + 1], rg.d_conics[o * 3], rg.d_conics[o * 3 + 1],
-                    rg.d_conics[o * 3 + 2], rg.d_opacities_2d[o]};
+            RGrad r{__builtin_nontemporal_load(rg.d_means2D + o * 2), __builtin_nontemporal_load(rg.d_means2D + o * 2 + 1),
+                    __builtin_nontemporal_load(rg.d_conics + o * 3), __builtin_nontemporal_load(rg.d_conics + o * 3 + 1),
+                    __builtin_nontemporal_load(rg.d_conics + o * 3 + 2),
+                    __builtin_nontemporal_load(rg.d_opacities_2d + o)};
             float gsc[3], grot[4], gop;
             geom_backward(P, g, c, r, gpos, gsc, grot, gop);
 #pragma unroll
@@ -364,8 +370,8 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
                             const int gl = j / E, k = j - gl * E;
                             pp[u] = prows[gl * ROW + k];
                             gg[u] = grows[gl * ROW + k];
-                            mm[u] = m[j];
-                            vv[u] = v[j];
+                            mm[u] = __builtin_nontemporal_load(m + j);
+                            vv[u] = __builtin_nontemporal_load(v + j);
                         }
                     }
 #pragma unroll
@@ -374,8 +380,8 @@ __global__ __launch_bounds__(256, 2) void preprocess_bwd_multi_kernel(gs_gaussia
                         if (j < total) {
                             pp[u] = adam_math(pp[u], &mm[u], &vv[u], gg[u], h.lr[3], h);
                             p[j] = pp[u];
-                            m[j] = mm[u];
-                            v[j] = vv[u];
+                            __builtin_nontemporal_store(mm[u], m + j);
+                            __builtin_nontemporal_store(vv[u], v + j);
                         }
                     }
                 }
