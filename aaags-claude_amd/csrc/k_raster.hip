@@ -229,9 +229,10 @@ __device__ __forceinline__ void fwd_tile(const RasterArgs& a, int tile, FwdSmem<
                 // (without this entry) once T would fall below 1e-4
                 const bool use = !done && power <= 0.0f && alpha >= 1.0f / 255.0f;
                 const float test_T = T * (1.0f - alpha);
-                const bool room = test_T >= 0.0001f;  // T and alpha are finite here
-                const bool take = use && room;
-                done = done || (use && !room);
+                const bool take = use && test_T >= 0.0001f;  // T and alpha are finite here
+                // used but no room left (one compare: the negated compare
+                // would be a second, since the compiler keeps NaN semantics)
+                done = done || (use != take);
                 // colour += c alpha T, with the weight alpha T selected in or out;
                 // the colour feeds no decision (T and alpha do), so its sums are
                 // contracted to FMAs
