@@ -530,7 +530,12 @@ struct Entry3 {
 };
 static_assert(sizeof(Entry3) == 112, "one LDS record per Gaussian");
 
-constexpr int R3_BATCH = 256;
+// Entries staged per round.  The k-buffers take 48 KB of LDS per workgroup;
+// 32 staged entries (3.5 KB) keep a workgroup at 52 KB, so three fit a CU
+// (256 entries, 28 KB, allowed two): the kernel is latency-bound, and the third
+// workgroup per CU hides more than the extra staging rounds cost
+// (3.75 -> 2.82 ms at 1M Gaussians, 1080p).
+constexpr int R3_BATCH = 32;
 
 // One workgroup per 16x16 tile, one pixel per lane (rasterizer_cpu.cpp:61-135).
 // Each lane's k-buffer is a 16-slot ring in LDS (slot-major, so the lanes of
@@ -569,7 +574,7 @@ __global__ __launch_bounds__(256) void raster3d_kernel(Cam3 c, const uint32_t* _
         // the workgroup stops once every pixel is finished
         if (__syncthreads_count(done ? 0 : 1) == 0) break;
         const uint32_t j = base + t;
-        if (j < r1) {
+        if (t < R3_BATCH && j < r1) {  // threads 0 .. R3_BATCH-1 stage one entry each
             const uint32_t id = values[j];
             const float4* g4 = reinterpret_cast<const float4*>(g2s + (size_t)id * 16);
             Entry3& e = sm[t];
